@@ -1,0 +1,69 @@
+// host_common.hpp -- host-side helpers shared by the entry points of libahead_hip.so:
+// error reporting, argument validation, the workspace and launch geometry.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/ahead_hip.h"
+#include "an_device.hpp"
+
+struct ahead_hip_ws {
+    int device = 0;
+    void *base = nullptr;  // device memory: WsHeader followed by per-operator regions
+    size_t bytes = 0;
+    uint32_t epoch = 0;    // last look-back epoch handed out
+};
+
+namespace ahead_host {
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define AHEAD_HIP_TRY(expr)                                    \
+    do {                                                       \
+        hipError_t _e = (expr);                                \
+        if (_e != hipSuccess) return ahead_host::hip_fail(_e, #expr); \
+    } while (0)
+
+constexpr size_t kHeaderBytes = sizeof(ahead_dev::WsHeader);
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Ensures the workspace holds `bytes` beyond the header.  Growing frees the old allocation
+// (hipFree waits for the device) and zero-fills the new one.
+int ws_ensure(ahead_hip_ws *ws, size_t bytes);
+
+// Next look-back epoch (1 .. 2^30-1).  On wrap-around the status region is cleared.
+int ws_next_epoch(ahead_hip_ws *ws, uint32_t *epoch);
+
+inline char *ws_region(ahead_hip_ws *ws, size_t offset) {
+    return static_cast<char *>(ws->base) + kHeaderBytes + offset;
+}
+inline ahead_dev::WsHeader *ws_header(ahead_hip_ws *ws) {
+    return static_cast<ahead_dev::WsHeader *>(ws->base);
+}
+
+int check_col(const ahead_col *c, const char *name, bool allow_void, bool need_encoded);
+
+inline uint64_t width_mask(uint32_t w) { return w >= 8 ? ~0ull : ((1ull << (8 * w)) - 1ull); }
+inline uint64_t bits_mask(uint32_t b) { return b >= 64 ? ~0ull : ((1ull << b) - 1ull); }
+
+// Look-back launch description: status words for S streams over num_tiles tiles.
+struct LbPlan {
+    uint32_t num_tiles = 0;
+    size_t status_bytes = 0;
+};
+inline LbPlan lb_plan(uint64_t n, uint32_t tile_rows, int streams) {
+    LbPlan p;
+    const uint64_t t = (n + tile_rows - 1) / tile_rows;
+    p.num_tiles = static_cast<uint32_t>(t == 0 ? 1 : t);
+    p.status_bytes = align_up(static_cast<size_t>(p.num_tiles) * streams * sizeof(uint64_t), 256);
+    return p;
+}
+int lb_args(ahead_hip_ws *ws, const LbPlan &p, size_t status_offset, ahead_dev::LbArgs *out);
+
+}  // namespace ahead_host

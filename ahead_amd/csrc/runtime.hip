@@ -1,0 +1,140 @@
+// runtime.hip -- error reporting and the workspace of libahead_hip.so.
+#include <stdio.h>
+
+#include <new>
+#include <string>
+
+#include "host_common.hpp"
+
+namespace ahead_host {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? AHEAD_ERR_NOMEM : AHEAD_ERR_HIP;
+}
+
+int ws_ensure(ahead_hip_ws *ws, size_t bytes) {
+    if (ws == nullptr) return fail(AHEAD_ERR_INVALID, "workspace is NULL");
+    const size_t need = kHeaderBytes + align_up(bytes, 256);
+    if (ws->base != nullptr && ws->bytes >= need) return AHEAD_OK;
+    size_t grow = ws->bytes + ws->bytes / 2;
+    const size_t target = need > grow ? need : grow;
+    int cur = 0;
+    AHEAD_HIP_TRY(hipGetDevice(&cur));
+    if (cur != ws->device) AHEAD_HIP_TRY(hipSetDevice(ws->device));
+    if (ws->base != nullptr) {
+        AHEAD_HIP_TRY(hipFree(ws->base));
+        ws->base = nullptr;
+        ws->bytes = 0;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, target);
+    if (e != hipSuccess) {
+        if (cur != ws->device) (void)hipSetDevice(cur);
+        return hip_fail(e, "hipMalloc(workspace)");
+    }
+    e = hipMemset(p, 0, target);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (cur != ws->device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        return hip_fail(e, "hipMemset(workspace)");
+    }
+    ws->base = p;
+    ws->bytes = target;
+    ws->epoch = 0;
+    return AHEAD_OK;
+}
+
+int ws_next_epoch(ahead_hip_ws *ws, uint32_t *epoch) {
+    if (ws->epoch + 1 >= (1u << 30)) {
+        // Status words of every earlier launch carry an epoch below 2^30; clear them all.
+        AHEAD_HIP_TRY(hipMemset(static_cast<char *>(ws->base) + kHeaderBytes, 0,
+                                ws->bytes - kHeaderBytes));
+        AHEAD_HIP_TRY(hipDeviceSynchronize());
+        ws->epoch = 0;
+    }
+    *epoch = ++ws->epoch;
+    return AHEAD_OK;
+}
+
+int lb_args(ahead_hip_ws *ws, const LbPlan &p, size_t status_offset, ahead_dev::LbArgs *out) {
+    uint32_t epoch = 0;
+    int st = ws_next_epoch(ws, &epoch);
+    if (st != AHEAD_OK) return st;
+    out->hdr = ws_header(ws);
+    out->status = reinterpret_cast<uint64_t *>(ws_region(ws, status_offset));
+    out->num_tiles = p.num_tiles;
+    out->epoch = epoch;
+    return AHEAD_OK;
+}
+
+int check_col(const ahead_col *c, const char *name, bool allow_void, bool need_encoded) {
+    if (c == nullptr) return fail(AHEAD_ERR_INVALID, std::string(name) + ": NULL column");
+    if (c->n > 0xFFFFFFFFull)
+        return fail(AHEAD_ERR_TOO_LARGE, std::string(name) + ": more than 2^32-1 rows");
+    if (c->data == nullptr && !allow_void && c->n > 0)
+        return fail(AHEAD_ERR_INVALID, std::string(name) + ": void column not allowed here");
+    if (c->data != nullptr && c->width != 1 && c->width != 2 && c->width != 4 && c->width != 8)
+        return fail(AHEAD_ERR_INVALID, std::string(name) + ": width must be 1, 2, 4 or 8");
+    if (need_encoded && !c->encoded)
+        return fail(AHEAD_ERR_INVALID, std::string(name) + ": must be an AN-encoded column");
+    if (c->encoded && c->data != nullptr && c->width < 2)
+        return fail(AHEAD_ERR_INVALID, std::string(name) + ": AN codes are 2, 4 or 8 bytes");
+    return AHEAD_OK;
+}
+
+}  // namespace ahead_host
+
+using namespace ahead_host;
+
+extern "C" {
+
+const char *ahead_hip_last_error(void) { return g_last_error.c_str(); }
+
+const char *ahead_hip_version(void) { return "ahead_hip 0.1 gfx950"; }
+
+int ahead_hip_ws_create(int device, ahead_hip_ws **ws) {
+    if (ws == nullptr) return fail(AHEAD_ERR_INVALID, "ws out-pointer is NULL");
+    int count = 0;
+    AHEAD_HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(AHEAD_ERR_INVALID, "no such device");
+    ahead_hip_ws *w = new (std::nothrow) ahead_hip_ws();
+    if (w == nullptr) return fail(AHEAD_ERR_NOMEM, "host allocation failed");
+    w->device = device;
+    const int st = ws_ensure(w, 1 << 20);
+    if (st != AHEAD_OK) {
+        delete w;
+        return st;
+    }
+    *ws = w;
+    return AHEAD_OK;
+}
+
+int ahead_hip_ws_destroy(ahead_hip_ws *ws) {
+    if (ws == nullptr) return AHEAD_OK;
+    if (ws->base != nullptr) {
+        hipError_t e = hipFree(ws->base);
+        ws->base = nullptr;
+        delete ws;
+        if (e != hipSuccess) return hip_fail(e, "hipFree(workspace)");
+        return AHEAD_OK;
+    }
+    delete ws;
+    return AHEAD_OK;
+}
+
+int ahead_hip_ws_reserve(ahead_hip_ws *ws, size_t bytes) { return ws_ensure(ws, bytes); }
+
+size_t ahead_hip_ws_bytes(const ahead_hip_ws *ws) { return ws == nullptr ? 0 : ws->bytes; }
+
+}  // extern "C"

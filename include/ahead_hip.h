@@ -1,0 +1,122 @@
+/*
+ * ahead_hip.h -- C ABI of the MI355X (gfx950) AN-hardened column operators.
+ *
+ * This is the drop-in boundary under the reference's operator templates
+ * (include/column_operators/OperatorsAN.hpp:33-298): every template of that header that lies
+ * on the hardened hot path has one extern "C" entry point here, and the C++ templates in
+ * include/ahead/column_operators/OperatorsAN.hpp marshal BAT<Head,Tail> arguments to them.
+ *
+ * Conventions
+ *  - All column and output pointers are DEVICE pointers (HBM).  No entry point copies data to
+ *    or from the host, and none allocates except through the workspace (ahead_hip_ws), which
+ *    grows on demand.
+ *  - Every entry point is asynchronous on `stream` (a hipStream_t; NULL = the null stream).
+ *  - Variable-length outputs are (pointer, capacity) pairs.  The operator writes element k of
+ *    an output only if k < capacity, and always writes the full element count to d_counts[]
+ *    (device memory).  A count larger than the capacity means "truncated": the caller sizes
+ *    the buffer from the count and calls again (operators are pure functions of their inputs).
+ *  - Error-position vectors (AN_indicator_vector, include/column_operators/ANbase.hpp:32) are
+ *    uint64 arrays of pos * A_OID, in the order the reference appends them.
+ *  - Return value: AHEAD_OK or an ahead_status error code; ahead_hip_last_error() gives text.
+ *  - A workspace must not be used by two streams at the same time.
+ */
+#ifndef AHEAD_HIP_H
+#define AHEAD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    AHEAD_OK = 0,
+    AHEAD_ERR_INVALID = 1,       /* bad argument (width, operator code, NULL pointer ...)      */
+    AHEAD_ERR_SIZE_MISMATCH = 2, /* the reference throws std::runtime_error here               */
+    AHEAD_ERR_HIP = 3,           /* HIP runtime error                                          */
+    AHEAD_ERR_NOMEM = 4,         /* device allocation failed (std::bad_alloc in the reference) */
+    AHEAD_ERR_TOO_LARGE = 5,     /* more than 2^32-1 rows (BAT sizes are oid_t, uint32)        */
+    AHEAD_ERR_DEVICE = 6         /* a kernel reported an internal failure                      */
+} ahead_status;
+
+/* Column descriptor: one BAT side (ColumnDescriptor + ColumnMetaData,
+ * include/column_storage/ColumnDescriptor.hpp:43-102, ColumnMetaData.hpp:150-210). */
+typedef struct {
+    const void *data;    /* device array, or NULL for a void (virtual dense OID) column   */
+    uint64_t n;          /* number of elements                                            */
+    uint32_t width;      /* bytes per element: 1, 2, 4, 8 (void: 4, oid_t)                */
+    uint32_t encoded;    /* 1: AN-encoded type (v2_res*_t), 0: plain                      */
+    uint32_t unenc_bits; /* bits of TypeMap<T>::v2_base_t::type_t (8, 16, 32, 64)         */
+    uint32_t reserved;
+    uint64_t a;          /* ColumnMetaData::AN_A                                          */
+    uint64_t a_inv;      /* ColumnMetaData::AN_Ainv                                       */
+    uint64_t max_u;      /* ColumnMetaData::AN_unencMaxU                                  */
+    uint64_t seqbase;    /* ColumnMetaData::seqbase (void columns)                        */
+} ahead_col;
+
+/* Predicate and combiner codes of selectAN (std::less ... std::not_equal_to; ahead::and_is,
+ * ahead::or_is from include/column_operators/functors.hpp). */
+enum { AHEAD_LT = 0, AHEAD_LE = 1, AHEAD_GT = 2, AHEAD_GE = 3, AHEAD_EQ = 4, AHEAD_NE = 5 };
+enum { AHEAD_COMB_NONE = 0, AHEAD_COMB_AND = 1, AHEAD_COMB_OR = 2 };
+
+typedef struct ahead_hip_ws ahead_hip_ws;
+
+/* ---- runtime --------------------------------------------------------------------------------- */
+const char *ahead_hip_last_error(void);
+const char *ahead_hip_version(void);
+/* Creates a workspace on `device` (device scratch for scan state, hash tables, partials). */
+int ahead_hip_ws_create(int device, ahead_hip_ws **ws);
+int ahead_hip_ws_destroy(ahead_hip_ws *ws);
+/* Grows the workspace to at least `bytes` now (so later calls do not allocate, e.g. under
+ * stream capture). */
+int ahead_hip_ws_reserve(ahead_hip_ws *ws, size_t bytes);
+size_t ahead_hip_ws_bytes(const ahead_hip_ws *ws);
+
+/* ---- selectAN ---------------------------------------------------------------------------------
+ * Replaces ahead::bat::ops::scalar::selectAN<Op> and selectAN<Op1,Op2,OpCombine>
+ * (OperatorsAN.hpp:163-195; lib/column_operators/AN/selectAN_scalar.tcc:66-95, 149-181).
+ * col: the encoded tail of a BAT<void, v2_res*_t>.  th1/th2 are ENCODED thresholds.
+ * Outputs: oids[] = (uint32)(seqbase+pos) * a_head, err[] = pos * a_oid (every row checked),
+ * d_counts[0] = #oids, d_counts[1] = #err.  d_sum (nullable): the sum of decoded values of
+ * matching, uncorrupted rows (select + check-and-decode + sum, fused), mod 2^64.            */
+int ahead_hip_select_an(const ahead_col *col, int op1, uint64_t th1, int op2, uint64_t th2,
+                        int comb, uint64_t a_head, uint64_t a_oid, uint64_t *oids,
+                        uint64_t oids_cap, uint64_t *err, uint64_t err_cap, uint64_t *d_counts,
+                        uint64_t *d_sum, ahead_hip_ws *ws, void *stream);
+
+/* ---- encode / check / decode ------------------------------------------------------------------
+ * encodeAN (OperatorsAN.hpp:137-141; encdecAN_scalar.tcc:44-67): out[i] = (code)v_i * a,
+ * out_width bytes per code.                                                                  */
+int ahead_hip_encode_an(const ahead_col *col, uint64_t a, uint32_t out_width, void *out,
+                        void *stream);
+/* checkAN (OperatorsAN.hpp:143-149; encdecAN_scalar.tcc:69-94): d_counts[0] = #err */
+int ahead_hip_check_an(const ahead_col *col, uint64_t a_oid, uint64_t *err, uint64_t err_cap,
+                       uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
+/* decodeAN (OperatorsAN.hpp:151-155; encdecAN_scalar.tcc:96-126): all rows decoded into
+ * dec[] (unenc_bits/8 bytes each), d_counts[0] = n, d_counts[1] = #err                       */
+int ahead_hip_decode_an(const ahead_col *col, uint64_t a_oid, void *dec, uint64_t *err,
+                        uint64_t err_cap, uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
+/* checkAndDecodeAN<void,Tail> (OperatorsAN.hpp:157-161; encdecAN_scalar.tcc:185-210):
+ * corrupt rows dropped; d_counts[0] = #dec, d_counts[1] = #err                               */
+int ahead_hip_check_and_decode_an(const ahead_col *col, uint64_t a_oid, void *dec,
+                                  uint64_t dec_cap, uint64_t *err, uint64_t err_cap,
+                                  uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
+
+/* ---- data generation (seeded, counter-based; the same streams as ahead_amd/datagen.py) ------
+ * Not part of the reference's operator API: the device-side generator for the synthetic
+ * columns of the benchmark workloads (the reference loads SSB tables from dbgen instead).
+ * out[i] = encode(lo + ((rand(seed, stream, i) >> 32) * (hi - lo + 1)) >> 32), a (0 = plain). */
+int ahead_hip_gen_uniform(void *out, uint32_t width, uint64_t n, uint64_t seed, uint64_t stream,
+                          uint64_t lo, uint64_t hi, uint64_t a, void *hstream);
+/* Flips one bit of each code i with rand(seed, stream, i) < threshold; the bit is
+ * ((rand(seed, stream+1, i) >> 32) * 8*width) >> 32.  Optionally appends nothing: the flipped
+ * positions follow from the same streams on the host.                                         */
+int ahead_hip_inject_flips(void *codes, uint32_t width, uint64_t n, uint64_t seed,
+                           uint64_t stream, uint64_t threshold, void *hstream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AHEAD_HIP_H */
