@@ -44,6 +44,21 @@ class AheadCol(ctypes.Structure):
     ]
 
 
+class Star1Args(ctypes.Structure):
+    _fields_ = [("fact_fk", AheadCol), ("fact_m", AheadCol), ("dim_key", AheadCol), ("dim_a1", AheadCol),
+                ("dim_a2", AheadCol), ("a_oid_join", ctypes.c_uint64), ("a_oid_err", ctypes.c_uint64),
+                ("a_oid_grp", ctypes.c_uint64), ("a_result", ctypes.c_uint64)]
+
+
+class Star1Out(ctypes.Structure):
+    _fields_ = [("sums", ctypes.c_void_p), ("groid", ctypes.c_void_p), ("key1", ctypes.c_void_p),
+                ("key2", ctypes.c_void_p), ("groups_cap", ctypes.c_uint64),
+                ("e_fk", ctypes.c_void_p), ("e_dk", ctypes.c_void_p), ("e_a1", ctypes.c_void_p),
+                ("e_a2", ctypes.c_void_p), ("e_m", ctypes.c_void_p),
+                ("e_fk_cap", ctypes.c_uint64), ("e_dk_cap", ctypes.c_uint64), ("e_a1_cap", ctypes.c_uint64),
+                ("e_a2_cap", ctypes.c_uint64), ("e_m_cap", ctypes.c_uint64), ("d_counts", ctypes.c_void_p)]
+
+
 _u64 = ctypes.c_uint64
 _u32 = ctypes.c_uint32
 _int = ctypes.c_int
@@ -56,12 +71,15 @@ _SIGS = {
     "ahead_hip_ws_destroy": ([_vp], _int),
     "ahead_hip_ws_reserve": ([_vp, ctypes.c_size_t], _int),
     "ahead_hip_ws_bytes": ([_vp], ctypes.c_size_t),
+    "ahead_hip_set_tiling": ([_u32, _u32], _int),
+    "ahead_hip_get_tiling": ([ctypes.POINTER(_u32), ctypes.POINTER(_u32)], _int),
     "ahead_hip_select_an": ([_colp, _int, _u64, _int, _u64, _int, _u64, _u64, _vp, _u64, _vp, _u64,
                              _vp, _vp, _vp, _vp], _int),
     "ahead_hip_encode_an": ([_colp, _u64, _u32, _vp, _vp], _int),
     "ahead_hip_check_an": ([_colp, _u64, _vp, _u64, _vp, _vp, _vp], _int),
     "ahead_hip_decode_an": ([_colp, _u64, _vp, _vp, _u64, _vp, _vp, _vp], _int),
     "ahead_hip_check_and_decode_an": ([_colp, _u64, _vp, _u64, _vp, _u64, _vp, _vp, _vp], _int),
+    "ahead_hip_star1_pipeline": ([ctypes.POINTER(Star1Args), ctypes.POINTER(Star1Out), _vp, _vp], _int),
     "ahead_hip_gen_uniform": ([_vp, _u32, _u64, _u64, _u64, _u64, _u64, _u64, _vp], _int),
     "ahead_hip_inject_flips": ([_vp, _u32, _u64, _u64, _u64, _u64, _vp], _int),
 }
@@ -199,6 +217,30 @@ def workspace(device: int = 0) -> Workspace:
     return _ws[device]
 
 
+def set_tiling(max_subtiles: int, min_tiles: int) -> None:
+    check(load().ahead_hip_set_tiling(max_subtiles, min_tiles))
+
+
+def get_tiling() -> tuple[int, int]:
+    a, b = _u32(), _u32()
+    check(load().ahead_hip_get_tiling(ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
+
+
+class tiling:
+    """Context manager that sets the operator tiling and restores the previous one."""
+
+    def __init__(self, max_subtiles: int, min_tiles: int):
+        self.new = (max_subtiles, min_tiles)
+
+    def __enter__(self):
+        self.old = get_tiling()
+        set_tiling(*self.new)
+
+    def __exit__(self, *exc):
+        set_tiling(*self.old)
+
+
 def _counts(buf: DeviceBuffer, k: int) -> list[int]:
     return [int(x) for x in buf.to_numpy(np.uint64, k)]
 
@@ -293,6 +335,12 @@ def encode_an(col: Column, a: int, out_width: int) -> np.ndarray:
     return out.to_numpy(np.dtype(f"u{out_width}"), col.n)
 
 
+def encode_void_into(buf: DeviceBuffer, n: int, a: int, out_width: int, seqbase: int = 0) -> None:
+    """buf[i] = (seqbase + i) * a: the codes of a dense key / OID column."""
+    d = Column.void(n, seqbase).desc()
+    check(load().ahead_hip_encode_an(ctypes.byref(d), a, out_width, buf.ptr, current_stream()))
+
+
 def gen_uniform(buf: DeviceBuffer, width: int, n: int, seed: int, stream_id: int, lo: int, hi: int,
                 a: int = 0) -> None:
     check(load().ahead_hip_gen_uniform(buf.ptr, width, n, seed, stream_id, lo, hi, a, current_stream()))
@@ -300,3 +348,63 @@ def gen_uniform(buf: DeviceBuffer, width: int, n: int, seed: int, stream_id: int
 
 def inject_flips(buf: DeviceBuffer, width: int, n: int, seed: int, stream_id: int, threshold: int) -> None:
     check(load().ahead_hip_inject_flips(buf.ptr, width, n, seed, stream_id, threshold, current_stream()))
+
+
+STAR1_FIELDS = ["sums", "groid", "key1", "key2", "e_fk", "e_dk", "e_a1", "e_a2", "e_m", "e_grp"]
+
+
+class Star1:
+    """Device buffers + argument blocks of one fused star-join configuration (reusable)."""
+
+    def __init__(self, fact_fk: Column, fact_m: Column, dim_key: Column, dim_a1: Column, dim_a2: Column,
+                 a_oid_join: int = an.A_OID, a_oid_err: int = an.A_OID_ERR, a_oid_grp: int = an.A_OID,
+                 a_result: int = an.A_RESULT, groups_cap: int = 8192, err_cap: int = 4096):
+        self.cols = (fact_fk, fact_m, dim_key, dim_a1, dim_a2)
+        self.args = Star1Args(fact_fk.desc(), fact_m.desc(), dim_key.desc(), dim_a1.desc(), dim_a2.desc(),
+                              a_oid_join, a_oid_err, a_oid_grp, a_result)
+        self.groups_cap = groups_cap
+        self.err_cap = err_cap
+        self._alloc()
+
+    def _alloc(self):
+        g, e = self.groups_cap, self.err_cap
+        self.bufs = {k: DeviceBuffer(8 * g) for k in ("sums", "groid", "key1", "key2")}
+        self.bufs.update({k: DeviceBuffer(8 * e) for k in ("e_fk", "e_dk", "e_a1", "e_a2", "e_m")})
+        self.counts = DeviceBuffer(8 * 9)
+        b = self.bufs
+        self.out = Star1Out(b["sums"].ptr, b["groid"].ptr, b["key1"].ptr, b["key2"].ptr, g,
+                            b["e_fk"].ptr, b["e_dk"].ptr, b["e_a1"].ptr, b["e_a2"].ptr, b["e_m"].ptr,
+                            e, e, e, e, e, self.counts.ptr)
+
+    def launch(self, stream: int | None = None) -> None:
+        s = current_stream() if stream is None else stream
+        st = load().ahead_hip_star1_pipeline(ctypes.byref(self.args), ctypes.byref(self.out), workspace().h, s)
+        if st:
+            check(st)
+
+    def results(self) -> dict:
+        """Synchronises, re-runs with larger buffers if an output was truncated, returns numpy."""
+        while True:
+            synchronize()
+            c = [int(x) for x in self.counts.to_numpy(np.uint64, 9)]
+            need_e = max(c[1:6])
+            if c[0] <= self.groups_cap and need_e <= self.err_cap:
+                break
+            self.groups_cap = max(self.groups_cap, c[0])
+            self.err_cap = max(self.err_cap, need_e)
+            self._alloc()
+            self.launch()
+        out = {"status": c[7], "kept": c[8]}
+        for k, i in (("sums", 0), ("groid", 0), ("key1", 0), ("key2", 0)):
+            out[k] = self.bufs[k].to_numpy(np.uint64, c[i])
+        for k, i in (("e_fk", 1), ("e_dk", 2), ("e_a1", 3), ("e_a2", 4), ("e_m", 5)):
+            out[k] = self.bufs[k].to_numpy(np.uint64, c[i])
+        out["e_grp"] = np.zeros(c[6], dtype=np.uint64)
+        return out
+
+
+def star1_pipeline(fact_fk: Column, fact_m: Column, dim_key: Column, dim_a1: Column, dim_a2: Column,
+                   **kw) -> dict:
+    st = Star1(fact_fk, fact_m, dim_key, dim_a1, dim_a2, **kw)
+    st.launch()
+    return st.results()

@@ -23,6 +23,8 @@ constexpr int kWaves = kBlock / 64;
 // microseconds; the bound only guarantees that no wave can spin forever if the workspace was
 // misused (e.g. shared by two streams).  On expiry the kernel raises the workspace fail flag.
 constexpr uint32_t kSpinLimit = 1u << 22;
+// Most 16 KiB sub-tiles one look-back tile may cover.
+constexpr uint32_t kMaxSubtiles = 16;
 
 // ---- workspace header (device) -----------------------------------------------------------------
 struct WsHeader {
@@ -41,6 +43,14 @@ __device__ __forceinline__ T an_mul(T x, T y) {
     } else {
         return x * y;
     }
+}
+
+// ---- 128-bit streaming loads --------------------------------------------------------------------------
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// 16-byte load with the non-temporal hint: column data is read exactly once.
+__device__ __forceinline__ u32x4 load16_nt(const void *p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
 }
 
 // ---- wave / block primitives -----------------------------------------------------------------------
@@ -176,6 +186,14 @@ __device__ __forceinline__ uint32_t take_tile(const LbArgs &lb, uint32_t *s_tile
 // Publishes the tile aggregates (tile 0 publishes its inclusive prefix directly), resolves the
 // exclusive prefixes by look-back in wave 0, publishes the inclusive prefixes and leaves the
 // exclusive prefixes in s_excl[S].  Must be called by all threads of the block.
+//
+// One look-back round examines a window of 64 x kLbPerLane predecessors: lane l owns the
+// kLbPerLane consecutive tiles [end - W + l*kLbPerLane, ...), and all S x kLbPerLane status loads
+// of a lane are issued together, so a round costs one device-scope load latency.  The widest
+// window matters: tiles far ahead of the last published inclusive prefix need one round per
+// window, and the prefix frontier advances one window per round.
+constexpr int kLbPerLane = 4;
+
 template <int S>
 __device__ __forceinline__ void lb_resolve(const LbArgs &lb, uint32_t tile, const uint32_t (&agg)[S],
                                            uint32_t *s_excl) {
@@ -191,41 +209,66 @@ __device__ __forceinline__ void lb_resolve(const LbArgs &lb, uint32_t tile, cons
 #pragma unroll
         for (int s = 0; s < S; ++s) excl[s] = 0;
         if (tile > 0) {
+            constexpr int L = kLbPerLane;
+            constexpr int W = 64 * L;
             const int lane = lane_id();
             uint32_t done = 0;
             int64_t end = tile;
             bool failed = false;
             while (done != (1u << S) - 1u) {
-                const int64_t pred = end - 64 + lane;
+                const int64_t first = end - W + static_cast<int64_t>(lane) * L;
+                uint64_t w[S][L];
+                // issue every load of this round before waiting on any of them
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    if (done & (1u << s)) continue;
-                    uint32_t flag = 2, val = 0;
-                    if (pred >= 0) {
-                        const uint64_t *w = &lb.status[static_cast<size_t>(s) * lb.num_tiles + pred];
-                        uint64_t x = lb_load(w);
+#pragma unroll
+                    for (int i = 0; i < L; ++i) {
+                        const int64_t pred = first + i;
+                        w[s][i] = (done & (1u << s)) || pred < 0
+                                      ? lb_word(lb.epoch, 2u, 0u)
+                                      : lb_load(&lb.status[static_cast<size_t>(s) * lb.num_tiles + pred]);
+                    }
+                }
+                // wait for predecessors that have not published yet
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+#pragma unroll
+                    for (int i = 0; i < L; ++i) {
                         uint32_t spins = 0;
-                        while ((static_cast<uint32_t>(x >> 34)) != lb.epoch) {
+                        while (static_cast<uint32_t>(w[s][i] >> 34) != lb.epoch) {
                             if (++spins >= kSpinLimit) {
                                 failed = true;
+                                w[s][i] = lb_word(lb.epoch, 2u, 0u);
                                 break;
                             }
                             __builtin_amdgcn_s_sleep(1);
-                            x = lb_load(w);
+                            w[s][i] = lb_load(&lb.status[static_cast<size_t>(s) * lb.num_tiles + first + i]);
                         }
-                        flag = failed ? 2u : static_cast<uint32_t>((x >> 32) & 3u);
-                        val = failed ? 0u : static_cast<uint32_t>(x);
-                    }
-                    const uint64_t pmask = __ballot(flag == 2u);
-                    if (pmask != 0) {
-                        const int hi = 63 - __clzll(static_cast<long long>(pmask));
-                        excl[s] += wave_sum(lane >= hi ? val : 0u);
-                        done |= 1u << s;
-                    } else {
-                        excl[s] += wave_sum(val);
                     }
                 }
-                end -= 64;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (done & (1u << s)) continue;
+                    // lane-local: sum from the highest inclusive prefix among my L tiles (if any)
+                    uint32_t part = 0;
+                    bool has_p = false;
+#pragma unroll
+                    for (int i = L - 1; i >= 0; --i) {
+                        if (!has_p) {
+                            part += static_cast<uint32_t>(w[s][i]);
+                            has_p = ((w[s][i] >> 32) & 3u) == 2u;
+                        }
+                    }
+                    const uint64_t pmask = __ballot(has_p);
+                    if (pmask != 0) {
+                        const int hi = 63 - __clzll(static_cast<long long>(pmask));
+                        excl[s] += wave_sum(lane >= hi ? part : 0u);
+                        done |= 1u << s;
+                    } else {
+                        excl[s] += wave_sum(part);
+                    }
+                }
+                end -= W;
             }
             if (__ballot(failed) != 0 && lane == 0) atomicOr(&lb.hdr->fail, 1u);
             if (lane == 0) {

@@ -49,6 +49,14 @@ inline ahead_dev::WsHeader *ws_header(ahead_hip_ws *ws) {
 
 int check_col(const ahead_col *c, const char *name, bool allow_void, bool need_encoded);
 
+// Multiplicative inverse of odd a modulo 2^bits (Newton iteration x <- x(2 - a x)); 0 for even a.
+inline uint64_t inverse_mod2k(uint64_t a, uint32_t bits) {
+    if ((a & 1ull) == 0) return 0;
+    uint64_t x = a;
+    for (int i = 0; i < 6; ++i) x *= 2ull - a * x;
+    return bits >= 64 ? x : (x & ((1ull << bits) - 1ull));
+}
+
 inline uint64_t width_mask(uint32_t w) { return w >= 8 ? ~0ull : ((1ull << (8 * w)) - 1ull); }
 inline uint64_t bits_mask(uint32_t b) { return b >= 64 ? ~0ull : ((1ull << b) - 1ull); }
 
@@ -65,5 +73,16 @@ inline LbPlan lb_plan(uint64_t n, uint32_t tile_rows, int streams) {
     return p;
 }
 int lb_args(ahead_hip_ws *ws, const LbPlan &p, size_t status_offset, ahead_dev::LbArgs *out);
+
+// Process-wide tiling knobs (ahead_hip_set_tiling).
+using ahead_dev::kMaxSubtiles;
+struct Tiling {
+    uint32_t max_subtiles = kMaxSubtiles;
+    uint32_t min_tiles = 4096;
+};
+Tiling tiling();
+// Sub-tiles per look-back tile for a stream of `subs` sub-tiles: the largest power of two
+// <= max_subtiles that still leaves >= min_tiles tiles.
+uint32_t subtiles_per_tile(uint64_t subs);
 
 }  // namespace ahead_host

@@ -78,6 +78,17 @@ int lb_args(ahead_hip_ws *ws, const LbPlan &p, size_t status_offset, ahead_dev::
     return AHEAD_OK;
 }
 
+static Tiling g_tiling;
+
+Tiling tiling() { return g_tiling; }
+
+uint32_t subtiles_per_tile(uint64_t subs) {
+    const Tiling t = g_tiling;
+    uint32_t it = 1;
+    while (2 * it <= t.max_subtiles && subs >= 2ull * it * t.min_tiles) it *= 2;
+    return it;
+}
+
 int check_col(const ahead_col *c, const char *name, bool allow_void, bool need_encoded) {
     if (c == nullptr) return fail(AHEAD_ERR_INVALID, std::string(name) + ": NULL column");
     if (c->n > 0xFFFFFFFFull)
@@ -136,5 +147,20 @@ int ahead_hip_ws_destroy(ahead_hip_ws *ws) {
 int ahead_hip_ws_reserve(ahead_hip_ws *ws, size_t bytes) { return ws_ensure(ws, bytes); }
 
 size_t ahead_hip_ws_bytes(const ahead_hip_ws *ws) { return ws == nullptr ? 0 : ws->bytes; }
+
+int ahead_hip_set_tiling(uint32_t max_subtiles, uint32_t min_tiles) {
+    if (max_subtiles < 1 || max_subtiles > kMaxSubtiles || (max_subtiles & (max_subtiles - 1)) != 0)
+        return fail(AHEAD_ERR_INVALID, "set_tiling: max_subtiles must be a power of two in [1, 16]");
+    if (min_tiles < 1) return fail(AHEAD_ERR_INVALID, "set_tiling: min_tiles must be >= 1");
+    g_tiling.max_subtiles = max_subtiles;
+    g_tiling.min_tiles = min_tiles;
+    return AHEAD_OK;
+}
+
+int ahead_hip_get_tiling(uint32_t *max_subtiles, uint32_t *min_tiles) {
+    if (max_subtiles) *max_subtiles = g_tiling.max_subtiles;
+    if (min_tiles) *min_tiles = g_tiling.min_tiles;
+    return AHEAD_OK;
+}
 
 }  // extern "C"

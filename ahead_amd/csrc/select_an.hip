@@ -8,15 +8,19 @@
 // with unsigned comparisons on the raw codes against thresholds the caller already encoded.
 // Corrupt rows still take part in (b).
 //
-// Kernel shape: one 256-thread workgroup per 16 KiB tile, 4 x 16-byte loads per thread
-// (8192 rows of 16-bit codes, 4096 of 32-bit, 2048 of 64-bit).  Each thread loads its 4 chunks
-// with coalesced 128-bit loads, evaluates the predicate and the check per row, and packs its
-// per-chunk match counts into one uint64 (16-bit fields) so that a single block scan yields the
-// in-order rank of every row.  The tile's prefix over all earlier tiles comes from the
-// decoupled look-back (an_device.hpp).  The fused decoded sum is reduced by the last tile to
-// finish (last_tile_sum), so one call is one kernel launch.  Matching row indices are staged in LDS by rank and
-// written out as one contiguous, coalesced run of 64-bit OIDs; error positions (rare) are
-// scattered straight to HBM.
+// Kernel shape: one 256-thread workgroup per tile of ITERS sub-tiles of 16 KiB (4 x 16-byte
+// loads per thread; 8192 rows of 16-bit codes, 4096 of 32-bit, 2048 of 64-bit).
+//  phase 1  streams the tile once with coalesced 128-bit non-temporal loads and keeps only one
+//           match bit per row (LDS) and one "has an error" bit per sub-tile;
+//  resolve  publishes the tile's counts and takes its prefix by decoupled look-back
+//           (an_device.hpp) -- one ticket and one look-back per ITERS sub-tiles;
+//  phase 2  per sub-tile, packs the per-chunk counts into one uint64 (16-bit fields) so that one
+//           block scan ranks every row in order, stages matching row indices in LDS and writes
+//           them as one contiguous, coalesced run of 64-bit OIDs.  OIDs depend only on the row
+//           index, so the input is never read twice.  Sub-tiles holding an error (rare) are
+//           re-read to rank their error positions, which are scattered.
+// The fused decoded sum leaves per-tile partials that k_sum_partials adds up; the kernel
+// boundary orders them without any device-wide fence.
 #include "host_common.hpp"
 
 using namespace ahead_dev;
@@ -40,7 +44,7 @@ struct SelectArgs {
     uint64_t err_cap;
     uint64_t *d_counts;
     uint64_t *partials;  // per-tile decoded sums (fused sum only)
-    uint64_t *d_sum;
+    uint32_t iters;      // 16 KiB sub-tiles per look-back tile (1 .. kMaxSubtiles)
     LbArgs lb;
 };
 
@@ -50,50 +54,36 @@ __device__ __forceinline__ bool in_range(CodeT x, CodeT lo, CodeT span, uint32_t
     return in != static_cast<bool>(neg);
 }
 
+// Loads the 4 x 16 bytes of one thread in sub-tile `sb` and evaluates predicate and check.
 template <typename CodeT, int COMB, bool SUM>
-__global__ __launch_bounds__(kBlock) void k_select(SelectArgs a) {
+__device__ __forceinline__ void eval_subtile(const SelectArgs &a, const CodeT *codes, bool aligned, uint64_t sb,
+                                             uint32_t &m, uint32_t &e, uint64_t &lsum) {
     constexpr int VEC = 16 / sizeof(CodeT);
     constexpr int CH = 4;
     constexpr int CHUNK = kBlock * VEC;
-    constexpr int TILE = CHUNK * CH;
-    static_assert(TILE <= 65536, "LDS row index must fit 16 bits");
-
-    __shared__ uint32_t s_tile;
-    __shared__ uint64_t s_scan[kWaves];
-    __shared__ uint32_t s_excl[2];
-    __shared__ uint16_t s_rows[TILE];
-
-    const uint32_t tile = take_tile(a.lb, &s_tile);
-    if (tile >= a.lb.num_tiles) return;
-    const uint64_t base = static_cast<uint64_t>(tile) * TILE;
-    const CodeT *codes = static_cast<const CodeT *>(a.codes);
-    const bool aligned = (reinterpret_cast<uintptr_t>(codes) & 15u) == 0;
-
+    const CodeT ainv = static_cast<CodeT>(a.a_inv);
+    const CodeT maxu = static_cast<CodeT>(a.max_u);
+    const CodeT lo1 = static_cast<CodeT>(a.lo1), sp1 = static_cast<CodeT>(a.span1);
+    const CodeT lo2 = static_cast<CodeT>(a.lo2), sp2 = static_cast<CodeT>(a.span2);
     CodeT v[CH][VEC];
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
-        const uint64_t r0 = base + static_cast<uint64_t>(j) * CHUNK + threadIdx.x * VEC;
+        const uint64_t r0 = sb + static_cast<uint64_t>(j) * CHUNK + threadIdx.x * VEC;
         if (aligned && r0 + VEC <= a.n) {
-            const uint4 q = *reinterpret_cast<const uint4 *>(codes + r0);
+            const u32x4 q = load16_nt(codes + r0);
             __builtin_memcpy(&v[j][0], &q, 16);
         } else {
 #pragma unroll
             for (int k = 0; k < VEC; ++k) v[j][k] = (r0 + k < a.n) ? codes[r0 + k] : CodeT(0);
         }
     }
-
-    const CodeT ainv = static_cast<CodeT>(a.a_inv);
-    const CodeT maxu = static_cast<CodeT>(a.max_u);
-    const CodeT lo1 = static_cast<CodeT>(a.lo1), sp1 = static_cast<CodeT>(a.span1);
-    const CodeT lo2 = static_cast<CodeT>(a.lo2), sp2 = static_cast<CodeT>(a.span2);
-
-    uint32_t mbits = 0, ebits = 0;
-    uint64_t lsum = 0;
+    m = 0;
+    e = 0;
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
 #pragma unroll
         for (int k = 0; k < VEC; ++k) {
-            const uint64_t row = base + static_cast<uint64_t>(j) * CHUNK + threadIdx.x * VEC + k;
+            const uint64_t row = sb + static_cast<uint64_t>(j) * CHUNK + threadIdx.x * VEC + k;
             const bool live = row < a.n;
             const CodeT x = v[j][k];
             const CodeT d = an_mul<CodeT>(x, ainv);
@@ -102,50 +92,106 @@ __global__ __launch_bounds__(kBlock) void k_select(SelectArgs a) {
             if constexpr (COMB == AHEAD_COMB_AND) hit = hit && in_range<CodeT>(x, lo2, sp2, a.valid2, a.neg2);
             if constexpr (COMB == AHEAD_COMB_OR) hit = hit || in_range<CodeT>(x, lo2, sp2, a.valid2, a.neg2);
             hit = hit && live;
-            mbits |= static_cast<uint32_t>(hit) << (j * VEC + k);
-            ebits |= static_cast<uint32_t>(bad) << (j * VEC + k);
+            m |= static_cast<uint32_t>(hit) << (j * VEC + k);
+            e |= static_cast<uint32_t>(bad) << (j * VEC + k);
             if constexpr (SUM) lsum += (hit && !bad) ? static_cast<uint64_t>(d) : 0ull;
         }
     }
+}
 
-    const uint64_t pm = pack_counts<CH, VEC>(mbits), pe = pack_counts<CH, VEC>(ebits);
-    uint64_t tot_m = 0, tot_e = 0, ex_e = 0;
-    const uint64_t ex_m = block_excl_scan<uint64_t>(pm, tot_m, s_scan);
-    const int any_err = __syncthreads_or(ebits != 0u);
-    if (any_err) ex_e = block_excl_scan<uint64_t>(pe, tot_e, s_scan);
+template <typename CodeT, int COMB, bool SUM>
+__global__ __launch_bounds__(kBlock) void k_select(SelectArgs a) {
+    constexpr int VEC = 16 / sizeof(CodeT);
+    constexpr int CH = 4;
+    constexpr int CHUNK = kBlock * VEC;
+    constexpr int SUB = CHUNK * CH;  // rows per 16 KiB sub-tile
+    static_assert(SUB <= 65536, "LDS row index must fit 16 bits");
 
-    const uint32_t agg[2] = {fields_sum(tot_m), fields_sum(tot_e)};
-    lb_resolve<2>(a.lb, tile, agg, s_excl);
-    const uint64_t xm = s_excl[0], xe = s_excl[1];
+    __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_scan[kWaves];
+    __shared__ uint32_t s_excl[2];
+    __shared__ uint32_t s_mb[kMaxSubtiles * kBlock];  // match bits of every row of the tile
+    __shared__ uint16_t s_rows[SUB];
 
-    // stage matching rows in LDS at their in-tile rank; error positions go straight to HBM
-    for_each_ranked<CH, VEC>(mbits, ex_m, tot_m, [&](uint32_t r, int j, int k) {
-        s_rows[r] = static_cast<uint16_t>(j * CHUNK + threadIdx.x * VEC + k);
-    });
-    if (any_err) {
-        for_each_ranked<CH, VEC>(ebits, ex_e, tot_e, [&](uint32_t r, int j, int k) {
-            const uint64_t idx = xe + r;
-            const uint64_t pos = base + static_cast<uint64_t>(j) * CHUNK + threadIdx.x * VEC + k;
-            if (idx < a.err_cap) a.err[idx] = pos * a.a_oid;
-        });
+    const uint32_t tile = take_tile(a.lb, &s_tile);
+    if (tile >= a.lb.num_tiles) return;
+    const uint32_t iters = a.iters;
+    const uint64_t base = static_cast<uint64_t>(tile) * iters * SUB;
+    const CodeT *codes = static_cast<const CodeT *>(a.codes);
+    const bool aligned = (reinterpret_cast<uintptr_t>(codes) & 15u) == 0;
+
+    // phase 1: stream the tile once; keep one match bit per row (LDS) and which sub-tiles hold
+    // an error (register bit mask)
+    uint32_t cnt_m = 0, cnt_e = 0, err_subs = 0;
+    uint64_t lsum = 0;
+#pragma unroll 2
+    for (uint32_t it = 0; it < iters; ++it) {
+        uint32_t m, e;
+        eval_subtile<CodeT, COMB, SUM>(a, codes, aligned, base + static_cast<uint64_t>(it) * SUB, m, e, lsum);
+        s_mb[it * kBlock + threadIdx.x] = m;
+        cnt_m += __popc(m);
+        cnt_e += __popc(e);
+        err_subs |= (e != 0u ? 1u : 0u) << it;
     }
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < agg[0]; k += kBlock) {
-        const uint64_t idx = xm + k;
-        if (idx < a.oids_cap) {
-            const uint32_t head = static_cast<uint32_t>(a.seqbase + base + s_rows[k]);
-            a.oids[idx] = static_cast<uint64_t>(head) * a.a_head;
+
+    // tile aggregates -> look-back -> exclusive prefixes of both streams
+    const uint32_t agg_m = block_sum<uint32_t>(cnt_m, reinterpret_cast<uint32_t *>(s_scan));
+    const uint32_t agg_e = block_sum<uint32_t>(cnt_e, reinterpret_cast<uint32_t *>(s_scan));
+    const uint32_t agg[2] = {agg_m, agg_e};
+    lb_resolve<2>(a.lb, tile, agg, s_excl);
+    uint64_t xm = s_excl[0], xe = s_excl[1];
+
+    // phase 2: per sub-tile, rank the matching rows, stage them in LDS, write one coalesced run
+    for (uint32_t it = 0; it < iters; ++it) {
+        const uint64_t sb = base + static_cast<uint64_t>(it) * SUB;
+        const uint32_t m = s_mb[it * kBlock + threadIdx.x];
+        uint64_t tot_m = 0;
+        const uint64_t ex_m = block_excl_scan<uint64_t>(pack_counts<CH, VEC>(m), tot_m, s_scan);
+        for_each_ranked<CH, VEC>(m, ex_m, tot_m, [&](uint32_t r, int j, int k) {
+            s_rows[r] = static_cast<uint16_t>(j * CHUNK + threadIdx.x * VEC + k);
+        });
+        if (agg_e != 0 && __syncthreads_or((err_subs >> it) & 1u)) {
+            // rare: recompute this sub-tile's error bits and write its positions in order
+            uint32_t mm, e;
+            uint64_t dummy = 0;
+            eval_subtile<CodeT, COMB, false>(a, codes, aligned, sb, mm, e, dummy);
+            uint64_t tot_e = 0;
+            const uint64_t ex_e = block_excl_scan<uint64_t>(pack_counts<CH, VEC>(e), tot_e, s_scan);
+            for_each_ranked<CH, VEC>(e, ex_e, tot_e, [&](uint32_t r, int j, int k) {
+                const uint64_t idx = xe + r;
+                const uint64_t pos = sb + static_cast<uint64_t>(j) * CHUNK + threadIdx.x * VEC + k;
+                if (idx < a.err_cap) a.err[idx] = pos * a.a_oid;
+            });
+            xe += fields_sum(tot_e);
         }
+        __syncthreads();
+        const uint32_t nm = fields_sum(tot_m);
+        for (uint32_t k = threadIdx.x; k < nm; k += kBlock) {
+            const uint64_t idx = xm + k;
+            if (idx < a.oids_cap) {
+                const uint32_t head = static_cast<uint32_t>(a.seqbase + sb + s_rows[k]);
+                a.oids[idx] = static_cast<uint64_t>(head) * a.a_head;
+            }
+        }
+        xm += nm;
     }
     if constexpr (SUM) {
-        __shared__ uint32_t s_last;
         const uint64_t ts = block_sum<uint64_t>(lsum, s_scan);
-        last_tile_sum(a.lb.hdr, a.partials, a.lb.num_tiles, tile, ts, a.d_sum, s_scan, &s_last);
+        if (threadIdx.x == 0) a.partials[tile] = ts;
     }
     if (tile == a.lb.num_tiles - 1 && threadIdx.x == 0) {
-        a.d_counts[0] = xm + agg[0];
-        a.d_counts[1] = xe + agg[1];
+        a.d_counts[0] = xm;
+        a.d_counts[1] = xe;
     }
+}
+
+// Adds the per-tile partial sums of a k_select launch (kernel order makes them visible).
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const uint64_t *partials, uint32_t n, uint64_t *out) {
+    __shared__ uint64_t s_scan[kWaves];
+    uint64_t s = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock) s += partials[i];
+    s = block_sum<uint64_t>(s, s_scan);
+    if (threadIdx.x == 0) *out = s;
 }
 
 // canonical interval form of one predicate in a code width with mask m
@@ -205,8 +251,13 @@ extern "C" int ahead_hip_select_an(const ahead_col *col, int op1, uint64_t th1, 
     Canon c1, c2;
     if (!canon(op1, th1, m, &c1)) return fail(AHEAD_ERR_INVALID, "select: bad op1");
     if (comb != AHEAD_COMB_NONE && !canon(op2, th2, m, &c2)) return fail(AHEAD_ERR_INVALID, "select: bad op2");
-    const uint32_t tile_rows = 4u * kBlock * (16u / col->width);
-    const LbPlan plan = lb_plan(col->n, tile_rows, 2);
+    // Sub-tiles of 16 KiB; one look-back ticket covers `iters` of them.  Fewer tickets keep the
+    // contended ticket atomic and the look-back off the critical path; iters stays small enough
+    // that the grid still holds >= min_tiles tiles (default 4096, 16 per CU) when it can.
+    const uint64_t sub_rows = 4ull * kBlock * (16u / col->width);
+    const uint64_t subs = (col->n + sub_rows - 1) / sub_rows;
+    const uint32_t iters = subtiles_per_tile(subs);
+    const LbPlan plan = lb_plan(col->n, static_cast<uint32_t>(sub_rows * iters), 2);
     const size_t partial_off = plan.status_bytes;
     const size_t need = partial_off + align_up(static_cast<size_t>(plan.num_tiles) * 8, 256);
     if ((st = ws_ensure(ws, need)) != AHEAD_OK) return st;
@@ -227,7 +278,7 @@ extern "C" int ahead_hip_select_an(const ahead_col *col, int op1, uint64_t th1, 
     a.err_cap = err == nullptr ? 0 : err_cap;
     a.d_counts = d_counts;
     a.partials = reinterpret_cast<uint64_t *>(ws_region(ws, partial_off));
-    a.d_sum = d_sum;
+    a.iters = iters;
     if ((st = lb_args(ws, plan, 0, &a.lb)) != AHEAD_OK) return st;
 
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -238,5 +289,9 @@ extern "C" int ahead_hip_select_an(const ahead_col *col, int op1, uint64_t th1, 
         default: dispatch<uint64_t>(a, comb, sum, s); break;
     }
     AHEAD_HIP_TRY(hipGetLastError());
+    if (sum) {
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, a.partials, plan.num_tiles, d_sum);
+        AHEAD_HIP_TRY(hipGetLastError());
+    }
     return AHEAD_OK;
 }

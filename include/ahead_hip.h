@@ -73,6 +73,12 @@ int ahead_hip_ws_destroy(ahead_hip_ws *ws);
 int ahead_hip_ws_reserve(ahead_hip_ws *ws, size_t bytes);
 size_t ahead_hip_ws_bytes(const ahead_hip_ws *ws);
 
+/* Tiling of the streaming operators: a look-back tile covers up to `max_subtiles` sub-tiles of
+ * 16 KiB, as many as keep at least `min_tiles` tiles in the grid.  Process-wide; defaults 16 and
+ * 4096.  Used by the autotuner and by tests that need multi-sub-tile tiles at small sizes.     */
+int ahead_hip_set_tiling(uint32_t max_subtiles, uint32_t min_tiles);
+int ahead_hip_get_tiling(uint32_t *max_subtiles, uint32_t *min_tiles);
+
 /* ---- selectAN ---------------------------------------------------------------------------------
  * Replaces ahead::bat::ops::scalar::selectAN<Op> and selectAN<Op1,Op2,OpCombine>
  * (OperatorsAN.hpp:163-195; lib/column_operators/AN/selectAN_scalar.tcc:66-95, 149-181).
@@ -102,6 +108,44 @@ int ahead_hip_decode_an(const ahead_col *col, uint64_t a_oid, void *dec, uint64_
 int ahead_hip_check_and_decode_an(const ahead_col *col, uint64_t a_oid, void *dec,
                                   uint64_t dec_cap, uint64_t *err, uint64_t err_cap,
                                   uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
+
+/* ---- fused star join over one dimension ------------------------------------------------------
+ * Replaces, for one dimension with two 8-bit group attributes, the hardened operator chain of
+ * tools/ssbm/ssbm-q21_continuous.cpp:97-181:
+ *   hashjoinAN(fact_fk, reverse(dim_key))                       (OperatorsAN.hpp:79-90)
+ *   fetchjoinAN(dim OIDs, dim_a1), fetchjoinAN(dim OIDs, dim_a2), fetchjoinAN(fact OIDs, fact_m)
+ *                                                               (OperatorsAN.hpp:92-106)
+ *   groupbyAN(a1), groupbyAN(a2, g1, |g1|)                      (OperatorsAN.hpp:108-123)
+ *   aggregate_sum_groupedAN<v2_resbigint_t>(m, g2, |g2|)        (OperatorsAN.hpp:125-134)
+ *   fetchjoinAN(VGIDtoROID, a1 / a2) for the group keys
+ * with the row-consistent policy: a joined row whose fetched attribute or measure is corrupt is
+ * reported by that fetch and dropped from group-by and sum.  Fused path: 64-bit FK / key /
+ * measure codes, restiny (16-bit code, 8-bit value) attributes, unique build keys.
+ * d_counts[0] = #groups, [1] = #e_fk, [2] = #e_dk, [3] = #e_a1, [4] = #e_a2, [5] = #e_m,
+ * [6] = #e_grp, [7] = status (0 ok; 1 = duplicate build keys and 2 = more than 8192 groups: the
+ * fused path does not apply and the caller runs the per-operator chain), [8] = kept rows.      */
+typedef struct {
+    ahead_col fact_fk;    /* BAT<void, resint> foreign-key codes                 */
+    ahead_col fact_m;     /* BAT<void, resint> measure codes                     */
+    ahead_col dim_key;    /* BAT<void, resint> dimension key codes (build side)  */
+    ahead_col dim_a1;     /* BAT<void, restiny> first group attribute            */
+    ahead_col dim_a2;     /* BAT<void, restiny> second group attribute           */
+    uint64_t a_oid_join;  /* A of the re-encoded join OIDs                       */
+    uint64_t a_oid_err;   /* A of error positions (16089 by default)             */
+    uint64_t a_oid_grp;   /* A of group ids / VGIDtoROID (65117 by default)      */
+    uint64_t a_result;    /* A of the grouped sums (65117 by default)            */
+} ahead_star1_args;
+
+typedef struct {
+    uint64_t *sums, *groid, *key1, *key2; /* per group, first-occurrence order      */
+    uint64_t groups_cap;
+    uint64_t *e_fk, *e_dk, *e_a1, *e_a2, *e_m;
+    uint64_t e_fk_cap, e_dk_cap, e_a1_cap, e_a2_cap, e_m_cap;
+    uint64_t *d_counts;                   /* device, 9 entries                      */
+} ahead_star1_out;
+
+int ahead_hip_star1_pipeline(const ahead_star1_args *in, const ahead_star1_out *out,
+                             ahead_hip_ws *ws, void *stream);
 
 /* ---- data generation (seeded, counter-based; the same streams as ahead_amd/datagen.py) ------
  * Not part of the reference's operator API: the device-side generator for the synthetic

@@ -140,3 +140,18 @@ def test_encode_parity(src):
         got = capi.encode_an(capi.Column.from_numpy(d, meta), a, ow)
         ref = O.encode_an(O.Col(d, meta), a, ow)
         assert np.array_equal(got.astype(np.uint64), ref)
+
+
+@pytest.mark.parametrize("max_sub", [2, 4, 16])
+@pytest.mark.parametrize("t,a", [(an.RESTINY, 233), (an.RESSHORT, 233), (an.RESINT, 65117)])
+def test_select_multi_subtile_tiles(max_sub, t, a):
+    # force look-back tiles of several 16 KiB sub-tiles at a small size (min_tiles = 3)
+    n = 1_000_003
+    _, codes, _ = make_codes(t, a, n, seed=max_sub, rate=5e-4, hi=100)
+    meta = an.an_meta(t, a)
+    ref = O.select_an(O.Col(codes, meta), capi.LT, 37 * a, capi.GT, 80 * a, capi.COMB_OR)
+    with capi.tiling(max_sub, 3):
+        got = capi.select_an(capi.Column.from_numpy(codes, meta), capi.LT, 37 * a, capi.GT, 80 * a, capi.COMB_OR,
+                             with_sum=True, err_cap=7)
+    for g, r in zip(got, ref):
+        assert np.array_equal(g, r)
