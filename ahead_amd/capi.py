@@ -44,6 +44,10 @@ class AheadCol(ctypes.Structure):
     ]
 
 
+class Buf(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("cap", ctypes.c_uint64)]
+
+
 class Star1Args(ctypes.Structure):
     _fields_ = [("fact_fk", AheadCol), ("fact_m", AheadCol), ("dim_key", AheadCol), ("dim_a1", AheadCol),
                 ("dim_a2", AheadCol), ("a_oid_join", ctypes.c_uint64), ("a_oid_err", ctypes.c_uint64),
@@ -80,6 +84,21 @@ _SIGS = {
     "ahead_hip_decode_an": ([_colp, _u64, _vp, _vp, _u64, _vp, _vp, _vp], _int),
     "ahead_hip_check_and_decode_an": ([_colp, _u64, _vp, _u64, _vp, _u64, _vp, _vp, _vp], _int),
     "ahead_hip_star1_pipeline": ([ctypes.POINTER(Star1Args), ctypes.POINTER(Star1Out), _vp, _vp], _int),
+    "ahead_hip_star1_build": ([ctypes.POINTER(Star1Args), ctypes.POINTER(Star1Out), _vp, _vp], _int),
+    "ahead_hip_star1_probe": ([ctypes.POINTER(Star1Args), ctypes.POINTER(Star1Out), _vp, _vp], _int),
+    "ahead_hip_fetchjoin_an": ([_colp, _colp, _int, _u64, _u64, Buf, Buf, Buf, _vp, _vp, _vp], _int),
+    "ahead_hip_matchjoin_pos_an": ([_colp, _colp, _colp, _int, _u64, _u32, _int, _u64, _u32, _u64, Buf, Buf, Buf,
+                                    Buf, Buf, _vp, _vp, _vp], _int),
+    "ahead_hip_hashjoin_an": ([_colp, _colp, _colp, _colp, _int, _int, _u64, _u32, _int, _u64, _u32, _u64, Buf, Buf,
+                               Buf, Buf, Buf, Buf, _vp, _vp, _vp], _int),
+    "ahead_hip_groupby_an": ([_colp, _colp, _u64, _u32, _vp, Buf, Buf, Buf, _vp, _vp, _vp], _int),
+    "ahead_hip_groupby2_an": ([_colp, _colp, _colp, _u64, _u32, _u64, _u32, _vp, Buf, Buf, Buf, Buf, _vp, _vp, _vp],
+                              _int),
+    "ahead_hip_aggregate_sum_grouped_an": ([_colp, _colp, _colp, _u64, _u64, _u64, _vp, Buf, Buf, Buf, _vp, _vp, _vp],
+                                           _int),
+    "ahead_hip_aggregate_mul_sum_an": ([_colp, _colp, _u64, _int, _u64, _u64, _vp, Buf, Buf, _vp, _vp, _vp], _int),
+    "ahead_hip_arithmetic_an": ([_colp, _colp, _colp, _colp, _int, _u64, _u32, _u64, _vp, Buf, Buf, Buf, Buf, _vp, _vp,
+                                 _vp], _int),
     "ahead_hip_gen_uniform": ([_vp, _u32, _u64, _u64, _u64, _u64, _u64, _u64, _vp], _int),
     "ahead_hip_inject_flips": ([_vp, _u32, _u64, _u64, _u64, _u64, _vp], _int),
 }
@@ -377,8 +396,19 @@ class Star1:
                             e, e, e, e, e, self.counts.ptr)
 
     def launch(self, stream: int | None = None) -> None:
+        """Build + probe (one full pipeline)."""
         s = current_stream() if stream is None else stream
         st = load().ahead_hip_star1_pipeline(ctypes.byref(self.args), ctypes.byref(self.out), workspace().h, s)
+        if st:
+            check(st)
+
+    def build(self, stream: int | None = None) -> None:
+        s = current_stream() if stream is None else stream
+        check(load().ahead_hip_star1_build(ctypes.byref(self.args), ctypes.byref(self.out), workspace().h, s))
+
+    def probe(self, stream: int | None = None) -> None:
+        s = current_stream() if stream is None else stream
+        st = load().ahead_hip_star1_probe(ctypes.byref(self.args), ctypes.byref(self.out), workspace().h, s)
         if st:
             check(st)
 
@@ -408,3 +438,166 @@ def star1_pipeline(fact_fk: Column, fact_m: Column, dim_key: Column, dim_a1: Col
     st = Star1(fact_fk, fact_m, dim_key, dim_a1, dim_a2, **kw)
     st.launch()
     return st.results()
+
+
+# ------------------------------------------------------------------------------------------------
+# per-operator wrappers (synchronous; outputs re-allocated and the call repeated on truncation)
+# ------------------------------------------------------------------------------------------------
+
+def _run(call, outs, ncounts, count_of):
+    """outs: list of [elem_bytes, cap]; call(bufs, counts) launches; count_of[i] = counts index of
+    output i.  Returns (numpy uint8 buffers list, counts)."""
+    while True:
+        bufs = [DeviceBuffer(max(1, eb * cap)) for eb, cap in outs]
+        cnt = DeviceBuffer(8 * ncounts, zero=True)
+        call(bufs, cnt)
+        synchronize()
+        c = _counts(cnt, ncounts)
+        grow = False
+        for i, (eb, cap) in enumerate(outs):
+            if count_of[i] is not None and c[count_of[i]] > cap:
+                outs[i][1] = c[count_of[i]]
+                grow = True
+        if not grow:
+            return bufs, c
+
+
+def _b(buf: DeviceBuffer, cap: int) -> Buf:
+    return Buf(buf.ptr, cap)
+
+
+def fetchjoin_an(oids: Column, col: Column, reenc: int = 0, a_reenc: int = 0, a_oid: int = an.A_OID_ERR,
+                 err_cap: int = 1024):
+    lib, ws = load(), workspace()
+    ow = col.meta.width if col.meta.type.encoded else (8 if col.meta.width >= 4 else 2 * col.meta.width)
+    outs = [[ow, oids.n], [8, err_cap], [8, err_cap]]
+    do, dc = oids.desc(), col.desc()
+
+    def call(b, cnt):
+        check(lib.ahead_hip_fetchjoin_an(ctypes.byref(do), ctypes.byref(dc), reenc, a_reenc & an.MASK64, a_oid,
+                                         _b(b[0], outs[0][1]), _b(b[1], outs[1][1]), _b(b[2], outs[2][1]), cnt.ptr,
+                                         ws.h, current_stream()))
+
+    b, c = _run(call, outs, 3, [0, 1, 2])
+    return (b[0].to_numpy(np.dtype(f"u{ow}"), c[0]), b[1].to_numpy(np.uint64, c[1]), b[2].to_numpy(np.uint64, c[2]))
+
+
+def matchjoin_pos_an(h1: Column, t1: Column, t2: Column, reenc_h=0, f_h1=0, wout_h=8, reenc_t=0, f_t2=0, wout_t=8,
+                     a_oid: int = an.A_OID_ERR, err_cap: int = 1024):
+    lib, ws = load(), workspace()
+    outs = [[wout_h, t1.n], [wout_t, t1.n], [8, err_cap], [8, err_cap], [8, err_cap]]
+    d1, d2, d3 = h1.desc(), t1.desc(), t2.desc()
+
+    def call(b, cnt):
+        check(lib.ahead_hip_matchjoin_pos_an(ctypes.byref(d1), ctypes.byref(d2), ctypes.byref(d3), reenc_h, f_h1,
+                                             wout_h, reenc_t, f_t2, wout_t, a_oid, *[_b(x, o[1]) for x, o in
+                                                                                   zip(b, outs)], cnt.ptr, ws.h,
+                                             current_stream()))
+
+    b, c = _run(call, outs, 4, [0, 0, 1, 2, 3])
+    return (b[0].to_numpy(np.dtype(f"u{wout_h}"), c[0]), b[1].to_numpy(np.dtype(f"u{wout_t}"), c[0]),
+            b[2].to_numpy(np.uint64, c[1]), b[3].to_numpy(np.uint64, c[2]), b[4].to_numpy(np.uint64, c[3]))
+
+
+def hashjoin_an(h1: Column, t1: Column, h2: Column, t2: Column, build_right=1, reenc_h=0, f_h1=0, wout_h=8,
+                reenc_t=0, f_t2=0, wout_t=8, a_oid: int = an.A_OID_ERR, out_cap: int | None = None,
+                err_cap: int = 1024):
+    lib, ws = load(), workspace()
+    cap = max(t1.n, t2.n) if out_cap is None else out_cap
+    outs = [[wout_h, cap], [wout_t, cap], [8, err_cap], [8, err_cap], [8, err_cap], [8, err_cap]]
+    ds = [x.desc() for x in (h1, t1, h2, t2)]
+
+    def call(b, cnt):
+        check(lib.ahead_hip_hashjoin_an(*[ctypes.byref(d) for d in ds], build_right, reenc_h, f_h1, wout_h, reenc_t,
+                                        f_t2, wout_t, a_oid, *[_b(x, o[1]) for x, o in zip(b, outs)], cnt.ptr, ws.h,
+                                        current_stream()))
+
+    b, c = _run(call, outs, 6, [0, 0, 1, 2, 3, 4])
+    if c[5] != 0:
+        raise AheadError(6, f"hashjoin status {c[5]}")
+    return (b[0].to_numpy(np.dtype(f"u{wout_h}"), c[0]), b[1].to_numpy(np.dtype(f"u{wout_t}"), c[0]),
+            *(b[i].to_numpy(np.uint64, c[i - 1]) for i in range(2, 6)))
+
+
+def groupby_an(head: Column, tail: Column, a_oid: int = an.A_OID, err_cap: int = 1024, table_bits: int = 0):
+    lib, ws = load(), workspace()
+    dh, dt = head.desc(), tail.desc()
+    while True:
+        outs = [[8, tail.n], [8, 4096], [8, err_cap], [8, err_cap]]
+
+        def call(b, cnt):
+            check(lib.ahead_hip_groupby_an(ctypes.byref(dh), ctypes.byref(dt), a_oid, table_bits, b[0].ptr,
+                                           _b(b[1], outs[1][1]), _b(b[2], outs[2][1]), _b(b[3], outs[3][1]), cnt.ptr,
+                                           ws.h, current_stream()))
+
+        b, c = _run(call, outs, 6, [None, 1, 2, 3])
+        if c[5] == 0:
+            break
+        table_bits = max(table_bits, 22) + 2
+    return (b[0].to_numpy(np.uint64, c[0]), b[1].to_numpy(np.uint64, c[1]), b[2].to_numpy(np.uint64, c[2]),
+            b[3].to_numpy(np.uint64, c[3]))
+
+
+def groupby2_an(head: Column, tail: Column, grouping: Column, num_groups: int, larger_bits: int,
+                a_oid: int = an.A_OID, err_cap: int = 1024, table_bits: int = 0):
+    lib, ws = load(), workspace()
+    dh, dt, dg = head.desc(), tail.desc(), grouping.desc()
+    while True:
+        outs = [[8, tail.n], [8, 4096], [8, err_cap], [8, err_cap], [8, err_cap]]
+
+        def call(b, cnt):
+            check(lib.ahead_hip_groupby2_an(ctypes.byref(dh), ctypes.byref(dt), ctypes.byref(dg), num_groups,
+                                            larger_bits, a_oid, table_bits, b[0].ptr, *[_b(x, o[1]) for x, o in
+                                                                                       zip(b[1:], outs[1:])],
+                                            cnt.ptr, ws.h, current_stream()))
+
+        b, c = _run(call, outs, 6, [None, 1, 2, 3, 4])
+        if c[5] == 0:
+            break
+        table_bits = max(table_bits, 22) + 2
+    return (b[0].to_numpy(np.uint64, c[0]), *(b[i].to_numpy(np.uint64, c[i]) for i in range(1, 5)))
+
+
+def aggregate_sum_grouped_an(head: Column, tail: Column, grouping: Column, num_groups: int,
+                             a_result: int = an.A_RESULT, a_oid: int = an.A_OID, err_cap: int = 1024):
+    lib, ws = load(), workspace()
+    outs = [[8, num_groups], [8, err_cap], [8, err_cap], [8, err_cap]]
+    ds = [x.desc() for x in (head, tail, grouping)]
+
+    def call(b, cnt):
+        check(lib.ahead_hip_aggregate_sum_grouped_an(*[ctypes.byref(d) for d in ds], num_groups, a_result, a_oid,
+                                                     b[0].ptr, *[_b(x, o[1]) for x, o in zip(b[1:], outs[1:])],
+                                                     cnt.ptr, ws.h, current_stream()))
+
+    b, c = _run(call, outs, 4, [None, 1, 2, 3])
+    return (b[0].to_numpy(np.uint64, num_groups), *(b[i].to_numpy(np.uint64, c[i]) for i in range(1, 4)))
+
+
+def aggregate_mul_sum_an(t1: Column, t2: Column, init: int = 0, result_encoded: int = 1, ra: int = an.A_RESULT,
+                         a_oid: int = an.A_OID_ERR, err_cap: int = 1024):
+    lib, ws = load(), workspace()
+    outs = [[8, 1], [8, err_cap], [8, err_cap]]
+    d1, d2 = t1.desc(), t2.desc()
+
+    def call(b, cnt):
+        check(lib.ahead_hip_aggregate_mul_sum_an(ctypes.byref(d1), ctypes.byref(d2), init, result_encoded, ra, a_oid,
+                                                 b[0].ptr, _b(b[1], outs[1][1]), _b(b[2], outs[2][1]), cnt.ptr, ws.h,
+                                                 current_stream()))
+
+    b, c = _run(call, outs, 3, [None, 1, 2])
+    return (int(b[0].to_numpy(np.uint64, 1)[0]), b[1].to_numpy(np.uint64, c[1]), b[2].to_numpy(np.uint64, c[2]))
+
+
+def arithmetic_an(h1: Column, t1: Column, h2: Column, t2: Column, op: int, a_result: int, result_width: int,
+                  a_oid: int = an.A_OID_ERR, err_cap: int = 1024):
+    lib, ws = load(), workspace()
+    outs = [[result_width, t1.n], [8, err_cap], [8, err_cap], [8, err_cap], [8, err_cap]]
+    ds = [x.desc() for x in (h1, t1, h2, t2)]
+
+    def call(b, cnt):
+        check(lib.ahead_hip_arithmetic_an(*[ctypes.byref(d) for d in ds], op, a_result, result_width, a_oid, b[0].ptr,
+                                          *[_b(x, o[1]) for x, o in zip(b[1:], outs[1:])], cnt.ptr, ws.h,
+                                          current_stream()))
+
+    b, c = _run(call, outs, 5, [None, 1, 2, 3, 4])
+    return (b[0].to_numpy(np.dtype(f"u{result_width}"), c[0]), *(b[i].to_numpy(np.uint64, c[i]) for i in range(1, 5)))

@@ -189,10 +189,9 @@ __device__ __forceinline__ uint32_t take_tile(const LbArgs &lb, uint32_t *s_tile
 //
 // One look-back round examines a window of 64 x kLbPerLane predecessors: lane l owns the
 // kLbPerLane consecutive tiles [end - W + l*kLbPerLane, ...), and all S x kLbPerLane status loads
-// of a lane are issued together, so a round costs one device-scope load latency.  The widest
-// window matters: tiles far ahead of the last published inclusive prefix need one round per
-// window, and the prefix frontier advances one window per round.
-constexpr int kLbPerLane = 4;
+// of a lane are issued together, so a round costs one device-scope load latency.  Measured on
+// MI355X, one status word per lane (a 64-tile window) gives the fastest scans.
+constexpr int kLbPerLane = 1;
 
 template <int S>
 __device__ __forceinline__ void lb_resolve(const LbArgs &lb, uint32_t tile, const uint32_t (&agg)[S],

@@ -1,0 +1,350 @@
+// hashjoin_an.hip -- hashjoinAN for gfx950 (hashjoinAN.tcc:77-222, entries :552-585).
+//
+// The reference builds google::dense_hash_map<unencoded key, vector<payload>> over one side and
+// probes it with the other.  Here:
+//  * build  : every build row is inserted under its decoded, width-truncated key
+//             (static_cast<h2unenc_t>(h2), :186) into an HBM hash table (htable.hpp) that counts
+//             the rows per key and keeps the first row.  Keys with several rows additionally get
+//             a row list: counts are scanned into offsets, rows are scattered into the lists and
+//             each list is sorted back into build (insertion) order.
+//  * checks : the build side's head/tail checks are an ordered two-stream pass on the row-tile
+//             engine (positions in build-scan order).
+//  * probe  : row-tile engine over the probe side: per row the head/tail checks, the key
+//             (decoded, guarded by `(larger_t)key <= max(build unencoded type)` and truncated to the
+//             build key width, :156-157 and :199-200 -- even for a corrupt probe key), and the
+//             matching rows' count; phase 2 writes the (probe payload, build payload) pairs of
+//             every probe row in probe order, build rows in insertion order.
+// Both hash sides (hash_side_t::right / ::left) and the re-encoding variants are supported.
+#include "htable.hpp"
+#include "rowtile.hpp"
+
+using namespace ahead_dev;
+
+namespace {
+
+constexpr uint32_t kMaxSortedList = 8192;  // longest per-key row list the build can order in LDS
+
+struct HjTable {
+    HTable ht;
+    uint32_t *cnt;    // [size + 1] rows per key
+    uint32_t *first;  // [size + 1] first (lowest) build row per key
+    uint32_t *off;    // [size + 1] list offsets (keys with several rows)
+    uint32_t *cur;    // [size + 1] fill cursors
+    uint32_t *rows;   // [n_build] row lists
+    uint32_t *flags;  // [0] several rows share a key, [1] table overflow, [2] list too long to order
+};
+
+struct BuildArgs {
+    HjTable t;
+    DCol key;
+    uint64_t key_mask;
+    uint64_t n;
+};
+
+__global__ __launch_bounds__(kBlock) void k_hj_insert(BuildArgs a) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t j = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; j < a.n; j += stride) {
+        const uint64_t k = a.key.dec_of(a.key.raw(j)) & a.key_mask;
+        const uint32_t s = a.t.ht.insert(k);
+        if (s == UINT32_MAX) continue;
+        if (atomicAdd(&a.t.cnt[s], 1u) != 0u) atomicOr(&a.t.flags[0], 1u);
+        atomicMin(&a.t.first[s], static_cast<uint32_t>(j));
+    }
+}
+
+// offsets of the row lists: a multi-entry scan over the table slots on the row-tile engine
+struct OffsetOp {
+    static constexpr int kErr = 0;
+    static constexpr bool kMulti = true, kMain = true, kAcc = false;
+    using State = uint8_t;
+    uint64_t n;
+    HjTable t;
+    __device__ uint32_t eval(uint64_t row, State &, uint32_t &, uint64_t &) const {
+        if (t.flags[0] == 0u) return 0;
+        const uint32_t c = t.cnt[row];
+        return c > 1u ? c : 0u;
+    }
+    __device__ void emit_main(uint64_t row, const State &, uint64_t pos, uint32_t) const {
+        t.off[row] = static_cast<uint32_t>(pos);
+    }
+    __device__ void emit_err(int, uint64_t, const State &, uint64_t) const {}
+    __device__ void tile_done(uint32_t, uint64_t) const {}
+    __device__ void finish(const uint64_t *) const {}
+};
+
+__global__ __launch_bounds__(kBlock) void k_hj_fill(BuildArgs a) {
+    if (a.t.flags[0] == 0u) return;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t j = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; j < a.n; j += stride) {
+        const uint64_t k = a.key.dec_of(a.key.raw(j)) & a.key_mask;
+        const uint32_t s = a.t.ht.find(k);
+        if (s == UINT32_MAX || a.t.cnt[s] < 2u) continue;
+        a.t.rows[a.t.off[s] + atomicAdd(&a.t.cur[s], 1u)] = static_cast<uint32_t>(j);
+    }
+}
+
+// Sorts every row list back into build order: one workgroup per list (grid-stride over slots),
+// bitonic sort in LDS.
+__global__ __launch_bounds__(kBlock) void k_hj_sort(HjTable t, uint32_t slots) {
+    __shared__ uint32_t s_v[kMaxSortedList];
+    if (t.flags[0] == 0u) return;
+    for (uint32_t s = blockIdx.x; s < slots; s += gridDim.x) {
+        const uint32_t c = t.cnt[s];
+        if (c < 2u) continue;
+        if (c > kMaxSortedList) {
+            if (threadIdx.x == 0) atomicOr(&t.flags[2], 1u);
+            continue;
+        }
+        uint32_t p2 = 1;
+        while (p2 < c) p2 <<= 1;
+        const uint32_t o = t.off[s];
+        for (uint32_t i = threadIdx.x; i < p2; i += kBlock) s_v[i] = i < c ? t.rows[o + i] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t k = 2; k <= p2; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < p2; i += kBlock) {
+                    const uint32_t l = i ^ j;
+                    if (l > i) {
+                        const uint32_t x = s_v[i], y = s_v[l];
+                        if ((x > y) == ((i & k) == 0)) {
+                            s_v[i] = y;
+                            s_v[l] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < c; i += kBlock) t.rows[o + i] = s_v[i];
+        __syncthreads();
+    }
+}
+
+// ordered checks of two columns (the build side's head and tail)
+struct Check2Op {
+    static constexpr int kErr = 2;
+    static constexpr bool kMulti = false, kMain = false, kAcc = false;
+    using State = uint8_t;
+    uint64_t n;
+    DCol c0, c1;
+    uint64_t a_oid;
+    uint64_t *err[2];
+    uint64_t cap[2];
+    uint64_t *counts[2];
+    __device__ uint32_t eval(uint64_t row, State &, uint32_t &errs, uint64_t &) const {
+        if (c0.bad_of(c0.raw(row))) errs |= 1u;
+        if (c1.bad_of(c1.raw(row))) errs |= 2u;
+        return 0;
+    }
+    __device__ void emit_main(uint64_t, const State &, uint64_t, uint32_t) const {}
+    __device__ void emit_err(int e, uint64_t row, const State &, uint64_t pos) const {
+        if (pos < cap[e]) err[e][pos] = row * a_oid;
+    }
+    __device__ void tile_done(uint32_t, uint64_t) const {}
+    __device__ void finish(const uint64_t *t) const {
+        *counts[0] = t[1];
+        *counts[1] = t[2];
+    }
+};
+
+struct ProbeState {
+    uint64_t pay;   // probe-side payload (raw)
+    uint32_t slot;
+};
+
+struct ProbeOp {
+    static constexpr int kErr = 2;  // probe head check, probe tail check
+    static constexpr bool kMulti = true, kMain = true, kAcc = false;
+    using State = ProbeState;
+    uint64_t n;
+    HjTable t;
+    DCol phead, ptail;        // probe BAT (head, tail)
+    DCol key;                 // the probe key column (decoded): ptail for a right build, phead for a left
+    DCol pay;                 // the probe payload column: phead for a right build, ptail for a left
+    DCol bpay;                // build payload column
+    uint64_t guard_mask, key_mask, build_max;
+    uint32_t build_right;
+    uint32_t reenc_p, reenc_b; // re-encode probe / build payload
+    uint64_t f_p, f_b, m_p, m_b;
+    uint32_t w_p, w_b;
+    void *out_h, *out_t;
+    uint64_t out_cap;
+    uint64_t a_oid;
+    uint64_t *err[2];
+    uint64_t cap[2];
+    uint64_t *counts;         // [0] pairs, error counts at [e_index[0]], [e_index[1]]
+    uint32_t e_index[2];
+
+    __device__ uint32_t eval(uint64_t row, State &st, uint32_t &errs, uint64_t &) const {
+        const uint64_t h = phead.raw(row), tl = ptail.raw(row);
+        if (phead.bad_of(h)) errs |= 1u;
+        if (ptail.bad_of(tl)) errs |= 2u;
+        const uint64_t kv = key.dec_of(build_right ? tl : h);
+        st.pay = build_right ? h : tl;
+        if ((kv & guard_mask) > build_max) return 0;
+        const uint32_t s = t.ht.find(kv & key_mask);
+        if (s == UINT32_MAX) return 0;
+        st.slot = s;
+        return t.cnt[s];
+    }
+    __device__ void emit_main(uint64_t, const State &st, uint64_t pos, uint32_t c) const {
+        const uint64_t pv = reenc_p ? (st.pay * f_p) & m_p : st.pay;
+        for (uint32_t k = 0; k < c; ++k) {
+            if (pos + k >= out_cap) break;
+            const uint32_t brow = c == 1u ? t.first[st.slot] : t.rows[t.off[st.slot] + k];
+            const uint64_t b = bpay.raw(brow);
+            const uint64_t bv = reenc_b ? (b * f_b) & m_b : b;
+            if (build_right) {
+                store_w(out_h, pos + k, w_p, pv);
+                store_w(out_t, pos + k, w_b, bv);
+            } else {
+                store_w(out_h, pos + k, w_b, bv);
+                store_w(out_t, pos + k, w_p, pv);
+            }
+        }
+    }
+    __device__ void emit_err(int e, uint64_t row, const State &, uint64_t pos) const {
+        if (pos < cap[e]) err[e][pos] = row * a_oid;
+    }
+    __device__ void tile_done(uint32_t, uint64_t) const {}
+    __device__ void finish(const uint64_t *tot) const {
+        counts[0] = tot[0];
+        counts[e_index[0]] = tot[1];
+        counts[e_index[1]] = tot[2];
+    }
+};
+
+__global__ void k_hj_status(const uint32_t *flags, uint64_t *d_counts) {
+    d_counts[5] = flags[1] ? 1u : (flags[2] ? 2u : 0u);
+}
+
+}  // namespace
+
+using namespace ahead_host;
+
+extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, const ahead_col *h2, const ahead_col *t2,
+                                     int build_right, int reenc_h, uint64_t f_h1, uint32_t wout_h, int reenc_t,
+                                     uint64_t f_t2, uint32_t wout_t, uint64_t a_oid, ahead_buf out_h, ahead_buf out_t,
+                                     ahead_buf e1, ahead_buf e2, ahead_buf e3, ahead_buf e4, uint64_t *d_counts,
+                                     ahead_hip_ws *ws, void *stream) {
+    int st;
+    if ((st = check_col(h1, "hashjoin.h1", true, false)) != AHEAD_OK) return st;
+    if ((st = check_col(t1, "hashjoin.t1", true, false)) != AHEAD_OK) return st;
+    if ((st = check_col(h2, "hashjoin.h2", true, false)) != AHEAD_OK) return st;
+    if ((st = check_col(t2, "hashjoin.t2", true, false)) != AHEAD_OK) return st;
+    if (h1->n != t1->n || h2->n != t2->n) return fail(AHEAD_ERR_SIZE_MISMATCH, "hashjoin: head and tail sizes differ");
+    if (d_counts == nullptr) return fail(AHEAD_ERR_INVALID, "hashjoin: d_counts is NULL");
+    for (uint32_t w : {wout_h, wout_t})
+        if (w != 1 && w != 2 && w != 4 && w != 8) return fail(AHEAD_ERR_INVALID, "hashjoin: output widths 1, 2, 4, 8");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (t1->n == 0 || t2->n == 0) {  // hashjoinAN.tcc:128: nothing is built, probed or checked
+        return fill_u64(d_counts, 6, 0, s);
+    }
+    const bool right = build_right != 0;
+    const ahead_col &bkey = right ? *h2 : *t1, &bpay = right ? *t2 : *h1;
+    const ahead_col &phead = right ? *h1 : *h2, &ptail = right ? *t1 : *t2;
+    const ahead_col &pkey = right ? *t1 : *h2;
+    const uint64_t nb = bkey.n, np = phead.n;
+    if (nb > 0x7FFFFFFFull) return fail(AHEAD_ERR_TOO_LARGE, "hashjoin: build side above 2^31 rows");
+
+    uint32_t bits = 4;
+    while ((1ull << bits) < 2ull * nb) ++bits;
+    const uint64_t slots = (1ull << bits) + 1;
+    // layout: [flags 256][keys][cnt][first][off][cur][rows][status...]
+    const size_t o_flags = 0, o_keys = 256, o_cnt = o_keys + align_up(slots * 8, 256),
+                 o_first = o_cnt + align_up(slots * 4, 256), o_off = o_first + align_up(slots * 4, 256),
+                 o_cur = o_off + align_up(slots * 4, 256), o_rows = o_cur + align_up(slots * 4, 256),
+                 o_status = o_rows + align_up(nb * 4, 256);
+    size_t stat = rowtile_status_bytes(np, 3);
+    const size_t s2 = rowtile_status_bytes(nb, 3), s3 = rowtile_status_bytes(slots, 1);
+    if (s2 > stat) stat = s2;
+    if (s3 > stat) stat = s3;
+    if ((st = ws_ensure(ws, o_status + stat)) != AHEAD_OK) return st;
+
+    HjTable t{};
+    t.flags = reinterpret_cast<uint32_t *>(ws_region(ws, o_flags));
+    t.ht.keys = reinterpret_cast<unsigned long long *>(ws_region(ws, o_keys));
+    t.ht.bits = bits;
+    t.ht.overflow = t.flags + 1;
+    t.cnt = reinterpret_cast<uint32_t *>(ws_region(ws, o_cnt));
+    t.first = reinterpret_cast<uint32_t *>(ws_region(ws, o_first));
+    t.off = reinterpret_cast<uint32_t *>(ws_region(ws, o_off));
+    t.cur = reinterpret_cast<uint32_t *>(ws_region(ws, o_cur));
+    t.rows = reinterpret_cast<uint32_t *>(ws_region(ws, o_rows));
+    if ((st = fill_u32(t.flags, 64, 0, s)) != AHEAD_OK) return st;
+    if ((st = fill_u64(reinterpret_cast<uint64_t *>(t.ht.keys), slots, 0, s)) != AHEAD_OK) return st;
+    if ((st = fill_u32(t.cnt, slots, 0, s)) != AHEAD_OK) return st;
+    if ((st = fill_u32(t.first, slots, 0xFFFFFFFFu, s)) != AHEAD_OK) return st;
+    if ((st = fill_u32(t.cur, slots, 0, s)) != AHEAD_OK) return st;
+
+    BuildArgs b{};
+    b.t = t;
+    b.key = dcol(bkey);
+    b.key_mask = bits_mask(bkey.data ? bkey.unenc_bits : 32);
+    b.n = nb;
+    const uint64_t gb = (nb + kBlock - 1) / kBlock;
+    const dim3 grid(static_cast<uint32_t>(gb < 4096 ? gb : 4096));
+    hipLaunchKernelGGL(k_hj_insert, grid, dim3(kBlock), 0, s, b);
+    AHEAD_HIP_TRY(hipGetLastError());
+    OffsetOp off{};
+    off.n = slots;
+    off.t = t;
+    if ((st = launch_rowtile(off, slots, 1, ws, o_status, s)) != AHEAD_OK) return st;
+    hipLaunchKernelGGL(k_hj_fill, grid, dim3(kBlock), 0, s, b);
+    hipLaunchKernelGGL(k_hj_sort, dim3(1024), dim3(kBlock), 0, s, t, static_cast<uint32_t>(slots));
+    AHEAD_HIP_TRY(hipGetLastError());
+
+    // build-side checks, in build-scan order: e3/e4 for a right build, e1/e2 for a left build
+    Check2Op chk{};
+    chk.n = nb;
+    chk.c0 = dcol(right ? *h2 : *h1);
+    chk.c1 = dcol(right ? *t2 : *t1);
+    chk.a_oid = a_oid;
+    const ahead_buf be0 = right ? e3 : e1, be1 = right ? e4 : e2;
+    chk.err[0] = static_cast<uint64_t *>(be0.data);
+    chk.cap[0] = be0.data ? be0.cap : 0;
+    chk.err[1] = static_cast<uint64_t *>(be1.data);
+    chk.cap[1] = be1.data ? be1.cap : 0;
+    chk.counts[0] = d_counts + (right ? 3 : 1);
+    chk.counts[1] = d_counts + (right ? 4 : 2);
+    if ((st = launch_rowtile(chk, nb, 3, ws, o_status, s)) != AHEAD_OK) return st;
+
+    ProbeOp p{};
+    p.n = np;
+    p.t = t;
+    p.phead = dcol(phead);
+    p.ptail = dcol(ptail);
+    p.key = dcol(pkey);
+    p.pay = dcol(right ? phead : ptail);
+    p.bpay = dcol(bpay);
+    const uint32_t pbits = pkey.data ? pkey.unenc_bits : 32, bbits = bkey.data ? bkey.unenc_bits : 32;
+    p.guard_mask = bits_mask(pbits > bbits ? pbits : bbits);  // larger_type<t1unenc_t, h2unenc_t>
+    p.build_max = bits_mask(bbits);                           // numeric_limits<h2unenc_t>::max()
+    p.key_mask = bits_mask(bbits);
+    p.build_right = right;
+    // probe payload = h1 (right build) / t2 (left build); build payload = t2 / h1
+    p.reenc_p = right ? reenc_h != 0 : reenc_t != 0;
+    p.reenc_b = right ? reenc_t != 0 : reenc_h != 0;
+    p.f_p = right ? f_h1 : f_t2;
+    p.f_b = right ? f_t2 : f_h1;
+    p.w_p = right ? wout_h : wout_t;
+    p.w_b = right ? wout_t : wout_h;
+    p.m_p = width_mask(p.w_p);
+    p.m_b = width_mask(p.w_b);
+    p.out_h = out_h.data;
+    p.out_t = out_t.data;
+    p.out_cap = (out_h.data && out_t.data) ? (out_h.cap < out_t.cap ? out_h.cap : out_t.cap) : 0;
+    p.a_oid = a_oid;
+    const ahead_buf pe0 = right ? e1 : e3, pe1 = right ? e2 : e4;
+    p.err[0] = static_cast<uint64_t *>(pe0.data);
+    p.cap[0] = pe0.data ? pe0.cap : 0;
+    p.err[1] = static_cast<uint64_t *>(pe1.data);
+    p.cap[1] = pe1.data ? pe1.cap : 0;
+    p.counts = d_counts;
+    p.e_index[0] = right ? 1 : 3;
+    p.e_index[1] = right ? 2 : 4;
+    if ((st = launch_rowtile(p, np, 3, ws, o_status, s)) != AHEAD_OK) return st;
+    hipLaunchKernelGGL(k_hj_status, dim3(1), dim3(1), 0, s, t.flags, d_counts);
+    AHEAD_HIP_TRY(hipGetLastError());
+    return AHEAD_OK;
+}

@@ -74,6 +74,10 @@ inline LbPlan lb_plan(uint64_t n, uint32_t tile_rows, int streams) {
 }
 int lb_args(ahead_hip_ws *ws, const LbPlan &p, size_t status_offset, ahead_dev::LbArgs *out);
 
+// Device fill kernels (graph-capture safe replacements for hipMemsetAsync).
+int fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s);
+int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
+
 // Process-wide tiling knobs (ahead_hip_set_tiling).
 using ahead_dev::kMaxSubtiles;
 struct Tiling {
@@ -84,5 +88,11 @@ Tiling tiling();
 // Sub-tiles per look-back tile for a stream of `subs` sub-tiles: the largest power of two
 // <= max_subtiles that still leaves >= min_tiles tiles.
 uint32_t subtiles_per_tile(uint64_t subs);
+
+// checkAN with its look-back status placed at `status_offset` of the workspace region (so that
+// composite operators can keep other data in the workspace meanwhile).
+int check_an_at(const ahead_col *col, uint64_t a_oid, uint64_t *err, uint64_t err_cap, uint64_t *d_counts,
+                ahead_hip_ws *ws, void *stream, size_t status_offset);
+size_t check_an_status_bytes(uint64_t n, uint32_t width);
 
 }  // namespace ahead_host

@@ -1,0 +1,58 @@
+// htable.hpp -- open-addressing hash table of 64-bit keys in HBM for the hash-join build and the
+// group-by dictionary.
+//
+// A slot word holds key + 1 (0 = empty) and is claimed with one 64-bit compare-and-swap, so no
+// lane ever waits for another lane's write.  The single key that key + 1 cannot represent
+// (2^64 - 1) lives in an extra slot at index `size`.  Linear probing; the table is sized to at
+// most half full, so every probe sequence ends at an empty slot.
+#pragma once
+
+#include "an_device.hpp"
+
+namespace ahead_dev {
+
+struct HTable {
+    unsigned long long *keys;  // [size + 1]
+    uint32_t bits;             // size = 2^bits
+    uint32_t *overflow;        // set when an insert found no free slot within the probe bound
+
+    __device__ __forceinline__ uint32_t size() const { return 1u << bits; }
+    __device__ __forceinline__ uint32_t home(uint64_t key) const {
+        // xor-fold: dense small keys map to themselves, wider keys are folded in
+        const uint64_t x = key ^ (key >> bits) ^ (key >> (2 * bits)) ^ (key >> 48);
+        return static_cast<uint32_t>(x) & (size() - 1u);
+    }
+
+    // Inserts key (if absent) and returns its slot, or UINT32_MAX if the table is full.
+    __device__ __forceinline__ uint32_t insert(uint64_t key) const {
+        if (key == ~0ull) {
+            atomicCAS(&keys[size()], 0ull, 1ull);
+            return size();
+        }
+        const unsigned long long word = static_cast<unsigned long long>(key) + 1ull;
+        uint32_t s = home(key);
+        for (uint32_t probes = 0; probes < size(); ++probes) {
+            const unsigned long long prev = atomicCAS(&keys[s], 0ull, word);
+            if (prev == 0ull || prev == word) return s;
+            s = (s + 1u) & (size() - 1u);
+        }
+        atomicOr(overflow, 1u);
+        return UINT32_MAX;
+    }
+
+    // Returns the slot of key, or UINT32_MAX if absent.
+    __device__ __forceinline__ uint32_t find(uint64_t key) const {
+        if (key == ~0ull) return keys[size()] != 0ull ? size() : UINT32_MAX;
+        const unsigned long long word = static_cast<unsigned long long>(key) + 1ull;
+        uint32_t s = home(key);
+        for (uint32_t probes = 0; probes < size(); ++probes) {
+            const unsigned long long k = keys[s];
+            if (k == word) return s;
+            if (k == 0ull) return UINT32_MAX;
+            s = (s + 1u) & (size() - 1u);
+        }
+        return UINT32_MAX;
+    }
+};
+
+}  // namespace ahead_dev

@@ -78,6 +78,34 @@ int lb_args(ahead_hip_ws *ws, const LbPlan &p, size_t status_offset, ahead_dev::
     return AHEAD_OK;
 }
 
+__global__ void k_fill64(uint64_t *p, uint64_t n, uint64_t v) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+__global__ void k_fill32(uint32_t *p, uint64_t n, uint32_t v) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+static uint32_t fill_grid(uint64_t n) {
+    const uint64_t b = (n + 255) / 256;
+    return static_cast<uint32_t>(b == 0 ? 1 : (b < 2048 ? b : 2048));
+}
+
+int fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s) {
+    if (n == 0) return AHEAD_OK;
+    hipLaunchKernelGGL(k_fill64, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v);
+    AHEAD_HIP_TRY(hipGetLastError());
+    return AHEAD_OK;
+}
+
+int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s) {
+    if (n == 0) return AHEAD_OK;
+    hipLaunchKernelGGL(k_fill32, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v);
+    AHEAD_HIP_TRY(hipGetLastError());
+    return AHEAD_OK;
+}
+
 static Tiling g_tiling;
 
 Tiling tiling() { return g_tiling; }

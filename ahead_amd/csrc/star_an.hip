@@ -14,16 +14,21 @@
 //                 (hashjoinAN.tcc:177-187) into an open-addressing table of
 //                 (key << 32 | row + 1) words; a repeated key raises the "not fused" flag
 //                 (the fused path needs unique build keys).
+//  k_star_dim     decodes and checks both attribute codes of every dimension row and marks the
+//                 (attr1, attr2) combinations of the valid rows;
+//  k_star_cid     numbers the marked combinations densely (combination id, cid);
+//  k_star_info    stores per dimension row its cid and its two attribute-error bits, so that a
+//                 probe needs one table word and one info word per fact row.
 //  k_star_probe   one workgroup per tile of ITERS x 2048 fact rows.  Phase 1 streams FK and
 //                 measure codes with 128-bit loads, decodes and checks the FK, probes the table
 //                 with the truncated decoded key even when the FK is corrupt (hashjoinAN.tcc:199),
-//                 fetches and checks both attribute codes and the measure, and accumulates the
-//                 measure code into an LDS table of (attr1, attr2) combinations together with the
-//                 combination's first tile-local row.  Per-row kept/error bits live in LDS
-//                 bitmaps.  Resolve: five look-back streams (kept rows, FK errors, attr1 errors,
-//                 attr2 errors, measure errors).  Phase 2 turns first rows into global kept-row
-//                 ranks, merges the LDS table into the global table with 64-bit atomics and writes
-//                 the error positions in order.
+//                 checks the measure, and adds the measure code of every kept row into per-wave
+//                 LDS accumulators indexed by cid (with the first tile-local row per cid).  Kept
+//                 rows form an LDS bitmap built from wave ballots; error rows (rare) set bits with
+//                 LDS atomics.  Resolve: five look-back streams (kept rows, FK errors, attr1
+//                 errors, attr2 errors, measure errors).  Phase 2 turns first rows into global
+//                 kept-row ranks, merges the accumulators into the global per-combination table
+//                 with 64-bit atomics and writes the error positions in order.
 //  k_star_final   orders the combinations by first kept-row rank (= binary group ids, since
 //                 (attr2, g1) <-> (attr1, attr2) is one-to-one), writes sums, VGIDtoROID and key
 //                 codes, and clears the global table for the next launch.
@@ -37,7 +42,8 @@ constexpr int kStarRowsPerSub = kBlock * 2 * 4;  // 2 x 64-bit codes per 16-byte
 constexpr int kStarMaxIters = 8;
 constexpr int kStarTileRows = kStarRowsPerSub * kStarMaxIters;  // 16384
 constexpr int kStarWords = kStarTileRows / 32;
-constexpr int kComboSlots = 1024;           // LDS combination table per tile
+constexpr uint32_t kLdsCids = 2048;         // combination ids accumulated in LDS per tile
+constexpr uint32_t kWaveCids = kLdsCids / kWaves;  // per-wave accumulators when cids fit
 constexpr uint32_t kCombos = 65536;         // 8-bit attr1 x 8-bit attr2
 constexpr uint32_t kFinalMaxGroups = 8192;  // groups the single-workgroup finalize can order
 enum StarStream { kKept = 0, kEFk = 1, kEA1 = 2, kEA2 = 3, kEM = 4, kStreams = 5 };
@@ -80,6 +86,65 @@ __global__ __launch_bounds__(kBlock) void k_star_build(BuildArgs a) {
     }
 }
 
+struct DimArgs {
+    const uint16_t *a1, *a2;
+    uint32_t n;
+    uint16_t a1_inv, a2_inv, a1_maxu, a2_maxu;
+    uint32_t *present;     // [kCombos / 32] bitmap of combinations of valid rows
+    uint16_t *combo_cid;   // [kCombos]
+    uint16_t *cid_combo;   // [kCombos]
+    uint32_t *info;        // [n]: cid | attr1 error << 30 | attr2 error << 31
+    uint32_t *flags;       // [1] = number of cids
+};
+
+__global__ __launch_bounds__(kBlock) void k_star_dim(DimArgs a) {
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < a.n; j += stride) {
+        const uint16_t d1 = an_mul<uint16_t>(a.a1[j], a.a1_inv), d2 = an_mul<uint16_t>(a.a2[j], a.a2_inv);
+        if (d1 <= a.a1_maxu && d2 <= a.a2_maxu) {
+            const uint32_t combo = (static_cast<uint32_t>(d1 & 0xFFu) << 8) | (d2 & 0xFFu);
+            atomicOr(&a.present[combo >> 5], 1u << (combo & 31u));
+        }
+    }
+}
+
+// One workgroup: dense numbering of the present combinations (in combination order).
+__global__ __launch_bounds__(kBlock) void k_star_cid(DimArgs a) {
+    __shared__ uint32_t s_red[kWaves];
+    constexpr int PER = (kCombos / 32) / kBlock;  // 8 bitmap words per thread
+    uint32_t w[PER], c = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        w[q] = a.present[threadIdx.x * PER + q];
+        c += __popc(w[q]);
+    }
+    uint32_t tot;
+    uint32_t cid = block_excl_scan<uint32_t>(c, tot, s_red);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        uint32_t bits = w[q];
+        while (bits) {
+            const uint32_t combo = (threadIdx.x * PER + q) * 32u + (__ffs(bits) - 1);
+            bits &= bits - 1u;
+            a.combo_cid[combo] = static_cast<uint16_t>(cid);
+            a.cid_combo[cid] = static_cast<uint16_t>(combo);
+            ++cid;
+        }
+    }
+    if (threadIdx.x == 0) a.flags[1] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_star_info(DimArgs a) {
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < a.n; j += stride) {
+        const uint16_t d1 = an_mul<uint16_t>(a.a1[j], a.a1_inv), d2 = an_mul<uint16_t>(a.a2[j], a.a2_inv);
+        const bool e1 = d1 > a.a1_maxu, e2 = d2 > a.a2_maxu;
+        uint32_t info = (static_cast<uint32_t>(e1) << 30) | (static_cast<uint32_t>(e2) << 31);
+        if (!e1 && !e2) info |= a.combo_cid[(static_cast<uint32_t>(d1 & 0xFFu) << 8) | (d2 & 0xFFu)];
+        a.info[j] = info;
+    }
+}
+
 struct ProbeArgs {
     const uint64_t *fk;
     const uint64_t *m;
@@ -87,10 +152,9 @@ struct ProbeArgs {
     uint32_t iters;
     uint64_t fk_inv, fk_maxu, key_mask;
     uint64_t m_inv, m_maxu;
-    const uint16_t *a1;
-    const uint16_t *a2;
+    const uint32_t *info;       // per dimension row: cid | attr error bits
+    const uint16_t *cid_combo;  // cid -> (attr1 << 8 | attr2)
     uint32_t n_dim;
-    uint16_t a1_inv, a2_inv, a1_maxu, a2_maxu;
     const uint64_t *table;
     uint32_t table_bits;
     const uint32_t *flags;
@@ -115,10 +179,10 @@ __device__ __forceinline__ int64_t probe_table(const uint64_t *table, uint32_t b
     return -1;
 }
 
-// Everything phase 2 needs to re-derive about one fact row (rare error paths only).
+// Everything phase 1 and the rare phase-2 paths need to know about one fact row.
 struct RowEval {
     bool joined, kept, e_fk, e_a1, e_a2, e_m;
-    uint32_t dim_row, combo;
+    uint32_t dim_row, cid;
 };
 
 __device__ __forceinline__ RowEval eval_row(const ProbeArgs &a, uint64_t fk, uint64_t m) {
@@ -129,15 +193,24 @@ __device__ __forceinline__ RowEval eval_row(const ProbeArgs &a, uint64_t fk, uin
     r.joined = j >= 0;
     if (r.joined) {
         r.dim_row = static_cast<uint32_t>(j);
-        const uint16_t d1 = an_mul<uint16_t>(a.a1[j], a.a1_inv);
-        const uint16_t d2 = an_mul<uint16_t>(a.a2[j], a.a2_inv);
-        r.e_a1 = d1 > a.a1_maxu;
-        r.e_a2 = d2 > a.a2_maxu;
+        const uint32_t info = a.info[j];
+        r.e_a1 = (info >> 30) & 1u;
+        r.e_a2 = (info >> 31) & 1u;
         r.e_m = (m * a.m_inv) > a.m_maxu;
         r.kept = !(r.e_a1 || r.e_a2 || r.e_m);
-        r.combo = (static_cast<uint32_t>(d1 & 0xFFu) << 8) | (d2 & 0xFFu);
+        r.cid = info & 0xFFFFu;
     }
     return r;
+}
+
+// 16 bits -> even bit positions of 32 (bit interleave)
+__device__ __forceinline__ uint32_t spread16(uint32_t x) {
+    x &= 0xFFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
 }
 
 __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
@@ -145,18 +218,17 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
     __shared__ uint32_t s_red[kWaves];
     __shared__ uint32_t s_excl[kStreams];
     __shared__ uint32_t s_bits[kStreams + 1][kStarWords];  // kept / error / overflow bitmaps by row
-    __shared__ uint32_t s_wpre[kStarWords];             // kept rows before each bitmap word
-    __shared__ uint32_t s_ckey[kComboSlots];            // combination + 1 (0 = free)
-    __shared__ uint32_t s_cfirst[kComboSlots];          // first tile-local row
-    __shared__ unsigned long long s_csum[kComboSlots];
+    __shared__ uint32_t s_wpre[kStarWords];                 // kept rows before each bitmap word
+    __shared__ unsigned long long s_sum[kLdsCids];          // per cid (per wave when cids fit)
+    __shared__ uint32_t s_first[kLdsCids];                  // first tile-local row per cid
     __shared__ uint32_t s_any_overflow;
-    __shared__ uint32_t s_flag;
+    __shared__ uint32_t s_flag[2];
 
     const uint32_t tile = take_tile(a.lb, &s_tile);
     if (tile >= a.lb.num_tiles) return;
-    if (threadIdx.x == 0) s_flag = a.flags[0];
+    if (threadIdx.x < 2) s_flag[threadIdx.x] = a.flags[threadIdx.x];
     __syncthreads();
-    if (s_flag != 0u) {
+    if (s_flag[0] != 0u) {
         // not applicable (duplicate build keys): publish empty counts so the chain drains
         const uint32_t zero[kStreams] = {0, 0, 0, 0, 0};
         lb_resolve<kStreams>(a.lb, tile, zero, s_excl);
@@ -166,11 +238,13 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
         }
         return;
     }
+    const uint32_t ncid = s_flag[1];
+    const bool per_wave = ncid <= kWaveCids;  // private accumulators per wave: no cross-wave contention
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     for (int i = threadIdx.x; i < (kStreams + 1) * kStarWords; i += kBlock) (&s_bits[0][0])[i] = 0u;
-    for (int i = threadIdx.x; i < kComboSlots; i += kBlock) {
-        s_ckey[i] = 0u;
-        s_cfirst[i] = 0xFFFFFFFFu;
-        s_csum[i] = 0ull;
+    for (uint32_t i = threadIdx.x; i < kLdsCids; i += kBlock) {
+        s_sum[i] = 0ull;
+        s_first[i] = 0xFFFFFFFFu;
     }
     if (threadIdx.x == 0) s_any_overflow = 0u;
     __syncthreads();
@@ -199,40 +273,87 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
                 }
             }
         }
+        // batched probe: all table words of this thread's 8 rows are requested before any is
+        // used, then all info words, so the dependent loads overlap instead of chaining
+        uint32_t key[4][2], slot[4][2];
+        uint64_t ent[4][2];
+        bool live[4][2], efk[4][2];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 const uint32_t local = sub + c * (kBlock * 2) + threadIdx.x * 2 + k;
-                if (base + local >= a.n) continue;
-                const RowEval r = eval_row(a, fk[c][k], mv[c][k]);
-                const uint32_t w = local >> 5, b = 1u << (local & 31u);
-                if (r.e_fk) { atomicOr(&s_bits[kEFk][w], b); cnt[kEFk]++; }
-                if (!r.joined) continue;
-                if (r.e_a1) { atomicOr(&s_bits[kEA1][w], b); cnt[kEA1]++; }
-                if (r.e_a2) { atomicOr(&s_bits[kEA2][w], b); cnt[kEA2]++; }
-                if (r.e_m) { atomicOr(&s_bits[kEM][w], b); cnt[kEM]++; }
-                if (!r.kept) continue;
-                atomicOr(&s_bits[kKept][w], b);
-                cnt[kKept]++;
-                // LDS combination table: open addressing on combo + 1
-                const uint32_t key = r.combo + 1u;
-                uint32_t s = (r.combo * 0x9E3779B1u) >> 22;  // 10-bit slot
-                bool placed = false;
-                for (int p = 0; p < kComboSlots; ++p) {
-                    const uint32_t prev = atomicCAS(&s_ckey[s], 0u, key);
-                    if (prev == 0u || prev == key) {
-                        atomicAdd(&s_csum[s], static_cast<unsigned long long>(mv[c][k]));
-                        atomicMin(&s_cfirst[s], local);
-                        placed = true;
-                        break;
+                live[c][k] = base + local < a.n;
+                const uint64_t d = fk[c][k] * a.fk_inv;
+                efk[c][k] = live[c][k] && d > a.fk_maxu;
+                key[c][k] = static_cast<uint32_t>(d & a.key_mask);
+                slot[c][k] = table_slot(key[c][k], a.table_bits);
+                ent[c][k] = live[c][k] ? a.table[slot[c][k]] : 0ull;
+            }
+        }
+        int64_t row[4][2];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                row[c][k] = -1;
+                if (!live[c][k] || ent[c][k] == 0ull) continue;
+                if (static_cast<uint32_t>(ent[c][k] >> 32) == key[c][k]) {
+                    row[c][k] = static_cast<uint32_t>(ent[c][k]) - 1u;
+                } else {  // collision: continue the linear probe (rare for dense keys)
+                    const uint32_t mask = (1u << a.table_bits) - 1u;
+                    uint32_t s2 = (slot[c][k] + 1u) & mask;
+                    for (uint32_t probes = 0; probes < mask; ++probes) {
+                        const uint64_t e = a.table[s2];
+                        if (e == 0ull) break;
+                        if (static_cast<uint32_t>(e >> 32) == key[c][k]) {
+                            row[c][k] = static_cast<uint32_t>(e) - 1u;
+                            break;
+                        }
+                        s2 = (s2 + 1u) & mask;
                     }
-                    s = (s + 1u) & (kComboSlots - 1u);
                 }
-                if (!placed) {  // table full: this row merges into the global table in phase 2
+            }
+        }
+        uint32_t info[4][2];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) info[c][k] = row[c][k] >= 0 ? a.info[row[c][k]] : 0u;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            bool kept[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const uint32_t local = sub + c * (kBlock * 2) + threadIdx.x * 2 + k;
+                kept[k] = false;
+                const uint32_t w = local >> 5, b = 1u << (local & 31u);
+                if (efk[c][k]) { atomicOr(&s_bits[kEFk][w], b); cnt[kEFk]++; }
+                if (row[c][k] < 0) continue;
+                const bool e1 = (info[c][k] >> 30) & 1u, e2 = (info[c][k] >> 31) & 1u;
+                const bool em = (mv[c][k] * a.m_inv) > a.m_maxu;
+                if (e1) { atomicOr(&s_bits[kEA1][w], b); cnt[kEA1]++; }
+                if (e2) { atomicOr(&s_bits[kEA2][w], b); cnt[kEA2]++; }
+                if (em) { atomicOr(&s_bits[kEM][w], b); cnt[kEM]++; }
+                if (e1 || e2 || em) continue;
+                kept[k] = true;
+                cnt[kKept]++;
+                const uint32_t cid = info[c][k] & 0xFFFFu;
+                if (cid < kLdsCids) {
+                    const uint32_t sl = per_wave ? wave * kWaveCids + cid : cid;
+                    atomicAdd(&s_sum[sl], static_cast<unsigned long long>(mv[c][k]));
+                    if (local < s_first[cid]) atomicMin(&s_first[cid], local);
+                } else {  // more combinations than LDS holds: merged through the global table in phase 2
                     atomicOr(&s_bits[kOverflow][w], b);
                     s_any_overflow = 1u;
                 }
+            }
+            // kept bitmap words from two ballots (even / odd rows of the wave's 128-row span)
+            const uint64_t b0 = __ballot(kept[0]), b1 = __ballot(kept[1]);
+            if (lane < 4) {
+                const uint32_t x0 = static_cast<uint32_t>(b0 >> (16 * lane)), x1 = static_cast<uint32_t>(b1 >> (16 * lane));
+                s_bits[kKept][(sub + c * (kBlock * 2) + wave * 128) / 32 + lane] = spread16(x0) | (spread16(x1) << 1);
             }
         }
     }
@@ -258,15 +379,23 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
         }
     }
     __syncthreads();
-    // phase 2b: merge the LDS combination table into the global table
+    // phase 2b: merge the LDS accumulators into the global per-combination table
     const uint32_t xk = s_excl[kKept];
-    for (int i = threadIdx.x; i < kComboSlots; i += kBlock) {
-        const uint32_t key = s_ckey[i];
-        if (key == 0u) continue;
-        const uint32_t combo = key - 1u, first = s_cfirst[i];
+    const uint32_t nslots = ncid < kLdsCids ? ncid : kLdsCids;
+    for (uint32_t cid = threadIdx.x; cid < nslots; cid += kBlock) {
+        const uint32_t first = s_first[cid];
+        if (first == 0xFFFFFFFFu) continue;
+        unsigned long long sum = 0ull;
+        if (per_wave) {
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) sum += s_sum[w * kWaveCids + cid];
+        } else {
+            sum = s_sum[cid];
+        }
+        const uint32_t combo = a.cid_combo[cid];
         const uint32_t w = first >> 5;
         const uint32_t rank = xk + s_wpre[w] + __popc(s_bits[kKept][w] & ((1u << (first & 31u)) - 1u));
-        atomicAdd(reinterpret_cast<unsigned long long *>(&a.g_sum[combo]), s_csum[i]);
+        atomicAdd(reinterpret_cast<unsigned long long *>(&a.g_sum[combo]), sum);
         atomicMax(&a.g_first[combo], 0xFFFFFFFFu - rank);
     }
     // phase 2b': rows whose combination found no LDS slot go straight to the global table
@@ -282,9 +411,10 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
                 const uint64_t row = base + w * 32u + b;
                 const RowEval r = eval_row(a, a.fk[row], a.m[row]);
                 const uint32_t rank = xk + s_wpre[w] + __popc(s_bits[kKept][w] & ((1u << b) - 1u));
-                atomicAdd(reinterpret_cast<unsigned long long *>(&a.g_sum[r.combo]),
+                const uint32_t combo = a.cid_combo[r.cid];
+                atomicAdd(reinterpret_cast<unsigned long long *>(&a.g_sum[combo]),
                           static_cast<unsigned long long>(a.m[row]));
-                atomicMax(&a.g_first[r.combo], 0xFFFFFFFFu - rank);
+                atomicMax(&a.g_first[combo], 0xFFFFFFFFu - rank);
             }
         }
     }
@@ -409,8 +539,9 @@ __global__ void k_clear_u64(uint64_t *p, uint64_t n) {
 
 using namespace ahead_host;
 
-extern "C" int ahead_hip_star1_pipeline(const ahead_star1_args *in, const ahead_star1_out *out,
-                                        ahead_hip_ws *ws, void *stream) {
+namespace {
+
+int star_validate(const ahead_star1_args *in, const ahead_star1_out *out) {
     if (in == nullptr || out == nullptr) return fail(AHEAD_ERR_INVALID, "star1: NULL arguments");
     int st;
     if ((st = check_col(&in->fact_fk, "star1.fact_fk", false, true)) != AHEAD_OK) return st;
@@ -430,78 +561,134 @@ extern "C" int ahead_hip_star1_pipeline(const ahead_star1_args *in, const ahead_
     if (out->d_counts == nullptr) return fail(AHEAD_ERR_INVALID, "star1: d_counts is NULL");
     if ((in->a_oid_grp & 1ull) == 0 || (in->fact_m.a & 1ull) == 0)
         return fail(AHEAD_ERR_INVALID, "star1: A values must be odd");
+    return AHEAD_OK;
+}
 
-    const uint32_t n_dim = static_cast<uint32_t>(in->dim_key.n);
-    uint32_t tbits = 4;
-    while ((1ull << tbits) < 2ull * n_dim) ++tbits;
-    const size_t table_bytes = align_up((1ull << tbits) * 8, 256);
-    const size_t gsum_bytes = align_up(kCombos * 8ull, 256), gfirst_bytes = align_up(kCombos * 4ull, 256);
+// Workspace layout of one star join: [flags 256][g_sum][g_first][table][look-back status].  The
+// dimension table survives between ahead_hip_star1_build and ahead_hip_star1_probe as long as no
+// other operator runs on the workspace in between.
+struct StarLayout {
+    uint32_t tbits = 4, iters = 1;
+    LbPlan plan;
+    size_t off_flags = 0, off_gsum = 0, off_gfirst = 0, off_table = 0, off_present = 0, off_combo_cid = 0,
+           off_cid_combo = 0, off_info = 0, off_status = 0, total = 0;
+};
+
+StarLayout star_layout(const ahead_star1_args *in) {
+    StarLayout L;
+    const uint64_t n_dim = in->dim_key.n;
+    while ((1ull << L.tbits) < 2ull * n_dim) ++L.tbits;
     const uint64_t subs = (in->fact_fk.n + kStarRowsPerSub - 1) / kStarRowsPerSub;
-    uint32_t iters = subtiles_per_tile(subs);
-    if (iters > static_cast<uint32_t>(kStarMaxIters)) iters = kStarMaxIters;
-    const LbPlan plan = lb_plan(in->fact_fk.n, kStarRowsPerSub * iters, kStreams);
-    // layout: [status][flags 256][g_sum][g_first][table]
-    const size_t off_flags = plan.status_bytes, off_gsum = off_flags + 256, off_gfirst = off_gsum + gsum_bytes,
-                 off_table = off_gfirst + gfirst_bytes;
-    if ((st = ws_ensure(ws, off_table + table_bytes)) != AHEAD_OK) return st;
-    hipStream_t s = static_cast<hipStream_t>(stream);
+    L.iters = subtiles_per_tile(subs);
+    if (L.iters > static_cast<uint32_t>(kStarMaxIters)) L.iters = kStarMaxIters;
+    L.plan = lb_plan(in->fact_fk.n, kStarRowsPerSub * L.iters, kStreams);
+    L.off_flags = 0;
+    L.off_gsum = 256;
+    L.off_gfirst = L.off_gsum + align_up(kCombos * 8ull, 256);
+    L.off_table = L.off_gfirst + align_up(kCombos * 4ull, 256);
+    L.off_present = L.off_table + align_up((1ull << L.tbits) * 8, 256);
+    L.off_combo_cid = L.off_present + align_up(kCombos / 8, 256);
+    L.off_cid_combo = L.off_combo_cid + align_up(kCombos * 2ull, 256);
+    L.off_info = L.off_cid_combo + align_up(kCombos * 2ull, 256);
+    L.off_status = L.off_info + align_up(n_dim * 4ull + 4, 256);
+    const size_t chk = check_an_status_bytes(in->dim_key.n, in->dim_key.width);
+    L.total = L.off_status + (L.plan.status_bytes > chk ? L.plan.status_bytes : chk);
+    return L;
+}
 
-    // dimension key check = hashjoinAN build-side error vector (hashjoinAN.tcc:179-181)
-    uint64_t *dk_counts = out->d_counts + 2;
+}  // namespace
+
+extern "C" int ahead_hip_star1_build(const ahead_star1_args *in, const ahead_star1_out *out, ahead_hip_ws *ws,
+                                     void *stream) {
+    int st = star_validate(in, out);
+    if (st != AHEAD_OK) return st;
+    const StarLayout L = star_layout(in);
+    if ((st = ws_ensure(ws, L.total)) != AHEAD_OK) return st;
+    hipStream_t s = static_cast<hipStream_t>(stream);
     BuildArgs b{};
     b.key = static_cast<const uint64_t *>(in->dim_key.data);
-    b.n = n_dim;
+    b.n = static_cast<uint32_t>(in->dim_key.n);
     b.a_inv = in->dim_key.a_inv;
     b.key_mask = bits_mask(in->dim_key.unenc_bits);
-    b.table = reinterpret_cast<uint64_t *>(ws_region(ws, off_table));
-    b.table_bits = tbits;
-    b.flags = reinterpret_cast<uint32_t *>(ws_region(ws, off_flags));
-    hipLaunchKernelGGL(k_clear_u64, dim3(256), dim3(256), 0, s, b.table, 1ull << tbits);
+    b.table = reinterpret_cast<uint64_t *>(ws_region(ws, L.off_table));
+    b.table_bits = L.tbits;
+    b.flags = reinterpret_cast<uint32_t *>(ws_region(ws, L.off_flags));
+    hipLaunchKernelGGL(k_clear_u64, dim3(256), dim3(256), 0, s, b.table, 1ull << L.tbits);
     hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(32), 0, s, reinterpret_cast<uint64_t *>(b.flags), 32ull);
+    // the group tables start empty; every finalize leaves them empty again
+    hipLaunchKernelGGL(k_clear_u64, dim3(256), dim3(256), 0, s, reinterpret_cast<uint64_t *>(ws_region(ws, L.off_gsum)),
+                       (L.off_table - L.off_gsum) / 8);
     AHEAD_HIP_TRY(hipGetLastError());
-    if (n_dim) {
-        const uint32_t g = (n_dim + kBlock - 1) / kBlock;
-        hipLaunchKernelGGL(k_star_build, dim3(g < 4096 ? g : 4096), dim3(kBlock), 0, s, b);
-        AHEAD_HIP_TRY(hipGetLastError());
+    DimArgs d{};
+    d.a1 = static_cast<const uint16_t *>(in->dim_a1.data);
+    d.a2 = static_cast<const uint16_t *>(in->dim_a2.data);
+    d.n = b.n;
+    d.a1_inv = static_cast<uint16_t>(in->dim_a1.a_inv);
+    d.a2_inv = static_cast<uint16_t>(in->dim_a2.a_inv);
+    d.a1_maxu = static_cast<uint16_t>(in->dim_a1.max_u);
+    d.a2_maxu = static_cast<uint16_t>(in->dim_a2.max_u);
+    d.present = reinterpret_cast<uint32_t *>(ws_region(ws, L.off_present));
+    d.combo_cid = reinterpret_cast<uint16_t *>(ws_region(ws, L.off_combo_cid));
+    d.cid_combo = reinterpret_cast<uint16_t *>(ws_region(ws, L.off_cid_combo));
+    d.info = reinterpret_cast<uint32_t *>(ws_region(ws, L.off_info));
+    d.flags = b.flags;
+    hipLaunchKernelGGL(k_clear_u64, dim3(8), dim3(256), 0, s, reinterpret_cast<uint64_t *>(d.present), kCombos / 64ull);
+    const uint32_t g = b.n ? (b.n + kBlock - 1) / kBlock : 1;
+    const dim3 gd(g < 4096 ? g : 4096);
+    if (b.n) {
+        hipLaunchKernelGGL(k_star_build, gd, dim3(kBlock), 0, s, b);
+        hipLaunchKernelGGL(k_star_dim, gd, dim3(kBlock), 0, s, d);
     }
+    hipLaunchKernelGGL(k_star_cid, dim3(1), dim3(kBlock), 0, s, d);
+    if (b.n) hipLaunchKernelGGL(k_star_info, gd, dim3(kBlock), 0, s, d);
+    AHEAD_HIP_TRY(hipGetLastError());
+    // the build side's error vector: checkAN over the dimension keys (hashjoinAN.tcc:179-181),
+    // using the status region, which the probe overwrites afterwards
+    return check_an_at(&in->dim_key, in->a_oid_err, out->e_dk, out->e_dk ? out->e_dk_cap : 0,
+                       out->d_counts + 2, ws, stream, L.off_status);
+}
 
+extern "C" int ahead_hip_star1_probe(const ahead_star1_args *in, const ahead_star1_out *out, ahead_hip_ws *ws,
+                                     void *stream) {
+    int st = star_validate(in, out);
+    if (st != AHEAD_OK) return st;
+    const StarLayout L = star_layout(in);
+    if (ws->bytes < kHeaderBytes + L.total)
+        return fail(AHEAD_ERR_INVALID, "star1_probe: the workspace holds no dimension table (run star1_build)");
+    hipStream_t s = static_cast<hipStream_t>(stream);
     ProbeArgs p{};
     p.fk = static_cast<const uint64_t *>(in->fact_fk.data);
     p.m = static_cast<const uint64_t *>(in->fact_m.data);
     p.n = static_cast<uint32_t>(in->fact_fk.n);
-    p.iters = iters;
+    p.iters = L.iters;
     p.fk_inv = in->fact_fk.a_inv;
     p.fk_maxu = in->fact_fk.max_u;
     p.key_mask = bits_mask(in->dim_key.unenc_bits);
     p.m_inv = in->fact_m.a_inv;
     p.m_maxu = in->fact_m.max_u;
-    p.a1 = static_cast<const uint16_t *>(in->dim_a1.data);
-    p.a2 = static_cast<const uint16_t *>(in->dim_a2.data);
-    p.n_dim = n_dim;
-    p.a1_inv = static_cast<uint16_t>(in->dim_a1.a_inv);
-    p.a2_inv = static_cast<uint16_t>(in->dim_a2.a_inv);
-    p.a1_maxu = static_cast<uint16_t>(in->dim_a1.max_u);
-    p.a2_maxu = static_cast<uint16_t>(in->dim_a2.max_u);
-    p.table = b.table;
-    p.table_bits = tbits;
-    p.flags = b.flags;
-    p.g_sum = reinterpret_cast<uint64_t *>(ws_region(ws, off_gsum));
-    p.g_first = reinterpret_cast<uint32_t *>(ws_region(ws, off_gfirst));
+    p.info = reinterpret_cast<const uint32_t *>(ws_region(ws, L.off_info));
+    p.cid_combo = reinterpret_cast<const uint16_t *>(ws_region(ws, L.off_cid_combo));
+    p.n_dim = static_cast<uint32_t>(in->dim_key.n);
+    p.table = reinterpret_cast<const uint64_t *>(ws_region(ws, L.off_table));
+    p.table_bits = L.tbits;
+    p.flags = reinterpret_cast<const uint32_t *>(ws_region(ws, L.off_flags));
+    p.g_sum = reinterpret_cast<uint64_t *>(ws_region(ws, L.off_gsum));
+    p.g_first = reinterpret_cast<uint32_t *>(ws_region(ws, L.off_gfirst));
     p.a_oid_err = in->a_oid_err;
     p.err[0] = out->e_fk; p.cap[0] = out->e_fk ? out->e_fk_cap : 0;
     p.err[1] = out->e_a1; p.cap[1] = out->e_a1 ? out->e_a1_cap : 0;
     p.err[2] = out->e_a2; p.cap[2] = out->e_a2 ? out->e_a2_cap : 0;
     p.err[3] = out->e_m; p.cap[3] = out->e_m ? out->e_m_cap : 0;
     p.d_counts = out->d_counts;
-    if ((st = lb_args(ws, plan, 0, &p.lb)) != AHEAD_OK) return st;
-    hipLaunchKernelGGL(k_star_probe, dim3(plan.num_tiles), dim3(kBlock), 0, s, p);
+    if ((st = lb_args(ws, L.plan, L.off_status, &p.lb)) != AHEAD_OK) return st;
+    hipLaunchKernelGGL(k_star_probe, dim3(L.plan.num_tiles), dim3(kBlock), 0, s, p);
     AHEAD_HIP_TRY(hipGetLastError());
 
     FinalArgs f{};
     f.g_sum = p.g_sum;
     f.g_first = p.g_first;
-    f.flags = b.flags;
-    f.sum_factor = in->a_result * (in->fact_m.a_inv != 0 ? inverse_mod2k(in->fact_m.a, 64) : 1ull);
+    f.flags = p.flags;
+    f.sum_factor = in->a_result * inverse_mod2k(in->fact_m.a, 64);
     f.a_oid_grp = in->a_oid_grp;
     f.a1_a = static_cast<uint16_t>(in->dim_a1.a);
     f.a2_a = static_cast<uint16_t>(in->dim_a2.a);
@@ -513,9 +700,12 @@ extern "C" int ahead_hip_star1_pipeline(const ahead_star1_args *in, const ahead_
     f.d_counts = out->d_counts;
     hipLaunchKernelGGL(k_star_final, dim3(1), dim3(1024), 0, s, f);
     AHEAD_HIP_TRY(hipGetLastError());
+    return AHEAD_OK;
+}
 
-    // dimension-key error vector (checkAN semantics over the build keys)
-    st = ahead_hip_check_an(&in->dim_key, in->a_oid_err, out->e_dk, out->e_dk ? out->e_dk_cap : 0, dk_counts,
-                            ws, stream);
-    return st;
+extern "C" int ahead_hip_star1_pipeline(const ahead_star1_args *in, const ahead_star1_out *out,
+                                        ahead_hip_ws *ws, void *stream) {
+    int st = ahead_hip_star1_build(in, out, ws, stream);
+    if (st != AHEAD_OK) return st;
+    return ahead_hip_star1_probe(in, out, ws, stream);
 }

@@ -62,6 +62,12 @@ enum { AHEAD_COMB_NONE = 0, AHEAD_COMB_AND = 1, AHEAD_COMB_OR = 2 };
 
 typedef struct ahead_hip_ws ahead_hip_ws;
 
+/* A device output buffer: entries at index >= cap are not written (see Conventions). */
+typedef struct {
+    void *data;
+    uint64_t cap;
+} ahead_buf;
+
 /* ---- runtime --------------------------------------------------------------------------------- */
 const char *ahead_hip_last_error(void);
 const char *ahead_hip_version(void);
@@ -109,6 +115,84 @@ int ahead_hip_check_and_decode_an(const ahead_col *col, uint64_t a_oid, void *de
                                   uint64_t dec_cap, uint64_t *err, uint64_t err_cap,
                                   uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
 
+/* ---- joins -----------------------------------------------------------------------------------
+ * fetchjoinAN (OperatorsAN.hpp:92-106; matchjoinAN.tcc:312-353).  oids: tail of a
+ * BAT<void, v2_resoid_t>; col: tail of BAT<void, T2>.  out: kept codes (TypeMap<T2>::
+ * v2_encoded_t width), re-encoded with a_reenc * inv(A_col) when reenc != 0.  err1: i * a_oid
+ * for OIDs beyond the column (t >= |col|), err2: t * a_oid for corrupt fetched codes (dropped).
+ * d_counts[0..2] = #out, #err1, #err2.                                                       */
+int ahead_hip_fetchjoin_an(const ahead_col *oids, const ahead_col *col, int reenc, uint64_t a_reenc,
+                           uint64_t a_oid, ahead_buf out, ahead_buf err1, ahead_buf err2,
+                           uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
+
+/* matchjoinAN, positional form <H1, v2_resoid_t, v2_void_t, T2> (OperatorsAN.hpp:33-58;
+ * matchjoinAN.tcc:240-301).  Pairs (h1 [* f_h1], t2[pos] [* f_t2]) with pos = t1 * A^-1;
+ * err1: corrupt h1 (pos1), err2: pos >= |t2| (pos1), err4: corrupt t2 (pos2, dropped).
+ * d_counts[0..3] = #pairs, #err1, #err2, #err4.                                              */
+int ahead_hip_matchjoin_pos_an(const ahead_col *h1, const ahead_col *t1, const ahead_col *t2,
+                               int reenc_h, uint64_t f_h1, uint32_t wout_h, int reenc_t,
+                               uint64_t f_t2, uint32_t wout_t, uint64_t a_oid, ahead_buf out_h,
+                               ahead_buf out_t, ahead_buf err1, ahead_buf err2, ahead_buf err4,
+                               uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
+
+/* hashjoinAN (OperatorsAN.hpp:60-90; hashjoinAN.tcc:77-222, :552-585) with the unencoded-key
+ * hash map.  build_right != 0 is hash_side_t::right (the default).  Pairs (h1, t2) in probe
+ * order, build rows in insertion order per key; f_h1 / f_t2 are the re-encoding factors
+ * mulIfEncoded(A_R, A^-1) of the re-encoding overloads.  e1..e4 check h1, t1, h2, t2 by scan
+ * position.  Build keys up to 64 bits; at most 8192 build rows per key.
+ * d_counts[0..5] = #pairs, #e1, #e2, #e3, #e4, status (0 ok, 2 = a key has > 8192 rows).   */
+int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, const ahead_col *h2,
+                          const ahead_col *t2, int build_right, int reenc_h, uint64_t f_h1,
+                          uint32_t wout_h, int reenc_t, uint64_t f_t2, uint32_t wout_t,
+                          uint64_t a_oid, ahead_buf out_h, ahead_buf out_t, ahead_buf e1,
+                          ahead_buf e2, ahead_buf e3, ahead_buf e4, uint64_t *d_counts,
+                          ahead_hip_ws *ws, void *stream);
+
+/* ---- grouping and aggregation -----------------------------------------------------------------
+ * groupbyAN unary (OperatorsAN.hpp:108-113; groupbyAN.tcc:51-94): voidtorgid[i] = gid * a_oid
+ * (n entries), vgidtoroid[g] = head(first row of g) * a_oid, ids by first occurrence.
+ * table_bits: log2 of the dictionary slots (0 = automatic, up to 2^22).
+ * d_counts[0..5] = n, #groups, #e_head, #e_tail, 0, status (0 ok, 1 = dictionary overflow:
+ * call again with a larger table_bits).                                                      */
+int ahead_hip_groupby_an(const ahead_col *head, const ahead_col *tail, uint64_t a_oid,
+                         uint32_t table_bits, uint64_t *voidtorgid, ahead_buf vgidtoroid,
+                         ahead_buf e_head, ahead_buf e_tail, uint64_t *d_counts,
+                         ahead_hip_ws *ws, void *stream);
+/* groupbyAN binary (OperatorsAN.hpp:115-123; groupbyAN.tcc:99-165): key
+ * t * num_groups + g in larger_bits (Tail::v2_larger_t) width; repeated keys return the RGID
+ * as stored in the reference's oid_t dictionary.  d_counts[4] = #e_group.                    */
+int ahead_hip_groupby2_an(const ahead_col *head, const ahead_col *tail, const ahead_col *grouping,
+                          uint64_t num_groups, uint32_t larger_bits, uint64_t a_oid,
+                          uint32_t table_bits, uint64_t *voidtorgid, ahead_buf vgidtoroid,
+                          ahead_buf e_head, ahead_buf e_tail, ahead_buf e_group,
+                          uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
+
+/* aggregate_sum_groupedAN (OperatorsAN.hpp:125-134; aggregateAN.tcc:41-119): sums[g] =
+ * sum of tail codes * (a_result * inv(A_tail)) mod 2^64 over rows with a valid group id
+ * (corrupt tails are added and reported).  A decoded group id above maxU or >= num_groups is
+ * reported in e_group and skipped.  d_counts[0..3] = num_groups, #e_head, #e_tail, #e_group. */
+int ahead_hip_aggregate_sum_grouped_an(const ahead_col *head, const ahead_col *tail,
+                                       const ahead_col *grouping, uint64_t num_groups,
+                                       uint64_t a_result, uint64_t a_oid, uint64_t *sums,
+                                       ahead_buf e_head, ahead_buf e_tail, ahead_buf e_group,
+                                       uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
+
+/* aggregate_mul_sumAN (OperatorsAN.hpp:206-215; aggregateAN_scalar.tcc:58-131):
+ * *d_total = (init + sum t1*t2) * (ra? * inv(A_t1) * inv(A_t2)) mod 2^64 over the shorter input.
+ * d_counts[0..2] = 1, #err1, #err2.                                                         */
+int ahead_hip_aggregate_mul_sum_an(const ahead_col *t1, const ahead_col *t2, uint64_t init,
+                                   int result_encoded, uint64_t ra, uint64_t a_oid,
+                                   uint64_t *d_total, ahead_buf err1, ahead_buf err2,
+                                   uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
+
+/* arithmeticAN (OperatorsAN.hpp:197-204; arithmeticAN_scalar.tcc:60-112): out[i] =
+ * op(d1_i * a_result, d2_i * a_result) in result_width bytes; op 0 add, 1 sub, 2 mul, 3 div
+ * (division by zero yields 0).  d_counts[0..4] = n, #e_h1, #e_t1, #e_h2, #e_t2.              */
+int ahead_hip_arithmetic_an(const ahead_col *h1, const ahead_col *t1, const ahead_col *h2,
+                            const ahead_col *t2, int op, uint64_t a_result, uint32_t result_width,
+                            uint64_t a_oid, void *out, ahead_buf eh1, ahead_buf et1, ahead_buf eh2,
+                            ahead_buf et2, uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
+
 /* ---- fused star join over one dimension ------------------------------------------------------
  * Replaces, for one dimension with two 8-bit group attributes, the hardened operator chain of
  * tools/ssbm/ssbm-q21_continuous.cpp:97-181:
@@ -146,6 +230,13 @@ typedef struct {
 
 int ahead_hip_star1_pipeline(const ahead_star1_args *in, const ahead_star1_out *out,
                              ahead_hip_ws *ws, void *stream);
+/* The two halves of ahead_hip_star1_pipeline: build (dimension hash table + dimension-key error
+ * vector, d_counts[2]) and probe (fact scan + group finalize).  A built table stays valid for any
+ * number of probes with the same dimension as long as no other operator uses the workspace.  */
+int ahead_hip_star1_build(const ahead_star1_args *in, const ahead_star1_out *out,
+                          ahead_hip_ws *ws, void *stream);
+int ahead_hip_star1_probe(const ahead_star1_args *in, const ahead_star1_out *out,
+                          ahead_hip_ws *ws, void *stream);
 
 /* ---- data generation (seeded, counter-based; the same streams as ahead_amd/datagen.py) ------
  * Not part of the reference's operator API: the device-side generator for the synthetic
