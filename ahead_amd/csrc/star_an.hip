@@ -17,8 +17,11 @@
 //  k_star_dim     decodes and checks both attribute codes of every dimension row and marks the
 //                 (attr1, attr2) combinations of the valid rows;
 //  k_star_cid     numbers the marked combinations densely (combination id, cid);
-//  k_star_info    stores per dimension row its cid and its two attribute-error bits, so that a
-//                 probe needs one table word and one info word per fact row.
+//  k_star_info    stores per dimension row its cid and its two attribute-error bits.
+//  k_star_range / k_star_map_clear / k_star_map_fill
+//                 when the valid keys span a range of at most 4 x |dim| + 1024 values, a direct
+//                 map of 16-bit entries (attr-error bits | cid + 1) indexed by key - kmin, so a
+//                 fact row costs one 2-byte gather; keys outside the range use the table.
 //  k_star_probe   one workgroup per tile of ITERS x 2048 fact rows.  Phase 1 streams FK and
 //                 measure codes with 128-bit loads, decodes and checks the FK, probes the table
 //                 with the truncated decoded key even when the FK is corrupt (hashjoinAN.tcc:199),
@@ -94,17 +97,91 @@ struct DimArgs {
     uint16_t *combo_cid;   // [kCombos]
     uint16_t *cid_combo;   // [kCombos]
     uint32_t *info;        // [n]: cid | attr1 error << 30 | attr2 error << 31
-    uint32_t *flags;       // [1] = number of cids
+    uint32_t *flags;       // [1] number of cids, [2] kmin, [3] kmax, [4] dense map in use
+    const uint64_t *key;   // dimension key codes
+    uint64_t key_inv, key_maxu, key_mask;
+    uint16_t *map;         // [map_cap]
+    uint32_t map_cap;
 };
 
+// Smallest and largest truncated key of the valid dimension keys (one atomic pair per workgroup).
+__global__ __launch_bounds__(kBlock) void k_star_range(DimArgs a) {
+    __shared__ uint32_t s_lo[kWaves], s_hi[kWaves];
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < a.n; j += stride) {
+        const uint64_t d = a.key[j] * a.key_inv;
+        if (d > a.key_maxu) continue;
+        const uint32_t k = static_cast<uint32_t>(d & a.key_mask);
+        lo = k < lo ? k : lo;
+        hi = k > hi ? k : hi;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint32_t l2 = __shfl_xor(lo, o, 64), h2 = __shfl_xor(hi, o, 64);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kWaves; ++w) {
+            lo = s_lo[w] < lo ? s_lo[w] : lo;
+            hi = s_hi[w] > hi ? s_hi[w] : hi;
+        }
+        atomicMin(&a.flags[2], lo);
+        atomicMax(&a.flags[3], hi);
+    }
+}
+
+// Decides whether the valid keys form a range the direct map can hold and clears that range.
+__global__ __launch_bounds__(kBlock) void k_star_map_clear(DimArgs a) {
+    const uint32_t lo = a.flags[2], hi = a.flags[3];
+    const bool dense = lo <= hi && static_cast<uint64_t>(hi) - lo < a.map_cap && a.flags[1] < 0x3FFEu;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.flags[4] = dense ? 1u : 0u;
+    if (!dense) return;
+    const uint32_t range = hi - lo + 1u;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < range; i += stride) a.map[i] = 0;
+}
+
+// map[key - kmin] = attr1 error << 15 | attr2 error << 14 | (cid + 1, or 0x3FFF if an attribute is
+// corrupt); 0 = no dimension row under that key.  Every dimension row whose truncated key lies in
+// the range enters, corrupt keys included (they are build entries of the reference's map too).
+__global__ __launch_bounds__(kBlock) void k_star_map_fill(DimArgs a) {
+    if (a.flags[4] == 0u) return;
+    const uint32_t lo = a.flags[2], range = a.flags[3] - lo + 1u;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < a.n; j += stride) {
+        const uint32_t k = static_cast<uint32_t>((a.key[j] * a.key_inv) & a.key_mask);
+        if (k - lo >= range) continue;
+        const uint32_t info = a.info[j];
+        const uint32_t e = info >> 30;
+        const uint32_t v = (e & 1u ? 0x8000u : 0u) | (e & 2u ? 0x4000u : 0u) | (e ? 0x3FFFu : (info & 0xFFFFu) + 1u);
+        a.map[k - lo] = static_cast<uint16_t>(v);
+    }
+}
+
+// Marks the combinations of valid rows: an LDS bitmap per workgroup, merged with one atomic per
+// nonzero word (a handful of distinct combinations would otherwise serialize on a few words).
 __global__ __launch_bounds__(kBlock) void k_star_dim(DimArgs a) {
+    __shared__ uint32_t s_present[kCombos / 32];
+    for (uint32_t i = threadIdx.x; i < kCombos / 32; i += kBlock) s_present[i] = 0u;
+    __syncthreads();
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < a.n; j += stride) {
         const uint16_t d1 = an_mul<uint16_t>(a.a1[j], a.a1_inv), d2 = an_mul<uint16_t>(a.a2[j], a.a2_inv);
         if (d1 <= a.a1_maxu && d2 <= a.a2_maxu) {
             const uint32_t combo = (static_cast<uint32_t>(d1 & 0xFFu) << 8) | (d2 & 0xFFu);
-            atomicOr(&a.present[combo >> 5], 1u << (combo & 31u));
+            atomicOr(&s_present[combo >> 5], 1u << (combo & 31u));
         }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kCombos / 32; i += kBlock) {
+        if (s_present[i]) atomicOr(&a.present[i], s_present[i]);
     }
 }
 
@@ -154,6 +231,7 @@ struct ProbeArgs {
     uint64_t m_inv, m_maxu;
     const uint32_t *info;       // per dimension row: cid | attr error bits
     const uint16_t *cid_combo;  // cid -> (attr1 << 8 | attr2)
+    const uint16_t *map;        // dense key range: key - kmin -> entry (see k_star_map_fill)
     uint32_t n_dim;
     const uint64_t *table;
     uint32_t table_bits;
@@ -222,11 +300,11 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
     __shared__ unsigned long long s_sum[kLdsCids];          // per cid (per wave when cids fit)
     __shared__ uint32_t s_first[kLdsCids];                  // first tile-local row per cid
     __shared__ uint32_t s_any_overflow;
-    __shared__ uint32_t s_flag[2];
+    __shared__ uint32_t s_flag[5];
 
     const uint32_t tile = take_tile(a.lb, &s_tile);
     if (tile >= a.lb.num_tiles) return;
-    if (threadIdx.x < 2) s_flag[threadIdx.x] = a.flags[threadIdx.x];
+    if (threadIdx.x < 5) s_flag[threadIdx.x] = a.flags[threadIdx.x];
     __syncthreads();
     if (s_flag[0] != 0u) {
         // not applicable (duplicate build keys): publish empty counts so the chain drains
@@ -239,6 +317,8 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
         return;
     }
     const uint32_t ncid = s_flag[1];
+    const bool dense = s_flag[4] != 0u;
+    const uint32_t kmin = s_flag[2], krange = s_flag[3] - s_flag[2] + 1u;
     const bool per_wave = ncid <= kWaveCids;  // private accumulators per wave: no cross-wave contention
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     for (int i = threadIdx.x; i < (kStreams + 1) * kStarWords; i += kBlock) (&s_bits[0][0])[i] = 0u;
@@ -273,11 +353,11 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
                 }
             }
         }
-        // batched probe: all table words of this thread's 8 rows are requested before any is
-        // used, then all info words, so the dependent loads overlap instead of chaining
+        // batched probe: the map (or table) words of this thread's 8 rows are requested before
+        // any is used, so the dependent gathers overlap instead of chaining
         uint32_t key[4][2], slot[4][2];
         uint64_t ent[4][2];
-        bool live[4][2], efk[4][2];
+        bool live[4][2], efk[4][2], in_map[4][2];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
 #pragma unroll
@@ -287,20 +367,32 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
                 const uint64_t d = fk[c][k] * a.fk_inv;
                 efk[c][k] = live[c][k] && d > a.fk_maxu;
                 key[c][k] = static_cast<uint32_t>(d & a.key_mask);
+                in_map[c][k] = dense && key[c][k] - kmin < krange;
                 slot[c][k] = table_slot(key[c][k], a.table_bits);
-                ent[c][k] = live[c][k] ? a.table[slot[c][k]] : 0ull;
+                ent[c][k] = !live[c][k] ? 0ull
+                          : (in_map[c][k] ? static_cast<uint64_t>(a.map[key[c][k] - kmin]) : a.table[slot[c][k]]);
             }
         }
+        bool joined[4][2];
         int64_t row[4][2];
+        uint32_t info[4][2];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 row[c][k] = -1;
+                joined[c][k] = false;
+                info[c][k] = 0u;
                 if (!live[c][k] || ent[c][k] == 0ull) continue;
+                if (in_map[c][k]) {
+                    const uint32_t e = static_cast<uint32_t>(ent[c][k]);
+                    joined[c][k] = true;
+                    info[c][k] = ((e >> 15) & 1u) << 30 | ((e >> 14) & 1u) << 31 | ((e & 0x3FFFu) - 1u);
+                    continue;
+                }
                 if (static_cast<uint32_t>(ent[c][k] >> 32) == key[c][k]) {
                     row[c][k] = static_cast<uint32_t>(ent[c][k]) - 1u;
-                } else {  // collision: continue the linear probe (rare for dense keys)
+                } else {  // collision: continue the linear probe
                     const uint32_t mask = (1u << a.table_bits) - 1u;
                     uint32_t s2 = (slot[c][k] + 1u) & mask;
                     for (uint32_t probes = 0; probes < mask; ++probes) {
@@ -313,13 +405,15 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
                         s2 = (s2 + 1u) & mask;
                     }
                 }
+                joined[c][k] = row[c][k] >= 0;
             }
         }
-        uint32_t info[4][2];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
 #pragma unroll
-            for (int k = 0; k < 2; ++k) info[c][k] = row[c][k] >= 0 ? a.info[row[c][k]] : 0u;
+            for (int k = 0; k < 2; ++k) {
+                if (row[c][k] >= 0) info[c][k] = a.info[row[c][k]];
+            }
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -330,7 +424,7 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
                 kept[k] = false;
                 const uint32_t w = local >> 5, b = 1u << (local & 31u);
                 if (efk[c][k]) { atomicOr(&s_bits[kEFk][w], b); cnt[kEFk]++; }
-                if (row[c][k] < 0) continue;
+                if (!joined[c][k]) continue;
                 const bool e1 = (info[c][k] >> 30) & 1u, e2 = (info[c][k] >> 31) & 1u;
                 const bool em = (mv[c][k] * a.m_inv) > a.m_maxu;
                 if (e1) { atomicOr(&s_bits[kEA1][w], b); cnt[kEA1]++; }
@@ -452,6 +546,7 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
 struct FinalArgs {
     uint64_t *g_sum;
     uint32_t *g_first;
+    const uint16_t *cid_combo;
     const uint32_t *flags;
     uint64_t sum_factor;   // A_result * A_measure^-1 (mod 2^64)
     uint64_t a_oid_grp;
@@ -468,7 +563,9 @@ __global__ __launch_bounds__(1024) void k_star_final(FinalArgs a) {
     if (threadIdx.x == 0) s_n = 0u;
     __syncthreads();
     const bool skip = a.flags[0] != 0u;
-    for (uint32_t c = threadIdx.x; c < kCombos; c += blockDim.x) {
+    const uint32_t ncid = a.flags[1];
+    for (uint32_t cid = threadIdx.x; cid < ncid; cid += blockDim.x) {
+        const uint32_t c = a.cid_combo[cid];
         const uint32_t f = a.g_first[c];
         if (f == 0u) continue;
         const uint32_t rank = 0xFFFFFFFFu - f;
@@ -479,7 +576,8 @@ __global__ __launch_bounds__(1024) void k_star_final(FinalArgs a) {
     const uint32_t n = s_n;
     if (skip || n > kFinalMaxGroups) {
         // status 1: not applicable (duplicate build keys) / 2: too many groups for this finalize
-        for (uint32_t c = threadIdx.x; c < kCombos; c += blockDim.x) {
+        for (uint32_t cid = threadIdx.x; cid < ncid; cid += blockDim.x) {
+            const uint32_t c = a.cid_combo[cid];
             a.g_first[c] = 0u;
             a.g_sum[c] = 0ull;
         }
@@ -571,7 +669,8 @@ struct StarLayout {
     uint32_t tbits = 4, iters = 1;
     LbPlan plan;
     size_t off_flags = 0, off_gsum = 0, off_gfirst = 0, off_table = 0, off_present = 0, off_combo_cid = 0,
-           off_cid_combo = 0, off_info = 0, off_status = 0, total = 0;
+           off_cid_combo = 0, off_info = 0, off_map = 0, off_status = 0, total = 0;
+    uint32_t map_cap = 0;
 };
 
 StarLayout star_layout(const ahead_star1_args *in) {
@@ -590,7 +689,9 @@ StarLayout star_layout(const ahead_star1_args *in) {
     L.off_combo_cid = L.off_present + align_up(kCombos / 8, 256);
     L.off_cid_combo = L.off_combo_cid + align_up(kCombos * 2ull, 256);
     L.off_info = L.off_cid_combo + align_up(kCombos * 2ull, 256);
-    L.off_status = L.off_info + align_up(n_dim * 4ull + 4, 256);
+    L.off_map = L.off_info + align_up(n_dim * 4ull + 4, 256);
+    L.map_cap = static_cast<uint32_t>(4ull * n_dim + 1024);
+    L.off_status = L.off_map + align_up(L.map_cap * 2ull, 256);
     const size_t chk = check_an_status_bytes(in->dim_key.n, in->dim_key.width);
     L.total = L.off_status + (L.plan.status_bytes > chk ? L.plan.status_bytes : chk);
     return L;
@@ -632,15 +733,27 @@ extern "C" int ahead_hip_star1_build(const ahead_star1_args *in, const ahead_sta
     d.cid_combo = reinterpret_cast<uint16_t *>(ws_region(ws, L.off_cid_combo));
     d.info = reinterpret_cast<uint32_t *>(ws_region(ws, L.off_info));
     d.flags = b.flags;
+    d.key = b.key;
+    d.key_inv = in->dim_key.a_inv;
+    d.key_maxu = in->dim_key.max_u;
+    d.key_mask = b.key_mask;
+    d.map = reinterpret_cast<uint16_t *>(ws_region(ws, L.off_map));
+    d.map_cap = L.map_cap;
     hipLaunchKernelGGL(k_clear_u64, dim3(8), dim3(256), 0, s, reinterpret_cast<uint64_t *>(d.present), kCombos / 64ull);
+    if ((st = fill_u32(b.flags + 2, 1, 0xFFFFFFFFu, s)) != AHEAD_OK) return st;  // kmin
     const uint32_t g = b.n ? (b.n + kBlock - 1) / kBlock : 1;
     const dim3 gd(g < 4096 ? g : 4096);
     if (b.n) {
         hipLaunchKernelGGL(k_star_build, gd, dim3(kBlock), 0, s, b);
-        hipLaunchKernelGGL(k_star_dim, gd, dim3(kBlock), 0, s, d);
+        hipLaunchKernelGGL(k_star_dim, dim3(g < 64 ? g : 64), dim3(kBlock), 0, s, d);
     }
     hipLaunchKernelGGL(k_star_cid, dim3(1), dim3(kBlock), 0, s, d);
-    if (b.n) hipLaunchKernelGGL(k_star_info, gd, dim3(kBlock), 0, s, d);
+    if (b.n) {
+        hipLaunchKernelGGL(k_star_info, gd, dim3(kBlock), 0, s, d);
+        hipLaunchKernelGGL(k_star_range, dim3(g < 128 ? g : 128), dim3(kBlock), 0, s, d);
+    }
+    hipLaunchKernelGGL(k_star_map_clear, dim3(512), dim3(kBlock), 0, s, d);
+    if (b.n) hipLaunchKernelGGL(k_star_map_fill, gd, dim3(kBlock), 0, s, d);
     AHEAD_HIP_TRY(hipGetLastError());
     // the build side's error vector: checkAN over the dimension keys (hashjoinAN.tcc:179-181),
     // using the status region, which the probe overwrites afterwards
@@ -668,6 +781,7 @@ extern "C" int ahead_hip_star1_probe(const ahead_star1_args *in, const ahead_sta
     p.m_maxu = in->fact_m.max_u;
     p.info = reinterpret_cast<const uint32_t *>(ws_region(ws, L.off_info));
     p.cid_combo = reinterpret_cast<const uint16_t *>(ws_region(ws, L.off_cid_combo));
+    p.map = reinterpret_cast<const uint16_t *>(ws_region(ws, L.off_map));
     p.n_dim = static_cast<uint32_t>(in->dim_key.n);
     p.table = reinterpret_cast<const uint64_t *>(ws_region(ws, L.off_table));
     p.table_bits = L.tbits;
@@ -687,6 +801,7 @@ extern "C" int ahead_hip_star1_probe(const ahead_star1_args *in, const ahead_sta
     FinalArgs f{};
     f.g_sum = p.g_sum;
     f.g_first = p.g_first;
+    f.cid_combo = p.cid_combo;
     f.flags = p.flags;
     f.sum_factor = in->a_result * inverse_mod2k(in->fact_m.a, 64);
     f.a_oid_grp = in->a_oid_grp;
