@@ -67,6 +67,32 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
     return LIB
 
 
+CXX = os.environ.get("CXX", "g++")
+CPP_TESTS = ROOT / "tests" / "cpp"
+
+
+def build_cpp_tests(verbose: bool = True) -> list[Path]:
+    """Compiles the C++ programs under tests/cpp/ (clients of include/ahead/ and the C ABI, checked
+    against the oracle) into tests/cpp/_build/.  Needs libahead_hip.so and the oracle built."""
+    out_dir = CPP_TESTS / "_build"
+    out_dir.mkdir(exist_ok=True)
+    outs = []
+    for src in sorted(CPP_TESTS.glob("*.cpp")):
+        exe = out_dir / src.stem
+        cmd = [CXX, "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-D__HIP_PLATFORM_AMD__",
+               "-I/opt/rocm/include", f"-I{ROOT / 'include'}", f"-I{ROOT / 'include' / 'ahead'}", str(src),
+               "-o", str(exe), f"-L{LIB_DIR}", "-lahead_hip", f"-L{ROOT / 'oracle' / '_build'}", "-loracle",
+               "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../../ahead_amd/lib",
+               "-Wl,-rpath,$ORIGIN/../../../oracle/_build", "-Wl,-rpath,/opt/rocm/lib"]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src.name}:\n{res.stdout}\n{res.stderr}")
+        outs.append(exe)
+    if verbose:
+        print(f"built {len(outs)} C++ test program(s) in {out_dir}", file=sys.stderr)
+    return outs
+
+
 def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--jobs", type=int, default=None)

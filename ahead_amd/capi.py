@@ -48,6 +48,14 @@ class Buf(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("cap", ctypes.c_uint64)]
 
 
+class SelectJob(ctypes.Structure):
+    _fields_ = [("col", AheadCol), ("op1", ctypes.c_int32), ("op2", ctypes.c_int32), ("comb", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("th1", ctypes.c_uint64), ("th2", ctypes.c_uint64),
+                ("a_head", ctypes.c_uint64), ("a_oid", ctypes.c_uint64), ("oids", ctypes.c_void_p),
+                ("oids_cap", ctypes.c_uint64), ("err", ctypes.c_void_p), ("err_cap", ctypes.c_uint64),
+                ("d_counts", ctypes.c_void_p), ("d_sum", ctypes.c_void_p)]
+
+
 class Star1Args(ctypes.Structure):
     _fields_ = [("fact_fk", AheadCol), ("fact_m", AheadCol), ("dim_key", AheadCol), ("dim_a1", AheadCol),
                 ("dim_a2", AheadCol), ("a_oid_join", ctypes.c_uint64), ("a_oid_err", ctypes.c_uint64),
@@ -89,6 +97,8 @@ _SIGS = {
     "ahead_hip_fetchjoin_an": ([_colp, _colp, _int, _u64, _u64, Buf, Buf, Buf, _vp, _vp, _vp], _int),
     "ahead_hip_matchjoin_pos_an": ([_colp, _colp, _colp, _int, _u64, _u32, _int, _u64, _u32, _u64, Buf, Buf, Buf,
                                     Buf, Buf, _vp, _vp, _vp], _int),
+    "ahead_hip_matchjoin_merge_an": ([_colp, _colp, _colp, _colp, _int, _u64, _u32, _int, _u64, _u32, _u64, Buf, Buf,
+                                      Buf, Buf, Buf, Buf, _vp, _vp, _vp], _int),
     "ahead_hip_hashjoin_an": ([_colp, _colp, _colp, _colp, _int, _int, _u64, _u32, _int, _u64, _u32, _u64, Buf, Buf,
                                Buf, Buf, Buf, Buf, _vp, _vp, _vp], _int),
     "ahead_hip_groupby_an": ([_colp, _colp, _u64, _u32, _vp, Buf, Buf, Buf, _vp, _vp, _vp], _int),
@@ -99,6 +109,10 @@ _SIGS = {
     "ahead_hip_aggregate_mul_sum_an": ([_colp, _colp, _u64, _int, _u64, _u64, _vp, Buf, Buf, _vp, _vp, _vp], _int),
     "ahead_hip_arithmetic_an": ([_colp, _colp, _colp, _colp, _int, _u64, _u32, _u64, _vp, Buf, Buf, Buf, Buf, _vp, _vp,
                                  _vp], _int),
+    "ahead_hip_select_batch_create": ([ctypes.POINTER(SelectJob), _u32, _u32, _u32, ctypes.POINTER(_vp)], _int),
+    "ahead_hip_select_batch_launch": ([_vp, _vp], _int),
+    "ahead_hip_select_batch_destroy": ([_vp], _int),
+    "ahead_hip_select_batch_info": ([_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)], _int),
     "ahead_hip_gen_uniform": ([_vp, _u32, _u64, _u64, _u64, _u64, _u64, _u64, _vp], _int),
     "ahead_hip_inject_flips": ([_vp, _u32, _u64, _u64, _u64, _u64, _vp], _int),
 }
@@ -291,6 +305,77 @@ def select_an(col: Column, op1: int, th1: int, op2: int = LT, th2: int = 0, comb
         err_cap = n_err
     total = int(dsum.to_numpy(np.uint64, 1)[0]) if dsum else None
     return oids.to_numpy(np.uint64, n_oids), err.to_numpy(np.uint64, n_err), total
+
+
+class SelectBatch:
+    """Many selectAN + decoded-sum instances in one persistent launch (ahead_hip_select_batch_*).
+
+    jobs: list of dicts with keys col (Column), op1, th1, and optionally op2, th2, comb, err_cap.
+    Outputs are allocated per job (OIDs: one per row; errors: err_cap, grown on overflow)."""
+
+    def __init__(self, jobs: list[dict], max_subtiles: int = 16, blocks: int = 0,
+                 a_head: int = an.A_OID, a_oid: int = an.A_OID_ERR):
+        self.lib = load()
+        self.jobs = jobs
+        self.max_subtiles, self.blocks = max_subtiles, blocks
+        self.a_head, self.a_oid = a_head, a_oid
+        self.err_caps = [int(j.get("err_cap", 1024)) for j in jobs]
+        self.h = None
+        self._create()
+
+    def _create(self):
+        n = len(self.jobs)
+        self.oids = [DeviceBuffer(max(8, 8 * j["col"].n)) for j in self.jobs]
+        self.errs = [DeviceBuffer(max(8, 8 * c)) for c in self.err_caps]
+        self.counts = DeviceBuffer(16 * n, zero=True)
+        self.sums = DeviceBuffer(8 * n, zero=True)
+        arr = (SelectJob * n)()
+        self._descs = []
+        for i, j in enumerate(self.jobs):
+            d = j["col"].desc()
+            self._descs.append(d)
+            arr[i] = SelectJob(d, j["op1"], j.get("op2", LT), j.get("comb", COMB_NONE), 0, j["th1"] & an.MASK64,
+                               j.get("th2", 0) & an.MASK64, self.a_head, self.a_oid, self.oids[i].ptr, j["col"].n,
+                               self.errs[i].ptr, self.err_caps[i], self.counts.ptr + 16 * i, self.sums.ptr + 8 * i)
+        h = ctypes.c_void_p()
+        check(self.lib.ahead_hip_select_batch_create(arr, n, self.max_subtiles, self.blocks, ctypes.byref(h)))
+        self.h = h
+
+    def info(self) -> tuple[int, int]:
+        t, b = ctypes.c_uint32(), ctypes.c_uint32()
+        check(self.lib.ahead_hip_select_batch_info(self.h, ctypes.byref(t), ctypes.byref(b)))
+        return t.value, b.value
+
+    def launch(self, stream: int | None = None) -> None:
+        check(self.lib.ahead_hip_select_batch_launch(self.h, current_stream() if stream is None else stream))
+
+    def results(self) -> list[tuple[np.ndarray, np.ndarray, int]]:
+        """Synchronises; re-creates and re-launches with larger error buffers if any overflowed."""
+        while True:
+            synchronize()
+            c = self.counts.to_numpy(np.uint64, 2 * len(self.jobs)).reshape(-1, 2)
+            over = [i for i in range(len(self.jobs)) if int(c[i, 1]) > self.err_caps[i]]
+            if not over:
+                break
+            for i in over:
+                self.err_caps[i] = int(c[i, 1])
+            self.close()
+            self._create()
+            self.launch()
+        sums = self.sums.to_numpy(np.uint64, len(self.jobs))
+        return [(self.oids[i].to_numpy(np.uint64, int(c[i, 0])), self.errs[i].to_numpy(np.uint64, int(c[i, 1])),
+                 int(sums[i])) for i in range(len(self.jobs))]
+
+    def close(self) -> None:
+        if self.h is not None and self.h.value:
+            check(self.lib.ahead_hip_select_batch_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def check_an(col: Column, a_oid: int = an.A_OID_ERR, err_cap: int = 4096) -> np.ndarray:
@@ -497,6 +582,23 @@ def matchjoin_pos_an(h1: Column, t1: Column, t2: Column, reenc_h=0, f_h1=0, wout
     b, c = _run(call, outs, 4, [0, 0, 1, 2, 3])
     return (b[0].to_numpy(np.dtype(f"u{wout_h}"), c[0]), b[1].to_numpy(np.dtype(f"u{wout_t}"), c[0]),
             b[2].to_numpy(np.uint64, c[1]), b[3].to_numpy(np.uint64, c[2]), b[4].to_numpy(np.uint64, c[3]))
+
+
+def matchjoin_merge_an(h1: Column, t1: Column, h2: Column, t2: Column, reenc_h=0, f_h1=0, wout_h=8, reenc_t=0,
+                       f_t2=0, wout_t=8, a_oid: int = an.A_OID_ERR, err_cap: int = 1024):
+    lib, ws = load(), workspace()
+    cap = max(min(t1.n, h2.n), 1)
+    outs = [[wout_h, cap], [wout_t, cap], [8, err_cap], [8, err_cap], [8, err_cap], [8, err_cap]]
+    d = [x.desc() for x in (h1, t1, h2, t2)]
+
+    def call(b, cnt):
+        check(lib.ahead_hip_matchjoin_merge_an(*[ctypes.byref(x) for x in d], reenc_h, f_h1, wout_h, reenc_t, f_t2,
+                                               wout_t, a_oid, *[_b(x, o[1]) for x, o in zip(b, outs)], cnt.ptr,
+                                               ws.h, current_stream()))
+
+    b, c = _run(call, outs, 5, [0, 0, 1, 2, 3, 4])
+    return (b[0].to_numpy(np.dtype(f"u{wout_h}"), c[0]), b[1].to_numpy(np.dtype(f"u{wout_t}"), c[0]),
+            *[b[2 + k].to_numpy(np.uint64, c[1 + k]) for k in range(4)])
 
 
 def hashjoin_an(h1: Column, t1: Column, h2: Column, t2: Column, build_right=1, reenc_h=0, f_h1=0, wout_h=8,

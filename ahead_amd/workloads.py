@@ -88,3 +88,74 @@ def star1_device(spec: Star1Spec):
     capi.inject_flips(a2, 2, nd, spec.seed, F_A2, th)
     return (capi.Column(fk, nf, spec.meta_key), capi.Column(m, nf, spec.meta_m), capi.Column(key, nd, spec.meta_key),
             capi.Column(a1, nd, spec.meta_attr), capi.Column(a2, nd, spec.meta_attr))
+
+
+# ------------------------------------------------------------------------------------------------
+# batched select + aggregate: many independent instances of uneven length and mixed code width
+# ------------------------------------------------------------------------------------------------
+
+# (column type, A): 8-bit values in 16-bit codes, 16-bit values in 32-bit codes (two As), 32-bit
+# values in 64-bit codes.  The three widths are equally likely; the 32-bit codes split between the
+# two As.
+BATCH_KINDS = [(an.RESTINY, 233), (an.RESSHORT, 233), (an.RESSHORT, 63877), (an.RESINT, 65117)]
+BATCH_VMAX = {2: 0xFF, 4: 0xFFFF, 8: 0xFFFFFFFF}
+
+
+@dataclass
+class BatchSpec:
+    count: int = 512
+    mu: float = float(np.log(7.07e5))   # lognormal length: median 707k rows
+    sigma: float = 1.42
+    n_min: int = 10_000
+    n_max: int = 50_000_000
+    sel_min: float = 0.01
+    sel_max: float = 0.50
+    rate: float = 1e-6
+    seed: int = 4242
+
+
+def batch_plan(spec: BatchSpec) -> list[dict]:
+    """Per instance: rows, column type, A, value range [lo, hi] of its predicate (seeded)."""
+    rng = np.random.default_rng(spec.seed)
+    n = np.clip(np.round(np.exp(rng.normal(spec.mu, spec.sigma, spec.count))), spec.n_min, spec.n_max)
+    width_class = rng.integers(0, 3, spec.count)
+    two_a = rng.integers(0, 2, spec.count)
+    sel = rng.uniform(spec.sel_min, spec.sel_max, spec.count)
+    start = rng.uniform(0.0, 1.0, spec.count)
+    plan = []
+    for i in range(spec.count):
+        kind = [0, 1 + int(two_a[i]), 3][int(width_class[i])]
+        t, a = BATCH_KINDS[kind]
+        vmax = BATCH_VMAX[t.width]
+        span = max(1, int(sel[i] * vmax))
+        lo = int(start[i] * (vmax - span))
+        plan.append({"n": int(n[i]), "type": t, "a": a, "lo": lo, "hi": lo + span, "vmax": vmax,
+                     "stream": 100 + 4 * i})  # values: stream; flips: stream+1 (hit), +2 (bit)
+    return plan
+
+
+def batch_device(spec: BatchSpec) -> list[dict]:
+    """The instances as select jobs (capi.SelectBatch / select_an arguments) over device columns
+    generated from the seeded streams: values uniform in [0, vmax], AN-coded, flipped at `rate`."""
+    from . import capi
+
+    th = datagen.flip_threshold(spec.rate)
+    jobs = []
+    for p in batch_plan(spec):
+        t, a = p["type"], p["a"]
+        buf = capi.DeviceBuffer(t.width * p["n"])
+        capi.gen_uniform(buf, t.width, p["n"], spec.seed, p["stream"], 0, p["vmax"], a)
+        capi.inject_flips(buf, t.width, p["n"], spec.seed, p["stream"] + 1, th)
+        col = capi.Column(buf, p["n"], an.an_meta(t, a))
+        jobs.append({"col": col, "op1": capi.GE, "th1": p["lo"] * a, "op2": capi.LE, "th2": p["hi"] * a,
+                     "comb": capi.COMB_AND, "err_cap": 256})
+    return jobs
+
+
+def batch_host_column(spec: BatchSpec, i: int) -> tuple[np.ndarray, an.ColMeta]:
+    """Instance i's codes built with numpy (the same streams as batch_device)."""
+    p = batch_plan(spec)[i]
+    t, a = p["type"], p["a"]
+    codes = datagen.encode(datagen.uniform(p["n"], spec.seed, p["stream"], 0, p["vmax"]), a, t.width)
+    datagen.inject_flips(codes, spec.seed, p["stream"] + 1, datagen.flip_threshold(spec.rate))
+    return codes, an.an_meta(t, a)

@@ -14,6 +14,12 @@ HBM.  Workloads:
   select_sum  : range select + check-and-decode + sum over 6,000,000 16-bit values in [1,50]
                 AN-coded to uint32 with A=233, 1e-6 seeded single-bit flips, predicate 1<=v<=24
                 on encoded bounds; outputs OIDs, decoded sum and sorted error positions.
+  batch       : 512 independent select + check-and-decode + sum instances, lengths lognormal
+                (median 707k rows, sigma 1.42, clipped to [1e4, 5e7]), code widths 16/32/64 bits
+                (8-bit values A=233; 16-bit values A=233 or 63877; 32-bit values A=65117), range
+                predicates of selectivity 1-50%, 1e-6 flips; one persistent launch (autotuned
+                tiling and workgroup count, replayed from a hipGraph) against one launch per
+                instance.  A step is the whole batch.
 
 The line printed carries the whole-job throughput (`value`), the dominant kernel's roofline
 (algorithmic bytes / its average duration from HIP events on the launch stream) and a CPU
@@ -41,7 +47,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="star1", choices=["star1", "select_sum"])
+    ap.add_argument("--workload", default="star1", choices=["star1", "select_sum", "batch"])
     ap.add_argument("--rows", type=int, default=None, help="override the row count (scaling studies)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -258,6 +264,146 @@ def time_star_probe(st, steps):
     return sum(e0.elapsed_time(e1) for e0, e1 in ev) / len(ev)
 
 
+def run_batch(args):
+    import numpy as np
+    import torch
+
+    from ahead_amd import autotune, capi, workloads as W
+
+    spec = W.BatchSpec()
+    if args.rows:
+        spec.count = max(1, min(1024, args.rows))
+    jobs = W.batch_device(spec)
+    lib = capi.load()
+    stream = capi.current_stream()
+    # the batched form: autotune, then time replays of a hipGraph capture of the tuned launch
+    (sub, blocks), tune_ms, table = autotune.tune_select_batch(jobs)
+    batch = capi.SelectBatch(jobs, max_subtiles=sub, blocks=blocks)
+    batch.launch()
+    res_b = batch.results()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g, stream=side):
+            batch.launch()
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    graph_ms = e0.elapsed_time(e1) / args.steps
+    # the same launch, not captured: the kernel's own duration on its stream
+    ek0, ek1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ek0.record()
+    for _ in range(args.steps):
+        batch.launch()
+    ek1.record()
+    torch.cuda.synchronize()
+    kern_ms = ek0.elapsed_time(ek1) / args.steps
+    res_g = batch.results()
+
+    # the plain form: one ahead_hip_select_an launch per instance (same outputs, own buffers)
+    ws = capi.workspace()
+    descs = [j["col"].desc() for j in jobs]
+    outs = [(capi.DeviceBuffer(8 * max(1, j["col"].n)), capi.DeviceBuffer(8 * 4096), capi.DeviceBuffer(16),
+             capi.DeviceBuffer(8)) for j in jobs]
+
+    def plain_step():
+        for j, d, (o, e, c, sm) in zip(jobs, descs, outs):
+            capi.check(lib.ahead_hip_select_an(ctypes.byref(d), j["op1"], j["th1"], j["op2"], j["th2"], j["comb"],
+                                               capi.an.A_OID, capi.an.A_OID_ERR, o.ptr, j["col"].n, e.ptr, 4096,
+                                               c.ptr, sm.ptr, ws.h, stream))
+
+    for _ in range(max(1, args.warmup // 2)):
+        plain_step()
+    torch.cuda.synchronize()
+    p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    plain_steps = max(1, args.steps // 2)
+    p0.record()
+    for _ in range(plain_steps):
+        plain_step()
+    p1.record()
+    torch.cuda.synchronize()
+    plain_ms = p0.elapsed_time(p1) / plain_steps
+    # parity between the forms (and between the eager and graph launches)
+    same = True
+    for (ob, eb, sb), (og, eg, sg), (o, e, c, sm) in zip(res_b, res_g, outs):
+        cnt = c.to_numpy(np.uint64, 2)
+        same &= bool(np.array_equal(ob, og) and np.array_equal(eb, eg) and sb == sg)
+        same &= int(cnt[0]) == ob.size and int(cnt[1]) == eb.size and int(sm.to_numpy(np.uint64, 1)[0]) == sb
+        same &= bool(np.array_equal(o.to_numpy(np.uint64, int(cnt[0])), ob))
+        same &= bool(np.array_equal(e.to_numpy(np.uint64, min(int(cnt[1]), 4096)), eb[:4096]))
+    rows = sum(j["col"].n for j in jobs)
+    matched = sum(r[0].size for r in res_b)
+    errors = sum(r[1].size for r in res_b)
+    algo_bytes = sum(j["col"].n * j["col"].meta.width for j in jobs) + 8 * matched + 8 * errors
+    achieved = algo_bytes / (kern_ms * 1e-3) / 1e9
+    res = {
+        "metric": "batched hardened select+sum: instances/s (one persistent launch vs one launch per instance)",
+        "value": spec.count * args.steps / el,
+        "unit": "instances/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16/u32/u64",
+        "data": "synthetic (seeded, generated on device)",
+        "config": {"workload": f"batch: {spec.count} select+check-and-decode+sum instances, lognormal lengths "
+                               f"(median 707k, sigma 1.42, [1e4, 5e7]), 16/32/64-bit codes, selectivity "
+                               f"{spec.sel_min:.0%}-{spec.sel_max:.0%}, flips {spec.rate}",
+                   "instances": spec.count, "rows": rows, "matched": matched, "errors": errors,
+                   "tuned": {"max_subtiles": sub, "blocks": blocks, "sweep": table},
+                   "graph_ms": graph_ms, "plain_ms": plain_ms,
+                   "speedup_vs_plain": plain_ms / graph_ms, "tuples_per_s": rows / (graph_ms * 1e-3),
+                   "parity_batched_vs_plain": same},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_select_batch (+sums)",
+                     "kernel_ms": kern_ms, "algorithmic_bytes": algo_bytes},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_batch(spec, jobs, res_b, args.cpu_seconds)
+    batch.close()
+    return res
+
+
+def cpu_baseline_batch(spec, jobs, res_b, seconds):
+    """The oracle's select on instances of the batch (host-built columns), in order, until the
+    time budget is spent; bit-exact check against the batched GPU results of the same instances."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import numpy as np
+
+    import oracle_lib as O
+    from ahead_amd import workloads as W
+
+    rows, done, el, ok = 0, 0, 0.0, True
+    for i, j in enumerate(jobs):
+        codes, meta = W.batch_host_column(spec, i)
+        t0 = time.perf_counter()
+        oids, err, s = O.select_an(O.Col(codes, meta), j["op1"], j["th1"], j["op2"], j["th2"], j["comb"])
+        el += time.perf_counter() - t0
+        ok &= bool(np.array_equal(oids, res_b[i][0]) and np.array_equal(err, res_b[i][1]) and s == res_b[i][2])
+        rows += codes.size
+        done += 1
+        if el >= seconds:
+            break
+    return {"value": done / el, "unit": "instances/s", "cores": 1, "kind": "port",
+            "sample": f"first {done} of {len(jobs)} instances ({rows} rows, {el:.1f} s of select time), single "
+                      f"thread, oracle/an_oracle.c -O3 -fno-tree-vectorize; {rows / el:.3e} tuples/s; "
+                      f"bit-exact vs GPU: {ok}"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -268,7 +414,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs an MI355X (cuda:0)"
     torch.cuda.set_device(0)
-    res = run_star1(args) if args.workload == "star1" else run_select_sum(args)
+    run = {"star1": run_star1, "select_sum": run_select_sum, "batch": run_batch}[args.workload]
+    res = run(args)
     print(json.dumps(res), flush=True)
 
 

@@ -97,6 +97,35 @@ int ahead_hip_select_an(const ahead_col *col, int op1, uint64_t th1, int op2, ui
                         uint64_t oids_cap, uint64_t *err, uint64_t err_cap, uint64_t *d_counts,
                         uint64_t *d_sum, ahead_hip_ws *ws, void *stream);
 
+/* ---- batched selectAN ---------------------------------------------------------------------------
+ * No counterpart in the reference's API: many independent selectAN (+ fused decoded sum)
+ * instances, each with the semantics of ahead_hip_select_an, run by one persistent launch.  A
+ * batch is created once from its jobs (columns, predicates and outputs are fixed; creation
+ * allocates and uploads, synchronously) and launched any number of times; a launch allocates
+ * nothing and sets no memory from the host, so it may be captured in a hipGraph and replayed.
+ * Tuning: max_subtiles (1, 2, 4, 8, 16) sub-tiles of 16 KiB per look-back tile, `blocks`
+ * resident workgroups (0 = 4 per CU).  At most 1024 jobs per batch.  Launches of one batch
+ * must not overlap.                                                                           */
+typedef struct {
+    ahead_col col;         /* encoded tail of a BAT<void, v2_res*_t>                         */
+    int32_t op1, op2, comb, reserved;
+    uint64_t th1, th2;     /* encoded thresholds                                              */
+    uint64_t a_head, a_oid;
+    uint64_t *oids;
+    uint64_t oids_cap;
+    uint64_t *err;
+    uint64_t err_cap;
+    uint64_t *d_counts;    /* device, 2 entries: #oids, #err                                  */
+    uint64_t *d_sum;       /* device, decoded sum of matching valid rows (nullable)          */
+} ahead_select_job;
+
+typedef struct ahead_hip_select_batch ahead_hip_select_batch;
+int ahead_hip_select_batch_create(const ahead_select_job *jobs, uint32_t njobs, uint32_t max_subtiles,
+                                  uint32_t blocks, ahead_hip_select_batch **out);
+int ahead_hip_select_batch_launch(ahead_hip_select_batch *batch, void *stream);
+int ahead_hip_select_batch_destroy(ahead_hip_select_batch *batch);
+int ahead_hip_select_batch_info(const ahead_hip_select_batch *batch, uint32_t *total_tiles, uint32_t *blocks);
+
 /* ---- encode / check / decode ------------------------------------------------------------------
  * encodeAN (OperatorsAN.hpp:137-141; encdecAN_scalar.tcc:44-67): out[i] = (code)v_i * a,
  * out_width bytes per code.                                                                  */
@@ -134,6 +163,19 @@ int ahead_hip_matchjoin_pos_an(const ahead_col *h1, const ahead_col *t1, const a
                                uint64_t f_t2, uint32_t wout_t, uint64_t a_oid, ahead_buf out_h,
                                ahead_buf out_t, ahead_buf err1, ahead_buf err2, ahead_buf err4,
                                uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
+
+/* matchjoinAN, merge form <H1, T1, H2, T2> (OperatorsAN.hpp:33-58; matchjoinAN.tcc:46-143):
+ * a two-cursor walk over decoded t1 (list 1) and h2 (list 2), both ascending, emitting
+ * (h1 [* f_h1], t2 [* f_t2]) on equal keys.  e1..e4 check h1, t1, h2, t2 where the reference's
+ * loops visit the rows (a list-2 row skipped after the top of an iteration is reported twice;
+ * matched list-1 rows are not checked).  Inputs that are not ascending are walked exactly as the
+ * reference does, on a slower single-thread path.  d_counts[0..4] = #pairs, #e1, #e2, #e3, #e4. */
+int ahead_hip_matchjoin_merge_an(const ahead_col *h1, const ahead_col *t1, const ahead_col *h2,
+                                 const ahead_col *t2, int reenc_h, uint64_t f_h1, uint32_t wout_h,
+                                 int reenc_t, uint64_t f_t2, uint32_t wout_t, uint64_t a_oid,
+                                 ahead_buf out_h, ahead_buf out_t, ahead_buf e1, ahead_buf e2,
+                                 ahead_buf e3, ahead_buf e4, uint64_t *d_counts, ahead_hip_ws *ws,
+                                 void *stream);
 
 /* hashjoinAN (OperatorsAN.hpp:60-90; hashjoinAN.tcc:77-222, :552-585) with the unencoded-key
  * hash map.  build_right != 0 is hash_side_t::right (the default).  Pairs (h1, t2) in probe

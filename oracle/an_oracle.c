@@ -301,6 +301,42 @@ void or_matchjoin_pos_an(const or_col *h1, const or_col *t1, const or_col *t2, i
     }
 }
 
+/* The reference's loop nest, restated: an outer loop over (pos1, pos2) while both lists have
+ * rows; inside it, a skip loop over list 1 while t1 < h2 and a skip loop over list 2 while
+ * t1 > h2 (matchjoinAN.tcc:95-134).  t1 / h2 are decoded in their own widths and compared as
+ * unsigned integers. */
+void or_matchjoin_merge_an(const or_col *h1, const or_col *t1, const or_col *h2, const or_col *t2,
+                           int reenc_h, uint64_t f_h1, int wout_h, int reenc_t, uint64_t f_t2,
+                           int wout_t, uint64_t a_oid, or_vec *out_h, or_vec *out_t,
+                           or_vec *err1, or_vec *err2, or_vec *err3, or_vec *err4) {
+    const uint64_t n1 = t1->n, n2 = h2->n;
+    const uint64_t mh = or_mask(wout_h), mt = or_mask(wout_t);
+    uint64_t p1 = 0, p2 = 0;
+    while (p1 < n1 && p2 < n2) {
+        uint64_t hv = or_dec(h2, p2);
+        if (or_is_corrupt(h2, p2)) vec_push(err3, p2 * a_oid);
+        if (or_is_corrupt(t2, p2)) vec_push(err4, p2 * a_oid);
+        uint64_t tv = or_dec(t1, p1);
+        int valid;
+        for (; (valid = p1 < n1) && tv < hv; ++p1, tv = p1 < n1 ? or_dec(t1, p1) : 0) {
+            if (or_is_corrupt(h1, p1)) vec_push(err1, p1 * a_oid);
+            if (or_is_corrupt(t1, p1)) vec_push(err2, p1 * a_oid);
+        }
+        if (!valid) continue;
+        for (; (valid = p2 < n2) && tv > hv; ++p2, hv = p2 < n2 ? or_dec(h2, p2) : 0) {
+            if (or_is_corrupt(h2, p2)) vec_push(err3, p2 * a_oid);
+            if (or_is_corrupt(t2, p2)) vec_push(err4, p2 * a_oid);
+        }
+        if (valid && tv == hv) {
+            const uint64_t h = or_get(h1, p1), t = or_get(t2, p2);
+            vec_push(out_h, reenc_h && h1->encoded ? (h * f_h1) & mh : h);
+            vec_push(out_t, reenc_t && t2->encoded ? (t * f_t2) & mt : t);
+            ++p1;
+            ++p2;
+        }
+    }
+}
+
 /* ---------------------------------------------------------------------------------------------- */
 /* hashjoinAN (unencoded hash map)                                                                 */
 /* ---------------------------------------------------------------------------------------------- */

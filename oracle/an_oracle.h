@@ -96,6 +96,16 @@ void or_matchjoin_pos_an(const or_col *h1, const or_col *t1, const or_col *t2, i
                          uint64_t a_oid,
                          or_vec *out_h, or_vec *out_t, or_vec *err1, or_vec *err2, or_vec *err4);
 
+/* matchjoinAN, merge form <H1, T1, H2, T2> on two lists ascending in T1 / H2
+ * (matchjoinAN.tcc:46-143): a two-pointer walk over decoded t1 and h2 that emits (h1, t2) on
+ * equality.  Rows are checked where the reference's loops visit them: list-2 rows at the top of
+ * every outer iteration and again when skipped, list-1 rows only when skipped.
+ * reenc_h / reenc_t multiply encoded outputs by f_h1 / f_t2 (mulIfEncoded). */
+void or_matchjoin_merge_an(const or_col *h1, const or_col *t1, const or_col *h2, const or_col *t2,
+                           int reenc_h, uint64_t f_h1, int wout_h, int reenc_t, uint64_t f_t2,
+                           int wout_t, uint64_t a_oid, or_vec *out_h, or_vec *out_t,
+                           or_vec *err1, or_vec *err2, or_vec *err3, or_vec *err4);
+
 /* hashjoinAN with an unencoded-key hash map (hashjoinAN.tcc:77-222).  build_right != 0 is
  * hash_side_t::right (the default).  reenc_h / reenc_t enable the re-encoding variants
  * (:561-585): outputs h1 * f_h1 and t2 * f_t2 with f = mulIfEncoded(A_R, A_inv).

@@ -1,0 +1,91 @@
+"""GPU parity of the merge form of matchjoinAN (matchjoinAN.tcc:46-143) against the oracle,
+bit-exact, error vectors included: sorted inputs (the chunked path), sorted inputs with long runs
+of equal keys, and unsorted or corrupted inputs (the single-thread path)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from ahead_amd import an, capi, datagen
+
+pytestmark = pytest.mark.gpu
+
+RESOID = an.an_meta(an.RESOID, an.A_OID)
+
+
+def same(got, ref):
+    assert len(got) == len(ref)
+    for k, (g, r) in enumerate(zip(got, ref)):
+        assert np.array_equal(np.asarray(g).astype(np.uint64), np.asarray(r).astype(np.uint64)), k
+
+
+def oid_list(keys, seed, stream, flip_rate=0.0):
+    codes = datagen.encode(keys, an.A_OID, 8)
+    if flip_rate:
+        datagen.inject_flips(codes, seed, stream, datagen.flip_threshold(flip_rate))
+    return codes
+
+
+def run_both(h1, t1, h2, t2, m_h1, m_t1, m_h2, m_t2, *args):
+    ref = O.matchjoin_merge_an(O.Col(h1, m_h1), O.Col(t1, m_t1), O.Col(h2, m_h2), O.Col(t2, m_t2), *args)
+    got = capi.matchjoin_merge_an(capi.Column.from_numpy(h1, m_h1), capi.Column.from_numpy(t1, m_t1),
+                                  capi.Column.from_numpy(h2, m_h2), capi.Column.from_numpy(t2, m_t2), *args, err_cap=2)
+    same(got, ref)
+    return ref
+
+
+@pytest.mark.parametrize("n1,n2,hi", [(0, 10, 50), (10, 0, 50), (1, 1, 0), (15, 40, 60), (16, 16, 20), (17, 100, 300),
+                                      (4095, 5000, 20_000), (4097, 300, 20_000), (100_003, 80_001, 400_000),
+                                      (200_000, 1000, 10_000_000), (1000, 200_000, 300_000)])
+def test_merge_sorted_oid_lists(n1, n2, hi):
+    # two selections' OID lists (strictly ascending), the Q1.x shape: BAT<resoid, resoid> both
+    seed = n1 + 3 * n2
+    k1 = np.unique(datagen.uniform(n1, seed, 1, 0, hi))
+    k2 = np.unique(datagen.uniform(n2, seed, 2, 0, hi))
+    t1, t2 = oid_list(k1, seed, 3), oid_list(k2, seed, 4)
+    ref = run_both(t1, t1, t2, t2, RESOID, RESOID, RESOID, RESOID)
+    assert ref[0].size == np.intersect1d(k1, k2).size
+
+
+@pytest.mark.parametrize("n1,n2,hi", [(5000, 5000, 30), (20_000, 3000, 7), (3000, 20_000, 700), (65_536, 65_536, 1)])
+def test_merge_sorted_with_runs(n1, n2, hi):
+    seed = n1 + n2 + hi
+    meta = an.an_meta(an.RESSHORT, 233)
+    k1 = np.sort(datagen.uniform(n1, seed, 1, 0, hi))
+    k2 = np.sort(datagen.uniform(n2, seed, 2, 0, hi))
+    h1 = oid_list(np.arange(n1, dtype=np.uint64), seed, 3)
+    t2 = datagen.encode(datagen.uniform(n2, seed, 4, 0, 200), 29, 2)
+    run_both(h1, datagen.encode(k1, 233, 4), datagen.encode(k2, 233, 4), t2, RESOID, meta, meta,
+             an.an_meta(an.RESTINY, 29))
+
+
+@pytest.mark.parametrize("n1,n2", [(300, 500), (5000, 4000), (20_000, 20_000)])
+def test_merge_unsorted_and_corrupted(n1, n2):
+    seed = n1 * 7 + n2
+    meta = an.an_meta(an.RESSHORT, 233)
+    # unsorted keys: the walk is the reference's two-cursor walk all the same
+    k1 = datagen.uniform(n1, seed, 1, 0, 50)
+    k2 = datagen.uniform(n2, seed, 2, 0, 50)
+    h1 = oid_list(np.arange(n1, dtype=np.uint64), seed, 3, 1e-3)
+    t2 = oid_list(np.arange(n2, dtype=np.uint64), seed, 4, 1e-3)
+    run_both(h1, datagen.encode(k1, 233, 4), datagen.encode(k2, 233, 4), t2, RESOID, meta, meta, RESOID)
+    # sorted keys with flipped bits in both key lists
+    c1 = datagen.encode(np.sort(datagen.uniform(n1, seed, 5, 0, 3 * n1)), 233, 4)
+    c2 = datagen.encode(np.sort(datagen.uniform(n2, seed, 6, 0, 3 * n1)), 233, 4)
+    datagen.inject_flips(c1, seed, 7, datagen.flip_threshold(2e-3))
+    datagen.inject_flips(c2, seed, 8, datagen.flip_threshold(2e-3))
+    run_both(h1, c1, c2, t2, RESOID, meta, meta, RESOID)
+
+
+def test_merge_reencode_and_widths():
+    n1, n2 = 30_000, 25_000
+    k1 = np.unique(datagen.uniform(n1, 91, 1, 0, 60_000))
+    k2 = np.unique(datagen.uniform(n2, 91, 2, 0, 60_000))
+    t1, h2 = oid_list(k1, 91, 3, 1e-4), oid_list(k2, 91, 4, 1e-4)
+    h1 = oid_list(np.arange(k1.size, dtype=np.uint64) + 5, 91, 5, 1e-4)
+    t2 = datagen.encode(datagen.uniform(k2.size, 91, 6, 0, 0xFFFF), 63877, 4)
+    m_t2 = an.an_meta(an.RESSHORT, 63877)
+    f_h1 = (16089 * an.A_OID_INV) & an.MASK64
+    f_t2 = (233 * an.inverse(63877, 32)) & 0xFFFFFFFF
+    run_both(h1, t1, h2, t2, RESOID, RESOID, RESOID, m_t2, 1, f_h1, 8, 1, f_t2, 4)
+    # head-only re-encoding zeroes the tail factor, as the reference does
+    run_both(h1, t1, h2, t2, RESOID, RESOID, RESOID, m_t2, 1, f_h1, 8, 1, 0, 4)

@@ -83,3 +83,16 @@ def test_star1_repeated_launches_are_identical():
     b = st.results()
     for k in KEYS:
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_star1_parity_full_size():
+    """The benchmark workload itself (60M fact rows, 200k keys, 1e-7 flips), bit-exact vs the oracle."""
+    spec = W.Star1Spec()
+    print(f"star1 full size: n_fact={spec.n_fact} n_dim={spec.n_dim} rate={spec.rate} seed={spec.seed}", flush=True)
+    h, ref, got = run_both(spec)
+    assert got["status"] == 0
+    for k in KEYS:
+        assert np.array_equal(got[k], ref[k]), k
+    assert got["sums"].size == spec.card1 * spec.card2
+    # every injected flip in the foreign keys is detected (single flips of A=63877 codes, 2^64/A > maxU)
+    assert np.array_equal(got["e_fk"], (h["flips"]["fk"].astype(np.uint64) * np.uint64(an.A_OID_ERR)))
