@@ -22,16 +22,18 @@
 //                 when the valid keys span a range of at most 4 x |dim| + 1024 values, a direct
 //                 map of 16-bit entries (attr-error bits | cid + 1) indexed by key - kmin, so a
 //                 fact row costs one 2-byte gather; keys outside the range use the table.
-//  k_star_probe   one workgroup per tile of ITERS x 2048 fact rows.  Phase 1 streams FK and
-//                 measure codes with 128-bit loads, decodes and checks the FK, probes the table
-//                 with the truncated decoded key even when the FK is corrupt (hashjoinAN.tcc:199),
-//                 checks the measure, and adds the measure code of every kept row into per-wave
-//                 LDS accumulators indexed by cid (with the first tile-local row per cid).  Kept
-//                 rows form an LDS bitmap built from wave ballots; error rows (rare) set bits with
-//                 LDS atomics.  Resolve: five look-back streams (kept rows, FK errors, attr1
-//                 errors, attr2 errors, measure errors).  Phase 2 turns first rows into global
-//                 kept-row ranks, merges the accumulators into the global per-combination table
-//                 with 64-bit atomics and writes the error positions in order.
+//  k_star_probe   persistent: 4 workgroups per CU take tiles of ITERS x 2048 fact rows in ticket
+//                 order.  Phase 1 streams FK and measure codes with 128-bit loads, decodes and
+//                 checks the FK, probes the map (or table) with the truncated decoded key even
+//                 when the FK is corrupt (hashjoinAN.tcc:199), checks the measure, and adds the
+//                 measure code of every kept row into per-wave LDS accumulators indexed by cid,
+//                 which live for the whole launch.  Kept rows form an LDS bitmap built from wave
+//                 ballots; error rows (rare) set bits with LDS atomics.  Resolve: five look-back
+//                 streams (kept rows, FK errors, attr1 errors, attr2 errors, measure errors).
+//                 Phase 2 turns the first row of every combination new to the workgroup into its
+//                 global kept-row rank and writes the error positions in order.  At the end each
+//                 workgroup writes its sums and first ranks to a partial table.
+//  k_star_reduce  one workgroup per cid adds the partial sums and takes the smallest first rank.
 //  k_star_final   orders the combinations by first kept-row rank (= binary group ids, since
 //                 (attr2, g1) <-> (attr1, attr2) is one-to-one), writes sums, VGIDtoROID and key
 //                 codes, and clears the global table for the next launch.
@@ -45,7 +47,7 @@ constexpr int kStarRowsPerSub = kBlock * 2 * 4;  // 2 x 64-bit codes per 16-byte
 constexpr int kStarMaxIters = 8;
 constexpr int kStarTileRows = kStarRowsPerSub * kStarMaxIters;  // 16384
 constexpr int kStarWords = kStarTileRows / 32;
-constexpr uint32_t kLdsCids = 2048;         // combination ids accumulated in LDS per tile
+constexpr uint32_t kLdsCids = 1024;         // combination ids accumulated in LDS per workgroup
 constexpr uint32_t kWaveCids = kLdsCids / kWaves;  // per-wave accumulators when cids fit
 constexpr uint32_t kCombos = 65536;         // 8-bit attr1 x 8-bit attr2
 constexpr uint32_t kFinalMaxGroups = 8192;  // groups the single-workgroup finalize can order
@@ -238,6 +240,8 @@ struct ProbeArgs {
     const uint32_t *flags;
     uint64_t *g_sum;    // [kCombos] sums of measure codes (mod 2^64)
     uint32_t *g_first;  // [kCombos] 0xFFFFFFFF - first kept-row rank (0 = absent)
+    unsigned long long *part_sum;  // [kLdsCids][grid] per-workgroup sums of LDS-accumulated cids
+    uint32_t *part_rank;           // [kLdsCids][grid] per-workgroup first kept-row ranks
     uint64_t a_oid_err;
     uint64_t *err[4];   // FK, attr1, attr2, measure error positions
     uint64_t cap[4];
@@ -298,248 +302,340 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
     __shared__ uint32_t s_bits[kStreams + 1][kStarWords];  // kept / error / overflow bitmaps by row
     __shared__ uint32_t s_wpre[kStarWords];                 // kept rows before each bitmap word
     __shared__ unsigned long long s_sum[kLdsCids];          // per cid (per wave when cids fit)
-    __shared__ uint32_t s_first[kLdsCids];                  // first tile-local row per cid
-    __shared__ uint32_t s_any_overflow;
+    __shared__ uint32_t s_tfirst[kLdsCids];                 // first tile-local row of a cid new to the block
+    __shared__ uint32_t s_rank[kLdsCids];                   // first kept-row rank of each cid (block)
+    __shared__ uint32_t s_new, s_any_overflow;
     __shared__ uint32_t s_flag[5];
 
-    const uint32_t tile = take_tile(a.lb, &s_tile);
-    if (tile >= a.lb.num_tiles) return;
     if (threadIdx.x < 5) s_flag[threadIdx.x] = a.flags[threadIdx.x];
-    __syncthreads();
-    if (s_flag[0] != 0u) {
-        // not applicable (duplicate build keys): publish empty counts so the chain drains
-        const uint32_t zero[kStreams] = {0, 0, 0, 0, 0};
-        lb_resolve<kStreams>(a.lb, tile, zero, s_excl);
-        if (tile == a.lb.num_tiles - 1 && threadIdx.x == 0) {
-            for (int e = 1; e < kStreams; ++e) a.d_counts[count_index(e)] = 0;
-            a.d_counts[8] = 0;
-        }
-        return;
+    for (int i = threadIdx.x; i < (kStreams + 1) * kStarWords; i += kBlock) (&s_bits[0][0])[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < kLdsCids; i += kBlock) {
+        s_sum[i] = 0ull;
+        s_tfirst[i] = 0xFFFFFFFFu;
+        s_rank[i] = 0xFFFFFFFFu;
     }
+    if (threadIdx.x == 0) {
+        s_new = 0u;
+        s_any_overflow = 0u;
+    }
+    __syncthreads();
+    const bool skip = s_flag[0] != 0u;
     const uint32_t ncid = s_flag[1];
     const bool dense = s_flag[4] != 0u;
     const uint32_t kmin = s_flag[2], krange = s_flag[3] - s_flag[2] + 1u;
     const bool per_wave = ncid <= kWaveCids;  // private accumulators per wave: no cross-wave contention
+    const uint32_t nslots = ncid < kLdsCids ? ncid : kLdsCids;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    for (int i = threadIdx.x; i < (kStreams + 1) * kStarWords; i += kBlock) (&s_bits[0][0])[i] = 0u;
-    for (uint32_t i = threadIdx.x; i < kLdsCids; i += kBlock) {
-        s_sum[i] = 0ull;
-        s_first[i] = 0xFFFFFFFFu;
-    }
-    if (threadIdx.x == 0) s_any_overflow = 0u;
-    __syncthreads();
-
-    const uint64_t base = static_cast<uint64_t>(tile) * a.iters * kStarRowsPerSub;
     const bool aligned = ((reinterpret_cast<uintptr_t>(a.fk) | reinterpret_cast<uintptr_t>(a.m)) & 15u) == 0;
-    uint32_t cnt[kStreams] = {0, 0, 0, 0, 0};
 
-    // phase 1
-    for (uint32_t it = 0; it < a.iters; ++it) {
-        const uint32_t sub = it * kStarRowsPerSub;
-        uint64_t fk[4][2], mv[4][2];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const uint64_t r0 = base + sub + c * (kBlock * 2) + threadIdx.x * 2;
-            if (aligned && r0 + 2 <= a.n) {
-                const u32x4 q = load16_nt(a.fk + r0);
-                const u32x4 w = load16_nt(a.m + r0);
-                __builtin_memcpy(&fk[c][0], &q, 16);
-                __builtin_memcpy(&mv[c][0], &w, 16);
-            } else {
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    fk[c][k] = r0 + k < a.n ? a.fk[r0 + k] : 0ull;
-                    mv[c][k] = r0 + k < a.n ? a.m[r0 + k] : 0ull;
-                }
+    // persistent: tiles are taken in ticket order until none is left
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&a.lb.hdr->ticket, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= a.lb.num_tiles) break;
+        if (skip) {
+            // not applicable (duplicate build keys): publish empty counts so the chain drains
+            const uint32_t zero[kStreams] = {0, 0, 0, 0, 0};
+            lb_resolve<kStreams>(a.lb, tile, zero, s_excl);
+            if (tile == a.lb.num_tiles - 1 && threadIdx.x == 0) {
+                for (int e = 1; e < kStreams; ++e) a.d_counts[count_index(e)] = 0;
+                a.d_counts[8] = 0;
             }
+            __syncthreads();
+            continue;
         }
-        // batched probe: the map (or table) words of this thread's 8 rows are requested before
-        // any is used, so the dependent gathers overlap instead of chaining
-        uint32_t key[4][2], slot[4][2];
-        uint64_t ent[4][2];
-        bool live[4][2], efk[4][2], in_map[4][2];
+        const uint64_t base = static_cast<uint64_t>(tile) * a.iters * kStarRowsPerSub;
+        uint32_t cnt[kStreams] = {0, 0, 0, 0, 0};
+
+        // phase 1
+        for (uint32_t it = 0; it < a.iters; ++it) {
+            const uint32_t sub = it * kStarRowsPerSub;
+            uint64_t fk[4][2], mv[4][2];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+            for (int c = 0; c < 4; ++c) {
+                const uint64_t r0 = base + sub + c * (kBlock * 2) + threadIdx.x * 2;
+                if (aligned && r0 + 2 <= a.n) {
+                    const u32x4 q = load16_nt(a.fk + r0);
+                    const u32x4 w = load16_nt(a.m + r0);
+                    __builtin_memcpy(&fk[c][0], &q, 16);
+                    __builtin_memcpy(&mv[c][0], &w, 16);
+                } else {
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const uint32_t local = sub + c * (kBlock * 2) + threadIdx.x * 2 + k;
-                live[c][k] = base + local < a.n;
-                const uint64_t d = fk[c][k] * a.fk_inv;
-                efk[c][k] = live[c][k] && d > a.fk_maxu;
-                key[c][k] = static_cast<uint32_t>(d & a.key_mask);
-                in_map[c][k] = dense && key[c][k] - kmin < krange;
-                slot[c][k] = table_slot(key[c][k], a.table_bits);
-                ent[c][k] = !live[c][k] ? 0ull
-                          : (in_map[c][k] ? static_cast<uint64_t>(a.map[key[c][k] - kmin]) : a.table[slot[c][k]]);
-            }
-        }
-        bool joined[4][2];
-        int64_t row[4][2];
-        uint32_t info[4][2];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                row[c][k] = -1;
-                joined[c][k] = false;
-                info[c][k] = 0u;
-                if (!live[c][k] || ent[c][k] == 0ull) continue;
-                if (in_map[c][k]) {
-                    const uint32_t e = static_cast<uint32_t>(ent[c][k]);
-                    joined[c][k] = true;
-                    info[c][k] = ((e >> 15) & 1u) << 30 | ((e >> 14) & 1u) << 31 | ((e & 0x3FFFu) - 1u);
-                    continue;
-                }
-                if (static_cast<uint32_t>(ent[c][k] >> 32) == key[c][k]) {
-                    row[c][k] = static_cast<uint32_t>(ent[c][k]) - 1u;
-                } else {  // collision: continue the linear probe
-                    const uint32_t mask = (1u << a.table_bits) - 1u;
-                    uint32_t s2 = (slot[c][k] + 1u) & mask;
-                    for (uint32_t probes = 0; probes < mask; ++probes) {
-                        const uint64_t e = a.table[s2];
-                        if (e == 0ull) break;
-                        if (static_cast<uint32_t>(e >> 32) == key[c][k]) {
-                            row[c][k] = static_cast<uint32_t>(e) - 1u;
-                            break;
-                        }
-                        s2 = (s2 + 1u) & mask;
+                    for (int k = 0; k < 2; ++k) {
+                        fk[c][k] = r0 + k < a.n ? a.fk[r0 + k] : 0ull;
+                        mv[c][k] = r0 + k < a.n ? a.m[r0 + k] : 0ull;
                     }
                 }
-                joined[c][k] = row[c][k] >= 0;
             }
-        }
+            // batched probe: the map (or table) words of this thread's 8 rows are requested before
+            // any is used, so the dependent gathers overlap instead of chaining
+            uint32_t key[4][2], slot[4][2];
+            uint64_t ent[4][2];
+            bool live[4][2], efk[4][2], in_map[4][2];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+            for (int c = 0; c < 4; ++c) {
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                if (row[c][k] >= 0) info[c][k] = a.info[row[c][k]];
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            bool kept[2];
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const uint32_t local = sub + c * (kBlock * 2) + threadIdx.x * 2 + k;
-                kept[k] = false;
-                const uint32_t w = local >> 5, b = 1u << (local & 31u);
-                if (efk[c][k]) { atomicOr(&s_bits[kEFk][w], b); cnt[kEFk]++; }
-                if (!joined[c][k]) continue;
-                const bool e1 = (info[c][k] >> 30) & 1u, e2 = (info[c][k] >> 31) & 1u;
-                const bool em = (mv[c][k] * a.m_inv) > a.m_maxu;
-                if (e1) { atomicOr(&s_bits[kEA1][w], b); cnt[kEA1]++; }
-                if (e2) { atomicOr(&s_bits[kEA2][w], b); cnt[kEA2]++; }
-                if (em) { atomicOr(&s_bits[kEM][w], b); cnt[kEM]++; }
-                if (e1 || e2 || em) continue;
-                kept[k] = true;
-                cnt[kKept]++;
-                const uint32_t cid = info[c][k] & 0xFFFFu;
-                if (cid < kLdsCids) {
-                    const uint32_t sl = per_wave ? wave * kWaveCids + cid : cid;
-                    atomicAdd(&s_sum[sl], static_cast<unsigned long long>(mv[c][k]));
-                    if (local < s_first[cid]) atomicMin(&s_first[cid], local);
-                } else {  // more combinations than LDS holds: merged through the global table in phase 2
-                    atomicOr(&s_bits[kOverflow][w], b);
-                    s_any_overflow = 1u;
+                for (int k = 0; k < 2; ++k) {
+                    const uint32_t local = sub + c * (kBlock * 2) + threadIdx.x * 2 + k;
+                    live[c][k] = base + local < a.n;
+                    const uint64_t d = fk[c][k] * a.fk_inv;
+                    efk[c][k] = live[c][k] && d > a.fk_maxu;
+                    key[c][k] = static_cast<uint32_t>(d & a.key_mask);
+                    in_map[c][k] = dense && key[c][k] - kmin < krange;
+                    slot[c][k] = table_slot(key[c][k], a.table_bits);
+                    ent[c][k] = !live[c][k] ? 0ull
+                              : (in_map[c][k] ? static_cast<uint64_t>(a.map[key[c][k] - kmin]) : a.table[slot[c][k]]);
                 }
             }
-            // kept bitmap words from two ballots (even / odd rows of the wave's 128-row span)
-            const uint64_t b0 = __ballot(kept[0]), b1 = __ballot(kept[1]);
-            if (lane < 4) {
-                const uint32_t x0 = static_cast<uint32_t>(b0 >> (16 * lane)), x1 = static_cast<uint32_t>(b1 >> (16 * lane));
-                s_bits[kKept][(sub + c * (kBlock * 2) + wave * 128) / 32 + lane] = spread16(x0) | (spread16(x1) << 1);
+            bool joined[4][2];
+            int64_t row[4][2];
+            uint32_t info[4][2];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    row[c][k] = -1;
+                    joined[c][k] = false;
+                    info[c][k] = 0u;
+                    if (!live[c][k] || ent[c][k] == 0ull) continue;
+                    if (in_map[c][k]) {
+                        const uint32_t e = static_cast<uint32_t>(ent[c][k]);
+                        joined[c][k] = true;
+                        info[c][k] = ((e >> 15) & 1u) << 30 | ((e >> 14) & 1u) << 31 | ((e & 0x3FFFu) - 1u);
+                        continue;
+                    }
+                    if (static_cast<uint32_t>(ent[c][k] >> 32) == key[c][k]) {
+                        row[c][k] = static_cast<uint32_t>(ent[c][k]) - 1u;
+                    } else {  // collision: continue the linear probe
+                        const uint32_t mask = (1u << a.table_bits) - 1u;
+                        uint32_t s2 = (slot[c][k] + 1u) & mask;
+                        for (uint32_t probes = 0; probes < mask; ++probes) {
+                            const uint64_t e = a.table[s2];
+                            if (e == 0ull) break;
+                            if (static_cast<uint32_t>(e >> 32) == key[c][k]) {
+                                row[c][k] = static_cast<uint32_t>(e) - 1u;
+                                break;
+                            }
+                            s2 = (s2 + 1u) & mask;
+                        }
+                    }
+                    joined[c][k] = row[c][k] >= 0;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    if (row[c][k] >= 0) info[c][k] = a.info[row[c][k]];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                bool kept[2];
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const uint32_t local = sub + c * (kBlock * 2) + threadIdx.x * 2 + k;
+                    kept[k] = false;
+                    const uint32_t w = local >> 5, b = 1u << (local & 31u);
+                    if (efk[c][k]) { atomicOr(&s_bits[kEFk][w], b); cnt[kEFk]++; }
+                    if (!joined[c][k]) continue;
+                    const bool e1 = (info[c][k] >> 30) & 1u, e2 = (info[c][k] >> 31) & 1u;
+                    const bool em = (mv[c][k] * a.m_inv) > a.m_maxu;
+                    if (e1) { atomicOr(&s_bits[kEA1][w], b); cnt[kEA1]++; }
+                    if (e2) { atomicOr(&s_bits[kEA2][w], b); cnt[kEA2]++; }
+                    if (em) { atomicOr(&s_bits[kEM][w], b); cnt[kEM]++; }
+                    if (e1 || e2 || em) continue;
+                    kept[k] = true;
+                    cnt[kKept]++;
+                    const uint32_t cid = info[c][k] & 0xFFFFu;
+                    if (cid < kLdsCids) {
+                        const uint32_t sl = per_wave ? wave * kWaveCids + cid : cid;
+                        atomicAdd(&s_sum[sl], static_cast<unsigned long long>(mv[c][k]));
+                        if (s_rank[cid] == 0xFFFFFFFFu && local < s_tfirst[cid]) {
+                            atomicMin(&s_tfirst[cid], local);
+                            s_new = 1u;
+                        }
+                    } else {  // more combinations than LDS holds: merged through the global table in phase 2
+                        atomicOr(&s_bits[kOverflow][w], b);
+                        s_any_overflow = 1u;
+                    }
+                }
+                // kept bitmap words from two ballots (even / odd rows of the wave's 128-row span)
+                const uint64_t b0 = __ballot(kept[0]), b1 = __ballot(kept[1]);
+                if (lane < 4) {
+                    const uint32_t x0 = static_cast<uint32_t>(b0 >> (16 * lane)), x1 = static_cast<uint32_t>(b1 >> (16 * lane));
+                    s_bits[kKept][(sub + c * (kBlock * 2) + wave * 128) / 32 + lane] = spread16(x0) | (spread16(x1) << 1);
+                }
             }
         }
-    }
+        // the tail of a tile shorter than the others holds no rows: its kept words must read 0
+        if (a.iters < kStarMaxIters) {
+            for (uint32_t w = a.iters * (kStarRowsPerSub / 32) + threadIdx.x; w < kStarWords; w += kBlock)
+                s_bits[kKept][w] = 0u;
+        }
 
-    // resolve: block totals, look-back
-    uint32_t agg[kStreams];
+        // resolve: block totals, look-back
+        uint32_t agg[kStreams];
 #pragma unroll
-    for (int s = 0; s < kStreams; ++s) agg[s] = block_sum<uint32_t>(cnt[s], s_red);
-    lb_resolve<kStreams>(a.lb, tile, agg, s_excl);
+        for (int s = 0; s < kStreams; ++s) agg[s] = block_sum<uint32_t>(cnt[s], s_red);
+        lb_resolve<kStreams>(a.lb, tile, agg, s_excl);
+        const uint32_t xk = s_excl[kKept];
+        const bool fresh = s_new != 0u;
+        const bool overflow = s_any_overflow != 0u;
 
-    // phase 2a: kept rows before every bitmap word of the tile
-    {
-        constexpr int PER = kStarWords / kBlock;  // 2 words per thread
-        uint32_t c0 = 0;
+        // phase 2a (only when the tile holds the block's first row of some combination, or rows that
+        // missed the LDS table): kept rows before every bitmap word of the tile
+        if (fresh || overflow) {
+            constexpr int PER = kStarWords / kBlock;  // 2 words per thread
+            uint32_t c0 = 0;
 #pragma unroll
-        for (int q = 0; q < PER; ++q) c0 += __popc(s_bits[kKept][threadIdx.x * PER + q]);
-        uint32_t tot;
-        uint32_t ex = block_excl_scan<uint32_t>(c0, tot, s_red);
+            for (int q = 0; q < PER; ++q) c0 += __popc(s_bits[kKept][threadIdx.x * PER + q]);
+            uint32_t tot;
+            uint32_t ex = block_excl_scan<uint32_t>(c0, tot, s_red);
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            s_wpre[threadIdx.x * PER + q] = ex;
-            ex += __popc(s_bits[kKept][threadIdx.x * PER + q]);
+            for (int q = 0; q < PER; ++q) {
+                s_wpre[threadIdx.x * PER + q] = ex;
+                ex += __popc(s_bits[kKept][threadIdx.x * PER + q]);
+            }
+            __syncthreads();
+        }
+        // phase 2b: first kept-row ranks of combinations new to this block
+        if (fresh) {
+            for (uint32_t cid = threadIdx.x; cid < nslots; cid += kBlock) {
+                const uint32_t first = s_tfirst[cid];
+                if (first == 0xFFFFFFFFu) continue;
+                const uint32_t w = first >> 5;
+                s_rank[cid] = xk + s_wpre[w] + __popc(s_bits[kKept][w] & ((1u << (first & 31u)) - 1u));
+                s_tfirst[cid] = 0xFFFFFFFFu;
+            }
+        }
+        // phase 2b': rows whose combination found no LDS slot go straight to the global table
+        if (overflow) {
+            constexpr int PER = kStarWords / kBlock;
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const uint32_t w = threadIdx.x * PER + q;
+                uint32_t bits = s_bits[kOverflow][w];
+                s_bits[kOverflow][w] = 0u;
+                while (bits) {
+                    const uint32_t b = __ffs(bits) - 1;
+                    bits &= bits - 1u;
+                    const uint64_t row = base + w * 32u + b;
+                    const RowEval r = eval_row(a, a.fk[row], a.m[row]);
+                    const uint32_t rank = xk + s_wpre[w] + __popc(s_bits[kKept][w] & ((1u << b) - 1u));
+                    const uint32_t combo = a.cid_combo[r.cid];
+                    atomicAdd(reinterpret_cast<unsigned long long *>(&a.g_sum[combo]),
+                              static_cast<unsigned long long>(a.m[row]));
+                    atomicMax(&a.g_first[combo], 0xFFFFFFFFu - rank);
+                }
+            }
+        }
+        // phase 2c: ordered error positions (rare): one pass per stream over the bitmap words
+#pragma unroll
+        for (int e = 1; e < kStreams; ++e) {
+            if (agg[e] == 0u) continue;
+            constexpr int PER = kStarWords / kBlock;
+            uint32_t c0 = 0;
+#pragma unroll
+            for (int q = 0; q < PER; ++q) c0 += __popc(s_bits[e][threadIdx.x * PER + q]);
+            uint32_t tot;
+            uint64_t idx = s_excl[e] + block_excl_scan<uint32_t>(c0, tot, s_red);
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                uint32_t bits = s_bits[e][threadIdx.x * PER + q];
+                s_bits[e][threadIdx.x * PER + q] = 0u;
+                while (bits) {
+                    const uint32_t local = (threadIdx.x * PER + q) * 32u + (__ffs(bits) - 1);
+                    bits &= bits - 1u;
+                    const uint64_t row = base + local;
+                    uint64_t pos = row;
+                    if (e == kEA1 || e == kEA2) pos = eval_row(a, a.fk[row], a.m[row]).dim_row;
+                    if (idx < a.cap[e - 1]) a.err[e - 1][idx] = pos * a.a_oid_err;
+                    ++idx;
+                }
+            }
+        }
+        if (tile == a.lb.num_tiles - 1 && threadIdx.x == 0) {
+#pragma unroll
+            for (int e = 1; e < kStreams; ++e) a.d_counts[count_index(e)] = s_excl[e] + agg[e];
+            a.d_counts[8] = s_excl[kKept] + agg[kKept];
+        }
+        __syncthreads();  // bitmaps, s_new and s_tile are reused by the next tile
+        if (threadIdx.x == 0) {
+            s_new = 0u;
+            s_any_overflow = 0u;
         }
     }
-    __syncthreads();
-    // phase 2b: merge the LDS accumulators into the global per-combination table
-    const uint32_t xk = s_excl[kKept];
+
+    // the block's accumulators -> its column of the partial tables (reduced by k_star_reduce)
+    if (!skip) {
+        for (uint32_t cid = threadIdx.x; cid < nslots; cid += kBlock) {
+            unsigned long long sum = 0ull;
+            if (per_wave) {
+#pragma unroll
+                for (int w = 0; w < kWaves; ++w) sum += s_sum[w * kWaveCids + cid];
+            } else {
+                sum = s_sum[cid];
+            }
+            a.part_sum[static_cast<size_t>(cid) * gridDim.x + blockIdx.x] = sum;
+            a.part_rank[static_cast<size_t>(cid) * gridDim.x + blockIdx.x] = s_rank[cid];
+        }
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t d = atomicAdd(&a.lb.hdr->done, 1u);
+        if (d == gridDim.x - 1) {  // every workgroup has taken its last ticket
+            __hip_atomic_store(&a.lb.hdr->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&a.lb.hdr->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// One workgroup per LDS-accumulated combination id: adds the per-workgroup partial sums and takes
+// the smallest first rank into the global per-combination table (cids beyond the LDS table reached
+// it directly from k_star_probe).
+__global__ __launch_bounds__(kBlock) void k_star_reduce(const unsigned long long *part_sum, const uint32_t *part_rank,
+                                                        uint32_t parts, const uint16_t *cid_combo,
+                                                        const uint32_t *flags, uint64_t *g_sum, uint32_t *g_first) {
+    __shared__ unsigned long long s_s[kWaves];
+    __shared__ uint32_t s_r[kWaves];
+    const uint32_t ncid = flags[1];
     const uint32_t nslots = ncid < kLdsCids ? ncid : kLdsCids;
-    for (uint32_t cid = threadIdx.x; cid < nslots; cid += kBlock) {
-        const uint32_t first = s_first[cid];
-        if (first == 0xFFFFFFFFu) continue;
-        unsigned long long sum = 0ull;
-        if (per_wave) {
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) sum += s_sum[w * kWaveCids + cid];
-        } else {
-            sum = s_sum[cid];
+    if (flags[0] != 0u) return;
+    for (uint32_t cid = blockIdx.x; cid < nslots; cid += gridDim.x) {
+        unsigned long long s = 0ull;
+        uint32_t r = 0xFFFFFFFFu;
+        for (uint32_t i = threadIdx.x; i < parts; i += kBlock) {
+            s += part_sum[static_cast<size_t>(cid) * parts + i];
+            const uint32_t x = part_rank[static_cast<size_t>(cid) * parts + i];
+            r = x < r ? x : r;
         }
-        const uint32_t combo = a.cid_combo[cid];
-        const uint32_t w = first >> 5;
-        const uint32_t rank = xk + s_wpre[w] + __popc(s_bits[kKept][w] & ((1u << (first & 31u)) - 1u));
-        atomicAdd(reinterpret_cast<unsigned long long *>(&a.g_sum[combo]), sum);
-        atomicMax(&a.g_first[combo], 0xFFFFFFFFu - rank);
-    }
-    // phase 2b': rows whose combination found no LDS slot go straight to the global table
-    if (s_any_overflow) {
-        constexpr int PER = kStarWords / kBlock;
+        s = wave_sum(s);
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const uint32_t w = threadIdx.x * PER + q;
-            uint32_t bits = s_bits[kOverflow][w];
-            while (bits) {
-                const uint32_t b = __ffs(bits) - 1;
-                bits &= bits - 1u;
-                const uint64_t row = base + w * 32u + b;
-                const RowEval r = eval_row(a, a.fk[row], a.m[row]);
-                const uint32_t rank = xk + s_wpre[w] + __popc(s_bits[kKept][w] & ((1u << b) - 1u));
-                const uint32_t combo = a.cid_combo[r.cid];
-                atomicAdd(reinterpret_cast<unsigned long long *>(&a.g_sum[combo]),
-                          static_cast<unsigned long long>(a.m[row]));
-                atomicMax(&a.g_first[combo], 0xFFFFFFFFu - rank);
+        for (int d = 32; d >= 1; d >>= 1) {
+            const uint32_t o = __shfl_xor(r, d);
+            r = o < r ? o : r;
+        }
+        if ((threadIdx.x & 63u) == 0) {
+            s_s[threadIdx.x >> 6] = s;
+            s_r[threadIdx.x >> 6] = r;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long ts = 0ull;
+            uint32_t tr = 0xFFFFFFFFu;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                ts += s_s[w];
+                tr = s_r[w] < tr ? s_r[w] : tr;
+            }
+            if (tr != 0xFFFFFFFFu) {
+                const uint32_t combo = cid_combo[cid];
+                g_sum[combo] = ts;
+                g_first[combo] = 0xFFFFFFFFu - tr;
             }
         }
-    }
-    // phase 2c: ordered error positions (rare): one pass per stream over the bitmap words
-#pragma unroll
-    for (int e = 1; e < kStreams; ++e) {
-        if (agg[e] == 0u) continue;
-        constexpr int PER = kStarWords / kBlock;
-        uint32_t c0 = 0;
-#pragma unroll
-        for (int q = 0; q < PER; ++q) c0 += __popc(s_bits[e][threadIdx.x * PER + q]);
-        uint32_t tot;
-        uint64_t idx = s_excl[e] + block_excl_scan<uint32_t>(c0, tot, s_red);
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            uint32_t bits = s_bits[e][threadIdx.x * PER + q];
-            while (bits) {
-                const uint32_t local = (threadIdx.x * PER + q) * 32u + (__ffs(bits) - 1);
-                bits &= bits - 1u;
-                const uint64_t row = base + local;
-                uint64_t pos = row;
-                if (e == kEA1 || e == kEA2) pos = eval_row(a, a.fk[row], a.m[row]).dim_row;
-                if (idx < a.cap[e - 1]) a.err[e - 1][idx] = pos * a.a_oid_err;
-                ++idx;
-            }
-        }
-    }
-    if (tile == a.lb.num_tiles - 1 && threadIdx.x == 0) {
-#pragma unroll
-        for (int e = 1; e < kStreams; ++e) a.d_counts[count_index(e)] = s_excl[e] + agg[e];
-        a.d_counts[8] = s_excl[kKept] + agg[kKept];
+        __syncthreads();
     }
 }
 
@@ -669,9 +765,27 @@ struct StarLayout {
     uint32_t tbits = 4, iters = 1;
     LbPlan plan;
     size_t off_flags = 0, off_gsum = 0, off_gfirst = 0, off_table = 0, off_present = 0, off_combo_cid = 0,
-           off_cid_combo = 0, off_info = 0, off_map = 0, off_status = 0, total = 0;
+           off_cid_combo = 0, off_info = 0, off_map = 0, off_psum = 0, off_prank = 0, off_status = 0, total = 0;
     uint32_t map_cap = 0;
+    uint32_t grid = 0;  // persistent probe workgroups
 };
+
+// Persistent probe grid: as many workgroups as can be resident at once (registers and LDS decide,
+// queried from the runtime), never more than there are tiles.
+uint32_t probe_grid(uint32_t tiles) {
+    static uint32_t resident = 0;
+    if (resident == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_star_probe), kBlock, 0) !=
+                hipSuccess || per_cu <= 0)
+            per_cu = 2;
+        resident = static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu);
+    }
+    return tiles < resident ? tiles : resident;
+}
 
 StarLayout star_layout(const ahead_star1_args *in) {
     StarLayout L;
@@ -691,7 +805,10 @@ StarLayout star_layout(const ahead_star1_args *in) {
     L.off_info = L.off_cid_combo + align_up(kCombos * 2ull, 256);
     L.off_map = L.off_info + align_up(n_dim * 4ull + 4, 256);
     L.map_cap = static_cast<uint32_t>(4ull * n_dim + 1024);
-    L.off_status = L.off_map + align_up(L.map_cap * 2ull, 256);
+    L.grid = probe_grid(L.plan.num_tiles);
+    L.off_psum = L.off_map + align_up(L.map_cap * 2ull, 256);
+    L.off_prank = L.off_psum + align_up(static_cast<size_t>(kLdsCids) * L.grid * 8, 256);
+    L.off_status = L.off_prank + align_up(static_cast<size_t>(kLdsCids) * L.grid * 4, 256);
     const size_t chk = check_an_status_bytes(in->dim_key.n, in->dim_key.width);
     L.total = L.off_status + (L.plan.status_bytes > chk ? L.plan.status_bytes : chk);
     return L;
@@ -788,6 +905,8 @@ extern "C" int ahead_hip_star1_probe(const ahead_star1_args *in, const ahead_sta
     p.flags = reinterpret_cast<const uint32_t *>(ws_region(ws, L.off_flags));
     p.g_sum = reinterpret_cast<uint64_t *>(ws_region(ws, L.off_gsum));
     p.g_first = reinterpret_cast<uint32_t *>(ws_region(ws, L.off_gfirst));
+    p.part_sum = reinterpret_cast<unsigned long long *>(ws_region(ws, L.off_psum));
+    p.part_rank = reinterpret_cast<uint32_t *>(ws_region(ws, L.off_prank));
     p.a_oid_err = in->a_oid_err;
     p.err[0] = out->e_fk; p.cap[0] = out->e_fk ? out->e_fk_cap : 0;
     p.err[1] = out->e_a1; p.cap[1] = out->e_a1 ? out->e_a1_cap : 0;
@@ -795,7 +914,9 @@ extern "C" int ahead_hip_star1_probe(const ahead_star1_args *in, const ahead_sta
     p.err[3] = out->e_m; p.cap[3] = out->e_m ? out->e_m_cap : 0;
     p.d_counts = out->d_counts;
     if ((st = lb_args(ws, L.plan, L.off_status, &p.lb)) != AHEAD_OK) return st;
-    hipLaunchKernelGGL(k_star_probe, dim3(L.plan.num_tiles), dim3(kBlock), 0, s, p);
+    hipLaunchKernelGGL(k_star_probe, dim3(L.grid), dim3(kBlock), 0, s, p);
+    hipLaunchKernelGGL(k_star_reduce, dim3(256), dim3(kBlock), 0, s, p.part_sum, p.part_rank, L.grid, p.cid_combo,
+                       p.flags, p.g_sum, p.g_first);
     AHEAD_HIP_TRY(hipGetLastError());
 
     FinalArgs f{};

@@ -20,6 +20,10 @@ HBM.  Workloads:
                 predicates of selectivity 1-50%, 1e-6 flips; one persistent launch (autotuned
                 tiling and workgroup count, replayed from a hipGraph) against one launch per
                 instance.  A step is the whole batch.
+  ssb         : the four star-schema pipelines (Q1.1, Q2.1, Q3.1, Q4.1 operator chains) over a
+                600,000,000-row fact table with 4 foreign keys into date (2,556 rows), customer
+                (3M), supplier (200k) and part (1.4M), integer-coded attributes; one operator
+                launch after another, as the reference's drivers run them.  A step runs all four.
 
 The line printed carries the whole-job throughput (`value`), the dominant kernel's roofline
 (algorithmic bytes / its average duration from HIP events on the launch stream) and a CPU
@@ -47,7 +51,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="star1", choices=["star1", "select_sum", "batch"])
+    ap.add_argument("--workload", default="star1", choices=["star1", "select_sum", "batch", "ssb"])
     ap.add_argument("--rows", type=int, default=None, help="override the row count (scaling studies)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -404,6 +408,92 @@ def cpu_baseline_batch(spec, jobs, res_b, seconds):
                       f"bit-exact vs GPU: {ok}"}
 
 
+SSB_BYTES_PER_ROW = {"P1": 20, "P2": 32, "P3": 32, "P4": 48}  # fact-column bytes each pipeline reads
+
+
+def run_ssb(args):
+    import torch
+
+    from ahead_amd import ssb
+
+    spec = ssb.SsbSpec()
+    if args.rows:
+        spec.n_fact = args.rows
+    t = ssb.device_tables(spec)
+    torch.cuda.synchronize()
+    results = {}
+    for name, fn in ssb.PIPELINES.items():
+        results[name] = fn(t)
+    torch.cuda.synchronize()
+    per = {name: 0.0 for name in ssb.PIPELINES}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for name, fn in ssb.PIPELINES.items():
+            q0 = time.perf_counter()
+            r = fn(t)
+            torch.cuda.synchronize()
+            per[name] += time.perf_counter() - q0
+            if r["rows"] != results[name]["rows"]:
+                raise RuntimeError(f"{name}: result changed between runs")
+    el = time.perf_counter() - t0
+    nq = len(ssb.PIPELINES) * args.steps
+    algo = sum(SSB_BYTES_PER_ROW[k] for k in ssb.PIPELINES) * spec.n_fact * args.steps
+    achieved = algo / el / 1e9
+    summary = {k: {"ms": per[k] / args.steps * 1e3, "rows": results[k]["rows"],
+                   "groups": results[k].get("groups", 1), "errors": results[k]["errors"].total()}
+               for k in ssb.PIPELINES}
+    res = {
+        "metric": "hardened star-schema queries/s (Q1.1, Q2.1, Q3.1, Q4.1 operator chains)",
+        "value": nq / el,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16/u32/u64",
+        "data": "synthetic (seeded; fact columns generated on device)",
+        "config": {"workload": f"ssb: fact {spec.n_fact} rows, date {spec.n_date}, customer {spec.n_cust}, supplier "
+                               f"{spec.n_supp}, part {spec.n_part}; flips {spec.rate} in selected columns",
+                   "pipelines": summary, "fact_rows_per_s": spec.n_fact * nq / el},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "P1-P4 operator chains (every launch, host waits between operators)",
+                     "kernel_ms": el / nq * 1e3, "algorithmic_bytes": algo // max(1, nq)},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_ssb(spec, args.cpu_seconds)
+    return res
+
+
+def cpu_baseline_ssb(spec_full, seconds):
+    """The same four operator chains on the oracle's operators (tests/oracle_bat.py), single thread,
+    on a fact sample scaled down until one round fits the time budget."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import dataclasses
+
+    import oracle_bat as OB
+    from ahead_amd import ssb
+
+    n = min(spec_full.n_fact, 3_000_000)
+    spec = dataclasses.replace(spec_full, n_fact=n)
+    t = OB.tables(spec)
+    done, el = 0, 0.0
+    while el < seconds:
+        for fn in ssb.PIPELINES.values():
+            t0 = time.perf_counter()
+            fn(t, ops=OB)
+            el += time.perf_counter() - t0
+            done += 1
+    return {"value": done / el * (n / spec_full.n_fact), "unit": "queries/s (scaled to the full fact table)",
+            "cores": 1, "kind": "port",
+            "sample": f"{done} pipeline runs on a {n}-row fact sample ({el:.1f} s), single thread, oracle operators "
+                      f"(oracle/an_oracle.c -O3 -fno-tree-vectorize) chained as on the GPU; rate scaled by "
+                      f"{n}/{spec_full.n_fact} rows"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -414,7 +504,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs an MI355X (cuda:0)"
     torch.cuda.set_device(0)
-    run = {"star1": run_star1, "select_sum": run_select_sum, "batch": run_batch}[args.workload]
+    run = {"star1": run_star1, "select_sum": run_select_sum, "batch": run_batch, "ssb": run_ssb}[args.workload]
     res = run(args)
     print(json.dumps(res), flush=True)
 
