@@ -6,6 +6,7 @@ If the library is missing, loading raises instead of falling back to anything.
 from __future__ import annotations
 
 import ctypes
+import threading
 import os
 from dataclasses import dataclass
 from pathlib import Path
@@ -241,13 +242,18 @@ class Workspace:
             pass
 
 
-_ws: dict[int, Workspace] = {}
+_ws_local = threading.local()
 
 
 def workspace(device: int = 0) -> Workspace:
-    if device not in _ws:
-        _ws[device] = Workspace(device)
-    return _ws[device]
+    """The calling thread's workspace on `device` (a workspace serves one stream at a time, and each
+    thread issues its work on its own current stream)."""
+    ws = getattr(_ws_local, "ws", None)
+    if ws is None:
+        ws = _ws_local.ws = {}
+    if device not in ws:
+        ws[device] = Workspace(device)
+    return ws[device]
 
 
 def set_tiling(max_subtiles: int, min_tiles: int) -> None:

@@ -176,21 +176,21 @@ class Errors:
         return sum(v.size for _, v in self.items)
 
 
-def p1(t: dict, ops=B) -> dict:
+def p1(t: dict, ops=B, year: int = 1993, disc_lo: int = 1, disc_hi: int = 3, qty_lt: int = 25) -> dict:
     """Q1.1 (ssbm-q11_continuous.cpp:69-121)."""
     E = Errors()
     LQ, LD, LO, LE = t["lo_quantity"], t["lo_discount"], t["lo_orderdate"], t["lo_extendedprice"]
     DY, DD = t["d_year"], t["d_datekey"]
-    s1, e = ops.selectAN(LQ, ops.LT, 25 * LQ.tail.meta.a)
+    s1, e = ops.selectAN(LQ, ops.LT, qty_lt * LQ.tail.meta.a)
     E.add("select_quantity", e)
-    s2, e = ops.selectAN(LD, ops.GE, 1 * LD.tail.meta.a, ops.LE, 3 * LD.tail.meta.a, ops.AND)
+    s2, e = ops.selectAN(LD, ops.GE, disc_lo * LD.tail.meta.a, ops.LE, disc_hi * LD.tail.meta.a, ops.AND)
     E.add("select_discount", e)
     r4 = ops.matchjoinAN(s1.mirror_head(), s2.mirror_head())
     E.add("match_selections", *r4[1:])
     bat5 = r4[0].mirror_head()
     r6 = ops.matchjoinAN(bat5, LO)
     E.add("fetch_orderdate", *r6[1:])
-    s7, e = ops.selectAN(DY, ops.EQ, 1993 * DY.tail.meta.a)
+    s7, e = ops.selectAN(DY, ops.EQ, year * DY.tail.meta.a)
     E.add("select_year", e)
     r9 = ops.matchjoinAN(s7.mirror_head(), DD)
     E.add("fetch_datekey", *r9[1:])
@@ -242,15 +242,15 @@ def _groups(ops, E: Errors, attrs: list, measure) -> dict:
     return {"sums": sums, "keys": keys, "groups": int(g[1].size())}
 
 
-def p2(t: dict, ops=B) -> dict:
+def p2(t: dict, ops=B, category: int = 12, s_region: int = 1) -> dict:
     """Q2.1 (ssbm-q21_continuous.cpp:84-181)."""
     E = Errors()
-    supp = _select_dim(ops, t, E, "s_suppkey", "s_region", (ops.EQ, 1), "s_region")
+    supp = _select_dim(ops, t, E, "s_suppkey", "s_region", (ops.EQ, s_region), "s_region")
     r5 = ops.hashjoinAN(t["lo_suppkey"], supp, (A_OIDX, A_OIDX_INV), (supp.tail.meta.a, supp.tail.meta.a_inv))
     E.add("join_supplier", *r5[1:])
     r7 = ops.matchjoinAN(r5[0].mirror_head(), t["lo_partkey"])
     E.add("fetch_partkey", *r7[1:])
-    part = _select_dim(ops, t, E, "p_partkey", "p_category", (ops.EQ, 12), "p_category")
+    part = _select_dim(ops, t, E, "p_partkey", "p_category", (ops.EQ, category), "p_category")
     rC = ops.hashjoinAN(r7[0], part)
     E.add("join_part", *rC[1:])
     rF = ops.matchjoinAN(rC[0].mirror_head(), t["lo_orderdate"])
@@ -285,18 +285,18 @@ def _join_dim_oids(ops, t: dict, E: Errors, rows, fact_fk: str, dim_key: str, ta
     return j[0].clear_head()
 
 
-def p3(t: dict, ops=B) -> dict:
+def p3(t: dict, ops=B, c_region: int = 2, s_region: int = 2, year_lo: int = 1992, year_hi: int = 1997) -> dict:
     """Q3.1 (ssbm-q31_continuous.cpp:89-212)."""
     E = Errors()
-    supp = _select_dim(ops, t, E, "s_suppkey", "s_region", (ops.EQ, 2), "s_region")
+    supp = _select_dim(ops, t, E, "s_suppkey", "s_region", (ops.EQ, s_region), "s_region")
     r5 = ops.hashjoinAN(t["lo_suppkey"], supp, (A_OIDX, A_OIDX_INV), (supp.tail.meta.a, supp.tail.meta.a_inv))
     E.add("join_supplier", *r5[1:])
-    cust = _select_dim(ops, t, E, "c_custkey", "c_region", (ops.EQ, 2), "c_region")
+    cust = _select_dim(ops, t, E, "c_custkey", "c_region", (ops.EQ, c_region), "c_region")
     r11 = ops.matchjoinAN(r5[0].mirror_head(), t["lo_custkey"])
     E.add("fetch_custkey", *r11[1:])
     r12 = ops.hashjoinAN(r11[0], cust)
     E.add("join_customer", *r12[1:])
-    date = _select_dim(ops, t, E, "d_datekey", "d_year", (ops.GE, 1992, ops.LE, 1997, ops.AND), "d_year")
+    date = _select_dim(ops, t, E, "d_datekey", "d_year", (ops.GE, year_lo, ops.LE, year_hi, ops.AND), "d_year")
     r18 = ops.matchjoinAN(r12[0].mirror_head(), t["lo_orderdate"])
     E.add("fetch_orderdate", *r18[1:])
     r19 = ops.hashjoinAN(r18[0], date)
@@ -318,19 +318,19 @@ def p3(t: dict, ops=B) -> dict:
     return g
 
 
-def p4(t: dict, ops=B) -> dict:
+def p4(t: dict, ops=B, c_region: int = 1, s_region: int = 1, mfgr: tuple = (0, 1)) -> dict:
     """Q4.1 (ssbm-q41_continuous.cpp:98-212)."""
     E = Errors()
-    part = _select_dim(ops, t, E, "p_partkey", "p_mfgr", (ops.EQ, 0, ops.EQ, 1, ops.OR), "p_mfgr")
+    part = _select_dim(ops, t, E, "p_partkey", "p_mfgr", (ops.EQ, mfgr[0], ops.EQ, mfgr[1], ops.OR), "p_mfgr")
     r5 = ops.hashjoinAN(t["lo_partkey"], part, (A_OIDX, A_OIDX_INV), (part.tail.meta.a, part.tail.meta.a_inv))
     E.add("join_part", *r5[1:])
     bat6 = r5[0].mirror_head()
-    supp = _select_dim(ops, t, E, "s_suppkey", "s_region", (ops.EQ, 1), "s_region")
+    supp = _select_dim(ops, t, E, "s_suppkey", "s_region", (ops.EQ, s_region), "s_region")
     r11 = ops.matchjoinAN(bat6, t["lo_suppkey"])
     E.add("fetch_suppkey", *r11[1:])
     r12 = ops.hashjoinAN(r11[0], supp)
     E.add("join_supplier", *r12[1:])
-    cust = _select_dim(ops, t, E, "c_custkey", "c_region", (ops.EQ, 1), "c_region")
+    cust = _select_dim(ops, t, E, "c_custkey", "c_region", (ops.EQ, c_region), "c_region")
     r18 = ops.matchjoinAN(r12[0].mirror_head(), t["lo_custkey"])
     E.add("fetch_custkey", *r18[1:])
     r19 = ops.hashjoinAN(r18[0], cust)
@@ -360,27 +360,33 @@ PIPELINES = {"P1": p1, "P2": p2, "P3": p3, "P4": p4}
 # textbook evaluation of the same queries on decoded host values (fault-free data), for tests
 # ------------------------------------------------------------------------------------------------
 
-def textbook(spec: SsbSpec, name: str) -> dict:
+def textbook(spec: SsbSpec, name: str, **q) -> dict:
     cache = {}
     v = lambda c: host_values(spec, c, cache)  # noqa: E731
     if name == "P1":
         year = v("d_year")[v("lo_orderdate").astype(np.int64)]
-        q, d = v("lo_quantity"), v("lo_discount")
-        m = (year == 1993) & (d >= 1) & (d <= 3) & (q < 25)
+        qty, d = v("lo_quantity"), v("lo_discount")
+        m = (year == q.get("year", 1993)) & (d >= q.get("disc_lo", 1)) & (d <= q.get("disc_hi", 3)) & \
+            (qty < q.get("qty_lt", 25))
         return {"result": int((v("lo_extendedprice")[m] * d[m]).sum()) & an.MASK64, "rows": int(m.sum())}
     fk = lambda c: v(c).astype(np.int64)  # noqa: E731
     if name == "P2":
-        m = (v("s_region")[fk("lo_suppkey")] == 1) & (v("p_category")[fk("lo_partkey")] == 12)
+        m = (v("s_region")[fk("lo_suppkey")] == q.get("s_region", 1)) & \
+            (v("p_category")[fk("lo_partkey")] == q.get("category", 12))
         keys = [v("d_year")[fk("lo_orderdate")][m], v("p_brand")[fk("lo_partkey")][m]]
         meas = v("lo_revenue")[m]
     elif name == "P3":
         yr = v("d_year")[fk("lo_orderdate")]
-        m = (v("c_region")[fk("lo_custkey")] == 2) & (v("s_region")[fk("lo_suppkey")] == 2) & (yr >= 1992) & (yr <= 1997)
+        m = (v("c_region")[fk("lo_custkey")] == q.get("c_region", 2)) & \
+            (v("s_region")[fk("lo_suppkey")] == q.get("s_region", 2)) & (yr >= q.get("year_lo", 1992)) & \
+            (yr <= q.get("year_hi", 1997))
         keys = [yr[m], v("s_nation")[fk("lo_suppkey")][m], v("c_nation")[fk("lo_custkey")][m]]
         meas = v("lo_revenue")[m]
     else:
         mf = v("p_mfgr")[fk("lo_partkey")]
-        m = (v("c_region")[fk("lo_custkey")] == 1) & (v("s_region")[fk("lo_suppkey")] == 1) & ((mf == 0) | (mf == 1))
+        mp = q.get("mfgr", (0, 1))
+        m = (v("c_region")[fk("lo_custkey")] == q.get("c_region", 1)) & \
+            (v("s_region")[fk("lo_suppkey")] == q.get("s_region", 1)) & ((mf == mp[0]) | (mf == mp[1]))
         keys = [v("d_year")[fk("lo_orderdate")][m], v("c_nation")[fk("lo_custkey")][m]]
         meas = (v("lo_revenue")[m].astype(np.int64) - v("lo_supplycost")[m].astype(np.int64)).astype(np.uint64)
     # groups in first-occurrence order of the (key tuple) over the qualifying rows in fact order

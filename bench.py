@@ -24,6 +24,10 @@ HBM.  Workloads:
                 600,000,000-row fact table with 4 foreign keys into date (2,556 rows), customer
                 (3M), supplier (200k) and part (1.4M), integer-coded attributes; one operator
                 launch after another, as the reference's drivers run them.  A step runs all four.
+  serve       : the same schema resident in HBM, queries of the four shapes with seeded random
+                predicate constants arriving on Poisson clocks; continuous batching on 4 GPU
+                slots (own streams, admission by HBM) against the unbatched path (one query at a
+                time); reports queries/s with p50/p99 latency per arrival rate.
 
 The line printed carries the whole-job throughput (`value`), the dominant kernel's roofline
 (algorithmic bytes / its average duration from HIP events on the launch stream) and a CPU
@@ -51,7 +55,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="star1", choices=["star1", "select_sum", "batch", "ssb"])
+    ap.add_argument("--workload", default="star1", choices=["star1", "select_sum", "batch", "ssb", "serve"])
     ap.add_argument("--rows", type=int, default=None, help="override the row count (scaling studies)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -408,6 +412,60 @@ def cpu_baseline_batch(spec, jobs, res_b, seconds):
                       f"bit-exact vs GPU: {ok}"}
 
 
+def run_serve(args):
+    import torch
+
+    from ahead_amd import serve, ssb
+
+    spec = ssb.SsbSpec()
+    if args.rows:
+        spec.n_fact = args.rows
+    tables = ssb.device_tables(spec)
+    torch.cuda.synchronize()
+    n_q = max(8, args.steps * 8)
+    # warm up every shape once
+    serve.Server(tables, spec.n_fact, workers=1).run(serve.random_queries(8, 0, 99), realtime=False)
+    # capacity of each form: everything arrives at once
+    sat = {}
+    for w in (1, 4):
+        qs = serve.Server(tables, spec.n_fact, workers=w).run(serve.random_queries(n_q, 0, 7), realtime=False)
+        sat[w] = serve.summarize(qs)
+    # Poisson arrivals at fractions of the unbatched capacity
+    sweep = []
+    base = sat[1]["qps"]
+    for frac in (0.5, 0.9, 1.5):
+        rate = base * frac
+        row = {"rate_qps": rate}
+        for w in (1, 4):
+            qs = serve.Server(tables, spec.n_fact, workers=w).run(serve.random_queries(n_q, rate, 11), realtime=True)
+            row[f"workers{w}"] = serve.summarize(qs)
+        sweep.append(row)
+    t0 = time.perf_counter()
+    final = serve.Server(tables, spec.n_fact, workers=4).run(serve.random_queries(n_q, 0, 13), realtime=False)
+    el = time.perf_counter() - t0
+    s4 = serve.summarize(final)
+    return {
+        "metric": "served star-schema queries/s (continuous batching, 4 slots) with p50/p99 latency",
+        "value": s4["qps"],
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": el / max(1, args.steps) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16/u32/u64",
+        "data": "synthetic (seeded; fact columns generated on device)",
+        "config": {"workload": f"serve: ssb schema with {spec.n_fact} fact rows resident in HBM; {n_q} queries "
+                               f"per run, shapes P1-P4 with seeded random predicates",
+                   "saturated": {"unbatched": sat[1], "batched_4_slots": sat[4]},
+                   "speedup_saturated": sat[4]["qps"] / sat[1]["qps"] if sat[1]["qps"] else None,
+                   "poisson_sweep": sweep, "final_batched": s4},
+        "roofline": None,
+    }
+
+
 SSB_BYTES_PER_ROW = {"P1": 20, "P2": 32, "P3": 32, "P4": 48}  # fact-column bytes each pipeline reads
 
 
@@ -504,7 +562,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs an MI355X (cuda:0)"
     torch.cuda.set_device(0)
-    run = {"star1": run_star1, "select_sum": run_select_sum, "batch": run_batch, "ssb": run_ssb}[args.workload]
+    run = {"star1": run_star1, "select_sum": run_select_sum, "batch": run_batch, "ssb": run_ssb,
+           "serve": run_serve}[args.workload]
     res = run(args)
     print(json.dumps(res), flush=True)
 

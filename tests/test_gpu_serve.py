@@ -1,0 +1,36 @@
+"""Served queries (ahead_amd.serve): every query's result, error vectors included, is the same when
+it runs alone (one query on the GPU at a time) and in the mix (four queries overlapping on four
+streams, all arriving at once)."""
+import pytest
+
+from ahead_amd import serve, ssb
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(n_fact=250_000, n_date=2556, n_cust=3_000, n_supp=200, n_part=1_400)
+
+
+def test_results_alone_equal_results_in_the_mix():
+    spec = ssb.SsbSpec(rate=2e-3, seed=21, **SMALL)
+    tables = ssb.device_tables(spec)
+    alone = serve.Server(tables, spec.n_fact, workers=1).run(serve.random_queries(24, 0, 5), realtime=False)
+    mixed = serve.Server(tables, spec.n_fact, workers=4).run(serve.random_queries(24, 0, 5), realtime=False)
+    shapes = set()
+    for a, m in zip(alone, mixed):
+        assert (a.qid, a.shape, a.params) == (m.qid, m.shape, m.params)
+        assert serve.same_result(a.result, m.result), (a.qid, a.shape, a.params)
+        shapes.add(a.shape)
+    assert shapes == {"P1", "P2", "P3", "P4"}
+    s = serve.summarize(mixed)
+    assert s["queries"] == 24 and s["p99_ms"] >= s["p50_ms"] > 0
+
+
+def test_admission_budget_serializes_when_memory_is_short():
+    spec = ssb.SsbSpec(rate=0.0, seed=22, **SMALL)
+    tables = ssb.device_tables(spec)
+    # a budget that fits one query at a time: queries must not overlap
+    srv = serve.Server(tables, spec.n_fact, workers=4, hbm_budget=28 * spec.n_fact)
+    qs = srv.run(serve.random_queries(8, 0, 6), realtime=False)
+    spans = sorted((q.start, q.finish) for q in qs)
+    for (s0, f0), (s1, f1) in zip(spans, spans[1:]):
+        assert s1 >= f0 - 1e-4
