@@ -161,8 +161,21 @@ def matchjoinAN(b1: BAT, b2: BAT, reenc_h: tuple[int, int] | None = None, reenc_
     rt = reencode and t2.meta.type.encoded
     d = [x.desc() for x in (h1, t1, h2, t2)]
     positional = t1.meta.type.name == "resoid" and h2.buf is None
-    if not positional and (t1.buf is None and h2.buf is None):
-        raise NotImplementedError("matchjoinAN <H1, void, void, T2> is not on the device path")
+    if t1.buf is None and h2.buf is None:  # <H1, void, void, T2>: seqbase alignment
+        r = _Runner(3)
+        cap = max(min(t1.n, h2.n), 1)
+        caps = [[th.width, cap], [tt.width, cap], [8, ERR_CAP], [8, ERR_CAP]]
+
+        def call(b, cnt):
+            return r.lib.ahead_hip_matchjoin_seq_an(*[ctypes.byref(x) for x in d], int(rh), f_h, th.width, int(rt),
+                                                    f_t, tt.width, a_oid, *[_b(x, c[1]) for x, c in zip(b, caps)],
+                                                    cnt, r.ws.h, current_stream())
+
+        bufs, c = r.run(call, caps, [0, 0, 1, 2])
+        mh = an.ColMeta(th, reenc_h[0], reenc_h[1]) if reencode else an.ColMeta(th, h1.meta.a, h1.meta.a_inv)
+        mt = an.ColMeta(tt, a_t, reenc_t[1] if reenc_t else 0) if reencode else an.ColMeta(tt, t2.meta.a, t2.meta.a_inv)
+        e = _errs(bufs[2:], c[1:3], [h1.meta.type.encoded, t2.meta.type.encoded])
+        return BAT(Column(bufs[0], c[0], mh), Column(bufs[1], c[0], mt)), e[0], None, None, e[1]
     r = _Runner(5 if not positional else 4)
     cap = max(t1.n if positional else min(t1.n, h2.n), 1)
     caps = [[th.width, cap], [tt.width, cap], [8, ERR_CAP], [8, ERR_CAP], [8, ERR_CAP], [8, ERR_CAP]]

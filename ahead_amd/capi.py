@@ -100,6 +100,10 @@ _SIGS = {
                                     Buf, Buf, _vp, _vp, _vp], _int),
     "ahead_hip_matchjoin_merge_an": ([_colp, _colp, _colp, _colp, _int, _u64, _u32, _int, _u64, _u32, _u64, Buf, Buf,
                                       Buf, Buf, Buf, Buf, _vp, _vp, _vp], _int),
+    "ahead_hip_matchjoin_seq_an": ([_colp, _colp, _colp, _colp, _int, _u64, _u32, _int, _u64, _u32, _u64, Buf, Buf,
+                                    Buf, Buf, _vp, _vp, _vp], _int),
+    "ahead_hip_check_and_decode2_an": ([_colp, _colp, _u64, _u32, _u32, _vp, _vp, _u64, Buf, Buf, _vp, _vp, _vp],
+                                       _int),
     "ahead_hip_hashjoin_an": ([_colp, _colp, _colp, _colp, _int, _int, _u64, _u32, _int, _u64, _u32, _u64, Buf, Buf,
                                Buf, Buf, Buf, Buf, _vp, _vp, _vp], _int),
     "ahead_hip_groupby_an": ([_colp, _colp, _u64, _u32, _vp, Buf, Buf, Buf, _vp, _vp, _vp], _int),
@@ -605,6 +609,42 @@ def matchjoin_merge_an(h1: Column, t1: Column, h2: Column, t2: Column, reenc_h=0
     b, c = _run(call, outs, 5, [0, 0, 1, 2, 3, 4])
     return (b[0].to_numpy(np.dtype(f"u{wout_h}"), c[0]), b[1].to_numpy(np.dtype(f"u{wout_t}"), c[0]),
             *[b[2 + k].to_numpy(np.uint64, c[1 + k]) for k in range(4)])
+
+
+def matchjoin_seq_an(h1: Column, t1: Column, h2: Column, t2: Column, reenc_h=0, f_h1=0, wout_h=8, reenc_t=0,
+                     f_t2=0, wout_t=8, a_oid: int = an.A_OID_ERR, err_cap: int = 1024):
+    lib, ws = load(), workspace()
+    cap = max(min(t1.n, h2.n), 1)
+    outs = [[wout_h, cap], [wout_t, cap], [8, err_cap], [8, err_cap]]
+    d = [x.desc() for x in (h1, t1, h2, t2)]
+
+    def call(b, cnt):
+        check(lib.ahead_hip_matchjoin_seq_an(*[ctypes.byref(x) for x in d], reenc_h, f_h1, wout_h, reenc_t, f_t2,
+                                             wout_t, a_oid, *[_b(x, o[1]) for x, o in zip(b, outs)], cnt.ptr, ws.h,
+                                             current_stream()))
+
+    b, c = _run(call, outs, 3, [0, 0, 1, 2])
+    return (b[0].to_numpy(np.dtype(f"u{wout_h}"), c[0]), b[1].to_numpy(np.dtype(f"u{wout_t}"), c[0]),
+            b[2].to_numpy(np.uint64, c[1]), b[3].to_numpy(np.uint64, c[2]))
+
+
+def check_and_decode2_an(head: Column, tail: Column, a_oid: int = an.A_OID_ERR, err_cap: int = 1024):
+    """checkAndDecodeAN<Head, Tail>: (decoded heads, decoded tails, head errors, tail errors)."""
+    lib, ws = load(), workspace()
+    wh = head.meta.type.unenc_bits // 8 if head.meta.type.encoded else head.meta.width
+    wt = tail.meta.type.unenc_bits // 8 if tail.meta.type.encoded else tail.meta.width
+    n = tail.n
+    outs = [[wh, max(n, 1)], [wt, max(n, 1)], [8, err_cap], [8, err_cap]]
+    dh, dt = head.desc(), tail.desc()
+
+    def call(b, cnt):
+        check(lib.ahead_hip_check_and_decode2_an(ctypes.byref(dh), ctypes.byref(dt), a_oid, wh, wt, b[0].ptr, b[1].ptr,
+                                                 outs[0][1], _b(b[2], outs[2][1]), _b(b[3], outs[3][1]), cnt.ptr,
+                                                 ws.h, current_stream()))
+
+    b, c = _run(call, outs, 3, [0, 0, 1, 2])
+    return (b[0].to_numpy(np.dtype(f"u{wh}"), c[0]), b[1].to_numpy(np.dtype(f"u{wt}"), c[0]),
+            b[2].to_numpy(np.uint64, c[1]), b[3].to_numpy(np.uint64, c[2]))
 
 
 def hashjoin_an(h1: Column, t1: Column, h2: Column, t2: Column, build_right=1, reenc_h=0, f_h1=0, wout_h=8,

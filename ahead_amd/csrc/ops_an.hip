@@ -4,8 +4,9 @@
 //   aggregate_sum_groupedAN     (aggregateAN.tcc:41-119)
 //   aggregate_mul_sumAN         (aggregateAN_scalar.tcc:58-131)
 //   arithmeticAN                (arithmeticAN_scalar.tcc:60-112)
-// Every check applies the code-width cast (static_cast<T>(code * A^-1)); see DESIGN.md's
-// divergence list for the few reference lines that omit it for 16-bit codes.
+// Every check applies the code-width cast (static_cast<T>(code * A^-1)).  A few reference lines
+// compare the int-promoted product of 16-bit codes without it (matchjoinAN.tcc:278 and :342,
+// hashjoinAN.tcc:153); no query path feeds 16-bit codes to them.
 #include "rowtile.hpp"
 
 using namespace ahead_dev;

@@ -283,21 +283,39 @@ namespace ahead {
                 template<typename Head, typename Tail>
                 std::tuple<BAT<typename Head::v2_unenc_t, typename Tail::v2_unenc_t> *, AN_indicator_vector *, AN_indicator_vector *>
                 checkAndDecodeAN(BAT<Head, Tail> *arg, resoid_t AOID = std::get<v2_resoid_t::AsBFW->size() - 1>(*v2_resoid_t::AsBFW)) {
-                    static_assert(device::is_void<Head>, "checkAndDecodeAN: <void, T> BATs only on the device path");
-                    typedef typename Tail::v2_unenc_t Unenc;
+                    typedef typename Head::v2_unenc_t UH;
+                    typedef typename Tail::v2_unenc_t UT;
                     const uint64_t n = device::rows(arg);
                     const ahead_col c = device::col(arg->tail, n);
-                    auto *r = new TempBAT<v2_void_t, Unenc>(ColumnDescriptor<v2_void_t>(arg->head.metaData), ColumnDescriptor<Unenc>(default_metadata<Unenc>()));
-                    void *dec = r->tail.container->overwrite(n);
                     auto & cx = device::Context::get();
                     const uint64_t *k = nullptr;
-                    do {
-                        device::raise(ahead_hip_check_and_decode_an(&c, AOID, dec, n, cx.err[0].data(), cx.err_cap, cx.d_counts(), cx.ws, cx.stream));
-                        k = cx.read_counts(2);
-                    } while (!cx.errors_fit( {k[1]}));
-                    const uint64_t n_dec = k[0], n_err = k[1];
-                    r->tail.container->resize(n_dec);
-                    return std::make_tuple(r, nullptr, cx.take(0, n_err, device::is_enc<Tail>));
+                    if constexpr (device::is_void<Head>) {
+                        // encdecAN_scalar.tcc:185-210: corrupt rows dropped
+                        auto *r = new TempBAT<v2_void_t, UT>(ColumnDescriptor<v2_void_t>(arg->head.metaData), ColumnDescriptor<UT>(default_metadata<UT>()));
+                        void *dec = r->tail.container->overwrite(n);
+                        do {
+                            device::raise(ahead_hip_check_and_decode_an(&c, AOID, dec, n, cx.err[0].data(), cx.err_cap, cx.d_counts(), cx.ws, cx.stream));
+                            k = cx.read_counts(2);
+                        } while (!cx.errors_fit( {k[1]}));
+                        const uint64_t n_dec = k[0], n_err = k[1];
+                        r->tail.container->resize(n_dec);
+                        return std::make_tuple(r, nullptr, cx.take(0, n_err, device::is_enc<Tail>));
+                    } else {
+                        // encdecAN_scalar.tcc:129-176: pairs kept only if both sides are valid
+                        const ahead_col h = device::col(arg->head, n);
+                        auto *r = new TempBAT<UH, UT>(ColumnDescriptor<UH>(default_metadata<UH>()), ColumnDescriptor<UT>(default_metadata<UT>()));
+                        void *dh = r->head.container->overwrite(n);
+                        void *dt = r->tail.container->overwrite(n);
+                        do {
+                            device::raise(ahead_hip_check_and_decode2_an(&h, &c, AOID, sizeof(typename UH::type_t), sizeof(typename UT::type_t), dh, dt, n,
+                                    cx.errbuf(0), cx.errbuf(1), cx.d_counts(), cx.ws, cx.stream));
+                            k = cx.read_counts(3);
+                        } while (!cx.errors_fit( {k[1], k[2]}));
+                        const uint64_t np = k[0], e1 = k[1], e2 = k[2];
+                        r->head.container->resize(np);
+                        r->tail.container->resize(np);
+                        return std::make_tuple(r, cx.take(0, e1, device::is_enc<Head>), cx.take(1, e2, device::is_enc<Tail>));
+                    }
                 }
 
                 template<template<typename > class Op, typename Head, typename Tail>
@@ -459,7 +477,20 @@ namespace ahead {
                         r->tail.container->resize(np);
                         return std::make_tuple(r, cx.take(0, e1, is_enc<H1>), cx.take(1, e2, is_enc<T1>), cx.take(2, e3, is_enc<H2>), cx.take(3, e4, is_enc<T2>));
                     } else {
-                        static_assert(dependent_false<T1>::value, "matchjoinAN: the <H1, void, void, T2> form is not on the device path");
+                        // <H1, void, void, T2>: virtual OIDs aligned by seqbase (matchjoinAN.tcc:146-224)
+                        const uint64_t cap = std::max<uint64_t>(std::min(n1, n2), 1);
+                        void *oh = r->head.container->overwrite(cap);
+                        void *ot = r->tail.container->overwrite(cap);
+                        const uint64_t *k = nullptr;
+                        do {
+                            raise(ahead_hip_matchjoin_seq_an(&h1, &t1, &h2, &t2, reenc_h, f_h1, sizeof(typename RH::type_t), reenc_t, f_t2, sizeof(typename RT::type_t),
+                                    AOID, ahead_buf {oh, cap}, ahead_buf {ot, cap}, cx.errbuf(0), cx.errbuf(3), cx.d_counts(), cx.ws, cx.stream));
+                            k = cx.read_counts(3);
+                        } while (!cx.errors_fit( {k[1], k[2]}));
+                        const uint64_t np = k[0], e1 = k[1], e4 = k[2];
+                        r->head.container->resize(np);
+                        r->tail.container->resize(np);
+                        return std::make_tuple(r, cx.take(0, e1, is_enc<H1>), nullptr, nullptr, cx.take(3, e4, is_enc<T2>));
                     }
                 }
 

@@ -144,6 +144,15 @@ int ahead_hip_check_and_decode_an(const ahead_col *col, uint64_t a_oid, void *de
                                   uint64_t dec_cap, uint64_t *err, uint64_t err_cap,
                                   uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
 
+/* checkAndDecodeAN<Head, Tail> with a non-void head (OperatorsAN.hpp:157-161;
+ * encdecAN_scalar.tcc:129-176): pairs decoded to wout_h / wout_t bytes, kept only if both sides
+ * are valid; err_h / err_t hold the positions of corrupt heads / tails.
+ * d_counts[0..2] = #pairs, #err_h, #err_t.                                                    */
+int ahead_hip_check_and_decode2_an(const ahead_col *head, const ahead_col *tail, uint64_t a_oid,
+                                   uint32_t wout_h, uint32_t wout_t, void *dec_h, void *dec_t,
+                                   uint64_t dec_cap, ahead_buf err_h, ahead_buf err_t,
+                                   uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
+
 /* ---- joins -----------------------------------------------------------------------------------
  * fetchjoinAN (OperatorsAN.hpp:92-106; matchjoinAN.tcc:312-353).  oids: tail of a
  * BAT<void, v2_resoid_t>; col: tail of BAT<void, T2>.  out: kept codes (TypeMap<T2>::
@@ -176,6 +185,17 @@ int ahead_hip_matchjoin_merge_an(const ahead_col *h1, const ahead_col *t1, const
                                  ahead_buf out_h, ahead_buf out_t, ahead_buf e1, ahead_buf e2,
                                  ahead_buf e3, ahead_buf e4, uint64_t *d_counts, ahead_hip_ws *ws,
                                  void *stream);
+
+/* matchjoinAN, form <H1, v2_void_t, v2_void_t, T2> (OperatorsAN.hpp:33-58;
+ * matchjoinAN.tcc:146-224): the virtual OIDs of list 1's tail (t1->seqbase + i) and list 2's
+ * head (h2->seqbase + j) are aligned by skipping the leading rows of the side that starts lower,
+ * then rows are paired one to one.  e1 checks h1 over the rows visited on list 1, e4 checks t2
+ * over those visited on list 2.  d_counts[0..2] = #pairs, #e1, #e4.                         */
+int ahead_hip_matchjoin_seq_an(const ahead_col *h1, const ahead_col *t1, const ahead_col *h2,
+                               const ahead_col *t2, int reenc_h, uint64_t f_h1, uint32_t wout_h,
+                               int reenc_t, uint64_t f_t2, uint32_t wout_t, uint64_t a_oid,
+                               ahead_buf out_h, ahead_buf out_t, ahead_buf e1, ahead_buf e4,
+                               uint64_t *d_counts, ahead_hip_ws *ws, void *stream);
 
 /* hashjoinAN (OperatorsAN.hpp:60-90; hashjoinAN.tcc:77-222, :552-585) with the unencoded-key
  * hash map.  build_right != 0 is hash_side_t::right (the default).  Pairs (h1, t2) in probe

@@ -337,6 +337,30 @@ void or_matchjoin_merge_an(const or_col *h1, const or_col *t1, const or_col *h2,
     }
 }
 
+void or_matchjoin_seq_an(const or_col *h1, const or_col *t1, const or_col *h2, const or_col *t2,
+                         int reenc_h, uint64_t f_h1, int wout_h, int reenc_t, uint64_t f_t2, int wout_t,
+                         uint64_t a_oid, or_vec *out_h, or_vec *out_t, or_vec *err1, or_vec *err4) {
+    const uint64_t n1 = t1->n, n2 = t2->n;
+    const uint64_t mh = or_mask(wout_h), mt = or_mask(wout_t);
+    uint64_t p1 = 0, p2 = 0;
+    if (n1 == 0 || n2 == 0) return; /* matchjoinAN.tcc:187 */
+    const uint32_t sb1 = (uint32_t)t1->seqbase, sb2 = (uint32_t)h2->seqbase;
+    if (sb1 < sb2) { /* :188-194: skip list-1 rows whose virtual OID is below list 2's first */
+        for (; p1 < n1 && p2 < n2 && (uint32_t)(sb1 + p1) < sb2; ++p1)
+            if (or_is_corrupt(h1, p1)) vec_push(err1, p1 * a_oid);
+    } else if (sb1 > sb2) { /* :195-201 */
+        for (; p1 < n1 && p2 < n2 && (uint32_t)(sb2 + p2) < sb1; ++p2)
+            if (or_is_corrupt(t2, p2)) vec_push(err4, p2 * a_oid);
+    }
+    for (; p1 < n1 && p2 < n2; ++p1, ++p2) { /* :203-215 */
+        if (or_is_corrupt(h1, p1)) vec_push(err1, p1 * a_oid);
+        if (or_is_corrupt(t2, p2)) vec_push(err4, p2 * a_oid);
+        const uint64_t h = or_get(h1, p1), t = or_get(t2, p2);
+        vec_push(out_h, reenc_h && h1->encoded ? (h * f_h1) & mh : h);
+        vec_push(out_t, reenc_t && t2->encoded ? (t * f_t2) & mt : t);
+    }
+}
+
 /* ---------------------------------------------------------------------------------------------- */
 /* hashjoinAN (unencoded hash map)                                                                 */
 /* ---------------------------------------------------------------------------------------------- */

@@ -106,6 +106,13 @@ void or_matchjoin_merge_an(const or_col *h1, const or_col *t1, const or_col *h2,
                            int wout_t, uint64_t a_oid, or_vec *out_h, or_vec *out_t,
                            or_vec *err1, or_vec *err2, or_vec *err3, or_vec *err4);
 
+/* matchjoinAN <H1, void, void, T2> (matchjoinAN.tcc:146-224): align the virtual OIDs
+ * t1->seqbase + i and h2->seqbase + j by skipping leading rows of the lower side (checking h1 resp.
+ * t2 on the skipped rows), then pair the rest one to one, checking both sides. */
+void or_matchjoin_seq_an(const or_col *h1, const or_col *t1, const or_col *h2, const or_col *t2,
+                         int reenc_h, uint64_t f_h1, int wout_h, int reenc_t, uint64_t f_t2, int wout_t,
+                         uint64_t a_oid, or_vec *out_h, or_vec *out_t, or_vec *err1, or_vec *err4);
+
 /* hashjoinAN with an unencoded-key hash map (hashjoinAN.tcc:77-222).  build_right != 0 is
  * hash_side_t::right (the default).  reenc_h / reenc_t enable the re-encoding variants
  * (:561-585): outputs h1 * f_h1 and t2 * f_t2 with f = mulIfEncoded(A_R, A_inv).

@@ -370,6 +370,42 @@ void test_star_chain() {
     delete da;
 }
 
+// checkAndDecodeAN on a BAT with an encoded head (encdecAN_scalar.tcc:129-176) and the seqbase-
+// aligned matchjoinAN <H1, void, void, T2> (matchjoinAN.tcc:146-224).
+void test_general_forms() {
+    const size_t n = 100003;
+    auto *a = column<v2_resint_t>(uniform<uint64_t>(n, 31, 0, 1000000, 65117, 1e-3), 65117);
+    auto *b = column<v2_resshort_t>(uniform<uint32_t>(n, 32, 0, 60000, 233, 1e-3), 233);
+    // <resint, void> from reverse, then mirror a <resint, resshort> BAT by pairing the two columns
+    auto pr = matchjoinAN(a->reverse(), b);    // <H1 = resint, void> x <void, resshort>: seqbase form
+    or_col ca = ocol_of(a->tail, n), cb = ocol_of(b->tail, n), v1 {}, v2 {};
+    v1.n = v2.n = n;
+    v1.width = v2.width = 4;
+    v1.unenc_bits = v2.unenc_bits = 32;
+    v1.a = v1.a_inv = v2.a = v2.a_inv = 1;
+    v1.max_u = v2.max_u = 0xFFFFFFFFull;
+    Vec mh, mt, m1, m4;
+    or_matchjoin_seq_an(&ca, &v1, &v2, &cb, 0, 0, 8, 0, 0, 4, 16089, &mh, &mt, &m1, &m4);
+    expect_eq("matchjoinAN <H1, void, void, T2> heads", head_of(std::get<0>(pr)), mh.v);
+    expect_eq("matchjoinAN <H1, void, void, T2> tails", tail_of(std::get<0>(pr)), mt.v);
+    expect_vec("matchjoinAN <H1, void, void, T2> e1", std::get<1>(pr), m1.v);
+    expect_vec("matchjoinAN <H1, void, void, T2> e4", std::get<4>(pr), m4.v);
+    auto cd = checkAndDecodeAN(std::get<0>(pr));
+    or_col ph = ocol_of(std::get<0>(pr)->head, 0), pt = ocol_of(std::get<0>(pr)->tail, 0);
+    Vec dh, dt, eh, et;
+    or_check_and_decode_an2(&ph, &pt, 16089, &dh, &dt, &eh, &et);
+    expect_eq("checkAndDecodeAN<resint, resshort> heads", head_of(std::get<0>(cd)), dh.v);
+    expect_eq("checkAndDecodeAN<resint, resshort> tails", tail_of(std::get<0>(cd)), dt.v);
+    expect_vec("checkAndDecodeAN<resint, resshort> head errors", std::get<1>(cd), eh.v);
+    expect_vec("checkAndDecodeAN<resint, resshort> tail errors", std::get<2>(cd), et.v);
+    delete std::get<0>(cd);
+    delete std::get<2>(pr);
+    delete std::get<3>(pr);
+    delete std::get<0>(pr);
+    delete a;
+    delete b;
+}
+
 }  // namespace
 
 int main() {
@@ -381,6 +417,7 @@ int main() {
     test_select_codec();
     test_q11_chain();
     test_star_chain();
+    test_general_forms();
     std::printf("%s: %d failure(s)\n", g_failures ? "FAILED" : "PASSED", g_failures);
     return g_failures ? 1 : 0;
 }

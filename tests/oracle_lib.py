@@ -69,6 +69,7 @@ def lib():
             "or_fetchjoin_an": ([c, c, i, u64, u64, v, v, v], None),
             "or_matchjoin_pos_an": ([c, c, c, i, u64, i, i, u64, i, u64, v, v, v, v, v], None),
             "or_matchjoin_merge_an": ([c, c, c, c, i, u64, i, i, u64, i, u64, v, v, v, v, v, v], None),
+            "or_matchjoin_seq_an": ([c, c, c, c, i, u64, i, i, u64, i, u64, v, v, v, v], None),
             "or_hashjoin_an": ([c, c, c, c, i, i, u64, i, i, u64, i, u64, v, v, v, v, v, v], None),
             "or_groupby_an": ([c, c, u64, v, v, v, v], None),
             "or_groupby2_an": ([c, c, c, u64, i, u64, v, v, v, v, v], i),
@@ -187,6 +188,14 @@ def matchjoin_merge_an(h1: Col, t1: Col, h2: Col, t2: Col, reenc_h=0, f_h1=0, wo
     vs = Vecs(6)
     lib().or_matchjoin_merge_an(h1.ref, t1.ref, h2.ref, t2.ref, reenc_h, f_h1, wout_h, reenc_t, f_t2, wout_t, a_oid,
                                 *vs.refs())
+    return tuple(vs.numpy())
+
+
+def matchjoin_seq_an(h1: Col, t1: Col, h2: Col, t2: Col, reenc_h=0, f_h1=0, wout_h=8, reenc_t=0, f_t2=0,
+                     wout_t=8, a_oid=an.A_OID_ERR):
+    vs = Vecs(4)
+    lib().or_matchjoin_seq_an(h1.ref, t1.ref, h2.ref, t2.ref, reenc_h, f_h1, wout_h, reenc_t, f_t2, wout_t, a_oid,
+                              *vs.refs())
     return tuple(vs.numpy())
 
 

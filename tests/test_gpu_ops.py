@@ -166,3 +166,32 @@ def test_arithmetic_parity(op, n):
     ref = O.arithmetic_an(O.void(n), O.Col(x, mx), O.void(n), O.Col(y, my), op, an.A_RESULT, 8)
     got = capi.arithmetic_an(capi.Column.void(n), gcol(x, mx), capi.Column.void(n), gcol(y, my), op, an.A_RESULT, 8)
     same(got, ref)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2049, 100_003])
+def test_check_and_decode2_parity(n):
+    h = datagen.encode(datagen.uniform(n, 71, 1, 0, 1 << 20), an.A_OID, 8)
+    t = datagen.encode(datagen.uniform(n, 71, 2, 0, 0xFFFF), 233, 4)
+    datagen.inject_flips(h, 71, 3, datagen.flip_threshold(2e-3))
+    datagen.inject_flips(t, 71, 5, datagen.flip_threshold(2e-3))
+    mt = an.an_meta(an.RESSHORT, 233)
+    ref = O.check_and_decode_an2(O.Col(h, RESOID), O.Col(t, mt))
+    got = capi.check_and_decode2_an(gcol(h, RESOID), gcol(t, mt), err_cap=2)
+    same(got, ref)
+
+
+@pytest.mark.parametrize("n1,n2,sb1,sb2", [(0, 10, 0, 0), (10, 0, 3, 0), (2049, 3000, 0, 0), (5000, 4000, 0, 700),
+                                           (4000, 5000, 900, 0), (100_003, 90_000, 12, 5), (300, 200, 1000, 0)])
+def test_matchjoin_seq_parity(n1, n2, sb1, sb2):
+    seed = n1 + n2 + sb1
+    h1 = datagen.encode(datagen.uniform(n1, seed, 1, 0, 1 << 20), an.A_OID, 8)
+    t2 = datagen.encode(datagen.uniform(n2, seed, 2, 0, 0xFFFF), 233, 4)
+    datagen.inject_flips(h1, seed, 3, datagen.flip_threshold(2e-3))
+    datagen.inject_flips(t2, seed, 5, datagen.flip_threshold(2e-3))
+    mt = an.an_meta(an.RESSHORT, 233)
+    f_h = (16089 * an.A_OID_INV) & an.MASK64
+    for args in [(0, 0, 8, 0, 0, 4), (1, f_h, 8, 1, (65117 * an.inverse(233, 32)) & 0xFFFFFFFF, 4)]:
+        ref = O.matchjoin_seq_an(O.Col(h1, RESOID), O.void(n1, sb1), O.void(n2, sb2), O.Col(t2, mt), *args)
+        got = capi.matchjoin_seq_an(gcol(h1, RESOID), capi.Column.void(n1, sb1), capi.Column.void(n2, sb2),
+                                    gcol(t2, mt), *args, err_cap=2)
+        same(got, ref)

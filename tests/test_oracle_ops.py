@@ -206,3 +206,51 @@ def test_arithmetic_matches_numpy(op):
     f = {0: lambda a, b: a + b, 1: lambda a, b: a - b, 2: lambda a, b: a * b, 3: lambda a, b: a // b if b else 0}[op]
     assert out.tolist() == [f(a, b) & M64 for a, b in zip(xs, ys)]
     assert all(e.size == 0 for e in errs)
+
+
+def test_check_and_decode2_flips_and_counts():
+    """checkAndDecodeAN<Head, Tail> (encdecAN_scalar.tcc:129-176): every injected single flip is
+    reported on its side (closed form I1), and a pair survives iff neither side was flipped."""
+    n = 20_000
+    hv = datagen.uniform(n, 61, 1, 0, 1 << 20)
+    tv = datagen.uniform(n, 61, 2, 0, 200)
+    h = datagen.encode(hv, an.A_OID, 8)
+    t = datagen.encode(tv, 29, 2)
+    fh = datagen.inject_flips(h, 61, 3, datagen.flip_threshold(2e-3))
+    ft = datagen.inject_flips(t, 61, 5, datagen.flip_threshold(2e-3))
+    dh, dt, eh, et = O.check_and_decode_an2(O.Col(h, RESOID), O.Col(t, an.an_meta(an.RESTINY, 29)))
+    assert np.array_equal(eh, fh.astype(np.uint64) * np.uint64(an.A_OID_ERR))
+    assert np.array_equal(et, ft.astype(np.uint64) * np.uint64(an.A_OID_ERR))
+    keep = np.ones(n, bool)
+    keep[fh.astype(np.int64)] = False
+    keep[ft.astype(np.int64)] = False
+    assert np.array_equal(dh, hv[keep]) and np.array_equal(dt, tv[keep])
+
+
+def test_matchjoin_seq_is_the_oid_range_intersection():
+    """matchjoinAN <H1, void, void, T2> (matchjoinAN.tcc:146-224): the pairs are the rows whose
+    virtual OIDs coincide, i.e. the intersection of [sb1, sb1+n1) and [sb2, sb2+n2); the rows the
+    reference visits on each side are those below the end of that intersection."""
+    rng = np.random.default_rng(3)
+    for _ in range(30):
+        n1, n2 = int(rng.integers(0, 400)), int(rng.integers(0, 400))
+        sb1, sb2 = int(rng.integers(0, 300)), int(rng.integers(0, 300))
+        h1 = datagen.encode(rng.integers(0, 1 << 20, n1).astype(np.uint64), an.A_OID, 8)
+        t2 = datagen.encode(rng.integers(0, 1 << 20, n2).astype(np.uint64), an.A_OID, 8)
+        f1 = datagen.inject_flips(h1, 7 + _, 1, datagen.flip_threshold(0.05))
+        f2 = datagen.inject_flips(t2, 7 + _, 3, datagen.flip_threshold(0.05))
+        oh, ot, e1, e4 = O.matchjoin_seq_an(O.Col(h1, RESOID), O.void(n1, sb1), O.void(n2, sb2), O.Col(t2, RESOID))
+        lo, hi = max(sb1, sb2), min(sb1 + n1, sb2 + n2)
+        m = max(0, hi - lo) if n1 and n2 else 0
+        assert np.array_equal(oh, h1[lo - sb1: lo - sb1 + m]) if m else oh.size == 0
+        assert np.array_equal(ot, t2[lo - sb2: lo - sb2 + m]) if m else ot.size == 0
+        if n1 and n2:
+            if m:    # skipped prefix of the lower side, then m pairs
+                v1, v2 = lo - sb1 + m, lo - sb2 + m
+            else:    # no overlap: only the lower side is walked, up to the end of a list
+                v1 = min(n1, sb2 - sb1) if sb1 < sb2 else 0
+                v2 = min(n2, sb1 - sb2) if sb2 < sb1 else 0
+            assert np.array_equal(e1, f1[f1 < v1].astype(np.uint64) * np.uint64(an.A_OID_ERR))
+            assert np.array_equal(e4, f2[f2 < v2].astype(np.uint64) * np.uint64(an.A_OID_ERR))
+        else:
+            assert e1.size == 0 and e4.size == 0
