@@ -449,6 +449,12 @@ def encode_an(col: Column, a: int, out_width: int) -> np.ndarray:
     return out.to_numpy(np.dtype(f"u{out_width}"), col.n)
 
 
+def encode_an_into(col: Column, a: int, out_width: int, out: DeviceBuffer) -> None:
+    """out[i] = (code)col[i] * a in out_width bytes, asynchronously on the current stream."""
+    d = col.desc()
+    check(load().ahead_hip_encode_an(ctypes.byref(d), a & an.MASK64, out_width, out.ptr, current_stream()))
+
+
 def encode_void_into(buf: DeviceBuffer, n: int, a: int, out_width: int, seqbase: int = 0) -> None:
     """buf[i] = (seqbase + i) * a: the codes of a dense key / OID column."""
     d = Column.void(n, seqbase).desc()
