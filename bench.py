@@ -28,6 +28,8 @@ HBM.  Workloads:
                 predicate constants arriving on Poisson clocks; continuous batching on 4 GPU
                 slots (own streams, admission by HBM) against the unbatched path (one query at a
                 time); reports queries/s with p50/p99 latency per arrival rate.
+  hbm_copy    : a plain device-to-device copy of 4 GiB (read 4 GiB, write 4 GiB): the calibration
+                kernel for the FETCH_SIZE / WRITE_SIZE counters behind `roofline.traffic`.
 
 The line printed carries the whole-job throughput (`value`), the dominant kernel's roofline
 (algorithmic bytes / its average duration from HIP events on the launch stream) and a CPU
@@ -55,7 +57,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="star1", choices=["star1", "select_sum", "batch", "ssb", "serve"])
+    ap.add_argument("--workload", default="star1", choices=["star1", "select_sum", "batch", "ssb", "serve", "hbm_copy"])
     ap.add_argument("--rows", type=int, default=None, help="override the row count (scaling studies)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -412,6 +414,32 @@ def cpu_baseline_batch(spec, jobs, res_b, seconds):
                       f"bit-exact vs GPU: {ok}"}
 
 
+def run_hbm_copy(args):
+    import torch
+
+    nbytes = args.rows or (4 << 30)
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(7)
+    dst = torch.empty_like(src)
+    for _ in range(max(1, args.warmup)):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+    return {"metric": "device copy GB/s (read + write)", "value": gbs, "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+            "config": {"workload": f"hbm_copy: {nbytes} bytes read and {nbytes} bytes written per step"},
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "traffic": 2 * nbytes, "kernel": "torch copy",
+                         "kernel_ms": ms, "algorithmic_bytes": 2 * nbytes}}
+
+
 def run_serve(args):
     import torch
 
@@ -563,7 +591,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs an MI355X (cuda:0)"
     torch.cuda.set_device(0)
     run = {"star1": run_star1, "select_sum": run_select_sum, "batch": run_batch, "ssb": run_ssb,
-           "serve": run_serve}[args.workload]
+           "serve": run_serve, "hbm_copy": run_hbm_copy}[args.workload]
     res = run(args)
     print(json.dumps(res), flush=True)
 
