@@ -76,6 +76,9 @@ def write_table(prefix: str | Path, columns: dict[str, tuple[str, np.ndarray]], 
     for n in names:
         typ, arr = columns[n]
         t, w = _type_of(typ)
+        if w not in (1, 2, 4, 8):  # fixed-width strings: rows of w bytes
+            np.ascontiguousarray(arr, dtype=np.uint8).reshape(-1, w).tofile(f"{prefix}_{n}.ahead")
+            continue
         np.ascontiguousarray(arr).astype(np.dtype(f"<u{w}")).tofile(f"{prefix}_{n}.ahead")
 
 

@@ -16,7 +16,7 @@ def test_load_codes_and_reencode_on_load(tmp_path):
     q = datagen.encode(qv, 233, 2)
     p = datagen.encode(pv, 65117, 8)
     flipped = datagen.inject_flips(q, 5, 3, datagen.flip_threshold(1e-3))
-    loader.write_table(tmp_path / "lo", {"quantity": ("RESTINY", q), "price": ("RESINT", p), "tag": ("STRING:5", q)})
+    loader.write_table(tmp_path / "lo", {"quantity": ("RESTINY", q), "price": ("RESINT", p), "tag": ("CHAR", (q & 0x7F).astype(np.uint8))})
     t = loader.load_table(tmp_path / "lo")
     assert set(t) == {"quantity", "price"}
     assert np.array_equal(t["quantity"].tail.to_numpy(), q) and np.array_equal(t["price"].tail.to_numpy(), p)

@@ -52,3 +52,12 @@ def test_write_read_roundtrip(tmp_path):
     assert np.array_equal(loader.read_column(tmp_path / "lo", "key", an.INT), k)
     m = loader.column_meta(an.RESSHORT)
     assert m.a == 63877 and (m.a * m.a_inv) % (1 << 32) == 1
+
+
+def test_string_columns_are_written_but_not_loaded(tmp_path):
+    n = 100
+    s = np.frombuffer(b"".join(b"abcde\0" for _ in range(n)), dtype=np.uint8)
+    loader.write_table(tmp_path / "t", {"name": ("STRING:5", s), "k": ("RESINT", np.arange(n, dtype=np.uint64) * 65117)})
+    assert (tmp_path / "t_name.ahead").stat().st_size == 6 * n
+    cols = {c[0]: c[2] for c in loader.read_header(tmp_path / "t")}
+    assert cols["name"] is None and cols["k"] is an.RESINT
