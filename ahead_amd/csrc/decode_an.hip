@@ -131,7 +131,7 @@ int launch_mode(const DecodeArgs &a, uint32_t unenc_bits, hipStream_t s) {
 }
 
 int run(int mode, const ahead_col *col, uint64_t a_oid, void *dec, uint64_t dec_cap, uint64_t *err,
-        uint64_t err_cap, uint64_t *d_counts, ahead_hip_ws *ws, void *stream, size_t status_offset = 0) {
+        uint64_t err_cap, uint64_t *d_counts, ahead_hip_ws *ws, void *stream) {
     using namespace ahead_host;
     int st = check_col(col, "decode.col", false, true);
     if (st != AHEAD_OK) return st;
@@ -141,7 +141,6 @@ int run(int mode, const ahead_col *col, uint64_t a_oid, void *dec, uint64_t dec_
     if (col->unenc_bits > 8 * col->width) return fail(AHEAD_ERR_INVALID, "decode: unenc_bits wider than the code");
     const uint32_t tile_rows = 4u * kBlock * (16u / col->width);
     const LbPlan plan = lb_plan(col->n, tile_rows, 2);
-    if ((st = ws_ensure(ws, status_offset + plan.status_bytes)) != AHEAD_OK) return st;
     DecodeArgs a{};
     a.codes = col->data;
     a.n = col->n;
@@ -153,7 +152,7 @@ int run(int mode, const ahead_col *col, uint64_t a_oid, void *dec, uint64_t dec_
     a.err = err;
     a.err_cap = err == nullptr ? 0 : err_cap;
     a.d_counts = d_counts;
-    if ((st = lb_args(ws, plan, status_offset, &a.lb)) != AHEAD_OK) return st;
+    if ((st = lb_args(ws, plan, &a.lb)) != AHEAD_OK) return st;
     hipStream_t s = static_cast<hipStream_t>(stream);
     switch (col->width) {
         case 2:
@@ -205,9 +204,9 @@ int launch_encode(const ahead_col *col, uint64_t a, uint32_t ow, void *out, hipS
 using namespace ahead_host;
 
 namespace ahead_host {
-int check_an_at(const ahead_col *col, uint64_t a_oid, uint64_t *err, uint64_t err_cap, uint64_t *d_counts,
-                ahead_hip_ws *ws, void *stream, size_t status_offset) {
-    return run(kCheck, col, a_oid, nullptr, 0, err, err_cap, d_counts, ws, stream, status_offset);
+int check_an_ws(const ahead_col *col, uint64_t a_oid, uint64_t *err, uint64_t err_cap, uint64_t *d_counts,
+                ahead_hip_ws *ws, void *stream) {
+    return run(kCheck, col, a_oid, nullptr, 0, err, err_cap, d_counts, ws, stream);
 }
 size_t check_an_status_bytes(uint64_t n, uint32_t width) {
     return lb_plan(n, 4u * kBlock * (16u / width), 2).status_bytes;

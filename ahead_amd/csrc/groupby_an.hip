@@ -179,9 +179,8 @@ int run_groupby(const ahead_col *head, const ahead_col *tail, const ahead_col *g
     }
     const uint64_t slots = (1ull << bits) + 1;
     const size_t o_flags = 0, o_keys = 256, o_first = o_keys + align_up(slots * 8, 256),
-                 o_gid = o_first + align_up(slots * 4, 256), o_status = o_gid + align_up(slots * 4, 256);
-    const size_t stat = rowtile_status_bytes(n, 4);
-    if ((st = ws_ensure(ws, o_status + stat)) != AHEAD_OK) return st;
+                 o_gid = o_first + align_up(slots * 4, 256), o_end = o_gid + align_up(slots * 4, 256);
+    if ((st = ws_ensure(ws, o_end)) != AHEAD_OK) return st;
     GrpTable t{};
     uint32_t *flags = reinterpret_cast<uint32_t *>(ws_region(ws, o_flags));
     t.ht.keys = reinterpret_cast<unsigned long long *>(ws_region(ws, o_keys));
@@ -217,7 +216,7 @@ int run_groupby(const ahead_col *head, const ahead_col *tail, const ahead_col *g
     f.groid = static_cast<uint64_t *>(vgidtoroid.data);
     f.groid_cap = vgidtoroid.data ? vgidtoroid.cap : 0;
     f.counts = d_counts;
-    if ((st = launch_rowtile(f, n, 1, ws, o_status, s)) != AHEAD_OK) return st;
+    if ((st = launch_rowtile(f, n, 1, ws, s)) != AHEAD_OK) return st;
     AssignOp a{};
     a.n = n;
     a.k = k;
@@ -231,7 +230,7 @@ int run_groupby(const ahead_col *head, const ahead_col *tail, const ahead_col *g
         a.cap[i] = e[i].data ? e[i].cap : 0;
     }
     a.counts = d_counts;
-    if ((st = launch_rowtile(a, n, 4, ws, o_status, s)) != AHEAD_OK) return st;
+    if ((st = launch_rowtile(a, n, 4, ws, s)) != AHEAD_OK) return st;
     hipLaunchKernelGGL(k_grp_status, dim3(1), dim3(1), 0, s, flags, d_counts);
     AHEAD_HIP_TRY(hipGetLastError());
     return AHEAD_OK;

@@ -250,16 +250,12 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     uint32_t bits = 4;
     while ((1ull << bits) < 2ull * nb) ++bits;
     const uint64_t slots = (1ull << bits) + 1;
-    // layout: [flags 256][keys][cnt][first][off][cur][rows][status...]
+    // layout: [flags 256][keys][cnt][first][off][cur][rows]
     const size_t o_flags = 0, o_keys = 256, o_cnt = o_keys + align_up(slots * 8, 256),
                  o_first = o_cnt + align_up(slots * 4, 256), o_off = o_first + align_up(slots * 4, 256),
                  o_cur = o_off + align_up(slots * 4, 256), o_rows = o_cur + align_up(slots * 4, 256),
-                 o_status = o_rows + align_up(nb * 4, 256);
-    size_t stat = rowtile_status_bytes(np, 3);
-    const size_t s2 = rowtile_status_bytes(nb, 3), s3 = rowtile_status_bytes(slots, 1);
-    if (s2 > stat) stat = s2;
-    if (s3 > stat) stat = s3;
-    if ((st = ws_ensure(ws, o_status + stat)) != AHEAD_OK) return st;
+                 o_end = o_rows + align_up(nb * 4, 256);
+    if ((st = ws_ensure(ws, o_end)) != AHEAD_OK) return st;
 
     HjTable t{};
     t.flags = reinterpret_cast<uint32_t *>(ws_region(ws, o_flags));
@@ -289,7 +285,7 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     OffsetOp off{};
     off.n = slots;
     off.t = t;
-    if ((st = launch_rowtile(off, slots, 1, ws, o_status, s)) != AHEAD_OK) return st;
+    if ((st = launch_rowtile(off, slots, 1, ws, s)) != AHEAD_OK) return st;
     hipLaunchKernelGGL(k_hj_fill, grid, dim3(kBlock), 0, s, b);
     hipLaunchKernelGGL(k_hj_sort, dim3(1024), dim3(kBlock), 0, s, t, static_cast<uint32_t>(slots));
     AHEAD_HIP_TRY(hipGetLastError());
@@ -307,7 +303,7 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     chk.cap[1] = be1.data ? be1.cap : 0;
     chk.counts[0] = d_counts + (right ? 3 : 1);
     chk.counts[1] = d_counts + (right ? 4 : 2);
-    if ((st = launch_rowtile(chk, nb, 3, ws, o_status, s)) != AHEAD_OK) return st;
+    if ((st = launch_rowtile(chk, nb, 3, ws, s)) != AHEAD_OK) return st;
 
     ProbeOp p{};
     p.n = np;
@@ -343,7 +339,7 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     p.counts = d_counts;
     p.e_index[0] = right ? 1 : 3;
     p.e_index[1] = right ? 2 : 4;
-    if ((st = launch_rowtile(p, np, 3, ws, o_status, s)) != AHEAD_OK) return st;
+    if ((st = launch_rowtile(p, np, 3, ws, s)) != AHEAD_OK) return st;
     hipLaunchKernelGGL(k_hj_status, dim3(1), dim3(1), 0, s, t.flags, d_counts);
     AHEAD_HIP_TRY(hipGetLastError());
     return AHEAD_OK;

@@ -10,11 +10,18 @@
 #include "../../include/ahead_hip.h"
 #include "an_device.hpp"
 
+// A workspace holds two device allocations.  `base` is the WsHeader followed by the operators'
+// scratch regions (hash tables, partials, offsets), laid out per operator.  `status` holds only
+// look-back status words: a word whose epoch equals the current launch's must have been written
+// by that launch, so no other data may ever be stored there (scratch data of an earlier operator
+// could carry any bit pattern, the current epoch included).
 struct ahead_hip_ws {
     int device = 0;
-    void *base = nullptr;  // device memory: WsHeader followed by per-operator regions
+    void *base = nullptr;    // WsHeader + scratch regions
     size_t bytes = 0;
-    uint32_t epoch = 0;    // last look-back epoch handed out
+    void *status = nullptr;  // look-back status words only
+    size_t status_bytes = 0;
+    uint32_t epoch = 0;      // last look-back epoch handed out
 };
 
 namespace ahead_host {
@@ -33,11 +40,15 @@ constexpr size_t kHeaderBytes = sizeof(ahead_dev::WsHeader);
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// Ensures the workspace holds `bytes` beyond the header.  Growing frees the old allocation
-// (hipFree waits for the device) and zero-fills the new one.
+// Ensures the scratch allocation holds `bytes` beyond the header.  Growing frees the old
+// allocation (hipFree waits for the device) and zero-fills the new one.
 int ws_ensure(ahead_hip_ws *ws, size_t bytes);
 
-// Next look-back epoch (1 .. 2^30-1).  On wrap-around the status region is cleared.
+// Ensures the status allocation holds `bytes`; grown the same way.  The epoch keeps counting
+// (a zero word carries epoch 0, which is never handed out).
+int ws_ensure_status(ahead_hip_ws *ws, size_t bytes);
+
+// Next look-back epoch (1 .. 2^30-1).  On wrap-around the status allocation is cleared.
 int ws_next_epoch(ahead_hip_ws *ws, uint32_t *epoch);
 
 inline char *ws_region(ahead_hip_ws *ws, size_t offset) {
@@ -72,7 +83,8 @@ inline LbPlan lb_plan(uint64_t n, uint32_t tile_rows, int streams) {
     p.status_bytes = align_up(static_cast<size_t>(p.num_tiles) * streams * sizeof(uint64_t), 256);
     return p;
 }
-int lb_args(ahead_hip_ws *ws, const LbPlan &p, size_t status_offset, ahead_dev::LbArgs *out);
+// Look-back arguments of one launch: a fresh epoch and the status allocation (grown to the plan).
+int lb_args(ahead_hip_ws *ws, const LbPlan &p, ahead_dev::LbArgs *out);
 
 // Device fill kernels (graph-capture safe replacements for hipMemsetAsync).
 int fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s);
@@ -89,10 +101,9 @@ Tiling tiling();
 // <= max_subtiles that still leaves >= min_tiles tiles.
 uint32_t subtiles_per_tile(uint64_t subs);
 
-// checkAN with its look-back status placed at `status_offset` of the workspace region (so that
-// composite operators can keep other data in the workspace meanwhile).
-int check_an_at(const ahead_col *col, uint64_t a_oid, uint64_t *err, uint64_t err_cap, uint64_t *d_counts,
-                ahead_hip_ws *ws, void *stream, size_t status_offset);
+// checkAN for composite operators (it uses no scratch region, so theirs stay intact).
+int check_an_ws(const ahead_col *col, uint64_t a_oid, uint64_t *err, uint64_t err_cap, uint64_t *d_counts,
+                ahead_hip_ws *ws, void *stream);
 size_t check_an_status_bytes(uint64_t n, uint32_t width);
 
 }  // namespace ahead_host

@@ -252,14 +252,14 @@ extern "C" int ahead_hip_matchjoin_merge_an(const ahead_col *h1, const ahead_col
 
     const uint64_t K = a.num_chunks ? a.num_chunks : 1;
     const size_t o_flags = 0, o_start = 256, o_offs = o_start + align_up(K * 8, 256);
-    const size_t o_status = o_offs + align_up(K * 4 * kMmStreams, 256);
+    const size_t o_end = o_offs + align_up(K * 4 * kMmStreams, 256);
     const LbPlan plan = lb_plan(a.num_chunks, kBlock, kMmStreams);
-    if ((st = ws_ensure(ws, o_status + plan.status_bytes)) != AHEAD_OK) return st;
+    if ((st = ws_ensure(ws, o_end)) != AHEAD_OK) return st;
     a.flags = reinterpret_cast<uint32_t *>(ws_region(ws, o_flags));
     a.start = reinterpret_cast<uint64_t *>(ws_region(ws, o_start));
     a.offs = reinterpret_cast<uint32_t *>(ws_region(ws, o_offs));
     LbArgs lb{};
-    if ((st = lb_args(ws, plan, o_status, &lb)) != AHEAD_OK) return st;
+    if ((st = lb_args(ws, plan, &lb)) != AHEAD_OK) return st;
 
     if ((st = fill_u32(a.flags, 1, 0u, s)) != AHEAD_OK) return st;
     const uint32_t grid = static_cast<uint32_t>((K + kBlock - 1) / kBlock);

@@ -127,7 +127,7 @@ extern "C" int ahead_hip_check_and_decode2_an(const ahead_col *head, const ahead
     op.err_cap[0] = err_h.data ? err_h.cap : 0;
     op.err_cap[1] = err_t.data ? err_t.cap : 0;
     op.d_counts = d_counts;
-    return launch_rowtile<CheckDecode2Op, kG>(op, op.n, 3, ws, 0, static_cast<hipStream_t>(stream));
+    return launch_rowtile<CheckDecode2Op, kG>(op, op.n, 3, ws, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int ahead_hip_matchjoin_seq_an(const ahead_col *h1, const ahead_col *t1, const ahead_col *h2,
@@ -172,5 +172,5 @@ extern "C" int ahead_hip_matchjoin_seq_an(const ahead_col *h1, const ahead_col *
     op.err_cap[0] = e1.data ? e1.cap : 0;
     op.err_cap[1] = e4.data ? e4.cap : 0;
     op.d_counts = d_counts;
-    return launch_rowtile<MatchSeqOp, kG>(op, op.n, 3, ws, 0, static_cast<hipStream_t>(stream));
+    return launch_rowtile<MatchSeqOp, kG>(op, op.n, 3, ws, static_cast<hipStream_t>(stream));
 }

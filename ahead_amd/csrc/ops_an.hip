@@ -325,7 +325,7 @@ extern "C" int ahead_hip_fetchjoin_an(const ahead_col *oids, const ahead_col *co
     op.e2 = static_cast<uint64_t *>(err2.data);
     op.e2_cap = err2.data ? err2.cap : 0;
     op.d_counts = d_counts;
-    return launch_rowtile(op, op.n, 3, ws, 0, static_cast<hipStream_t>(stream));
+    return launch_rowtile(op, op.n, 3, ws, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int ahead_hip_matchjoin_pos_an(const ahead_col *h1, const ahead_col *t1, const ahead_col *t2, int reenc_h,
@@ -365,7 +365,7 @@ extern "C" int ahead_hip_matchjoin_pos_an(const ahead_col *h1, const ahead_col *
         op.cap[i] = e[i].data ? e[i].cap : 0;
     }
     op.d_counts = d_counts;
-    return launch_rowtile(op, op.n, 4, ws, 0, static_cast<hipStream_t>(stream));
+    return launch_rowtile(op, op.n, 4, ws, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int ahead_hip_arithmetic_an(const ahead_col *h1, const ahead_col *t1, const ahead_col *h2, const ahead_col *t2,
@@ -400,7 +400,7 @@ extern "C" int ahead_hip_arithmetic_an(const ahead_col *h1, const ahead_col *t1,
         op.cap[i] = e[i].data ? e[i].cap : 0;
     }
     op.d_counts = d_counts;
-    return launch_rowtile(op, op.n, 5, ws, 0, static_cast<hipStream_t>(stream));
+    return launch_rowtile(op, op.n, 5, ws, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int ahead_hip_aggregate_mul_sum_an(const ahead_col *t1, const ahead_col *t2, uint64_t init,
@@ -417,9 +417,8 @@ extern "C" int ahead_hip_aggregate_mul_sum_an(const ahead_col *t1, const ahead_c
     op.t2 = dcol(*t2);
     op.a_oid = a_oid;
     const LbPlan plan = lb_plan(op.n, kG * kBlock, 3);
-    const size_t part_off = plan.status_bytes;
-    if ((st = ws_ensure(ws, part_off + align_up(plan.num_tiles * 8ull, 256))) != AHEAD_OK) return st;
-    op.partials = reinterpret_cast<uint64_t *>(ws_region(ws, part_off));
+    if ((st = ws_ensure(ws, plan.num_tiles * 8ull)) != AHEAD_OK) return st;
+    op.partials = reinterpret_cast<uint64_t *>(ws_region(ws, 0));
     const ahead_buf e[2] = {err1, err2};
     for (int i = 0; i < 2; ++i) {
         op.err[i] = static_cast<uint64_t *>(e[i].data);
@@ -428,7 +427,7 @@ extern "C" int ahead_hip_aggregate_mul_sum_an(const ahead_col *t1, const ahead_c
     op.d_counts = d_counts;
     hipStream_t s = static_cast<hipStream_t>(stream);
     uint32_t tiles = 0;
-    if ((st = launch_rowtile(op, op.n, 3, ws, 0, s, &tiles)) != AHEAD_OK) return st;
+    if ((st = launch_rowtile(op, op.n, 3, ws, s, &tiles)) != AHEAD_OK) return st;
     // aggregateAN_scalar.tcc:83-85: RA * inv(A_T1) * inv(A_T2) over 64 bits
     const uint64_t enc = (result_encoded ? ra : 1ull) * (t1->encoded ? inverse_mod2k(t1->a, 64) : 1ull) *
                          (t2->encoded ? inverse_mod2k(t2->a, 64) : 1ull);
@@ -476,5 +475,5 @@ extern "C" int ahead_hip_aggregate_sum_grouped_an(const ahead_col *head, const a
     AggSumOp op{};
     op.a = a;
     op.n = a.n;
-    return launch_rowtile(op, op.n, 4, ws, 0, s);
+    return launch_rowtile(op, op.n, 4, ws, s);
 }

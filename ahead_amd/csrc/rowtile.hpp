@@ -154,16 +154,15 @@ ahead_dev::DCol dcol(const ahead_col &c);
 // width of TypeMap<T>::v2_encoded_t::type_t for a column of the given kind
 uint32_t encoded_width(const ahead_col &c);
 
-// Launches one row-tile operator over n rows with `streams` look-back streams (1 + Op::kErr),
-// its status placed at `status_offset` of the workspace region.
+// Launches one row-tile operator over n rows with `streams` look-back streams (1 + Op::kErr).
+// It uses no scratch region of the workspace: an operator's own regions stay intact.
 template <class Op, int G = 8>
-int launch_rowtile(const Op &op, uint64_t n, int streams, ahead_hip_ws *ws, size_t status_offset, hipStream_t s,
+int launch_rowtile(const Op &op, uint64_t n, int streams, ahead_hip_ws *ws, hipStream_t s,
                    uint32_t *num_tiles = nullptr) {
     const LbPlan plan = lb_plan(n, G * ahead_dev::kBlock, streams);
-    int st = ws_ensure(ws, status_offset + plan.status_bytes);
-    if (st != AHEAD_OK) return st;
     ahead_dev::LbArgs lb{};
-    if ((st = lb_args(ws, plan, status_offset, &lb)) != AHEAD_OK) return st;
+    int st;
+    if ((st = lb_args(ws, plan, &lb)) != AHEAD_OK) return st;
     hipLaunchKernelGGL((ahead_dev::k_rowtile<Op, G>), dim3(plan.num_tiles), dim3(ahead_dev::kBlock), 0, s, op, lb);
     AHEAD_HIP_TRY(hipGetLastError());
     if (num_tiles) *num_tiles = plan.num_tiles;

@@ -51,28 +51,61 @@ int ws_ensure(ahead_hip_ws *ws, size_t bytes) {
     }
     ws->base = p;
     ws->bytes = target;
-    ws->epoch = 0;
+    return AHEAD_OK;
+}
+
+int ws_ensure_status(ahead_hip_ws *ws, size_t bytes) {
+    if (ws == nullptr) return fail(AHEAD_ERR_INVALID, "workspace is NULL");
+    const size_t need = align_up(bytes > 0 ? bytes : 1, 256);
+    if (ws->status != nullptr && ws->status_bytes >= need) return AHEAD_OK;
+    const size_t grow = ws->status_bytes + ws->status_bytes / 2;
+    const size_t target = need > grow ? need : grow;
+    int cur = 0;
+    AHEAD_HIP_TRY(hipGetDevice(&cur));
+    if (cur != ws->device) AHEAD_HIP_TRY(hipSetDevice(ws->device));
+    if (ws->status != nullptr) {
+        AHEAD_HIP_TRY(hipFree(ws->status));
+        ws->status = nullptr;
+        ws->status_bytes = 0;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, target);
+    if (e != hipSuccess) {
+        if (cur != ws->device) (void)hipSetDevice(cur);
+        return hip_fail(e, "hipMalloc(workspace status)");
+    }
+    e = hipMemset(p, 0, target);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (cur != ws->device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        return hip_fail(e, "hipMemset(workspace status)");
+    }
+    ws->status = p;
+    ws->status_bytes = target;
     return AHEAD_OK;
 }
 
 int ws_next_epoch(ahead_hip_ws *ws, uint32_t *epoch) {
     if (ws->epoch + 1 >= (1u << 30)) {
         // Status words of every earlier launch carry an epoch below 2^30; clear them all.
-        AHEAD_HIP_TRY(hipMemset(static_cast<char *>(ws->base) + kHeaderBytes, 0,
-                                ws->bytes - kHeaderBytes));
-        AHEAD_HIP_TRY(hipDeviceSynchronize());
+        if (ws->status != nullptr) {
+            AHEAD_HIP_TRY(hipMemset(ws->status, 0, ws->status_bytes));
+            AHEAD_HIP_TRY(hipDeviceSynchronize());
+        }
         ws->epoch = 0;
     }
     *epoch = ++ws->epoch;
     return AHEAD_OK;
 }
 
-int lb_args(ahead_hip_ws *ws, const LbPlan &p, size_t status_offset, ahead_dev::LbArgs *out) {
-    uint32_t epoch = 0;
-    int st = ws_next_epoch(ws, &epoch);
+int lb_args(ahead_hip_ws *ws, const LbPlan &p, ahead_dev::LbArgs *out) {
+    int st = ws_ensure_status(ws, p.status_bytes);
     if (st != AHEAD_OK) return st;
+    uint32_t epoch = 0;
+    if ((st = ws_next_epoch(ws, &epoch)) != AHEAD_OK) return st;
     out->hdr = ws_header(ws);
-    out->status = reinterpret_cast<uint64_t *>(ws_region(ws, status_offset));
+    out->status = static_cast<uint64_t *>(ws->status);
     out->num_tiles = p.num_tiles;
     out->epoch = epoch;
     return AHEAD_OK;
@@ -150,9 +183,10 @@ int ahead_hip_ws_create(int device, ahead_hip_ws **ws) {
     ahead_hip_ws *w = new (std::nothrow) ahead_hip_ws();
     if (w == nullptr) return fail(AHEAD_ERR_NOMEM, "host allocation failed");
     w->device = device;
-    const int st = ws_ensure(w, 1 << 20);
+    int st = ws_ensure(w, 1 << 20);
+    if (st == AHEAD_OK) st = ws_ensure_status(w, 1 << 16);
     if (st != AHEAD_OK) {
-        delete w;
+        (void)ahead_hip_ws_destroy(w);
         return st;
     }
     *ws = w;
@@ -161,20 +195,25 @@ int ahead_hip_ws_create(int device, ahead_hip_ws **ws) {
 
 int ahead_hip_ws_destroy(ahead_hip_ws *ws) {
     if (ws == nullptr) return AHEAD_OK;
-    if (ws->base != nullptr) {
-        hipError_t e = hipFree(ws->base);
-        ws->base = nullptr;
-        delete ws;
-        if (e != hipSuccess) return hip_fail(e, "hipFree(workspace)");
-        return AHEAD_OK;
+    hipError_t e = hipSuccess;
+    if (ws->base != nullptr) e = hipFree(ws->base);
+    if (ws->status != nullptr) {
+        const hipError_t e2 = hipFree(ws->status);
+        if (e == hipSuccess) e = e2;
     }
     delete ws;
+    if (e != hipSuccess) return hip_fail(e, "hipFree(workspace)");
     return AHEAD_OK;
 }
 
-int ahead_hip_ws_reserve(ahead_hip_ws *ws, size_t bytes) { return ws_ensure(ws, bytes); }
+int ahead_hip_ws_reserve(ahead_hip_ws *ws, size_t bytes) {
+    const int st = ws_ensure(ws, bytes);
+    return st != AHEAD_OK ? st : ws_ensure_status(ws, bytes);
+}
 
-size_t ahead_hip_ws_bytes(const ahead_hip_ws *ws) { return ws == nullptr ? 0 : ws->bytes; }
+size_t ahead_hip_ws_bytes(const ahead_hip_ws *ws) {
+    return ws == nullptr ? 0 : ws->bytes + ws->status_bytes;
+}
 
 int ahead_hip_set_tiling(uint32_t max_subtiles, uint32_t min_tiles) {
     if (max_subtiles < 1 || max_subtiles > kMaxSubtiles || (max_subtiles & (max_subtiles - 1)) != 0)

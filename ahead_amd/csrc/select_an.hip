@@ -109,9 +109,7 @@ extern "C" int ahead_hip_select_an(const ahead_col *col, int op1, uint64_t th1, 
     const uint64_t subs = (col->n + sub_rows - 1) / sub_rows;
     const uint32_t iters = subtiles_per_tile(subs);
     const LbPlan plan = lb_plan(col->n, static_cast<uint32_t>(sub_rows * iters), 2);
-    const size_t partial_off = plan.status_bytes;
-    const size_t need = partial_off + align_up(static_cast<size_t>(plan.num_tiles) * 8, 256);
-    if ((st = ws_ensure(ws, need)) != AHEAD_OK) return st;
+    if ((st = ws_ensure(ws, static_cast<size_t>(plan.num_tiles) * 8)) != AHEAD_OK) return st;
 
     SelectArgs a{};
     a.codes = col->data;
@@ -128,9 +126,9 @@ extern "C" int ahead_hip_select_an(const ahead_col *col, int op1, uint64_t th1, 
     a.err = err;
     a.err_cap = err == nullptr ? 0 : err_cap;
     a.d_counts = d_counts;
-    a.partials = reinterpret_cast<uint64_t *>(ws_region(ws, partial_off));
+    a.partials = reinterpret_cast<uint64_t *>(ws_region(ws, 0));
     a.iters = iters;
-    if ((st = lb_args(ws, plan, 0, &a.lb)) != AHEAD_OK) return st;
+    if ((st = lb_args(ws, plan, &a.lb)) != AHEAD_OK) return st;
 
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bool sum = d_sum != nullptr;

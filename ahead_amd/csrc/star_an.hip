@@ -758,14 +758,15 @@ int star_validate(const ahead_star1_args *in, const ahead_star1_out *out) {
     return AHEAD_OK;
 }
 
-// Workspace layout of one star join: [flags 256][g_sum][g_first][table][look-back status].  The
+// Workspace layout of one star join: [flags 256][g_sum][g_first][table]...[partials].  The
 // dimension table survives between ahead_hip_star1_build and ahead_hip_star1_probe as long as no
 // other operator runs on the workspace in between.
 struct StarLayout {
     uint32_t tbits = 4, iters = 1;
     LbPlan plan;
     size_t off_flags = 0, off_gsum = 0, off_gfirst = 0, off_table = 0, off_present = 0, off_combo_cid = 0,
-           off_cid_combo = 0, off_info = 0, off_map = 0, off_psum = 0, off_prank = 0, off_status = 0, total = 0;
+           off_cid_combo = 0, off_info = 0, off_map = 0, off_psum = 0, off_prank = 0, total = 0;
+    size_t status_bytes = 0;  // scan state of the probe and of the build-side check
     uint32_t map_cap = 0;
     uint32_t grid = 0;  // persistent probe workgroups
 };
@@ -808,9 +809,9 @@ StarLayout star_layout(const ahead_star1_args *in) {
     L.grid = probe_grid(L.plan.num_tiles);
     L.off_psum = L.off_map + align_up(L.map_cap * 2ull, 256);
     L.off_prank = L.off_psum + align_up(static_cast<size_t>(kLdsCids) * L.grid * 8, 256);
-    L.off_status = L.off_prank + align_up(static_cast<size_t>(kLdsCids) * L.grid * 4, 256);
+    L.total = L.off_prank + align_up(static_cast<size_t>(kLdsCids) * L.grid * 4, 256);
     const size_t chk = check_an_status_bytes(in->dim_key.n, in->dim_key.width);
-    L.total = L.off_status + (L.plan.status_bytes > chk ? L.plan.status_bytes : chk);
+    L.status_bytes = L.plan.status_bytes > chk ? L.plan.status_bytes : chk;
     return L;
 }
 
@@ -822,6 +823,8 @@ extern "C" int ahead_hip_star1_build(const ahead_star1_args *in, const ahead_sta
     if (st != AHEAD_OK) return st;
     const StarLayout L = star_layout(in);
     if ((st = ws_ensure(ws, L.total)) != AHEAD_OK) return st;
+    // the probe's scan state too, so that the probe allocates nothing
+    if ((st = ws_ensure_status(ws, L.status_bytes)) != AHEAD_OK) return st;
     hipStream_t s = static_cast<hipStream_t>(stream);
     BuildArgs b{};
     b.key = static_cast<const uint64_t *>(in->dim_key.data);
@@ -872,10 +875,9 @@ extern "C" int ahead_hip_star1_build(const ahead_star1_args *in, const ahead_sta
     hipLaunchKernelGGL(k_star_map_clear, dim3(512), dim3(kBlock), 0, s, d);
     if (b.n) hipLaunchKernelGGL(k_star_map_fill, gd, dim3(kBlock), 0, s, d);
     AHEAD_HIP_TRY(hipGetLastError());
-    // the build side's error vector: checkAN over the dimension keys (hashjoinAN.tcc:179-181),
-    // using the status region, which the probe overwrites afterwards
-    return check_an_at(&in->dim_key, in->a_oid_err, out->e_dk, out->e_dk ? out->e_dk_cap : 0,
-                       out->d_counts + 2, ws, stream, L.off_status);
+    // the build side's error vector: checkAN over the dimension keys (hashjoinAN.tcc:179-181)
+    return check_an_ws(&in->dim_key, in->a_oid_err, out->e_dk, out->e_dk ? out->e_dk_cap : 0,
+                       out->d_counts + 2, ws, stream);
 }
 
 extern "C" int ahead_hip_star1_probe(const ahead_star1_args *in, const ahead_star1_out *out, ahead_hip_ws *ws,
@@ -913,7 +915,7 @@ extern "C" int ahead_hip_star1_probe(const ahead_star1_args *in, const ahead_sta
     p.err[2] = out->e_a2; p.cap[2] = out->e_a2 ? out->e_a2_cap : 0;
     p.err[3] = out->e_m; p.cap[3] = out->e_m ? out->e_m_cap : 0;
     p.d_counts = out->d_counts;
-    if ((st = lb_args(ws, L.plan, L.off_status, &p.lb)) != AHEAD_OK) return st;
+    if ((st = lb_args(ws, L.plan, &p.lb)) != AHEAD_OK) return st;
     hipLaunchKernelGGL(k_star_probe, dim3(L.grid), dim3(kBlock), 0, s, p);
     hipLaunchKernelGGL(k_star_reduce, dim3(256), dim3(kBlock), 0, s, p.part_sum, p.part_rank, L.grid, p.cid_combo,
                        p.flags, p.g_sum, p.g_first);

@@ -74,9 +74,11 @@ const char *ahead_hip_version(void);
 /* Creates a workspace on `device` (device scratch for scan state, hash tables, partials). */
 int ahead_hip_ws_create(int device, ahead_hip_ws **ws);
 int ahead_hip_ws_destroy(ahead_hip_ws *ws);
-/* Grows the workspace to at least `bytes` now (so later calls do not allocate, e.g. under
- * stream capture). */
+/* Grows the workspace now (so later calls do not allocate, e.g. under stream capture): its
+ * scratch to at least `bytes` and its scan-state area to at least `bytes` too.  A reservation
+ * covers every call whose scratch plus scan state fits in `bytes`. */
 int ahead_hip_ws_reserve(ahead_hip_ws *ws, size_t bytes);
+/* Device bytes the workspace holds (scratch + scan state). */
 size_t ahead_hip_ws_bytes(const ahead_hip_ws *ws);
 
 /* Tiling of the streaming operators: a look-back tile covers up to `max_subtiles` sub-tiles of

@@ -1,8 +1,9 @@
 """Served queries (ahead_amd.serve): every query's result, error vectors included, is the same when
 it runs alone (one query on the GPU at a time) and in the mix (four queries overlapping on four
-streams, all arriving at once)."""
+streams, all arriving at once), and both equal the oracle's operator chain for the query."""
 import pytest
 
+import oracle_bat as OB
 from ahead_amd import serve, ssb
 
 pytestmark = pytest.mark.gpu
@@ -15,10 +16,14 @@ def test_results_alone_equal_results_in_the_mix():
     tables = ssb.device_tables(spec)
     alone = serve.Server(tables, spec.n_fact, workers=1).run(serve.random_queries(24, 0, 5), realtime=False)
     mixed = serve.Server(tables, spec.n_fact, workers=4).run(serve.random_queries(24, 0, 5), realtime=False)
+    host = OB.tables(spec)
     shapes = set()
     for a, m in zip(alone, mixed):
         assert (a.qid, a.shape, a.params) == (m.qid, m.shape, m.params)
-        assert serve.same_result(a.result, m.result), (a.qid, a.shape, a.params)
+        ref = ssb.PIPELINES[a.shape](host, ops=OB, **a.params)
+        print(a.qid, a.shape, a.params, "rows", a.result["rows"], m.result["rows"], ref["rows"], flush=True)
+        assert serve.same_result(a.result, ref), ("alone", a.qid, a.shape, a.params)
+        assert serve.same_result(m.result, ref), ("mixed", a.qid, a.shape, a.params)
         shapes.add(a.shape)
     assert shapes == {"P1", "P2", "P3", "P4"}
     s = serve.summarize(mixed)
