@@ -82,8 +82,12 @@ class _Runner:
         self.cnt = DeviceBuffer(8 * ncounts)
         self.ncounts = ncounts
 
-    def run(self, call, caps: list[list[int]], count_of: list[int]):
-        """caps: [elem_bytes, cap] per output; count_of[i]: counts index bounding output i."""
+    def run(self, call, caps: list[list[int]], count_of: list[int], shrink: tuple = ()):
+        """caps: [elem_bytes, cap] per output; count_of[i]: counts index bounding output i.
+        Outputs listed in `shrink` (sized by an upper bound, e.g. every row of a join's probe
+        side) are copied into buffers of their actual size when they came out at most half full,
+        so that results held by a caller occupy what they use and the bound-sized buffer goes
+        back to the allocator for the next operator."""
         while True:
             bufs = [DeviceBuffer(max(8, eb * cap)) for eb, cap in caps]
             check(call(bufs, self.cnt.ptr))
@@ -94,7 +98,12 @@ class _Runner:
                     caps[i][1] = c[count_of[i]]
                     grow = True
             if not grow:
-                return bufs, c
+                break
+        for i in shrink:
+            need = max(8, caps[i][0] * c[count_of[i]])
+            if 2 * need <= bufs[i].nbytes:
+                bufs[i] = bufs[i].prefix_copy(need)
+        return bufs, c
 
 
 def _b(buf: DeviceBuffer, cap: int) -> Buf:
@@ -119,7 +128,7 @@ def selectAN(bat: BAT, op1: int, th1: int, op2: int = LT, th2: int = 0, comb: in
                                          a_oid, b[0].ptr, caps[0][1], b[1].ptr, caps[1][1], cnt, None, r.ws.h,
                                          current_stream())
 
-    bufs, c = r.run(call2, caps, [0, 1])
+    bufs, c = r.run(call2, caps, [0, 1], shrink=(0,))
     oids = Column(bufs[0], c[0], RESOID_META)
     return BAT(oids, None), bufs[1].to_numpy(np.uint64, c[1])
 
@@ -138,7 +147,7 @@ def fetchjoinAN(oids: BAT, col: BAT, reenc: tuple[int, int] | None = None, a_oid
                                             a_oid, _b(b[0], caps[0][1]), _b(b[1], caps[1][1]), _b(b[2], caps[2][1]),
                                             cnt, r.ws.h, current_stream())
 
-    bufs, c = r.run(call, caps, [0, 1, 2])
+    bufs, c = r.run(call, caps, [0, 1, 2], shrink=(0,))
     m = col.tail.meta
     meta = an.ColMeta(t_out, reenc[0], reenc[1]) if reenc else an.ColMeta(t_out, m.a, m.a_inv)
     out = BAT(Column.void(c[0]), Column(bufs[0], c[0], meta))
@@ -171,7 +180,7 @@ def matchjoinAN(b1: BAT, b2: BAT, reenc_h: tuple[int, int] | None = None, reenc_
                                                     f_t, tt.width, a_oid, *[_b(x, c[1]) for x, c in zip(b, caps)],
                                                     cnt, r.ws.h, current_stream())
 
-        bufs, c = r.run(call, caps, [0, 0, 1, 2])
+        bufs, c = r.run(call, caps, [0, 0, 1, 2], shrink=(0, 1))
         mh = an.ColMeta(th, reenc_h[0], reenc_h[1]) if reencode else an.ColMeta(th, h1.meta.a, h1.meta.a_inv)
         mt = an.ColMeta(tt, a_t, reenc_t[1] if reenc_t else 0) if reencode else an.ColMeta(tt, t2.meta.a, t2.meta.a_inv)
         e = _errs(bufs[2:], c[1:3], [h1.meta.type.encoded, t2.meta.type.encoded])
@@ -188,7 +197,7 @@ def matchjoinAN(b1: BAT, b2: BAT, reenc_h: tuple[int, int] | None = None, reenc_
                                                     *[_b(x, c[1]) for x, c in zip(b, caps)], cnt, r.ws.h,
                                                     current_stream())
 
-        bufs, c = r.run(call, caps, [0, 0, 1, 2, 3])
+        bufs, c = r.run(call, caps, [0, 0, 1, 2, 3], shrink=(0, 1))
         e = _errs(bufs[2:], c[1:], [h1.meta.type.encoded, True, t2.meta.type.encoded])
         errs = (e[0], e[1], None, e[2])
     else:
@@ -197,7 +206,7 @@ def matchjoinAN(b1: BAT, b2: BAT, reenc_h: tuple[int, int] | None = None, reenc_
                                                       f_t, tt.width, a_oid, *[_b(x, c[1]) for x, c in zip(b, caps)],
                                                       cnt, r.ws.h, current_stream())
 
-        bufs, c = r.run(call, caps, [0, 0, 1, 2, 3, 4])
+        bufs, c = r.run(call, caps, [0, 0, 1, 2, 3, 4], shrink=(0, 1))
         errs = tuple(_errs(bufs[2:], c[1:5], [h1.meta.type.encoded, t1.meta.type.encoded, h2.meta.type.encoded,
                                               t2.meta.type.encoded]))
     mh = an.ColMeta(th, reenc_h[0], reenc_h[1]) if reencode else an.ColMeta(th, h1.meta.a, h1.meta.a_inv)
@@ -228,7 +237,7 @@ def hashjoinAN(b1: BAT, b2: BAT, reenc_h: tuple[int, int] | None = None, reenc_t
                                            th.width, int(bool(reenc_t)), f_t, tt.width, a_oid,
                                            *[_b(x, c[1]) for x, c in zip(b, caps)], cnt, r.ws.h, current_stream())
 
-    bufs, c = r.run(call, caps, [0, 0, 1, 2, 3, 4])
+    bufs, c = r.run(call, caps, [0, 0, 1, 2, 3, 4], shrink=(0, 1))
     if c[5] != 0:
         raise capi.AheadError(capi.AHEAD_ERR_DEVICE, "hashjoinAN: a build key has more rows than the device join supports")
     mh = an.ColMeta(th, reenc_h[0], reenc_h[1]) if reenc_h else an.ColMeta(th, h1.meta.a, h1.meta.a_inv)

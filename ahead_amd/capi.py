@@ -170,6 +170,13 @@ class DeviceBuffer:
     def ptr(self) -> int:
         return self.t.data_ptr()
 
+    def prefix_copy(self, nbytes: int) -> "DeviceBuffer":
+        """A new buffer holding the first nbytes (a device copy on the current stream)."""
+        b = DeviceBuffer.__new__(DeviceBuffer)
+        b.t = self.t[: max(int(nbytes), 1)].clone()
+        b.nbytes = int(nbytes)
+        return b
+
     @staticmethod
     def from_numpy(arr: np.ndarray, device: int = 0) -> "DeviceBuffer":
         torch = _torch()

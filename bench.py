@@ -451,27 +451,35 @@ def run_serve(args):
     tables = ssb.device_tables(spec)
     torch.cuda.synchronize()
     n_q = max(8, args.steps * 8)
-    # warm up every shape once
-    serve.Server(tables, spec.n_fact, workers=1).run(serve.random_queries(8, 0, 99), realtime=False)
+    # one long-lived server per form; each is warmed up on every shape before it is measured
+    servers = {w: serve.Server(tables, spec.n_fact, workers=w) for w in (1, 4)}
+    for w, srv in servers.items():
+        srv.run(serve.random_queries(4 * w, 0, 99), realtime=False)
+    torch.cuda.reset_peak_memory_stats()
+    retries0 = torch.cuda.memory_stats().get("num_alloc_retries", 0)
     # capacity of each form: everything arrives at once
     sat = {}
-    for w in (1, 4):
-        qs = serve.Server(tables, spec.n_fact, workers=w).run(serve.random_queries(n_q, 0, 7), realtime=False)
-        sat[w] = serve.summarize(qs)
+    for w, srv in servers.items():
+        sat[w] = serve.summarize(srv.run(serve.random_queries(n_q, 0, 7), realtime=False))
     # Poisson arrivals at fractions of the unbatched capacity
     sweep = []
     base = sat[1]["qps"]
     for frac in (0.5, 0.9, 1.5):
         rate = base * frac
         row = {"rate_qps": rate}
-        for w in (1, 4):
-            qs = serve.Server(tables, spec.n_fact, workers=w).run(serve.random_queries(n_q, rate, 11), realtime=True)
-            row[f"workers{w}"] = serve.summarize(qs)
+        for w, srv in servers.items():
+            row[f"workers{w}"] = serve.summarize(srv.run(serve.random_queries(n_q, rate, 11), realtime=True))
         sweep.append(row)
     t0 = time.perf_counter()
-    final = serve.Server(tables, spec.n_fact, workers=4).run(serve.random_queries(n_q, 0, 13), realtime=False)
+    final = servers[4].run(serve.random_queries(n_q, 0, 13), realtime=False)
     el = time.perf_counter() - t0
     s4 = serve.summarize(final)
+    for srv in servers.values():
+        srv.close()
+    mem = torch.cuda.memory_stats()
+    memory = {"alloc_retries": mem.get("num_alloc_retries", 0) - retries0,
+              "peak_allocated_gb": mem.get("allocated_bytes.all.peak", 0) / 1e9,
+              "peak_reserved_gb": mem.get("reserved_bytes.all.peak", 0) / 1e9}
     return {
         "metric": "served star-schema queries/s (continuous batching, 4 slots) with p50/p99 latency",
         "value": s4["qps"],
@@ -489,7 +497,7 @@ def run_serve(args):
                                f"per run, shapes P1-P4 with seeded random predicates",
                    "saturated": {"unbatched": sat[1], "batched_4_slots": sat[4]},
                    "speedup_saturated": sat[4]["qps"] / sat[1]["qps"] if sat[1]["qps"] else None,
-                   "poisson_sweep": sweep, "final_batched": s4},
+                   "poisson_sweep": sweep, "final_batched": s4, "device_memory": memory},
         "roofline": None,
     }
 
@@ -507,10 +515,14 @@ def run_ssb(args):
         spec.n_fact = args.rows
     t = ssb.device_tables(spec)
     torch.cuda.synchronize()
-    results = {}
+    results, peak = {}, {}
     for name, fn in ssb.PIPELINES.items():
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated()
+        torch.cuda.reset_peak_memory_stats()
         results[name] = fn(t)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        peak[name] = (torch.cuda.max_memory_allocated() - base) / spec.n_fact
     per = {name: 0.0 for name in ssb.PIPELINES}
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -526,7 +538,8 @@ def run_ssb(args):
     algo = sum(SSB_BYTES_PER_ROW[k] for k in ssb.PIPELINES) * spec.n_fact * args.steps
     achieved = algo / el / 1e9
     summary = {k: {"ms": per[k] / args.steps * 1e3, "rows": results[k]["rows"],
-                   "groups": results[k].get("groups", 1), "errors": results[k]["errors"].total()}
+                   "groups": results[k].get("groups", 1), "errors": results[k]["errors"].total(),
+                   "peak_device_bytes_per_fact_row": round(peak[k], 2)}
                for k in ssb.PIPELINES}
     res = {
         "metric": "hardened star-schema queries/s (Q1.1, Q2.1, Q3.1, Q4.1 operator chains)",

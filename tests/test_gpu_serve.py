@@ -14,8 +14,11 @@ SMALL = dict(n_fact=250_000, n_date=2556, n_cust=3_000, n_supp=200, n_part=1_400
 def test_results_alone_equal_results_in_the_mix():
     spec = ssb.SsbSpec(rate=2e-3, seed=21, **SMALL)
     tables = ssb.device_tables(spec)
-    alone = serve.Server(tables, spec.n_fact, workers=1).run(serve.random_queries(24, 0, 5), realtime=False)
-    mixed = serve.Server(tables, spec.n_fact, workers=4).run(serve.random_queries(24, 0, 5), realtime=False)
+    with serve.Server(tables, spec.n_fact, workers=1) as srv:
+        alone = srv.run(serve.random_queries(24, 0, 5), realtime=False)
+    with serve.Server(tables, spec.n_fact, workers=4) as srv:
+        mixed = srv.run(serve.random_queries(24, 0, 5), realtime=False)
+        again = srv.run(serve.random_queries(24, 0, 5), realtime=False)  # the same threads, warm
     host = OB.tables(spec)
     shapes = set()
     for a, m in zip(alone, mixed):
@@ -24,6 +27,7 @@ def test_results_alone_equal_results_in_the_mix():
         print(a.qid, a.shape, a.params, "rows", a.result["rows"], m.result["rows"], ref["rows"], flush=True)
         assert serve.same_result(a.result, ref), ("alone", a.qid, a.shape, a.params)
         assert serve.same_result(m.result, ref), ("mixed", a.qid, a.shape, a.params)
+        assert serve.same_result(again[a.qid].result, ref), ("mixed again", a.qid, a.shape, a.params)
         shapes.add(a.shape)
     assert shapes == {"P1", "P2", "P3", "P4"}
     s = serve.summarize(mixed)
@@ -34,8 +38,8 @@ def test_admission_budget_serializes_when_memory_is_short():
     spec = ssb.SsbSpec(rate=0.0, seed=22, **SMALL)
     tables = ssb.device_tables(spec)
     # a budget that fits one query at a time: queries must not overlap
-    srv = serve.Server(tables, spec.n_fact, workers=4, hbm_budget=28 * spec.n_fact)
-    qs = srv.run(serve.random_queries(8, 0, 6), realtime=False)
+    with serve.Server(tables, spec.n_fact, workers=4, hbm_budget=28 * spec.n_fact) as srv:
+        qs = srv.run(serve.random_queries(8, 0, 6), realtime=False)
     spans = sorted((q.start, q.finish) for q in qs)
     for (s0, f0), (s1, f1) in zip(spans, spans[1:]):
         assert s1 >= f0 - 1e-4
