@@ -56,6 +56,13 @@ constexpr int kOverflow = kStreams;  // extra bitmap: kept rows whose combinatio
 // d_counts index of each error stream
 __host__ __device__ constexpr int count_index(int e) { return e == kEFk ? 1 : (e == kEA1 ? 3 : (e == kEA2 ? 4 : 5)); }
 
+// The fast probe (k_star_probe2) runs when the build found it applicable (flags[5]) and the host
+// conditions hold; the general kernels run otherwise.  Both are launched; each returns at once
+// when it is not the one that applies.
+__device__ __forceinline__ bool fast_path(const uint32_t *flags, uint32_t fast_ok) {
+    return fast_ok != 0u && flags[5] != 0u;
+}
+
 __device__ __forceinline__ uint32_t table_slot(uint32_t key, uint32_t bits) {
     // xor-fold: dense small keys map to themselves (no collisions), wide keys are folded in
     return (key ^ (key >> bits) ^ (key >> 2 * bits)) & ((1u << bits) - 1u);
@@ -247,6 +254,14 @@ struct ProbeArgs {
     uint64_t cap[4];
     uint64_t *d_counts; // error counts at count_index(e), [8] kept rows
     LbArgs lb;
+    uint32_t fast_ok;   // host conditions of the fast probe (the general probe stands down when it runs)
+    // fast probe tables the same launch turns into ordered error positions (k_star_errfix)
+    uint64_t rows_per_wave;
+    uint32_t waves2;
+    const uint32_t *wcnt;      // [kStreams][waves2] per-wave counts
+    const uint32_t *wpre;      // [kStreams][waves2] exclusive prefixes over waves
+    const uint32_t *werows;    // [waves2] rows with any error per wave
+    const uint64_t *elist;     // [waves2][kErrList] the first of them: flags << 32 | row
 };
 
 __device__ __forceinline__ int64_t probe_table(const uint64_t *table, uint32_t bits, uint32_t key) {
@@ -285,6 +300,64 @@ __device__ __forceinline__ RowEval eval_row(const ProbeArgs &a, uint64_t fk, uin
     return r;
 }
 
+constexpr uint32_t kErrList = 16;  // error rows a fast-probe wave records in row order
+
+// Fast path: one wave per probe wave that saw errors writes that wave's error positions, each
+// stream at its exclusive prefix over the earlier waves (positions as in the general probe's
+// phase 2c).  From the recorded rows when they all fit the list, else by rescanning the range.
+__device__ void star_errfix(const ProbeArgs &a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t per_block = blockDim.x / 64u;
+    for (uint32_t gw = blockIdx.x * per_block + (threadIdx.x >> 6); gw < a.waves2; gw += gridDim.x * per_block) {
+        uint32_t any = 0;
+#pragma unroll
+        for (int e = 1; e < kStreams; ++e) any |= a.wcnt[e * a.waves2 + gw];
+        if (any == 0u) continue;
+        uint64_t idx[kStreams];
+#pragma unroll
+        for (int e = 1; e < kStreams; ++e) idx[e] = a.wpre[e * a.waves2 + gw];
+        const uint32_t nrows = a.werows[gw];
+        const uint64_t begin = static_cast<uint64_t>(gw) * a.rows_per_wave;
+        const uint64_t end = begin + a.rows_per_wave < a.n ? begin + a.rows_per_wave : a.n;
+        const bool listed = nrows <= kErrList;
+        const uint64_t span = listed ? nrows : end - begin;
+        for (uint64_t j0 = 0; j0 < span; j0 += 64u) {
+            const uint64_t j = j0 + lane;
+            bool f[kStreams] = {false, false, false, false, false};
+            uint64_t row = 0, dim_row = 0;
+            if (j < span) {
+                if (listed) {
+                    const uint64_t ent = a.elist[static_cast<size_t>(gw) * kErrList + j];
+                    row = static_cast<uint32_t>(ent);
+                    const uint32_t fl = static_cast<uint32_t>(ent >> 32);
+#pragma unroll
+                    for (int e = 1; e < kStreams; ++e) f[e] = (fl >> (e - 1)) & 1u;
+                    if (f[kEA1] || f[kEA2]) dim_row = eval_row(a, a.fk[row], a.m[row]).dim_row;
+                } else {
+                    row = begin + j;
+                    const RowEval r = eval_row(a, a.fk[row], a.m[row]);
+                    f[kEFk] = r.e_fk;
+                    f[kEA1] = r.joined && r.e_a1;
+                    f[kEA2] = r.joined && r.e_a2;
+                    f[kEM] = r.joined && r.e_m;
+                    dim_row = r.dim_row;
+                }
+            }
+#pragma unroll
+            for (int e = 1; e < kStreams; ++e) {
+                const uint64_t b = __ballot(f[e]);
+                if (f[e]) {
+                    const uint64_t i = idx[e] + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
+                    const uint64_t pos = (e == kEA1 || e == kEA2) ? dim_row : row;
+                    if (i < a.cap[e - 1]) a.err[e - 1][i] = pos * a.a_oid_err;
+                }
+                idx[e] += static_cast<uint64_t>(__popcll(b));
+            }
+        }
+    }
+}
+
 // 16 bits -> even bit positions of 32 (bit interleave)
 __device__ __forceinline__ uint32_t spread16(uint32_t x) {
     x &= 0xFFFFu;
@@ -307,6 +380,10 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
     __shared__ uint32_t s_new, s_any_overflow;
     __shared__ uint32_t s_flag[5];
 
+    if (fast_path(a.flags, a.fast_ok)) {  // the fast probe ran: only its error positions remain
+        star_errfix(a);
+        return;
+    }
     if (threadIdx.x < 5) s_flag[threadIdx.x] = a.flags[threadIdx.x];
     for (int i = threadIdx.x; i < (kStreams + 1) * kStarWords; i += kBlock) (&s_bits[0][0])[i] = 0u;
     for (uint32_t i = threadIdx.x; i < kLdsCids; i += kBlock) {
@@ -591,14 +668,70 @@ __global__ __launch_bounds__(kBlock) void k_star_probe(ProbeArgs a) {
     }
 }
 
+struct Reduce2Args {
+    const unsigned long long *part_sum, *part_first;
+    uint32_t parts;          // fast probe workgroups
+    const uint32_t *wpre;    // [kStreams][waves] exclusive prefixes over waves
+    uint32_t waves;
+    uint64_t *g_sum;
+    uint32_t *g_first;
+};
+
+// Fast path: one workgroup per combination id adds the per-workgroup partial sums; the first
+// kept-row rank is the kept rows of all earlier waves plus the rank inside the first wave holding it.
+__device__ void star_reduce2(const Reduce2Args &a, const uint16_t *cid_combo, uint32_t ncid) {
+    __shared__ unsigned long long s_s[kWaves], s_f[kWaves];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    for (uint32_t cid = blockIdx.x; cid < ncid; cid += gridDim.x) {
+        unsigned long long sm = 0ull, f = ~0ull;
+        for (uint32_t i = threadIdx.x; i < a.parts; i += blockDim.x) {
+            sm += a.part_sum[static_cast<size_t>(cid) * a.parts + i];
+            const unsigned long long x = a.part_first[static_cast<size_t>(cid) * a.parts + i];
+            f = x < f ? x : f;
+        }
+        sm = wave_sum(sm);
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const unsigned long long o = __shfl_xor(f, d);
+            f = o < f ? o : f;
+        }
+        if (lane == 0) {
+            s_s[wv] = sm;
+            s_f[wv] = f;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            sm = 0ull;
+            f = ~0ull;
+            for (uint32_t w = 0; w < blockDim.x / 64u; ++w) {
+                sm += s_s[w];
+                f = s_f[w] < f ? s_f[w] : f;
+            }
+            if (f != ~0ull) {  // some kept row has this combination
+                const uint32_t gw = static_cast<uint32_t>(f >> 32);
+                const uint32_t rank = a.wpre[kKept * a.waves + gw] + static_cast<uint32_t>(f);
+                const uint32_t combo = cid_combo[cid];
+                a.g_sum[combo] = sm;
+                a.g_first[combo] = 0xFFFFFFFFu - rank;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // One workgroup per LDS-accumulated combination id: adds the per-workgroup partial sums and takes
 // the smallest first rank into the global per-combination table (cids beyond the LDS table reached
 // it directly from k_star_probe).
 __global__ __launch_bounds__(kBlock) void k_star_reduce(const unsigned long long *part_sum, const uint32_t *part_rank,
                                                         uint32_t parts, const uint16_t *cid_combo,
-                                                        const uint32_t *flags, uint64_t *g_sum, uint32_t *g_first) {
+                                                        const uint32_t *flags, uint64_t *g_sum, uint32_t *g_first,
+                                                        uint32_t fast_ok, Reduce2Args r2) {
     __shared__ unsigned long long s_s[kWaves];
     __shared__ uint32_t s_r[kWaves];
+    if (fast_path(flags, fast_ok)) {
+        star_reduce2(r2, cid_combo, flags[1]);
+        return;
+    }
     const uint32_t ncid = flags[1];
     const uint32_t nslots = ncid < kLdsCids ? ncid : kLdsCids;
     if (flags[0] != 0u) return;
@@ -636,6 +769,291 @@ __global__ __launch_bounds__(kBlock) void k_star_reduce(const unsigned long long
             }
         }
         __syncthreads();
+    }
+}
+
+// ---- fast probe: LDS-resident key map, one contiguous fact range per wave ----------------------
+//
+// Applies when the dense key map is in use, there are at most 126 combinations and no build key
+// repeats (flags[5], set by k_star_pack).  Every dimension key of the map gets a 7-bit code:
+// 0 = no dimension row, 1..126 = combination id + 1 of a row with clean attributes,
+// 127 = look up the full 16-bit map entry (attribute errors).  Nine codes fill a 64-bit word;
+// the words of the first kPackKeys keys of the range sit in LDS for the whole launch, the codes
+// of the remaining keys in a byte map in HBM (L2-resident), and keys outside the range go to
+// the hash table.  Each wave owns one contiguous range of fact rows and keeps, in LDS, its own
+// sums per combination and the wave-local kept-row rank of each combination's first row; its
+// kept-row and error counts go to a per-wave table.  Nothing in the scan waits on another wave
+// or workgroup.  Rows with errors (rare) are recorded per wave in row order.  k_star_scan2 turns
+// the per-wave counts into prefixes; the general kernels' launches, which stand down on this path,
+// then finish it: k_star_reduce adds the per-workgroup sums and places the first ranks
+// (star_reduce2), k_star_probe writes the ordered error positions (star_errfix).
+constexpr int kP2Block = 1024;
+constexpr int kP2Waves = kP2Block / 64;
+constexpr int kP2Seg = 512;                 // rows per wave iteration: 4 x 128 (2 rows per lane)
+constexpr int kPackWords = 16384;           // 128 KiB of LDS
+constexpr uint32_t kPackKeys = kPackWords * 9u;
+constexpr uint32_t kP2Cids = 128;
+constexpr uint32_t kFastMaxCids = 126;
+constexpr uint32_t kCodeFull = 127;
+
+__device__ __forceinline__ uint32_t code_of_entry(uint32_t e) {
+    if (e == 0u) return 0u;
+    if (e & 0xC000u) return kCodeFull;
+    const uint32_t c1 = e & 0x3FFFu;  // combination id + 1
+    return c1 <= kFastMaxCids ? c1 : kCodeFull;
+}
+
+struct PackArgs {
+    const uint16_t *map;
+    uint64_t *packed;  // [kPackWords]
+    uint8_t *code8;    // [range - kPackKeys] codes of the keys beyond the packed ones
+    uint32_t *flags;
+};
+
+// Builds the packed codes and the byte map from the 16-bit map, and decides flags[5].
+__global__ __launch_bounds__(kBlock) void k_star_pack(PackArgs a) {
+    const bool fast = a.flags[0] == 0u && a.flags[4] != 0u && a.flags[1] <= kFastMaxCids;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.flags[5] = fast ? 1u : 0u;
+    if (!fast) return;
+    const uint32_t range = a.flags[3] - a.flags[2] + 1u;
+    const uint32_t kl = range < kPackKeys ? range : kPackKeys;
+    const uint32_t words = (kl + 8u) / 9u;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < words; q += stride) {
+        uint64_t w = 0ull;
+        for (uint32_t r = 0; r < 9u; ++r) {
+            const uint32_t k = q * 9u + r;
+            if (k < kl) w |= static_cast<uint64_t>(code_of_entry(a.map[k])) << (7u * r);
+        }
+        a.packed[q] = w;
+    }
+    for (uint32_t k = kl + blockIdx.x * kBlock + threadIdx.x; k < range; k += stride)
+        a.code8[k - kl] = static_cast<uint8_t>(code_of_entry(a.map[k]));
+}
+
+struct Probe2Args {
+    const uint64_t *fk, *m;
+    uint64_t n;
+    uint64_t rows_per_wave;  // multiple of kP2Seg
+    uint64_t fk_inv, fk_maxu, key_mask, m_inv, m_maxu;
+    const uint64_t *packed;
+    const uint8_t *code8;
+    const uint16_t *map;
+    const uint64_t *table;
+    uint32_t table_bits;
+    const uint32_t *info;
+    const uint32_t *flags;
+    uint32_t fast_ok;                 // host-side conditions (16-byte aligned fact columns, n > 0)
+    unsigned long long *part_sum;     // [kP2Cids][grid]
+    unsigned long long *part_first;   // [kP2Cids][grid]: global wave index << 32 | wave-local rank
+    uint32_t *wcnt;                   // [kStreams][waves]: kept, FK, attr1, attr2, measure errors
+    uint32_t *werows;                 // [waves] rows with any error
+    uint64_t *elist;                  // [waves][kErrList] the first of them, in row order
+};
+
+__global__ __launch_bounds__(kP2Block) void k_star_probe2(Probe2Args a) {
+    __shared__ uint64_t s_map[kPackWords];
+    __shared__ unsigned long long s_sum[kP2Waves][kP2Cids];
+    __shared__ uint32_t s_first[kP2Waves][kP2Cids];
+    if (!fast_path(a.flags, a.fast_ok)) return;
+    const uint32_t kmin = a.flags[2], krange = a.flags[3] - a.flags[2] + 1u;
+    const uint32_t kl = krange < kPackKeys ? krange : kPackKeys;
+    const uint32_t words = (kl + 8u) / 9u;
+    for (uint32_t q = threadIdx.x; q < words; q += kP2Block) s_map[q] = a.packed[q];
+    for (uint32_t i = threadIdx.x; i < kP2Waves * kP2Cids; i += kP2Block) {
+        (&s_sum[0][0])[i] = 0ull;
+        (&s_first[0][0])[i] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t gw = blockIdx.x * kP2Waves + wave;
+    const uint64_t begin = static_cast<uint64_t>(gw) * a.rows_per_wave;
+    const uint64_t end = begin + a.rows_per_wave < a.n ? begin + a.rows_per_wave : a.n;
+    uint32_t kept_before = 0;           // wave-uniform: kept rows of this wave so far
+    uint32_t err_rows = 0;              // wave-uniform: rows with any error so far
+    uint32_t ecnt[4] = {0, 0, 0, 0};    // per lane
+    unsigned long long *wsum = s_sum[wave];
+    uint32_t *wfirst = s_first[wave];
+
+    for (uint64_t seg = begin; seg < end; seg += kP2Seg) {
+        uint64_t fk[4][2], mv[4][2];
+        const bool full = seg + kP2Seg <= end;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const uint64_t r0 = seg + c * 128u + lane * 2u;
+            if (full) {
+                const u32x4 q = load16_nt(a.fk + r0);
+                const u32x4 w = load16_nt(a.m + r0);
+                __builtin_memcpy(&fk[c][0], &q, 16);
+                __builtin_memcpy(&mv[c][0], &w, 16);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    fk[c][k] = r0 + k < end ? a.fk[r0 + k] : 0ull;
+                    mv[c][k] = r0 + k < end ? a.m[r0 + k] : 0ull;
+                }
+            }
+        }
+        uint32_t code[4][2];
+        bool live[4][2], efk[4][2];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                live[c][k] = full || seg + c * 128u + lane * 2u + k < end;
+                const uint64_t d = fk[c][k] * a.fk_inv;
+                efk[c][k] = live[c][k] && d > a.fk_maxu;
+                ecnt[0] += efk[c][k] ? 1u : 0u;
+                const uint32_t rel = static_cast<uint32_t>(d & a.key_mask) - kmin;
+                uint32_t cd;
+                if (rel < kl) {
+                    const uint32_t q = rel / 9u;
+                    cd = static_cast<uint32_t>(s_map[q] >> (7u * (rel - q * 9u))) & 127u;
+                } else if (rel < krange) {
+                    cd = a.code8[rel - kl];
+                } else {
+                    cd = kCodeFull;
+                }
+                code[c][k] = live[c][k] ? cd : 0u;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            bool kept[2];
+            uint32_t cid[2], eflags[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                uint32_t cd = code[c][k];
+                bool e1 = false, e2 = false;
+                if (cd == kCodeFull) {  // rare: attribute errors, or a key outside the map range
+                    const uint64_t d = fk[c][k] * a.fk_inv;
+                    const uint32_t key = static_cast<uint32_t>(d & a.key_mask);
+                    uint32_t inf = 0u;
+                    bool joined = false;
+                    if (key - kmin < krange) {
+                        const uint32_t e = a.map[key - kmin];
+                        joined = e != 0u;
+                        inf = ((e >> 15) & 1u) << 30 | ((e >> 14) & 1u) << 31 | ((e & 0x3FFFu) - 1u);
+                    } else {
+                        const int64_t j = probe_table(a.table, a.table_bits, key);
+                        joined = j >= 0;
+                        if (joined) inf = a.info[j];
+                    }
+                    e1 = joined && ((inf >> 30) & 1u);
+                    e2 = joined && ((inf >> 31) & 1u);
+                    cd = joined ? ((inf & 0xFFFFu) + 1u) : 0u;
+                }
+                const bool joined = cd != 0u;
+                const bool em = joined && (mv[c][k] * a.m_inv) > a.m_maxu;
+                ecnt[1] += e1 ? 1u : 0u;
+                ecnt[2] += e2 ? 1u : 0u;
+                ecnt[3] += em ? 1u : 0u;
+                eflags[k] = (efk[c][k] ? 1u : 0u) | (e1 ? 2u : 0u) | (e2 ? 4u : 0u) | (em ? 8u : 0u);
+                kept[k] = joined && !e1 && !e2 && !em;
+                cid[k] = cd - 1u;
+            }
+            // wave-local kept-row ranks in row order (row = seg + c * 128 + lane * 2 + k)
+            const uint64_t b0 = __ballot(kept[0]), b1 = __ballot(kept[1]);
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b0 >> 32),
+                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b0), 0u)) +
+                                   __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b1 >> 32),
+                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b1), 0u));
+            // Both rows' "first of its combination in this wave" tests read the table before either
+            // row updates it: a lower lane's second row precedes a higher lane's first row.
+            bool fresh[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) fresh[k] = kept[k] && wfirst[cid[k]] == 0xFFFFFFFFu;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                if (!kept[k]) continue;
+                atomicAdd(&wsum[cid[k]], static_cast<unsigned long long>(mv[c][k]));
+                if (fresh[k]) atomicMin(&wfirst[cid[k]], kept_before + below + (k == 1 && kept[0] ? 1u : 0u));
+            }
+            kept_before += static_cast<uint32_t>(__popcll(b0) + __popcll(b1));
+            // rows with errors (rare): recorded in row order while the wave's list has room
+            const uint64_t x0 = __ballot(eflags[0] != 0u), x1 = __ballot(eflags[1] != 0u);
+            if ((x0 | x1) != 0ull) {
+                const uint32_t xb = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(x0 >> 32),
+                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(x0), 0u)) +
+                                    __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(x1 >> 32),
+                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(x1), 0u));
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    if (eflags[k] == 0u) continue;
+                    const uint32_t i = err_rows + xb + (k == 1 && eflags[0] != 0u ? 1u : 0u);
+                    if (i < kErrList)
+                        a.elist[static_cast<size_t>(gw) * kErrList + i] =
+                            (static_cast<uint64_t>(eflags[k]) << 32) | (seg + c * 128u + lane * 2u + k);
+                }
+                err_rows += static_cast<uint32_t>(__popcll(x0) + __popcll(x1));
+            }
+        }
+    }
+    // per-wave counts
+#pragma unroll
+    for (int e = 0; e < 4; ++e) ecnt[e] = wave_sum(ecnt[e]);
+    const uint32_t waves = gridDim.x * kP2Waves;
+    if (lane == 0) {
+        a.werows[gw] = err_rows;
+        a.wcnt[kKept * waves + gw] = kept_before;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) a.wcnt[(e + 1) * waves + gw] = ecnt[e];
+    }
+    __syncthreads();
+    // the workgroup's column of the partial tables: sums, and its first wave holding each cid
+    if (threadIdx.x < kP2Cids) {
+        const uint32_t cid = threadIdx.x;
+        unsigned long long sum = 0ull, first = ~0ull;
+#pragma unroll
+        for (int w = 0; w < kP2Waves; ++w) {
+            sum += s_sum[w][cid];
+            if (first == ~0ull && s_first[w][cid] != 0xFFFFFFFFu)
+                first = (static_cast<unsigned long long>(blockIdx.x * kP2Waves + w) << 32) | s_first[w][cid];
+        }
+        a.part_sum[static_cast<size_t>(cid) * gridDim.x + blockIdx.x] = sum;
+        a.part_first[static_cast<size_t>(cid) * gridDim.x + blockIdx.x] = first;
+    }
+}
+
+// One workgroup of 1024 threads: exclusive prefixes over the waves of every per-wave count (the
+// error positions' offsets and the kept rows before each wave) and the totals.
+__global__ __launch_bounds__(1024) void k_star_scan2(const uint32_t *wcnt, uint32_t *wpre, uint32_t waves,
+                                                      const uint32_t *flags, uint32_t fast_ok, uint64_t *d_counts) {
+    __shared__ uint32_t s_w[16];
+    if (!fast_path(flags, fast_ok)) return;
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    for (int s = 0; s < kStreams; ++s) {
+        uint32_t carry = 0;
+        for (uint32_t base = 0; base < waves; base += 4096u) {
+            // 4 consecutive waves per thread, loaded together
+            uint32_t v[4], t = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t i = base + threadIdx.x * 4u + q;
+                v[q] = i < waves ? wcnt[s * waves + i] : 0u;
+                t += v[q];
+            }
+            const uint32_t inc = wave_incl_scan(t);
+            if (lane == 63u) s_w[wv] = inc;
+            __syncthreads();
+            uint32_t pre = carry, tot = 0;
+#pragma unroll
+            for (int w = 0; w < 16; ++w) {
+                if (w < static_cast<int>(wv)) pre += s_w[w];
+                tot += s_w[w];
+            }
+            __syncthreads();
+            pre += inc - t;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t i = base + threadIdx.x * 4u + q;
+                if (i < waves) wpre[s * waves + i] = pre;
+                pre += v[q];
+            }
+            carry += tot;
+        }
+        if (threadIdx.x == 0) d_counts[s == kKept ? 8 : count_index(s)] = carry;
     }
 }
 
@@ -766,10 +1184,30 @@ struct StarLayout {
     LbPlan plan;
     size_t off_flags = 0, off_gsum = 0, off_gfirst = 0, off_table = 0, off_present = 0, off_combo_cid = 0,
            off_cid_combo = 0, off_info = 0, off_map = 0, off_psum = 0, off_prank = 0, total = 0;
+    size_t off_packed = 0, off_code8 = 0, off_psum2 = 0, off_pfirst2 = 0, off_wcnt = 0, off_wpre = 0,
+           off_werows = 0, off_elist = 0;
+    uint32_t grid2 = 0, waves2 = 0;  // fast probe workgroups and waves
+    uint64_t rows_per_wave = 0;
     size_t status_bytes = 0;  // scan state of the probe and of the build-side check
     uint32_t map_cap = 0;
     uint32_t grid = 0;  // persistent probe workgroups
 };
+
+// Fast probe grid: one workgroup per resident slot (its LDS map allows one per CU).
+uint32_t probe2_grid() {
+    static uint32_t resident = 0;
+    if (resident == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_star_probe2),
+                                                         kP2Block, 0) != hipSuccess || per_cu <= 0)
+            per_cu = 1;
+        resident = static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu);
+    }
+    return resident;
+}
 
 // Persistent probe grid: as many workgroups as can be resident at once (registers and LDS decide,
 // queried from the runtime), never more than there are tiles.
@@ -809,7 +1247,19 @@ StarLayout star_layout(const ahead_star1_args *in) {
     L.grid = probe_grid(L.plan.num_tiles);
     L.off_psum = L.off_map + align_up(L.map_cap * 2ull, 256);
     L.off_prank = L.off_psum + align_up(static_cast<size_t>(kLdsCids) * L.grid * 8, 256);
-    L.total = L.off_prank + align_up(static_cast<size_t>(kLdsCids) * L.grid * 4, 256);
+    L.grid2 = probe2_grid();
+    L.waves2 = L.grid2 * kP2Waves;
+    const uint64_t per = (in->fact_fk.n + L.waves2 - 1) / L.waves2;
+    L.rows_per_wave = (per + kP2Seg - 1) / kP2Seg * kP2Seg;
+    L.off_packed = L.off_prank + align_up(static_cast<size_t>(kLdsCids) * L.grid * 4, 256);
+    L.off_code8 = L.off_packed + align_up(kPackWords * 8ull, 256);
+    L.off_psum2 = L.off_code8 + align_up(L.map_cap, 256);
+    L.off_pfirst2 = L.off_psum2 + align_up(static_cast<size_t>(kP2Cids) * L.grid2 * 8, 256);
+    L.off_wcnt = L.off_pfirst2 + align_up(static_cast<size_t>(kP2Cids) * L.grid2 * 8, 256);
+    L.off_wpre = L.off_wcnt + align_up(static_cast<size_t>(kStreams) * L.waves2 * 4, 256);
+    L.off_werows = L.off_wpre + align_up(static_cast<size_t>(kStreams) * L.waves2 * 4, 256);
+    L.off_elist = L.off_werows + align_up(static_cast<size_t>(L.waves2) * 4, 256);
+    L.total = L.off_elist + align_up(static_cast<size_t>(L.waves2) * kErrList * 8, 256);
     const size_t chk = check_an_status_bytes(in->dim_key.n, in->dim_key.width);
     L.status_bytes = L.plan.status_bytes > chk ? L.plan.status_bytes : chk;
     return L;
@@ -874,6 +1324,12 @@ extern "C" int ahead_hip_star1_build(const ahead_star1_args *in, const ahead_sta
     }
     hipLaunchKernelGGL(k_star_map_clear, dim3(512), dim3(kBlock), 0, s, d);
     if (b.n) hipLaunchKernelGGL(k_star_map_fill, gd, dim3(kBlock), 0, s, d);
+    PackArgs pk{};
+    pk.map = d.map;
+    pk.packed = reinterpret_cast<uint64_t *>(ws_region(ws, L.off_packed));
+    pk.code8 = reinterpret_cast<uint8_t *>(ws_region(ws, L.off_code8));
+    pk.flags = b.flags;
+    hipLaunchKernelGGL(k_star_pack, dim3(64), dim3(kBlock), 0, s, pk);
     AHEAD_HIP_TRY(hipGetLastError());
     // the build side's error vector: checkAN over the dimension keys (hashjoinAN.tcc:179-181)
     return check_an_ws(&in->dim_key, in->a_oid_err, out->e_dk, out->e_dk ? out->e_dk_cap : 0,
@@ -915,10 +1371,59 @@ extern "C" int ahead_hip_star1_probe(const ahead_star1_args *in, const ahead_sta
     p.err[2] = out->e_a2; p.cap[2] = out->e_a2 ? out->e_a2_cap : 0;
     p.err[3] = out->e_m; p.cap[3] = out->e_m ? out->e_m_cap : 0;
     p.d_counts = out->d_counts;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(p.fk) | reinterpret_cast<uintptr_t>(p.m)) & 15u) == 0;
+    p.fast_ok = aligned && p.n > 0 ? 1u : 0u;
     if ((st = lb_args(ws, L.plan, &p.lb)) != AHEAD_OK) return st;
+
+    Probe2Args q{};
+    q.fk = p.fk;
+    q.m = p.m;
+    q.n = p.n;
+    q.rows_per_wave = L.rows_per_wave;
+    q.fk_inv = p.fk_inv;
+    q.fk_maxu = p.fk_maxu;
+    q.key_mask = p.key_mask;
+    q.m_inv = p.m_inv;
+    q.m_maxu = p.m_maxu;
+    q.packed = reinterpret_cast<const uint64_t *>(ws_region(ws, L.off_packed));
+    q.code8 = reinterpret_cast<const uint8_t *>(ws_region(ws, L.off_code8));
+    q.map = p.map;
+    q.table = p.table;
+    q.table_bits = p.table_bits;
+    q.info = p.info;
+    q.flags = p.flags;
+    q.fast_ok = p.fast_ok;
+    q.part_sum = reinterpret_cast<unsigned long long *>(ws_region(ws, L.off_psum2));
+    q.part_first = reinterpret_cast<unsigned long long *>(ws_region(ws, L.off_pfirst2));
+    q.wcnt = reinterpret_cast<uint32_t *>(ws_region(ws, L.off_wcnt));
+    q.werows = reinterpret_cast<uint32_t *>(ws_region(ws, L.off_werows));
+    q.elist = reinterpret_cast<uint64_t *>(ws_region(ws, L.off_elist));
+    uint32_t *wpre = reinterpret_cast<uint32_t *>(ws_region(ws, L.off_wpre));
+    p.rows_per_wave = L.rows_per_wave;
+    p.waves2 = L.waves2;
+    p.wcnt = q.wcnt;
+    p.wpre = wpre;
+    p.werows = q.werows;
+    p.elist = q.elist;
+    Reduce2Args r2{};
+    r2.part_sum = q.part_sum;
+    r2.part_first = q.part_first;
+    r2.parts = L.grid2;
+    r2.wpre = wpre;
+    r2.waves = L.waves2;
+    r2.g_sum = p.g_sum;
+    r2.g_first = p.g_first;
+
+    // fast path: probe2 + scan2, then the general kernels' launches finish it (or, when it does not
+    // apply, probe2 and scan2 return at once and the general kernels run)
+    if (p.fast_ok) {
+        hipLaunchKernelGGL(k_star_probe2, dim3(L.grid2), dim3(kP2Block), 0, s, q);
+        hipLaunchKernelGGL(k_star_scan2, dim3(1), dim3(1024), 0, s, q.wcnt, wpre, L.waves2, p.flags, p.fast_ok,
+                           p.d_counts);
+    }
     hipLaunchKernelGGL(k_star_probe, dim3(L.grid), dim3(kBlock), 0, s, p);
     hipLaunchKernelGGL(k_star_reduce, dim3(256), dim3(kBlock), 0, s, p.part_sum, p.part_rank, L.grid, p.cid_combo,
-                       p.flags, p.g_sum, p.g_first);
+                       p.flags, p.g_sum, p.g_first, p.fast_ok, r2);
     AHEAD_HIP_TRY(hipGetLastError());
 
     FinalArgs f{};

@@ -34,6 +34,46 @@ def test_star1_parity(n_fact, n_dim, rate, seed):
         assert np.array_equal(got[k], ref[k]), k
 
 
+@pytest.mark.parametrize("card1,card2,n_dim,rate,seed", [
+    (18, 7, 3_000, 2e-3, 41),     # 126 combinations: the most the LDS-map probe takes
+    (127, 1, 3_000, 2e-3, 42),    # 127 combinations: the general probe
+    (25, 5, 300_000, 1e-3, 43),   # keys beyond the LDS-resident part of the map (byte map in HBM)
+    (25, 5, 3_000, 1e-2, 44),     # errors in almost every wave: every wave rescanned for positions
+])
+def test_star1_parity_fast_probe_boundaries(card1, card2, n_dim, rate, seed):
+    spec = W.Star1Spec(n_fact=1_000_003, n_dim=n_dim, card1=card1, card2=card2, rate=rate, seed=seed)
+    _, ref, got = run_both(spec)
+    assert got["status"] == 0
+    for k in KEYS:
+        assert np.array_equal(got[k], ref[k]), k
+
+
+def _offset_column(arr, meta, offset):
+    """A device column whose data starts `offset` bytes into its allocation."""
+    base = capi.DeviceBuffer.from_numpy(np.concatenate([np.zeros(offset, np.uint8), arr.view(np.uint8)]))
+    b = capi.DeviceBuffer.__new__(capi.DeviceBuffer)
+    b.t = base.t[offset:]
+    b.nbytes = arr.nbytes
+    return capi.Column(b, arr.size, meta)
+
+
+def test_star1_parity_general_probe_on_unaligned_fact_columns():
+    # fact columns 8 bytes off a 16-byte boundary: the general probe runs on the same data
+    spec = W.Star1Spec(n_fact=700_001, n_dim=5_000, rate=1e-3, seed=45)
+    h = W.star1_host(spec)
+    oc = (O.Col(h["fk"], spec.meta_key), O.Col(h["m"], spec.meta_m), O.Col(h["key"], spec.meta_key),
+          O.Col(h["a1"], spec.meta_attr), O.Col(h["a2"], spec.meta_attr))
+    ref = O.star1_pipeline(*oc)
+    dc = (_offset_column(h["fk"], spec.meta_key, 8), _offset_column(h["m"], spec.meta_m, 8),
+          capi.Column.from_numpy(h["key"], spec.meta_key), capi.Column.from_numpy(h["a1"], spec.meta_attr),
+          capi.Column.from_numpy(h["a2"], spec.meta_attr))
+    assert dc[0].buf.ptr % 16 == 8
+    got = capi.star1_pipeline(*dc)
+    assert got["status"] == 0
+    for k in KEYS:
+        assert np.array_equal(got[k], ref[k]), k
+
+
 def test_star1_parity_small_tiles_and_many_groups():
     # 256 x 256 attribute values: up to 65536 groups (more than one tile's LDS table holds)
     spec = W.Star1Spec(n_fact=400_000, n_dim=70_000, card1=256, card2=256, rate=1e-4, seed=11)
