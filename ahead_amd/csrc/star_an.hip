@@ -309,14 +309,11 @@ __device__ void star_errfix(const ProbeArgs &a) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t per_block = blockDim.x / 64u;
     for (uint32_t gw = blockIdx.x * per_block + (threadIdx.x >> 6); gw < a.waves2; gw += gridDim.x * per_block) {
-        uint32_t any = 0;
-#pragma unroll
-        for (int e = 1; e < kStreams; ++e) any |= a.wcnt[e * a.waves2 + gw];
-        if (any == 0u) continue;
+        const uint32_t nrows = a.werows[gw];  // rows with any error
+        if (nrows == 0u) continue;
         uint64_t idx[kStreams];
 #pragma unroll
         for (int e = 1; e < kStreams; ++e) idx[e] = a.wpre[e * a.waves2 + gw];
-        const uint32_t nrows = a.werows[gw];
         const uint64_t begin = static_cast<uint64_t>(gw) * a.rows_per_wave;
         const uint64_t end = begin + a.rows_per_wave < a.n ? begin + a.rows_per_wave : a.n;
         const bool listed = nrows <= kErrList;
@@ -790,7 +787,7 @@ __global__ __launch_bounds__(kBlock) void k_star_reduce(const unsigned long long
 constexpr int kP2Block = 1024;
 constexpr int kP2Waves = kP2Block / 64;
 constexpr int kP2Seg = 512;                 // rows per wave iteration: 4 x 128 (2 rows per lane)
-constexpr int kPackWords = 16384;           // 128 KiB of LDS
+constexpr int kPackWords = 19200;           // 150 KiB of LDS
 constexpr uint32_t kPackKeys = kPackWords * 9u;
 constexpr uint32_t kP2Cids = 128;
 constexpr uint32_t kFastMaxCids = 126;
@@ -852,18 +849,19 @@ struct Probe2Args {
 };
 
 __global__ __launch_bounds__(kP2Block) void k_star_probe2(Probe2Args a) {
+    // LDS (of 160 KiB): the packed codes, the workgroup's sums per combination (LDS atomics
+    // conflict only within one instruction, so one table serves all waves), per-wave first ranks
     __shared__ uint64_t s_map[kPackWords];
-    __shared__ unsigned long long s_sum[kP2Waves][kP2Cids];
+    __shared__ unsigned long long s_sum[kP2Cids];
     __shared__ uint32_t s_first[kP2Waves][kP2Cids];
+    static_assert(sizeof(s_map) + sizeof(s_sum) + sizeof(s_first) <= 160 * 1024, "LDS budget");
     if (!fast_path(a.flags, a.fast_ok)) return;
     const uint32_t kmin = a.flags[2], krange = a.flags[3] - a.flags[2] + 1u;
     const uint32_t kl = krange < kPackKeys ? krange : kPackKeys;
     const uint32_t words = (kl + 8u) / 9u;
     for (uint32_t q = threadIdx.x; q < words; q += kP2Block) s_map[q] = a.packed[q];
-    for (uint32_t i = threadIdx.x; i < kP2Waves * kP2Cids; i += kP2Block) {
-        (&s_sum[0][0])[i] = 0ull;
-        (&s_first[0][0])[i] = 0xFFFFFFFFu;
-    }
+    for (uint32_t i = threadIdx.x; i < kP2Waves * kP2Cids; i += kP2Block) (&s_first[0][0])[i] = 0xFFFFFFFFu;
+    for (uint32_t i = threadIdx.x; i < kP2Cids; i += kP2Block) s_sum[i] = 0ull;
     __syncthreads();
 
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -872,8 +870,8 @@ __global__ __launch_bounds__(kP2Block) void k_star_probe2(Probe2Args a) {
     const uint64_t end = begin + a.rows_per_wave < a.n ? begin + a.rows_per_wave : a.n;
     uint32_t kept_before = 0;           // wave-uniform: kept rows of this wave so far
     uint32_t err_rows = 0;              // wave-uniform: rows with any error so far
+    uint32_t unseen = a.flags[1];       // wave-uniform: combinations without a first row in this wave
     uint32_t ecnt[4] = {0, 0, 0, 0};    // per lane
-    unsigned long long *wsum = s_sum[wave];
     uint32_t *wfirst = s_first[wave];
 
     for (uint64_t seg = begin; seg < end; seg += kP2Seg) {
@@ -961,14 +959,21 @@ __global__ __launch_bounds__(kP2Block) void k_star_probe2(Probe2Args a) {
                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b1), 0u));
             // Both rows' "first of its combination in this wave" tests read the table before either
             // row updates it: a lower lane's second row precedes a higher lane's first row.
-            bool fresh[2];
 #pragma unroll
-            for (int k = 0; k < 2; ++k) fresh[k] = kept[k] && wfirst[cid[k]] == 0xFFFFFFFFu;
+            for (int k = 0; k < 2; ++k)
+                if (kept[k]) atomicAdd(&s_sum[cid[k]], static_cast<unsigned long long>(mv[c][k]));
+            if (unseen != 0u) {
+                bool fresh[2], first[2] = {false, false};
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                if (!kept[k]) continue;
-                atomicAdd(&wsum[cid[k]], static_cast<unsigned long long>(mv[c][k]));
-                if (fresh[k]) atomicMin(&wfirst[cid[k]], kept_before + below + (k == 1 && kept[0] ? 1u : 0u));
+                for (int k = 0; k < 2; ++k) fresh[k] = kept[k] && wfirst[cid[k]] == 0xFFFFFFFFu;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    // the one lane whose update finds the entry empty counts the combination as seen
+                    if (fresh[k])
+                        first[k] = atomicMin(&wfirst[cid[k]], kept_before + below + (k == 1 && kept[0] ? 1u : 0u)) ==
+                                   0xFFFFFFFFu;
+                }
+                unseen -= static_cast<uint32_t>(__popcll(__ballot(first[0])) + __popcll(__ballot(first[1])));
             }
             kept_before += static_cast<uint32_t>(__popcll(b0) + __popcll(b1));
             // rows with errors (rare): recorded in row order while the wave's list has room
@@ -1004,10 +1009,10 @@ __global__ __launch_bounds__(kP2Block) void k_star_probe2(Probe2Args a) {
     // the workgroup's column of the partial tables: sums, and its first wave holding each cid
     if (threadIdx.x < kP2Cids) {
         const uint32_t cid = threadIdx.x;
-        unsigned long long sum = 0ull, first = ~0ull;
+        const unsigned long long sum = s_sum[cid];
+        unsigned long long first = ~0ull;
 #pragma unroll
         for (int w = 0; w < kP2Waves; ++w) {
-            sum += s_sum[w][cid];
             if (first == ~0ull && s_first[w][cid] != 0xFFFFFFFFu)
                 first = (static_cast<unsigned long long>(blockIdx.x * kP2Waves + w) << 32) | s_first[w][cid];
         }
@@ -1016,14 +1021,15 @@ __global__ __launch_bounds__(kP2Block) void k_star_probe2(Probe2Args a) {
     }
 }
 
-// One workgroup of 1024 threads: exclusive prefixes over the waves of every per-wave count (the
-// error positions' offsets and the kept rows before each wave) and the totals.
+// One workgroup of 1024 threads per count stream: exclusive prefixes over the waves (the error
+// positions' offsets and the kept rows before each wave) and the stream's total.
 __global__ __launch_bounds__(1024) void k_star_scan2(const uint32_t *wcnt, uint32_t *wpre, uint32_t waves,
                                                       const uint32_t *flags, uint32_t fast_ok, uint64_t *d_counts) {
     __shared__ uint32_t s_w[16];
     if (!fast_path(flags, fast_ok)) return;
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    for (int s = 0; s < kStreams; ++s) {
+    {
+        const int s = static_cast<int>(blockIdx.x);
         uint32_t carry = 0;
         for (uint32_t base = 0; base < waves; base += 4096u) {
             // 4 consecutive waves per thread, loaded together
@@ -1099,6 +1105,29 @@ __global__ __launch_bounds__(1024) void k_star_final(FinalArgs a) {
             a.d_counts[0] = 0;
             a.d_counts[6] = 0;
             a.d_counts[7] = skip ? 1u : 2u;
+        }
+        return;
+    }
+    if (n <= blockDim.x) {
+        // few groups: each key's place is the number of smaller keys (ranks are distinct)
+        if (threadIdx.x < n) {
+            const unsigned long long key = s_key[threadIdx.x];
+            uint32_t g = 0;
+            for (uint32_t j = 0; j < n; ++j) g += s_key[j] < key ? 1u : 0u;
+            const uint32_t c = static_cast<uint32_t>(key & 0xFFFFu);
+            if (g < a.cap) {
+                a.sums[g] = a.g_sum[c] * a.sum_factor;
+                a.groid[g] = (key >> 16) * a.a_oid_grp;
+                a.key1[g] = static_cast<uint16_t>(static_cast<uint32_t>(c >> 8) * a.a1_a);
+                a.key2[g] = static_cast<uint16_t>(static_cast<uint32_t>(c & 0xFFu) * a.a2_a);
+            }
+            a.g_first[c] = 0u;
+            a.g_sum[c] = 0ull;
+        }
+        if (threadIdx.x == 0) {
+            a.d_counts[0] = n;
+            a.d_counts[6] = 0;  // the grouped sum's group-id check: every kept row has a valid group
+            a.d_counts[7] = 0;
         }
         return;
     }
@@ -1418,7 +1447,7 @@ extern "C" int ahead_hip_star1_probe(const ahead_star1_args *in, const ahead_sta
     // apply, probe2 and scan2 return at once and the general kernels run)
     if (p.fast_ok) {
         hipLaunchKernelGGL(k_star_probe2, dim3(L.grid2), dim3(kP2Block), 0, s, q);
-        hipLaunchKernelGGL(k_star_scan2, dim3(1), dim3(1024), 0, s, q.wcnt, wpre, L.waves2, p.flags, p.fast_ok,
+        hipLaunchKernelGGL(k_star_scan2, dim3(kStreams), dim3(1024), 0, s, q.wcnt, wpre, L.waves2, p.flags, p.fast_ok,
                            p.d_counts);
     }
     hipLaunchKernelGGL(k_star_probe, dim3(L.grid), dim3(kBlock), 0, s, p);
