@@ -90,6 +90,18 @@ def cpu_baseline_select(codes_np, meta, lo, hi, seconds, expect):
     }
 
 
+def profiled_traffic(name: str):
+    """HBM bytes per launch of a workload's dominant kernel from its committed rocprofv3 counter
+    summary (profiles/<name>.md, line `traffic_bytes_per_launch: N`), or None."""
+    path = ROOT / "profiles" / f"{name}.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1])
+    return None
+
+
 def run_select_sum(args):
     import numpy as np
     import torch
@@ -154,7 +166,8 @@ def run_select_sum(args):
                                "1e-6 flips, 1<=v<=24" % n, "rows": n, "matches": n_oids, "errors": n_err,
                    "decoded_sum": total},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic("select_sum_traffic_1e9") if n == 1_000_000_000 else None,
                      "kernel": "k_select<uint32_t,AND,SUM>", "kernel_ms": kern_ms,
                      "algorithmic_bytes": algo_bytes},
     }
@@ -219,7 +232,7 @@ def run_star1(args):
     el = time.perf_counter() - t0
     step_ms = sum(e0.elapsed_time(e1) for e0, e1 in ev) / len(ev)
     res_out = st.results()
-    # the dominant kernel (k_star_probe) timed alone with HIP events on the launch stream
+    # the probe side (dominant kernel k_star_probe2) timed with HIP events on the launch stream
     probe_ms = time_star_probe(st, args.steps)
     nf = spec.n_fact
     algo_bytes = 16 * nf  # FK + measure codes read once; dimension (2.4 MB) and outputs excluded
@@ -245,7 +258,10 @@ def run_star1(args):
                    "errors": {k: int(res_out[k].size) for k in ("e_fk", "e_dk", "e_a1", "e_a2", "e_m")},
                    "step_kernels_ms": step_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_star_probe (+k_star_final)",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic("star1_traffic_60M") if (nf, spec.n_dim) == (60_000_000, 200_000)
+                     else None,
+                     "kernel": "probe side: k_star_probe2 (+ k_star_scan2, k_star_probe, k_star_reduce, k_star_final)",
                      "kernel_ms": probe_ms, "algorithmic_bytes": algo_bytes},
     }
     if not args.no_cpu:
@@ -254,8 +270,8 @@ def run_star1(args):
 
 
 def time_star_probe(st, steps):
-    """Average duration of the fact-side launch (k_star_probe + the one-workgroup k_star_final)
-    from HIP events on the launch stream, with the dimension table built once beforehand."""
+    """Average duration of the probe side (k_star_probe2 and the small kernels that finish it) from
+    HIP events on the launch stream, with the dimension table built once beforehand."""
     import torch
 
     from ahead_amd import capi
