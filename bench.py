@@ -57,7 +57,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="star1", choices=["star1", "select_sum", "batch", "ssb", "serve", "hbm_copy"])
+    ap.add_argument("--workload", default="star1", choices=["star1", "select_sum", "ops", "batch", "ssb", "serve",
+                                                                 "hbm_copy"])
     ap.add_argument("--rows", type=int, default=None, help="override the row count (scaling studies)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -430,6 +431,157 @@ def cpu_baseline_batch(spec, jobs, res_b, seconds):
                       f"bit-exact vs GPU: {ok}"}
 
 
+def run_ops(args):
+    """Every hardened operator alone at the join + group-by workload's sizes (60 M fact rows, 200 k
+    dimension keys): HIP-event time of one entry-point call on preallocated outputs, algorithmic
+    bytes (inputs read once, outputs written once), and the fraction of HBM peak."""
+    import numpy as np
+    import torch
+
+    from ahead_amd import an, capi, workloads as W
+
+    lib = capi.load()
+    spec = W.Star1Spec()
+    if args.rows:
+        spec.n_fact = args.rows
+    n, nd = spec.n_fact, spec.n_dim
+    fk, m, key, a1, a2 = W.star1_device(spec)
+    A_OID, A_ERR = an.A_OID, an.A_OID_ERR
+    resoid = an.an_meta(an.RESOID, A_OID)
+    # derived fact-side columns: OIDs into the dimension, 8-bit fact attribute, group ids, a second measure
+    oids = capi.DeviceBuffer(8 * n)
+    capi.gen_uniform(oids, 8, n, spec.seed, 31, 0, nd - 1, A_OID)
+    c_oids = capi.Column(oids, n, resoid)
+    fattr = capi.DeviceBuffer(2 * n)
+    capi.gen_uniform(fattr, 2, n, spec.seed, 32, 0, 24, W.A_ATTR)
+    c_fattr = capi.Column(fattr, n, spec.meta_attr)
+    gids = capi.DeviceBuffer(8 * n)
+    capi.gen_uniform(gids, 8, n, spec.seed, 33, 0, 124, A_OID)
+    c_gids = capi.Column(gids, n, resoid)
+    m2 = capi.DeviceBuffer(8 * n)
+    capi.gen_uniform(m2, 8, n, spec.seed, 34, 0, 90_000, W.A_MEASURE)
+    c_m2 = capi.Column(m2, n, spec.meta_m)
+    void_n = capi.Column.void(n)
+    void_d = capi.Column.void(nd)
+    ws = capi.workspace()
+    stream = capi.current_stream()
+    ecap = 1 << 16
+    errs = [capi.DeviceBuffer(8 * ecap) for _ in range(4)]
+    eb = [capi.Buf(e.ptr, ecap) for e in errs]
+    cnt = capi.DeviceBuffer(8 * 8)
+    out_a = capi.DeviceBuffer(8 * n)
+    out_b = capi.DeviceBuffer(8 * n)
+    total = capi.DeviceBuffer(8)
+    sums = capi.DeviceBuffer(8 * 1024)
+    torch.cuda.synchronize()
+
+    def desc(c):
+        return ctypes.byref(c.desc())
+
+    def timed(call, reps):
+        d = []
+        for i in range(reps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            capi.check(call())
+            e1.record()
+            if i:
+                d.append((e0, e1))
+        torch.cuda.synchronize()
+        return sum(a.elapsed_time(b) for a, b in d) / len(d)
+
+    def counts(k):
+        return [int(x) for x in cnt.to_numpy(np.uint64, k)]
+
+    reps = max(3, min(args.steps, 20))
+    ops = []
+
+    def record(name, call, bytes_of, note=""):
+        ms = timed(call, reps)
+        c = counts(8)
+        b = bytes_of(c)
+        gbs = b / (ms * 1e-3) / 1e9
+        ops.append({"op": name, "ms": ms, "algorithmic_bytes": b, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                    "counts": c[:6], "note": note})
+
+    # selectAN<>=, <=> on the measure codes (about half the rows match)
+    mid = (90_000 + 10_500_000) // 2
+    record("selectAN (range, 64-bit codes)",
+           lambda: lib.ahead_hip_select_an(desc(m), capi.GE, 90_000 * W.A_MEASURE, capi.LE, mid * W.A_MEASURE,
+                                           capi.COMB_AND, A_OID, A_ERR, out_a.ptr, n, errs[0].ptr, ecap, cnt.ptr,
+                                           None, ws.h, stream),
+           lambda c: 8 * n + 8 * c[0] + 8 * c[1])
+    record("checkAN (64-bit codes)",
+           lambda: lib.ahead_hip_check_an(desc(m), A_ERR, errs[0].ptr, ecap, cnt.ptr, ws.h, stream),
+           lambda c: 8 * n + 8 * c[0])
+    record("checkAndDecodeAN (64-bit codes -> 32-bit values)",
+           lambda: lib.ahead_hip_check_and_decode_an(desc(m), A_ERR, out_a.ptr, n, errs[0].ptr, ecap, cnt.ptr, ws.h,
+                                                     stream),
+           lambda c: 8 * n + 4 * c[0] + 8 * c[1])
+    record("fetchjoinAN (60M OIDs into a 200k-row 16-bit column)",
+           lambda: lib.ahead_hip_fetchjoin_an(desc(c_oids), desc(a1), 0, 0, A_ERR, capi.Buf(out_a.ptr, n), eb[0], eb[1],
+                                              cnt.ptr, ws.h, stream),
+           lambda c: 8 * n + 2 * c[0] + 8 * (c[1] + c[2]), "dimension column (400 KB) cache-resident, not counted")
+    # hashjoinAN(fact <void, FK>, reverse(dim <void, key>)): pairs (fact OID, dim OID)
+    record("hashjoinAN (60M probe rows, 200k build keys)",
+           lambda: lib.ahead_hip_hashjoin_an(desc(void_n), desc(fk), desc(key), desc(void_d), 1, 0, 0, 4, 0, 0, 4,
+                                             A_ERR, capi.Buf(out_a.ptr, n), capi.Buf(out_b.ptr, n), *eb, cnt.ptr, ws.h,
+                                             stream),
+           lambda c: 8 * n + 8 * nd + 8 * c[0] + 8 * (c[1] + c[2] + c[3] + c[4]),
+           "output pairs as 4-byte OIDs (void heads); build side (1.6 MB) counted once")
+    record("groupbyAN unary (60M 16-bit codes, 25 groups)",
+           lambda: lib.ahead_hip_groupby_an(desc(void_n), desc(c_fattr), A_OID, 0, out_a.ptr,
+                                            capi.Buf(out_b.ptr, n), eb[0], eb[1], cnt.ptr, ws.h, stream),
+           lambda c: 2 * n + 8 * n + 8 * c[1] + 8 * (c[2] + c[3]))
+    record("aggregate_sum_groupedAN (60M measures, 125 groups)",
+           lambda: lib.ahead_hip_aggregate_sum_grouped_an(desc(void_n), desc(m), desc(c_gids), 125, an.A_RESULT, A_OID,
+                                                          sums.ptr, eb[0], eb[1], eb[2], cnt.ptr, ws.h, stream),
+           lambda c: 8 * n + 8 * n + 8 * 125 + 8 * (c[1] + c[2] + c[3]))
+    record("aggregate_mul_sumAN (60M 64-bit x 16-bit codes)",
+           lambda: lib.ahead_hip_aggregate_mul_sum_an(desc(m), desc(c_fattr), 0, 1, an.A_RESULT, A_ERR, total.ptr, eb[0],
+                                                      eb[1], cnt.ptr, ws.h, stream),
+           lambda c: 8 * n + 2 * n + 8 * (c[1] + c[2]))
+    record("arithmeticAN SUB (60M + 60M 64-bit codes -> 64-bit)",
+           lambda: lib.ahead_hip_arithmetic_an(desc(void_n), desc(m), desc(void_n), desc(c_m2), 1, an.A_RESULT, 8, A_ERR,
+                                               out_a.ptr, *eb, cnt.ptr, ws.h, stream),
+           lambda c: 8 * n + 8 * n + 8 * n + 8 * (c[1] + c[2] + c[3] + c[4]))
+    # matchjoinAN merge form on two ascending OID lists (the Q1.1 selection intersection)
+    capi.check(lib.ahead_hip_select_an(desc(m), capi.LE, mid * W.A_MEASURE, capi.LT, 0, capi.COMB_NONE, A_OID, A_ERR,
+                                       out_a.ptr, n, errs[0].ptr, ecap, cnt.ptr, None, ws.h, stream))
+    na = counts(1)[0]
+    capi.check(lib.ahead_hip_select_an(desc(c_fattr), capi.LE, 12 * W.A_ATTR, capi.LT, 0, capi.COMB_NONE, A_OID, A_ERR,
+                                       out_b.ptr, n, errs[0].ptr, ecap, cnt.ptr, None, ws.h, stream))
+    nb = counts(1)[0]
+    la, lb = capi.Column(out_a, na, resoid), capi.Column(out_b, nb, resoid)
+    mo_h, mo_t = capi.DeviceBuffer(8 * min(na, nb)), capi.DeviceBuffer(8 * min(na, nb))
+    record("matchjoinAN merge (two ascending OID lists)",
+           lambda: lib.ahead_hip_matchjoin_merge_an(desc(la), desc(la), desc(lb), desc(lb), 0, 0, 8, 0, 0, 8, A_ERR,
+                                                    capi.Buf(mo_h.ptr, min(na, nb)), capi.Buf(mo_t.ptr, min(na, nb)),
+                                                    *eb, cnt.ptr, ws.h, stream),
+           lambda c: 8 * na + 8 * nb + 16 * c[0] + 8 * (c[1] + c[2] + c[3] + c[4]),
+           f"lists of {na} and {nb} OIDs; matched rows' second side re-read counted once")
+    worst = min(ops, key=lambda o: o["frac"])
+    return {
+        "metric": "AN-hardened operators alone at the 60M-row join workload's sizes: % of HBM peak (lowest operator)",
+        "value": worst["frac"],
+        "unit": "fraction of HBM peak",
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": 1,
+        "ms_per_step": sum(o["ms"] for o in ops),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16/u32/u64",
+        "data": "synthetic (seeded, generated on device)",
+        "config": {"workload": f"ops: {n} fact rows, {nd} dimension keys; one entry-point call per operator",
+                   "ops": ops},
+        "roofline": {"bound": "hbm", "achieved": worst["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": worst["frac"], "traffic": None, "kernel": worst["op"], "kernel_ms": worst["ms"],
+                     "algorithmic_bytes": worst["algorithmic_bytes"]},
+    }
+
+
 def run_hbm_copy(args):
     import torch
 
@@ -620,7 +772,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs an MI355X (cuda:0)"
     torch.cuda.set_device(0)
     run = {"star1": run_star1, "select_sum": run_select_sum, "batch": run_batch, "ssb": run_ssb,
-           "serve": run_serve, "hbm_copy": run_hbm_copy}[args.workload]
+           "serve": run_serve, "hbm_copy": run_hbm_copy, "ops": run_ops}[args.workload]
     res = run(args)
     print(json.dumps(res), flush=True)
 
