@@ -7,7 +7,7 @@
 // Every check applies the code-width cast (static_cast<T>(code * A^-1)).  A few reference lines
 // compare the int-promoted product of 16-bit codes without it (matchjoinAN.tcc:278 and :342,
 // hashjoinAN.tcc:153); no query path feeds 16-bit codes to them.
-#include "rowtile.hpp"
+#include "maptile.hpp"
 
 using namespace ahead_dev;
 
@@ -32,6 +32,46 @@ struct FetchOp {
     uint64_t e1_cap, e2_cap;
     uint64_t *d_counts;
 
+    static constexpr bool kBatched = true;
+    static constexpr int kOuts = 1;
+    __device__ void out_arrays(void *(&p)[2], uint32_t (&w)[2]) const {
+        p[0] = out;
+        w[0] = ow;
+        p[1] = nullptr;
+        w[1] = 0;
+    }
+    template <int G>
+    __device__ void eval_n(const uint64_t (&row)[G], const bool (&live)[G], State (&st)[G], uint32_t (&errs)[G],
+                           uint32_t (&cnt)[G], uint64_t &) const {
+        uint64_t o[G], t[G], v[G];
+        bool in[G];
+        oids.raw_n<G>(row, live, o);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            t[g] = o[g] * oids.a_inv;
+            in[g] = live[g] && t[g] < col.n;
+        }
+        col.raw_n<G>(t, in, v);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            errs[g] = 0;
+            cnt[g] = 0;
+            st[g] = 0;
+            if (!live[g]) continue;
+            if (!in[g]) {  // normative bound (the reference reads one past the end at t == |col|)
+                errs[g] = 1u;
+                continue;
+            }
+            const uint64_t x = v[g] & omask;
+            if (col.encoded && ((x * col.a_inv) & omask) > col.max_u) {
+                errs[g] = 2u;
+                st[g] = t[g];
+                continue;
+            }
+            st[g] = reenc ? (x * f) & omask : x;
+            cnt[g] = 1;
+        }
+    }
     __device__ uint32_t eval(uint64_t row, State &st, uint32_t &errs, uint64_t &) const {
         const uint64_t t = oids.raw(row) * oids.a_inv;
         if (t >= col.n) {  // normative bound (the reference reads one past the end at t == |col|)
@@ -135,6 +175,38 @@ struct ArithOp {
     uint64_t cap[4];
     uint64_t *d_counts;
 
+    __device__ uint64_t combine(uint64_t c1, uint64_t c2) const {
+        const uint64_t x = (t1.dec_of(c1) * a_result) & rmask, y = (t2.dec_of(c2) * a_result) & rmask;
+        uint64_t r;
+        switch (op) {
+            case 0: r = x + y; break;
+            case 1: r = x - y; break;
+            case 2: r = x * y; break;
+            default: r = y ? x / y : 0; break;
+        }
+        return r & rmask;
+    }
+    static constexpr bool kBatched = true;
+    template <int G>
+    __device__ void eval_n(const uint64_t (&row)[G], const bool (&live)[G], State (&)[G], uint32_t (&errs)[G],
+                           uint32_t (&cnt)[G], uint64_t &) const {
+        uint64_t c1[G], c2[G], a1[G], a2[G];
+        t1.raw_n<G>(row, live, c1);
+        t2.raw_n<G>(row, live, c2);
+        if (h1.encoded) h1.raw_n<G>(row, live, a1);
+        if (h2.encoded) h2.raw_n<G>(row, live, a2);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            cnt[g] = 0;
+            errs[g] = 0;
+            if (!live[g]) continue;
+            if (h1.encoded && h1.bad_of(a1[g])) errs[g] |= 1u;
+            if (t1.bad_of(c1[g])) errs[g] |= 2u;
+            if (h2.encoded && h2.bad_of(a2[g])) errs[g] |= 4u;
+            if (t2.bad_of(c2[g])) errs[g] |= 8u;
+            store_w(out, row[g], rw, combine(c1[g], c2[g]));
+        }
+    }
     __device__ uint32_t eval(uint64_t row, State &, uint32_t &errs, uint64_t &) const {
         if (h1.bad_of(h1.raw(row))) errs |= 1u;
         const uint64_t c1 = t1.raw(row), c2 = t2.raw(row);
@@ -176,6 +248,20 @@ struct MulSumOp {
     uint64_t cap[2];
     uint64_t *d_counts;
 
+    static constexpr bool kBatched = true;
+    template <int G>
+    __device__ void eval_n(const uint64_t (&row)[G], const bool (&live)[G], State (&)[G], uint32_t (&errs)[G],
+                           uint32_t (&cnt)[G], uint64_t &acc) const {
+        uint64_t c1[G], c2[G];
+        t1.raw_n<G>(row, live, c1);
+        t2.raw_n<G>(row, live, c2);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            cnt[g] = 0;
+            errs[g] = (live[g] && t1.bad_of(c1[g]) ? 1u : 0u) | (live[g] && t2.bad_of(c2[g]) ? 2u : 0u);
+            acc += c1[g] * c2[g];  // rows past the end load 0
+        }
+    }
     __device__ uint32_t eval(uint64_t row, State &, uint32_t &errs, uint64_t &acc) const {
         const uint64_t c1 = t1.raw(row), c2 = t2.raw(row);
         if (t1.bad_of(c1)) errs |= 1u;
@@ -226,6 +312,25 @@ struct AggSumOp {
     using State = uint8_t;
     AggSumArgs a;
     uint64_t n;
+    static constexpr bool kBatched = true;
+    template <int G>
+    __device__ void eval_n(const uint64_t (&row)[G], const bool (&live)[G], State (&)[G], uint32_t (&errs)[G],
+                           uint32_t (&cnt)[G], uint64_t &) const {
+        uint64_t h[G], t[G], gr[G];
+        if (a.head.encoded) a.head.raw_n<G>(row, live, h);
+        a.tail.raw_n<G>(row, live, t);
+        a.grp.raw_n<G>(row, live, gr);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            cnt[g] = 0;
+            errs[g] = 0;
+            if (!live[g]) continue;
+            if (a.head.encoded && a.head.bad_of(h[g])) errs[g] |= 1u;
+            if (a.tail.bad_of(t[g])) errs[g] |= 2u;
+            const uint64_t gi = gr[g] * a.grp.a_inv;
+            if (gi > a.grp.max_u || gi >= a.num_groups) errs[g] |= 4u;
+        }
+    }
     __device__ uint32_t eval(uint64_t row, State &, uint32_t &errs, uint64_t &) const {
         if (a.head.bad_of(a.head.raw(row))) errs |= 1u;
         if (a.tail.bad_of(a.tail.raw(row))) errs |= 2u;
@@ -325,6 +430,8 @@ extern "C" int ahead_hip_fetchjoin_an(const ahead_col *oids, const ahead_col *co
     op.e2 = static_cast<uint64_t *>(err2.data);
     op.e2_cap = err2.data ? err2.cap : 0;
     op.d_counts = d_counts;
+    // the streaming engine stages entries at their row's place: it needs room for every row
+    if (op.out != nullptr && op.out_cap >= op.n) return launch_map(op, ws, 0, static_cast<hipStream_t>(stream));
     return launch_rowtile(op, op.n, 3, ws, static_cast<hipStream_t>(stream));
 }
 
@@ -400,7 +507,7 @@ extern "C" int ahead_hip_arithmetic_an(const ahead_col *h1, const ahead_col *t1,
         op.cap[i] = e[i].data ? e[i].cap : 0;
     }
     op.d_counts = d_counts;
-    return launch_rowtile(op, op.n, 5, ws, static_cast<hipStream_t>(stream));
+    return launch_map(op, ws, 0, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int ahead_hip_aggregate_mul_sum_an(const ahead_col *t1, const ahead_col *t2, uint64_t init,
@@ -416,8 +523,9 @@ extern "C" int ahead_hip_aggregate_mul_sum_an(const ahead_col *t1, const ahead_c
     op.t1 = dcol(*t1);
     op.t2 = dcol(*t2);
     op.a_oid = a_oid;
-    const LbPlan plan = lb_plan(op.n, kG * kBlock, 3);
-    if ((st = ws_ensure(ws, plan.num_tiles * 8ull)) != AHEAD_OK) return st;
+    const uint32_t waves = map_waves<MulSumOp>(op.n);  // one partial per wave of the streaming engine
+    const size_t o_map = align_up(waves * 8ull, 256);
+    if ((st = ws_ensure(ws, o_map)) != AHEAD_OK) return st;
     op.partials = reinterpret_cast<uint64_t *>(ws_region(ws, 0));
     const ahead_buf e[2] = {err1, err2};
     for (int i = 0; i < 2; ++i) {
@@ -426,12 +534,11 @@ extern "C" int ahead_hip_aggregate_mul_sum_an(const ahead_col *t1, const ahead_c
     }
     op.d_counts = d_counts;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    uint32_t tiles = 0;
-    if ((st = launch_rowtile(op, op.n, 3, ws, s, &tiles)) != AHEAD_OK) return st;
+    if ((st = launch_map(op, ws, o_map, s)) != AHEAD_OK) return st;
     // aggregateAN_scalar.tcc:83-85: RA * inv(A_T1) * inv(A_T2) over 64 bits
     const uint64_t enc = (result_encoded ? ra : 1ull) * (t1->encoded ? inverse_mod2k(t1->a, 64) : 1ull) *
                          (t2->encoded ? inverse_mod2k(t2->a, 64) : 1ull);
-    hipLaunchKernelGGL(k_mulsum_final, dim3(1), dim3(kBlock), 0, s, op.partials, tiles, init, enc, d_total);
+    hipLaunchKernelGGL(k_mulsum_final, dim3(1), dim3(kBlock), 0, s, op.partials, waves, init, enc, d_total);
     AHEAD_HIP_TRY(hipGetLastError());
     return AHEAD_OK;
 }
@@ -475,5 +582,5 @@ extern "C" int ahead_hip_aggregate_sum_grouped_an(const ahead_col *head, const a
     AggSumOp op{};
     op.a = a;
     op.n = a.n;
-    return launch_rowtile(op, op.n, 4, ws, s);
+    return launch_map(op, ws, 0, s);
 }

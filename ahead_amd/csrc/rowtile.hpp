@@ -22,6 +22,8 @@
 //   __device__ void finish(const uint64_t *totals) const;          // last tile: stream totals
 #pragma once
 
+#include <type_traits>
+
 #include "host_common.hpp"
 
 namespace ahead_dev {
@@ -31,6 +33,13 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     return __popcll(mask & below);
 }
+
+// An operator may evaluate all G rows of a thread at once (eval_n): its loads then go out as one
+// batch per input column instead of one row after another.
+template <class Op, class = void>
+struct has_eval_n : std::false_type {};
+template <class Op>
+struct has_eval_n<Op, std::void_t<decltype(Op::kBatched)>> : std::true_type {};
 
 template <class Op, int G>
 __global__ __launch_bounds__(kBlock) void k_rowtile(Op op, LbArgs lb) {
@@ -50,12 +59,34 @@ __global__ __launch_bounds__(kBlock) void k_rowtile(Op op, LbArgs lb) {
     typename Op::State st[G];
     uint32_t cnt[G], errs[G];
     uint64_t acc = 0;
+    // All G rows are evaluated before any wave-wide step, so the loads of different rows (and
+    // their dependent gathers) are in flight together rather than one row after another.
+    if constexpr (has_eval_n<Op>::value) {
+        uint64_t rows[G];
+        bool live[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            rows[g] = base + static_cast<uint64_t>(g) * kBlock + threadIdx.x;
+            live[g] = rows[g] < op.n;
+        }
+        op.template eval_n<G>(rows, live, st, errs, cnt, acc);
+    } else if (base + G * kBlock <= op.n) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            errs[g] = 0;
+            cnt[g] = op.eval(base + static_cast<uint64_t>(g) * kBlock + threadIdx.x, st[g], errs[g], acc);
+        }
+    } else {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const uint64_t row = base + static_cast<uint64_t>(g) * kBlock + threadIdx.x;
+            cnt[g] = 0;
+            errs[g] = 0;
+            if (row < op.n) cnt[g] = op.eval(row, st[g], errs[g], acc);
+        }
+    }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const uint64_t row = base + static_cast<uint64_t>(g) * kBlock + threadIdx.x;
-        cnt[g] = 0;
-        errs[g] = 0;
-        if (row < op.n) cnt[g] = op.eval(row, st[g], errs[g], acc);
         uint32_t wt = 0;
         if constexpr (Op::kMain) {
             if constexpr (Op::kMulti) wt = wave_sum(cnt[g]);
@@ -128,6 +159,30 @@ __device__ __forceinline__ void store_w(void *p, uint64_t i, uint32_t w, uint64_
     }
 }
 
+// G elements at once, width chosen once: within each case the G loads are straight-line,
+// unconditional code (a row with ok[g] false loads element 0 and yields 0), so they are all issued
+// before the first is waited for.  The caller guarantees element 0 exists.
+template <int G, typename T>
+__device__ __forceinline__ void load_rows_t(const void *p, const uint64_t (&i)[G], const bool (&ok)[G],
+                                            uint64_t (&v)[G]) {
+    const T *q = static_cast<const T *>(p);
+    T x[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) x[g] = q[ok[g] ? i[g] : 0ull];
+#pragma unroll
+    for (int g = 0; g < G; ++g) v[g] = ok[g] ? static_cast<uint64_t>(x[g]) : 0ull;
+}
+template <int G>
+__device__ __forceinline__ void load_rows(const void *p, uint32_t w, const uint64_t (&i)[G], const bool (&ok)[G],
+                                          uint64_t (&v)[G]) {
+    switch (w) {
+        case 1: load_rows_t<G, uint8_t>(p, i, ok, v); break;
+        case 2: load_rows_t<G, uint16_t>(p, i, ok, v); break;
+        case 4: load_rows_t<G, uint32_t>(p, i, ok, v); break;
+        default: load_rows_t<G, uint64_t>(p, i, ok, v); break;
+    }
+}
+
 // One BAT side on the device: raw element, decode (mulIfEncoded) and check.
 struct DCol {
     const void *data;  // NULL: void column (seqbase + i as oid_t)
@@ -141,6 +196,19 @@ struct DCol {
 
     __device__ __forceinline__ uint64_t raw(uint64_t i) const {
         return data ? load_w(data, i, width) : static_cast<uint64_t>(static_cast<uint32_t>(seqbase + i));
+    }
+    // raw() of G rows at once (void columns compute seqbase + i)
+    template <int G>
+    __device__ __forceinline__ void raw_n(const uint64_t (&i)[G], const bool (&ok)[G], uint64_t (&v)[G]) const {
+        if (data && n > 0) {
+            load_rows<G>(data, width, i, ok, v);
+        } else if (data) {  // empty column: no row can be in range
+#pragma unroll
+            for (int g = 0; g < G; ++g) v[g] = 0ull;
+        } else {
+#pragma unroll
+            for (int g = 0; g < G; ++g) v[g] = static_cast<uint64_t>(static_cast<uint32_t>(seqbase + i[g]));
+        }
     }
     __device__ __forceinline__ uint64_t dec_of(uint64_t x) const { return encoded ? (x * a_inv) & mask : x; }
     __device__ __forceinline__ bool bad_of(uint64_t x) const { return encoded && ((x * a_inv) & mask) > max_u; }
