@@ -5,9 +5,10 @@
 //   checkAN           :69-94   err += i*A_OID for every i with (c_i * A^-1 mod 2^k) > maxU
 //   decodeAN          :96-126  every row decoded (truncated to the unencoded width), err as above
 //   checkAndDecodeAN  :185-210 corrupt rows are dropped from the decoded output, err as above
-// The tile layout, in-order compaction and look-back are those of select_an.hip; decoded values
-// are staged in LDS by rank and leave the tile as one contiguous run.
-#include "host_common.hpp"
+// Two engines: k_decode (tiles with look-back, decoded values staged in LDS by rank; used when a
+// check-and-decode output holds fewer entries than the column has rows) and the streaming map
+// engine of maptile.hpp (DecodeMapOp: every other call).
+#include "maptile.hpp"
 
 using namespace ahead_dev;
 
@@ -130,6 +131,104 @@ int launch_mode(const DecodeArgs &a, uint32_t unenc_bits, hipStream_t s) {
     return AHEAD_OK;
 }
 
+// One row of checkAN / decodeAN / checkAndDecodeAN for the streaming engine: codes of CodeT, decoded
+// values stored at the runtime width ow (the unencoded width).
+template <typename CodeT, int MODE>
+struct DecodeMapOp {
+    static constexpr int kErr = 1;
+    static constexpr bool kMulti = false, kMain = MODE == kCheckDecode, kAcc = false;
+    static constexpr bool kBatched = true;
+    static constexpr int kOuts = 1;
+    using State = uint64_t;  // the decoded value
+    uint64_t n;
+    const CodeT *codes;
+    CodeT a_inv, max_u;
+    uint32_t ow;
+    uint64_t a_oid;
+    void *dec;
+    uint64_t dec_cap;
+    uint64_t *err;
+    uint64_t err_cap;
+    uint64_t *d_counts;
+
+    __device__ void out_arrays(void *(&p)[2], uint32_t (&w)[2]) const {
+        p[0] = dec;
+        w[0] = ow;
+        p[1] = nullptr;
+        w[1] = 0;
+    }
+    template <int G>
+    __device__ void eval_n(const uint64_t (&row)[G], const bool (&live)[G], State (&st)[G], uint32_t (&errs)[G],
+                           uint32_t (&cnt)[G], uint64_t &) const {
+        CodeT c[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) c[g] = codes[live[g] ? row[g] : 0ull];  // n > 0 whenever a row is live
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const CodeT d = an_mul<CodeT>(c[g], a_inv);
+            const bool bad = live[g] && d > max_u;
+            errs[g] = bad ? 1u : 0u;
+            st[g] = d;
+            cnt[g] = (MODE == kCheckDecode && live[g] && !bad) ? 1u : 0u;
+            if (MODE == kDecode && live[g]) store_w(dec, row[g], ow, d);
+        }
+    }
+    __device__ uint32_t eval(uint64_t row, State &st, uint32_t &errs, uint64_t &) const {
+        const CodeT d = an_mul<CodeT>(codes[row], a_inv);
+        const bool bad = d > max_u;
+        errs = bad ? 1u : 0u;
+        st = d;
+        if (MODE == kDecode) store_w(dec, row, ow, d);
+        return (MODE == kCheckDecode && !bad) ? 1u : 0u;
+    }
+    __device__ void emit_main(uint64_t, const State &st, uint64_t pos, uint32_t) const {
+        if (pos < dec_cap) store_w(dec, pos, ow, st);
+    }
+    __device__ void emit_err(int, uint64_t row, const State &, uint64_t pos) const {
+        if (pos < err_cap) err[pos] = row * a_oid;
+    }
+    __device__ void tile_done(uint32_t, uint64_t) const {}
+    __device__ void finish(const uint64_t *t) const {
+        if (MODE == kCheckDecode) {
+            d_counts[0] = t[0];
+            d_counts[1] = t[1];
+        } else if (MODE == kDecode) {
+            d_counts[0] = n;
+            d_counts[1] = t[1];
+        } else {
+            d_counts[0] = t[1];
+        }
+    }
+};
+
+template <typename CodeT, int MODE>
+int launch_map_mode(const ahead_col *col, uint64_t a_oid, void *dec, uint64_t dec_cap, uint64_t *err, uint64_t err_cap,
+                    uint64_t *d_counts, ahead_hip_ws *ws, hipStream_t s, size_t offset = 0) {
+    DecodeMapOp<CodeT, MODE> op{};
+    op.n = col->n;
+    op.codes = static_cast<const CodeT *>(col->data);
+    op.a_inv = static_cast<CodeT>(col->a_inv);
+    op.max_u = static_cast<CodeT>(col->max_u);
+    op.ow = col->unenc_bits / 8;
+    op.a_oid = a_oid;
+    op.dec = dec;
+    op.dec_cap = dec == nullptr ? 0 : dec_cap;
+    op.err = err;
+    op.err_cap = err == nullptr ? 0 : err_cap;
+    op.d_counts = d_counts;
+    return ahead_host::launch_map(op, ws, offset, s);
+}
+
+template <int MODE>
+int launch_map_width(const ahead_col *col, uint64_t a_oid, void *dec, uint64_t dec_cap, uint64_t *err, uint64_t err_cap,
+                     uint64_t *d_counts, ahead_hip_ws *ws, hipStream_t s, size_t offset = 0) {
+    switch (col->width) {
+        case 2: return launch_map_mode<uint16_t, MODE>(col, a_oid, dec, dec_cap, err, err_cap, d_counts, ws, s, offset);
+        case 4: return launch_map_mode<uint32_t, MODE>(col, a_oid, dec, dec_cap, err, err_cap, d_counts, ws, s, offset);
+        default: return launch_map_mode<uint64_t, MODE>(col, a_oid, dec, dec_cap, err, err_cap, d_counts, ws, s, offset);
+    }
+}
+
 int run(int mode, const ahead_col *col, uint64_t a_oid, void *dec, uint64_t dec_cap, uint64_t *err,
         uint64_t err_cap, uint64_t *d_counts, ahead_hip_ws *ws, void *stream) {
     using namespace ahead_host;
@@ -139,6 +238,16 @@ int run(int mode, const ahead_col *col, uint64_t a_oid, void *dec, uint64_t dec_
     if (mode != kCheck && dec == nullptr && col->n) return fail(AHEAD_ERR_INVALID, "decode: NULL output");
     if (err == nullptr && err_cap) return fail(AHEAD_ERR_INVALID, "decode: NULL err with capacity");
     if (col->unenc_bits > 8 * col->width) return fail(AHEAD_ERR_INVALID, "decode: unenc_bits wider than the code");
+    if (col->unenc_bits != 8 && col->unenc_bits != 16 && col->unenc_bits != 32 && col->unenc_bits != 64)
+        return fail(AHEAD_ERR_INVALID, "decode: unenc_bits must be 8, 16, 32 or 64");
+    {
+        // the streaming engine, unless a check-and-decode output cannot hold a staged entry per row
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (mode == kCheck) return launch_map_width<kCheck>(col, a_oid, nullptr, 0, err, err_cap, d_counts, ws, s);
+        if (mode == kDecode) return launch_map_width<kDecode>(col, a_oid, dec, col->n, err, err_cap, d_counts, ws, s);
+        if (dec != nullptr && dec_cap >= col->n)
+            return launch_map_width<kCheckDecode>(col, a_oid, dec, dec_cap, err, err_cap, d_counts, ws, s);
+    }
     const uint32_t tile_rows = 4u * kBlock * (16u / col->width);
     const LbPlan plan = lb_plan(col->n, tile_rows, 2);
     DecodeArgs a{};
@@ -205,11 +314,23 @@ using namespace ahead_host;
 
 namespace ahead_host {
 int check_an_ws(const ahead_col *col, uint64_t a_oid, uint64_t *err, uint64_t err_cap, uint64_t *d_counts,
-                ahead_hip_ws *ws, void *stream) {
-    return run(kCheck, col, a_oid, nullptr, 0, err, err_cap, d_counts, ws, stream);
+                ahead_hip_ws *ws, void *stream, size_t offset) {
+    int st = check_col(col, "check.col", false, true);
+    if (st != AHEAD_OK) return st;
+    if (d_counts == nullptr) return fail(AHEAD_ERR_INVALID, "check: d_counts is NULL");
+    if (err == nullptr && err_cap) return fail(AHEAD_ERR_INVALID, "check: NULL err with capacity");
+    return launch_map_width<kCheck>(col, a_oid, nullptr, 0, err, err_cap, d_counts, ws, static_cast<hipStream_t>(stream),
+                                    offset);
 }
 size_t check_an_status_bytes(uint64_t n, uint32_t width) {
     return lb_plan(n, 4u * kBlock * (16u / width), 2).status_bytes;
+}
+size_t check_an_scratch_bytes(uint64_t n, uint32_t width) {
+    switch (width) {
+        case 2: return map_scratch_bytes<DecodeMapOp<uint16_t, kCheck>>(n);
+        case 4: return map_scratch_bytes<DecodeMapOp<uint32_t, kCheck>>(n);
+        default: return map_scratch_bytes<DecodeMapOp<uint64_t, kCheck>>(n);
+    }
 }
 }  // namespace ahead_host
 

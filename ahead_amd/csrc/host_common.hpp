@@ -101,9 +101,11 @@ Tiling tiling();
 // <= max_subtiles that still leaves >= min_tiles tiles.
 uint32_t subtiles_per_tile(uint64_t subs);
 
-// checkAN for composite operators (it uses no scratch region, so theirs stay intact).
+// checkAN for composite operators: its scratch tables go at `offset` of the workspace, after the
+// caller's own regions; the caller grows the workspace to offset + check_an_scratch_bytes first.
 int check_an_ws(const ahead_col *col, uint64_t a_oid, uint64_t *err, uint64_t err_cap, uint64_t *d_counts,
-                ahead_hip_ws *ws, void *stream);
+                ahead_hip_ws *ws, void *stream, size_t offset);
 size_t check_an_status_bytes(uint64_t n, uint32_t width);
+size_t check_an_scratch_bytes(uint64_t n, uint32_t width);
 
 }  // namespace ahead_host

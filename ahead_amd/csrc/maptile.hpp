@@ -343,8 +343,19 @@ uint32_t map_grid(uint64_t n) {
     return static_cast<uint32_t>(g == 0 ? 1 : g);
 }
 
+// Workspace bytes of the map engine's per-wave tables for n rows.
+template <class Op>
+size_t map_scratch_bytes(uint64_t n) {
+    using namespace ahead_dev;
+    constexpr int NS = MapTraits<Op>::NS;
+    const size_t waves = static_cast<size_t>(map_grid<Op>(n)) * kWaves;
+    return 2 * align_up((NS + 1) * waves * 4, 256) + align_up(waves * 4, 256) + align_up(waves * kMapErrList * 8, 256) +
+           align_up(waves * 2 * kMaxShift * 8, 256) + 256;
+}
+
 // Launches the map engine for op over op.n rows.  Scratch: per-wave tables in the workspace region
-// at `offset` (the operator's own regions come before it).
+// at `offset`.  A caller that keeps regions of its own before `offset` must have grown the
+// workspace to offset + map_scratch_bytes<Op>(n) beforehand (growing moves the workspace).
 template <class Op>
 int launch_map(const Op &op, ahead_hip_ws *ws, size_t offset, hipStream_t s) {
     using namespace ahead_dev;
@@ -360,6 +371,8 @@ int launch_map(const Op &op, ahead_hip_ws *ws, size_t offset, hipStream_t s) {
                  o_flags = o_side + align_up(static_cast<size_t>(waves) * 2 * kMaxShift * 8, 256),
                  o_end = o_flags + 256;
     int st;
+    if (offset > 0 && ws->bytes < kHeaderBytes + o_end)
+        return fail(AHEAD_ERR_INVALID, "map engine: the workspace was not grown before its regions were placed");
     if ((st = ws_ensure(ws, o_end)) != AHEAD_OK) return st;
     MapArgs m{};
     m.rows_per_wave = per;

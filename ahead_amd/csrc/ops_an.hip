@@ -525,7 +525,7 @@ extern "C" int ahead_hip_aggregate_mul_sum_an(const ahead_col *t1, const ahead_c
     op.a_oid = a_oid;
     const uint32_t waves = map_waves<MulSumOp>(op.n);  // one partial per wave of the streaming engine
     const size_t o_map = align_up(waves * 8ull, 256);
-    if ((st = ws_ensure(ws, o_map)) != AHEAD_OK) return st;
+    if ((st = ws_ensure(ws, o_map + map_scratch_bytes<MulSumOp>(op.n))) != AHEAD_OK) return st;
     op.partials = reinterpret_cast<uint64_t *>(ws_region(ws, 0));
     const ahead_buf e[2] = {err1, err2};
     for (int i = 0; i < 2; ++i) {

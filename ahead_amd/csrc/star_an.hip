@@ -1214,7 +1214,7 @@ struct StarLayout {
     size_t off_flags = 0, off_gsum = 0, off_gfirst = 0, off_table = 0, off_present = 0, off_combo_cid = 0,
            off_cid_combo = 0, off_info = 0, off_map = 0, off_psum = 0, off_prank = 0, total = 0;
     size_t off_packed = 0, off_code8 = 0, off_psum2 = 0, off_pfirst2 = 0, off_wcnt = 0, off_wpre = 0,
-           off_werows = 0, off_elist = 0;
+           off_werows = 0, off_elist = 0, off_check = 0;
     uint32_t grid2 = 0, waves2 = 0;  // fast probe workgroups and waves
     uint64_t rows_per_wave = 0;
     size_t status_bytes = 0;  // scan state of the probe and of the build-side check
@@ -1288,7 +1288,9 @@ StarLayout star_layout(const ahead_star1_args *in) {
     L.off_wpre = L.off_wcnt + align_up(static_cast<size_t>(kStreams) * L.waves2 * 4, 256);
     L.off_werows = L.off_wpre + align_up(static_cast<size_t>(kStreams) * L.waves2 * 4, 256);
     L.off_elist = L.off_werows + align_up(static_cast<size_t>(L.waves2) * 4, 256);
-    L.total = L.off_elist + align_up(static_cast<size_t>(L.waves2) * kErrList * 8, 256);
+    L.off_check = L.off_elist + align_up(static_cast<size_t>(L.waves2) * kErrList * 8, 256);
+    // the build-side key check's scratch (the streaming engine's per-wave tables) comes last
+    L.total = L.off_check + check_an_scratch_bytes(in->dim_key.n, in->dim_key.width);
     const size_t chk = check_an_status_bytes(in->dim_key.n, in->dim_key.width);
     L.status_bytes = L.plan.status_bytes > chk ? L.plan.status_bytes : chk;
     return L;
@@ -1362,7 +1364,7 @@ extern "C" int ahead_hip_star1_build(const ahead_star1_args *in, const ahead_sta
     AHEAD_HIP_TRY(hipGetLastError());
     // the build side's error vector: checkAN over the dimension keys (hashjoinAN.tcc:179-181)
     return check_an_ws(&in->dim_key, in->a_oid_err, out->e_dk, out->e_dk ? out->e_dk_cap : 0,
-                       out->d_counts + 2, ws, stream);
+                       out->d_counts + 2, ws, stream, L.off_check);
 }
 
 extern "C" int ahead_hip_star1_probe(const ahead_star1_args *in, const ahead_star1_out *out, ahead_hip_ws *ws,
