@@ -46,6 +46,7 @@ struct MapArgs {
     uint64_t *elist;   // [waves][kMapErrList] error bits << 32 | row
     uint64_t *side;    // [waves][kMaxShift] staged entries saved before a move
     uint32_t *flags;   // [0] dropped rows in all (k_map_scan)
+    uint64_t *totals;  // [NS + 1] per stream (k_map_scan)
 };
 
 template <class Op>
@@ -74,11 +75,31 @@ __device__ __forceinline__ void map_eval(const Op &op, const uint64_t (&rows)[kM
     }
 }
 
+// An operator may keep per-workgroup accumulators in LDS (kSharedWords 64-bit words):
+// shared_init / shared_flush bracket the scan, and eval_shared replaces eval_n.
+template <class Op, class = void>
+struct has_shared : std::false_type {};
+template <class Op>
+struct has_shared<Op, std::void_t<decltype(Op::kSharedWords)>> : std::true_type {};
+template <class Op, bool = has_shared<Op>::value>
+struct SharedWords {
+    static constexpr int value = 1;
+};
+template <class Op>
+struct SharedWords<Op, true> {
+    static constexpr int value = Op::kSharedWords;
+};
+
 // Rows of one wave's range; lane l handles rows seg + g * 64 + l (one coalesced run per g).
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
     static_assert(!Op::kMulti, "the map engine takes operators whose rows emit at most one entry");
     constexpr int NS = MapTraits<Op>::NS;
+    __shared__ unsigned long long s_acc[SharedWords<Op>::value];
+    if constexpr (has_shared<Op>::value) {
+        op.shared_init(s_acc);
+        __syncthreads();
+    }
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
     const uint64_t begin = static_cast<uint64_t>(gw) * m.rows_per_wave;
@@ -96,7 +117,11 @@ __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
         }
         typename Op::State st[kMapG];
         uint32_t errs[kMapG], cnt[kMapG];
-        map_eval(op, rows, live, st, errs, cnt, acc);
+        if constexpr (has_shared<Op>::value) {
+            op.template eval_shared<kMapG>(rows, live, st, errs, cnt, acc, s_acc);
+        } else {
+            map_eval(op, rows, live, st, errs, cnt, acc);
+        }
 #pragma unroll
         for (int g = 0; g < kMapG; ++g) {
             if constexpr (Op::kMain) {
@@ -118,6 +143,10 @@ __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
             }
         }
     }
+    if constexpr (has_shared<Op>::value) {
+        __syncthreads();
+        op.shared_flush(s_acc);
+    }
 #pragma unroll
     for (int e = 0; e < Op::kErr; ++e) ecnt[e] = wave_sum(ecnt[e]);
     if constexpr (Op::kAcc) {
@@ -133,49 +162,49 @@ __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
     }
 }
 
-// One workgroup of 1024 threads: prefixes of every per-wave count, the totals and the drops.
+// One workgroup of 1024 threads per count stream (NS + 1 of them): exclusive prefixes over the
+// waves and the stream total.
 template <class Op>
 __global__ __launch_bounds__(1024) void k_map_scan(Op op, MapArgs m) {
-    constexpr int NS = MapTraits<Op>::NS;
     __shared__ uint32_t s_w[16];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    uint64_t totals[NS + 1];
+    const size_t s = blockIdx.x;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < m.waves; base += 8192u) {
+        uint32_t v[8], t = 0;
 #pragma unroll
-    for (int s = 0; s <= NS; ++s) {
-        uint32_t carry = 0;
-        for (uint32_t base = 0; base < m.waves; base += 4096u) {
-            uint32_t v[4], t = 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t i = base + threadIdx.x * 4u + q;
-                v[q] = i < m.waves ? m.wcnt[static_cast<size_t>(s) * m.waves + i] : 0u;
-                t += v[q];
-            }
-            const uint32_t inc = wave_incl_scan(t);
-            if (lane == 63u) s_w[wv] = inc;
-            __syncthreads();
-            uint32_t pre = carry, tot = 0;
-#pragma unroll
-            for (int w = 0; w < 16; ++w) {
-                if (w < static_cast<int>(wv)) pre += s_w[w];
-                tot += s_w[w];
-            }
-            __syncthreads();
-            pre += inc - t;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t i = base + threadIdx.x * 4u + q;
-                if (i < m.waves) m.wpre[static_cast<size_t>(s) * m.waves + i] = pre;
-                pre += v[q];
-            }
-            carry += tot;
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t i = base + threadIdx.x * 8u + q;
+            v[q] = i < m.waves ? m.wcnt[s * m.waves + i] : 0u;
+            t += v[q];
         }
-        totals[s] = carry;
+        const uint32_t inc = wave_incl_scan(t);
+        if (lane == 63u) s_w[wv] = inc;
+        __syncthreads();
+        uint32_t pre = carry, tot = 0;
+#pragma unroll
+        for (int w = 0; w < 16; ++w) {
+            if (w < static_cast<int>(wv)) pre += s_w[w];
+            tot += s_w[w];
+        }
+        __syncthreads();
+        pre += inc - t;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t i = base + threadIdx.x * 8u + q;
+            if (i < m.waves) m.wpre[s * m.waves + i] = pre;
+            pre += v[q];
+        }
+        carry += tot;
     }
-    if (threadIdx.x == 0) {
-        op.finish(totals);
-        m.flags[0] = Op::kMain ? static_cast<uint32_t>(totals[NS] - totals[0]) : 0u;
-    }
+    if (threadIdx.x == 0) m.totals[s] = carry;
+}
+
+// Dropped rows in all (kMain operators): live rows minus main entries.
+template <class Op>
+__device__ __forceinline__ uint32_t map_total_drops(const MapArgs &m) {
+    constexpr int NS = MapTraits<Op>::NS;
+    return Op::kMain ? static_cast<uint32_t>(m.totals[NS] - m.totals[0]) : 0u;
 }
 
 // Rows dropped before wave gw (kMain operators).
@@ -193,7 +222,7 @@ __device__ __forceinline__ uint64_t map_load_entry(const void *p, uint32_t w, ui
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_map_save(Op op, MapArgs m) {
     if constexpr (Op::kMain) {
-        const uint32_t total = m.flags[0];
+        const uint32_t total = map_total_drops<Op>(m);
         if (total == 0u || total > kMaxShift) return;
         const uint32_t lane = threadIdx.x & 63u;
         const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
@@ -225,8 +254,9 @@ __global__ __launch_bounds__(kBlock) void k_map_fix(Op op, MapArgs m) {
     if (gw >= m.waves) return;
     const uint64_t begin = static_cast<uint64_t>(gw) * m.rows_per_wave;
     const uint64_t end = begin + m.rows_per_wave < op.n ? begin + m.rows_per_wave : op.n;
-    const uint32_t total_drops = Op::kMain ? m.flags[0] : 0u;
+    const uint32_t total_drops = map_total_drops<Op>(m);
     const bool rewrite = total_drops > kMaxShift;
+    if (gw == 0 && lane == 0) op.finish(m.totals);
 
     // error positions
     const uint32_t nrows = m.werows[gw];
@@ -297,22 +327,38 @@ __global__ __launch_bounds__(kBlock) void k_map_fix(Op op, MapArgs m) {
         void *p[2];
         uint32_t w[2];
         op.out_arrays(p, w);
-        for (uint32_t k0 = 0; k0 < kept; k0 += 64u) {
-            const uint32_t k = k0 + lane;
-            uint64_t v[2] = {0, 0};
-            if (k < kept) {
+        // 512 entries per step (8 per lane): all are read before any is written; a step's writes
+        // land on entries of this step or earlier ones only (they move left)
+        constexpr int kPer = 8;
+        for (uint32_t k0 = 0; k0 < kept; k0 += 64u * kPer) {
+            uint64_t v[2][kPer];
 #pragma unroll
-                for (int a = 0; a < Op::kOuts; ++a)
-                    v[a] = k >= kept - keep
-                               ? m.side[(static_cast<size_t>(gw) * 2 + a) * kMaxShift + (k - (kept - keep))]
-                               : map_load_entry(p[a], w[a], begin + k);
+            for (int a = 0; a < Op::kOuts; ++a) {
+                uint64_t idx[kPer];
+                bool from_stage[kPer];
+#pragma unroll
+                for (int q = 0; q < kPer; ++q) {
+                    const uint32_t k = k0 + q * 64u + lane;
+                    from_stage[q] = k < kept - keep;
+                    idx[q] = begin + k;
+                }
+                load_rows<kPer>(p[a], w[a], idx, from_stage, v[a]);
+#pragma unroll
+                for (int q = 0; q < kPer; ++q) {
+                    const uint32_t k = k0 + q * 64u + lane;
+                    if (k < kept && !from_stage[q])
+                        v[a][q] = m.side[(static_cast<size_t>(gw) * 2 + a) * kMaxShift + (k - (kept - keep))];
+                }
             }
-            // every lane's read of this step is complete before any lane writes over it
             __builtin_amdgcn_s_waitcnt(0);
             __builtin_amdgcn_wave_barrier();
-            if (k < kept) {
 #pragma unroll
-                for (int a = 0; a < Op::kOuts; ++a) store_w(p[a], first + k, w[a], v[a]);
+            for (int q = 0; q < kPer; ++q) {
+                const uint32_t k = k0 + q * 64u + lane;
+                if (k < kept) {
+#pragma unroll
+                    for (int a = 0; a < Op::kOuts; ++a) store_w(p[a], first + k, w[a], v[a][q]);
+                }
             }
         }
     }
@@ -369,7 +415,7 @@ int launch_map(const Op &op, ahead_hip_ws *ws, size_t offset, hipStream_t s) {
                  o_elist = o_erows + align_up(static_cast<size_t>(waves) * 4, 256),
                  o_side = o_elist + align_up(static_cast<size_t>(waves) * kMapErrList * 8, 256),
                  o_flags = o_side + align_up(static_cast<size_t>(waves) * 2 * kMaxShift * 8, 256),
-                 o_end = o_flags + 256;
+                 o_end = o_flags + 256;  // flags, then totals
     int st;
     if (offset > 0 && ws->bytes < kHeaderBytes + o_end)
         return fail(AHEAD_ERR_INVALID, "map engine: the workspace was not grown before its regions were placed");
@@ -383,8 +429,10 @@ int launch_map(const Op &op, ahead_hip_ws *ws, size_t offset, hipStream_t s) {
     m.elist = reinterpret_cast<uint64_t *>(ws_region(ws, o_elist));
     m.side = reinterpret_cast<uint64_t *>(ws_region(ws, o_side));
     m.flags = reinterpret_cast<uint32_t *>(ws_region(ws, o_flags));
+    m.totals = reinterpret_cast<uint64_t *>(ws_region(ws, o_flags + 64));
+    static_assert(64 + 8 * (NS + 1) <= 256, "totals fit the flags block");
     hipLaunchKernelGGL(k_map<Op>, dim3(grid), dim3(kBlock), 0, s, op, m);
-    hipLaunchKernelGGL(k_map_scan<Op>, dim3(1), dim3(1024), 0, s, op, m);
+    hipLaunchKernelGGL(k_map_scan<Op>, dim3(NS + 1), dim3(1024), 0, s, op, m);
     if constexpr (Op::kMain) hipLaunchKernelGGL(k_map_save<Op>, dim3(grid), dim3(kBlock), 0, s, op, m);
     hipLaunchKernelGGL(k_map_fix<Op>, dim3(grid), dim3(kBlock), 0, s, op, m);
     AHEAD_HIP_TRY(hipGetLastError());

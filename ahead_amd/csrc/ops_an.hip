@@ -291,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_mulsum_final(const uint64_t *partial
 }
 
 // ---------------------------------------------------------------------------------------------------
-// Grouped sum: LDS-private accumulators per tile when the groups fit, merged with 64-bit
+// Grouped sum: LDS-private accumulators per workgroup when the groups fit, merged with 64-bit
 // atomics; otherwise straight 64-bit atomics on the output.
 constexpr uint32_t kAggLds = 2048;
 
@@ -306,23 +306,53 @@ struct AggSumArgs {
 };
 
 struct AggSumOp {
-    // checks only; the additions happen in k_aggsum (one pass each, both stream the input once)
+    // checks and sums in one pass: per-workgroup LDS sums when the groups fit (merged into the
+    // output with 64-bit atomics at the end), else 64-bit atomics on the output directly
     static constexpr int kErr = 3;
+    static constexpr int kSharedWords = kAggLds;
     static constexpr bool kMulti = false, kMain = false, kAcc = false;
     using State = uint8_t;
     AggSumArgs a;
     uint64_t n;
     static constexpr bool kBatched = true;
+    __device__ void shared_init(unsigned long long *s) const {
+        const uint32_t ng = a.num_groups <= kAggLds ? static_cast<uint32_t>(a.num_groups) : 0u;
+        for (uint32_t i = threadIdx.x; i < ng; i += blockDim.x) s[i] = 0ull;
+    }
+    __device__ void shared_flush(unsigned long long *s) const {
+        if (a.num_groups > kAggLds) return;
+        for (uint32_t i = threadIdx.x; i < a.num_groups; i += blockDim.x)
+            if (s[i] != 0ull) atomicAdd(reinterpret_cast<unsigned long long *>(&a.sums[i]), s[i]);
+    }
+    // aggregateAN.tcc:84-112: a corrupt tail is still added (and reported); a row whose group id
+    // is corrupt or out of range is reported and skipped
     template <int G>
-    __device__ void eval_n(const uint64_t (&row)[G], const bool (&live)[G], State (&)[G], uint32_t (&errs)[G],
-                           uint32_t (&cnt)[G], uint64_t &) const {
-        uint64_t h[G], t[G], gr[G];
+    __device__ void eval_shared(const uint64_t (&row)[G], const bool (&live)[G], State (&st)[G], uint32_t (&errs)[G],
+                                uint32_t (&cnt)[G], uint64_t &acc, unsigned long long *s) const {
+        uint64_t t[G], gr[G];
+        eval_rows<G>(row, live, errs, t, gr);
+        const bool lds = a.num_groups <= kAggLds;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            cnt[g] = 0;
+            if (!live[g] || (errs[g] & 4u)) continue;
+            const uint64_t gi = gr[g] * a.grp.a_inv;
+            const unsigned long long add = t[g] * a.enc;
+            if (lds) atomicAdd(&s[gi], add);
+            else atomicAdd(reinterpret_cast<unsigned long long *>(&a.sums[gi]), add);
+        }
+        (void)st;
+        (void)acc;
+    }
+    template <int G>
+    __device__ void eval_rows(const uint64_t (&row)[G], const bool (&live)[G], uint32_t (&errs)[G], uint64_t (&t)[G],
+                              uint64_t (&gr)[G]) const {
+        uint64_t h[G];
         if (a.head.encoded) a.head.raw_n<G>(row, live, h);
         a.tail.raw_n<G>(row, live, t);
         a.grp.raw_n<G>(row, live, gr);
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            cnt[g] = 0;
             errs[g] = 0;
             if (!live[g]) continue;
             if (a.head.encoded && a.head.bad_of(h[g])) errs[g] |= 1u;
@@ -330,6 +360,14 @@ struct AggSumOp {
             const uint64_t gi = gr[g] * a.grp.a_inv;
             if (gi > a.grp.max_u || gi >= a.num_groups) errs[g] |= 4u;
         }
+    }
+    template <int G>
+    __device__ void eval_n(const uint64_t (&row)[G], const bool (&live)[G], State (&)[G], uint32_t (&errs)[G],
+                           uint32_t (&cnt)[G], uint64_t &) const {
+        uint64_t t[G], gr[G];
+        eval_rows<G>(row, live, errs, t, gr);
+#pragma unroll
+        for (int g = 0; g < G; ++g) cnt[g] = 0;
     }
     __device__ uint32_t eval(uint64_t row, State &, uint32_t &errs, uint64_t &) const {
         if (a.head.bad_of(a.head.raw(row))) errs |= 1u;
@@ -349,29 +387,6 @@ struct AggSumOp {
     }
 };
 
-__global__ __launch_bounds__(kBlock) void k_aggsum(AggSumArgs a, uint32_t rows_per_block) {
-    __shared__ unsigned long long s_acc[kAggLds];
-    const bool lds = a.num_groups <= kAggLds;
-    if (lds) {
-        for (uint32_t i = threadIdx.x; i < a.num_groups; i += kBlock) s_acc[i] = 0ull;
-        __syncthreads();
-    }
-    const uint64_t r0 = static_cast<uint64_t>(blockIdx.x) * rows_per_block;
-    const uint64_t r1 = r0 + rows_per_block < a.n ? r0 + rows_per_block : a.n;
-    for (uint64_t row = r0 + threadIdx.x; row < r1; row += kBlock) {
-        const uint64_t g = a.grp.raw(row) * a.grp.a_inv;
-        if (g > a.grp.max_u || g >= a.num_groups) continue;
-        const uint64_t add = a.tail.raw(row) * a.enc;
-        if (lds) atomicAdd(&s_acc[g], static_cast<unsigned long long>(add));
-        else atomicAdd(reinterpret_cast<unsigned long long *>(&a.sums[g]), static_cast<unsigned long long>(add));
-    }
-    if (lds) {
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < a.num_groups; i += kBlock) {
-            if (s_acc[i]) atomicAdd(reinterpret_cast<unsigned long long *>(&a.sums[i]), s_acc[i]);
-        }
-    }
-}
 
 __global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -574,9 +589,6 @@ extern "C" int ahead_hip_aggregate_sum_grouped_an(const ahead_col *head, const a
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (num_groups) {
         hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, s, sums, num_groups, 0ull);
-        const uint32_t rows_per_block = 16384;
-        const uint64_t blocks = (a.n + rows_per_block - 1) / rows_per_block;
-        if (blocks) hipLaunchKernelGGL(k_aggsum, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, s, a, rows_per_block);
         AHEAD_HIP_TRY(hipGetLastError());
     }
     AggSumOp op{};
