@@ -7,6 +7,11 @@
 //             the rows per key and keeps the first row.  Keys with several rows additionally get
 //             a row list: counts are scanned into offsets, rows are scattered into the lists and
 //             each list is sorted back into build (insertion) order.
+//  * dense  : when the build keys span a range of at most 4 x |build| + 1024 values, a direct
+//             map over that range (u32 per key: 0 = no row, first row + 1 for a key with one row,
+//             bit 31 | table slot for a key with several) lets a probe row make one gather into a
+//             compact, cache-resident array instead of probing the table and its count and
+//             first-row arrays.
 //  * checks : the build side's head/tail checks are an ordered two-stream pass on the row-tile
 //             engine (positions in build-scan order).
 //  * probe  : row-tile engine over the probe side: per row the head/tail checks, the key
@@ -24,14 +29,19 @@ namespace {
 
 constexpr uint32_t kMaxSortedList = 8192;  // longest per-key row list the build can order in LDS
 
+constexpr uint32_t kDmapMulti = 0x80000000u;  // direct-map entry: the key has several rows (low bits: slot)
+
 struct HjTable {
     HTable ht;
+    uint32_t *dmap;   // [map_cap] direct map (flags[4] != 0: in use)
+    uint32_t map_cap;
     uint32_t *cnt;    // [size + 1] rows per key
     uint32_t *first;  // [size + 1] first (lowest) build row per key
     uint32_t *off;    // [size + 1] list offsets (keys with several rows)
     uint32_t *cur;    // [size + 1] fill cursors
     uint32_t *rows;   // [n_build] row lists
-    uint32_t *flags;  // [0] several rows share a key, [1] table overflow, [2] list too long to order
+    uint32_t *flags;  // [0] several rows share a key, [1] table overflow, [2] list too long to order,
+                      // [3] smallest build key, [5] largest build key, [4] direct map in use
 };
 
 struct BuildArgs {
@@ -49,6 +59,62 @@ __global__ __launch_bounds__(kBlock) void k_hj_insert(BuildArgs a) {
         if (s == UINT32_MAX) continue;
         if (atomicAdd(&a.t.cnt[s], 1u) != 0u) atomicOr(&a.t.flags[0], 1u);
         atomicMin(&a.t.first[s], static_cast<uint32_t>(j));
+    }
+}
+
+// Smallest and largest (truncated, decoded) build key: one atomic pair per workgroup.  Keys of
+// 32 bits or less only (the direct map is indexed by key - smallest).
+__global__ __launch_bounds__(kBlock) void k_hj_range(BuildArgs a) {
+    __shared__ uint32_t s_lo[kWaves], s_hi[kWaves];
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t j = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; j < a.n; j += stride) {
+        const uint32_t k = static_cast<uint32_t>(a.key.dec_of(a.key.raw(j)) & a.key_mask);
+        lo = k < lo ? k : lo;
+        hi = k > hi ? k : hi;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint32_t l2 = __shfl_xor(lo, o, 64), h2 = __shfl_xor(hi, o, 64);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kWaves; ++w) {
+            lo = s_lo[w] < lo ? s_lo[w] : lo;
+            hi = s_hi[w] > hi ? s_hi[w] : hi;
+        }
+        atomicMin(&a.t.flags[3], lo);
+        atomicMax(&a.t.flags[5], hi);
+    }
+}
+
+// Decides whether the direct map applies and clears it over the key range.
+__global__ __launch_bounds__(kBlock) void k_hj_dmap_clear(HjTable t) {
+    const uint32_t lo = t.flags[3], hi = t.flags[5];
+    const bool dense = t.flags[1] == 0u && lo <= hi && static_cast<uint64_t>(hi) - lo < t.map_cap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) t.flags[4] = dense ? 1u : 0u;
+    if (!dense) return;
+    const uint32_t range = hi - lo + 1u;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < range; i += gridDim.x * kBlock) t.dmap[i] = 0u;
+}
+
+// dmap[key - smallest] = first row + 1 (one row) or kDmapMulti | slot (several rows).
+__global__ __launch_bounds__(kBlock) void k_hj_dmap_fill(BuildArgs a) {
+    if (a.t.flags[4] == 0u) return;
+    const uint32_t lo = a.t.flags[3];
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t j = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; j < a.n; j += stride) {
+        const uint64_t k = a.key.dec_of(a.key.raw(j)) & a.key_mask;
+        const uint32_t s = a.t.ht.find(k);
+        if (s == UINT32_MAX) continue;  // not reached: every build key is in the table
+        const uint32_t e = a.t.cnt[s] == 1u ? a.t.first[s] + 1u : (kDmapMulti | s);
+        a.t.dmap[static_cast<uint32_t>(k) - lo] = e;
     }
 }
 
@@ -149,7 +215,8 @@ struct Check2Op {
 
 struct ProbeState {
     uint64_t pay;   // probe-side payload (raw)
-    uint32_t slot;
+    uint32_t slot;  // table slot (keys with several rows)
+    uint32_t brow;  // the build row (keys with one row)
 };
 
 struct ProbeOp {
@@ -182,16 +249,33 @@ struct ProbeOp {
         const uint64_t kv = key.dec_of(build_right ? tl : h);
         st.pay = build_right ? h : tl;
         if ((kv & guard_mask) > build_max) return 0;
-        const uint32_t s = t.ht.find(kv & key_mask);
+        const uint64_t k = kv & key_mask;
+        if (t.flags[4] != 0u) {  // direct map over [flags[3], flags[5]] (set by the build)
+            const uint64_t rel = k - t.flags[3];
+            if (rel > static_cast<uint64_t>(t.flags[5] - t.flags[3])) return 0;
+            const uint32_t e = t.dmap[rel];
+            if (e == 0u) return 0;
+            if ((e & kDmapMulti) == 0u) {
+                st.brow = e - 1u;
+                return 1;
+            }
+            st.slot = e & ~kDmapMulti;
+            return t.cnt[st.slot];
+        }
+        const uint32_t s = t.ht.find(k);
         if (s == UINT32_MAX) return 0;
         st.slot = s;
+        if (t.cnt[s] == 1u) {
+            st.brow = t.first[s];
+            return 1;
+        }
         return t.cnt[s];
     }
     __device__ void emit_main(uint64_t, const State &st, uint64_t pos, uint32_t c) const {
         const uint64_t pv = reenc_p ? (st.pay * f_p) & m_p : st.pay;
         for (uint32_t k = 0; k < c; ++k) {
             if (pos + k >= out_cap) break;
-            const uint32_t brow = c == 1u ? t.first[st.slot] : t.rows[t.off[st.slot] + k];
+            const uint32_t brow = c == 1u ? st.brow : t.rows[t.off[st.slot] + k];
             const uint64_t b = bpay.raw(brow);
             const uint64_t bv = reenc_b ? (b * f_b) & m_b : b;
             if (build_right) {
@@ -250,11 +334,13 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     uint32_t bits = 4;
     while ((1ull << bits) < 2ull * nb) ++bits;
     const uint64_t slots = (1ull << bits) + 1;
-    // layout: [flags 256][keys][cnt][first][off][cur][rows]
+    // layout: [flags 256][keys][cnt][first][off][cur][rows][direct map]
+    const uint32_t key_bits = bkey.data ? bkey.unenc_bits : 32;
+    const uint64_t map_cap = key_bits <= 32 ? 4ull * nb + 1024 : 0;  // direct map only for <= 32-bit keys
     const size_t o_flags = 0, o_keys = 256, o_cnt = o_keys + align_up(slots * 8, 256),
                  o_first = o_cnt + align_up(slots * 4, 256), o_off = o_first + align_up(slots * 4, 256),
                  o_cur = o_off + align_up(slots * 4, 256), o_rows = o_cur + align_up(slots * 4, 256),
-                 o_end = o_rows + align_up(nb * 4, 256);
+                 o_dmap = o_rows + align_up(nb * 4, 256), o_end = o_dmap + align_up(map_cap * 4, 256);
     if ((st = ws_ensure(ws, o_end)) != AHEAD_OK) return st;
 
     HjTable t{};
@@ -267,7 +353,10 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     t.off = reinterpret_cast<uint32_t *>(ws_region(ws, o_off));
     t.cur = reinterpret_cast<uint32_t *>(ws_region(ws, o_cur));
     t.rows = reinterpret_cast<uint32_t *>(ws_region(ws, o_rows));
+    t.dmap = reinterpret_cast<uint32_t *>(ws_region(ws, o_dmap));
+    t.map_cap = static_cast<uint32_t>(map_cap);
     if ((st = fill_u32(t.flags, 64, 0, s)) != AHEAD_OK) return st;
+    if ((st = fill_u32(t.flags + 3, 1, 0xFFFFFFFFu, s)) != AHEAD_OK) return st;  // smallest key
     if ((st = fill_u64(reinterpret_cast<uint64_t *>(t.ht.keys), slots, 0, s)) != AHEAD_OK) return st;
     if ((st = fill_u32(t.cnt, slots, 0, s)) != AHEAD_OK) return st;
     if ((st = fill_u32(t.first, slots, 0xFFFFFFFFu, s)) != AHEAD_OK) return st;
@@ -281,6 +370,11 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     const uint64_t gb = (nb + kBlock - 1) / kBlock;
     const dim3 grid(static_cast<uint32_t>(gb < 4096 ? gb : 4096));
     hipLaunchKernelGGL(k_hj_insert, grid, dim3(kBlock), 0, s, b);
+    if (map_cap) {
+        hipLaunchKernelGGL(k_hj_range, dim3(gb < 128 ? static_cast<uint32_t>(gb) : 128u), dim3(kBlock), 0, s, b);
+        hipLaunchKernelGGL(k_hj_dmap_clear, dim3(512), dim3(kBlock), 0, s, t);
+        hipLaunchKernelGGL(k_hj_dmap_fill, grid, dim3(kBlock), 0, s, b);
+    }
     AHEAD_HIP_TRY(hipGetLastError());
     OffsetOp off{};
     off.n = slots;

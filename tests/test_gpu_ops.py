@@ -61,10 +61,13 @@ def test_matchjoin_positional_parity(n):
         same(got, ref)
 
 
+# build_flips: with flipped build keys the key range spans 2^32 and the probe uses the hash
+# table; without, the keys are dense and the probe uses the direct map (one row or several)
+@pytest.mark.parametrize("build_flips", [True, False])
 @pytest.mark.parametrize("build_right", [1, 0])
 @pytest.mark.parametrize("n1,n2,kmax", [(0, 10, 5), (10, 0, 5), (1, 1, 0), (5000, 300, 120), (100_003, 20_000, 25_000),
-                                        (30_000, 3_000, 100)])
-def test_hashjoin_parity(build_right, n1, n2, kmax):
+                                        (30_000, 3_000, 100), (50_000, 40_000, 1_000_000)])
+def test_hashjoin_parity(build_flips, build_right, n1, n2, kmax):
     seed = n1 + 7 * n2 + build_right
     k1 = datagen.uniform(n1, seed, 1, 0, kmax)
     k2 = datagen.uniform(n2, seed, 2, 0, kmax)
@@ -72,7 +75,8 @@ def test_hashjoin_parity(build_right, n1, n2, kmax):
     c1 = datagen.encode(k1, 63877, 8)
     c2 = datagen.encode(k2, 63877, 8)
     datagen.inject_flips(c1, seed, 3, datagen.flip_threshold(2e-3))
-    datagen.inject_flips(c2, seed, 5, datagen.flip_threshold(2e-3))
+    if build_flips:
+        datagen.inject_flips(c2, seed, 5, datagen.flip_threshold(2e-3))
     h1 = datagen.encode(datagen.uniform(n1, seed, 7, 0, 10**6), an.A_OID, 8)
     t2 = datagen.encode(datagen.uniform(n2, seed, 8, 0, 10**6), an.A_OID, 8)
     f = (an.A_OID * an.inverse(an.A_OID, 64)) & an.MASK64
