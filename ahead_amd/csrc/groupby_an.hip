@@ -1,19 +1,22 @@
 // groupby_an.hip -- groupbyAN, unary and binary, for gfx950 (groupbyAN.tcc:51-94 and :99-165).
 //
-// Group ids are numbered by first occurrence in scan order.  Three passes:
+// Group ids are numbered by first occurrence in scan order:
 //  k_grp_insert  every row's key enters an HBM hash table (htable.hpp) that keeps the key's
 //                first row (atomicMin); a workgroup first de-duplicates its 8192 rows in an LDS
 //                table, so a column with few distinct keys costs one global atomic per key and
-//                workgroup instead of one per row.
-//  FirstsOp      (row-tile engine) rows that are their key's first row form an ordered
-//                compaction: the rank of a first row is its group id.  It writes VGIDtoROID and
-//                the slot -> group-id map.
-//  AssignOp      (row-tile engine) writes VOIDtoRGID for every row and the ordered error vectors.
+//                workgroup instead of one per row.  A row reads the LDS table before it writes:
+//                once a key and an earlier first row are there, it makes no atomic at all.
+//  k_grp_collect / k_grp_sort   (up to kGrpFastMax distinct keys) the occupied slots are
+//                gathered and ordered by first row in one workgroup: the position of a key is its
+//                group id.  They write VGIDtoROID and the slot -> group-id map.
+//  FirstsOp      (row-tile engine; only with more distinct keys) the same by an ordered
+//                compaction of the rows that are their key's first row.
+//  AssignOp      (streaming engine) writes VOIDtoRGID for every row and the ordered error vectors.
 // Keys: unary static_cast<tunenc_t>(decoded tail) (:79); binary
 // (largerTail_t)t * numGroups + g in the larger type's width (:144).  The binary dictionary
 // stores RGIDs as oid_t (:122), so a repeated key yields (uint32)(gid * A_OID) (:152).
 #include "htable.hpp"
-#include "rowtile.hpp"
+#include "maptile.hpp"
 
 using namespace ahead_dev;
 
@@ -21,6 +24,8 @@ namespace {
 
 constexpr uint32_t kGrpRowsPerBlock = 8192;
 constexpr uint32_t kGrpLdsSlots = 2048;
+constexpr uint32_t kGrpFastMax = 8192;  // distinct keys the one-workgroup ordering takes
+// flags: [0] table overflow, [8] distinct keys collected, [9] more than kGrpFastMax (compaction path)
 
 struct GrpKey {
     DCol tail, grp;
@@ -28,13 +33,34 @@ struct GrpKey {
     uint64_t tail_mask;  // unary: unencoded width of the tail type
     uint64_t lmask;      // binary: width of the larger type
     uint64_t num_groups;
-    __device__ __forceinline__ uint64_t key(uint64_t row) const {
-        const uint64_t t = tail.dec_of(tail.raw(row));
+    __device__ __forceinline__ uint64_t key_of(uint64_t tail_code, uint64_t grp_code) const {
+        const uint64_t t = tail.dec_of(tail_code);
         if (!binary) return t & tail_mask;
-        const uint64_t g = grp.raw(row) * grp.a_inv;
+        const uint64_t g = grp_code * grp.a_inv;
         return ((t & lmask) * (num_groups & lmask) + (g & lmask)) & lmask;
     }
+    __device__ __forceinline__ uint64_t key(uint64_t row) const {
+        return key_of(tail.raw(row), binary ? grp.raw(row) : 0ull);
+    }
+    // keys of G rows, their loads issued together
+    template <int G>
+    __device__ __forceinline__ void keys(const uint64_t (&row)[G], const bool (&live)[G], uint64_t (&out)[G]) const {
+        uint64_t tc[G], gc[G];
+        tail.raw_n<G>(row, live, tc);
+        if (binary) grp.raw_n<G>(row, live, gc);
+#pragma unroll
+        for (int g = 0; g < G; ++g) out[g] = key_of(tc[g], binary ? gc[g] : 0ull);
+    }
 };
+
+// Global insert of a key with its (block's) first row; reads first, so that a key already
+// present with an earlier first row costs no atomic (blocks run roughly in row order, and
+// otherwise every workgroup would serialise on the same few slots).
+__device__ __forceinline__ void grp_global_insert(const HTable &ht, uint32_t *first, uint64_t key, uint32_t row) {
+    uint32_t gs = ht.find(key);
+    if (gs == UINT32_MAX) gs = ht.insert(key);
+    if (gs != UINT32_MAX && first[gs] > row) atomicMin(&first[gs], row);
+}
 
 struct GrpTable {
     HTable ht;
@@ -52,34 +78,108 @@ __global__ __launch_bounds__(kBlock) void k_grp_insert(GrpKey k, GrpTable t, uin
     __syncthreads();
     const uint64_t r0 = static_cast<uint64_t>(blockIdx.x) * kGrpRowsPerBlock;
     const uint64_t r1 = r0 + kGrpRowsPerBlock < n ? r0 + kGrpRowsPerBlock : n;
-    for (uint64_t row = r0 + threadIdx.x; row < r1; row += kBlock) {
-        const uint64_t key = k.key(row);
+    constexpr int B = 8;  // rows per thread whose key loads go out together
+    for (uint64_t base = r0 + threadIdx.x; base < r1; base += B * kBlock) {
+      uint64_t rows[B], keys[B];
+      bool live[B];
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+          rows[q] = base + static_cast<uint64_t>(q) * kBlock;
+          live[q] = rows[q] < r1;
+      }
+      k.keys<B>(rows, live, keys);
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        if (!live[q]) continue;
+        const uint64_t row = rows[q];
+        const uint64_t key = keys[q];
         bool placed = false;
         if (key != ~0ull) {
             const unsigned long long w = static_cast<unsigned long long>(key) + 1ull;
             uint32_t s = static_cast<uint32_t>((key * 0x9E3779B97F4A7C15ull) >> 53);  // 11-bit slot
             for (uint32_t p = 0; p < 16; ++p) {
-                const unsigned long long prev = atomicCAS(&s_key[s], 0ull, w);
+                unsigned long long prev = s_key[s];
+                if (prev == 0ull) prev = atomicCAS(&s_key[s], 0ull, w);
                 if (prev == 0ull || prev == w) {
-                    atomicMin(&s_first[s], static_cast<uint32_t>(row));
+                    // rows of a thread come in increasing order: most find an earlier first row
+                    if (s_first[s] > static_cast<uint32_t>(row)) atomicMin(&s_first[s], static_cast<uint32_t>(row));
                     placed = true;
                     break;
                 }
                 s = (s + 1u) & (kGrpLdsSlots - 1u);
             }
         }
-        if (!placed) {  // LDS table crowded (or the all-ones key): go to the global table directly
-            const uint32_t gs = t.ht.insert(key);
-            if (gs != UINT32_MAX) atomicMin(&t.first[gs], static_cast<uint32_t>(row));
-        }
+        if (!placed)  // LDS table crowded (or the all-ones key): go to the global table directly
+            grp_global_insert(t.ht, t.first, key, static_cast<uint32_t>(row));
+      }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kGrpLdsSlots; i += kBlock) {
         const unsigned long long w = s_key[i];
         if (w == 0ull) continue;
-        const uint32_t gs = t.ht.insert(static_cast<uint64_t>(w) - 1ull);
-        if (gs != UINT32_MAX) atomicMin(&t.first[gs], s_first[i]);
+        grp_global_insert(t.ht, t.first, static_cast<uint64_t>(w) - 1ull, s_first[i]);
     }
+}
+
+// Occupied table slots -> list[] as (first row << 32 | slot), while at most kGrpFastMax.
+__global__ __launch_bounds__(kBlock) void k_grp_collect(GrpTable t, uint32_t *flags, uint64_t *list) {
+    const uint32_t slots = t.ht.size() + 1u;
+    for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < slots; s += gridDim.x * kBlock) {
+        if (t.ht.keys[s] == 0ull) continue;
+        const uint32_t i = atomicAdd(&flags[8], 1u);
+        if (i < kGrpFastMax) list[i] = (static_cast<uint64_t>(t.first[s]) << 32) | s;
+    }
+}
+
+struct GrpSortArgs {
+    GrpTable t;
+    uint32_t *flags;
+    const uint64_t *list;
+    DCol head;
+    uint32_t binary;
+    uint64_t a_oid;
+    uint64_t *groid;
+    uint64_t groid_cap;
+    uint64_t *counts;
+};
+
+// One workgroup: orders the collected keys by first row (bitonic sort in LDS); a key's position
+// is its group id.  More than kGrpFastMax keys: raises flags[9] for the compaction path.
+__global__ __launch_bounds__(1024) void k_grp_sort(GrpSortArgs a) {
+    __shared__ unsigned long long s_v[kGrpFastMax];
+    const uint32_t d = a.flags[8];
+    if (d > kGrpFastMax) {
+        if (threadIdx.x == 0) a.flags[9] = 1u;
+        return;
+    }
+    uint32_t p2 = 1;
+    while (p2 < d) p2 <<= 1;
+    for (uint32_t i = threadIdx.x; i < p2; i += blockDim.x) s_v[i] = i < d ? a.list[i] : ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= p2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < p2; i += blockDim.x) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const unsigned long long x = s_v[i], y = s_v[l];
+                    if ((x > y) == ((i & k) == 0)) {
+                        s_v[i] = y;
+                        s_v[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t g = threadIdx.x; g < d; g += blockDim.x) {
+        const uint32_t slot = static_cast<uint32_t>(s_v[g]);
+        const uint64_t row = s_v[g] >> 32;
+        a.t.gid[slot] = g;
+        // unary: iter->head() * AOID (raw head, groupbyAN.tcc:87); binary: decoded head * AOID (:155)
+        const uint64_t h = a.binary ? a.head.dec_of(a.head.raw(row)) : a.head.raw(row);
+        if (g < a.groid_cap) a.groid[g] = h * a.a_oid;
+    }
+    if (threadIdx.x == 0) a.counts[1] = d;
 }
 
 struct FirstsOp {
@@ -94,6 +194,9 @@ struct FirstsOp {
     uint64_t *groid;
     uint64_t groid_cap;
     uint64_t *counts;
+    const uint32_t *flags;
+    // runs only when the keys were too many for k_grp_sort
+    __device__ bool skip() const { return flags[9] == 0u; }
     __device__ uint32_t eval(uint64_t row, State &st, uint32_t &, uint64_t &) const {
         const uint32_t s = t.ht.find(k.key(row));
         st = s;
@@ -123,6 +226,50 @@ struct AssignOp {
     uint64_t *err[3];
     uint64_t cap[3];
     uint64_t *counts;
+    static constexpr bool kBatched = true;
+    template <int G>
+    __device__ void eval_n(const uint64_t (&row)[G], const bool (&live)[G], State (&)[G], uint32_t (&errs)[G],
+                           uint32_t (&cnt)[G], uint64_t &) const {
+        uint64_t tc[G], gc[G], hc[G], key[G], w[G];
+        k.tail.raw_n<G>(row, live, tc);
+        if (k.binary) k.grp.raw_n<G>(row, live, gc);
+        if (head.encoded) head.raw_n<G>(row, live, hc);
+        uint32_t slot[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            key[g] = k.key_of(tc[g], k.binary ? gc[g] : 0ull);
+            slot[g] = key[g] == ~0ull ? t.ht.size() : t.ht.home(key[g]);
+        }
+        // the home slots of all G keys are read together; a key found elsewhere (a collision)
+        // continues the probe on its own
+#pragma unroll
+        for (int g = 0; g < G; ++g) w[g] = live[g] ? t.ht.keys[slot[g]] : 0ull;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const bool at_home = key[g] == ~0ull ? w[g] != 0ull : w[g] == key[g] + 1ull;
+            if (live[g] && !at_home) slot[g] = t.ht.find(key[g]);
+        }
+        uint32_t gid[G], fr[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const bool ok = live[g] && slot[g] != UINT32_MAX;
+            gid[g] = ok ? t.gid[slot[g]] : 0u;
+            fr[g] = ok ? t.first[slot[g]] : 0u;
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            cnt[g] = 0;
+            errs[g] = 0;
+            if (!live[g]) continue;
+            if (head.encoded && head.bad_of(hc[g])) errs[g] |= 1u;
+            if (k.tail.bad_of(tc[g])) errs[g] |= 2u;
+            if (k.binary && (gc[g] * k.grp.a_inv) > k.grp.max_u) errs[g] |= 4u;
+            if (slot[g] == UINT32_MAX) continue;  // only after a table overflow (reported in the status)
+            const uint64_t rg = static_cast<uint64_t>(gid[g]) * a_oid;
+            const bool first = fr[g] == static_cast<uint32_t>(row[g]);
+            vgid[row[g]] = (k.binary && !first) ? static_cast<uint64_t>(static_cast<uint32_t>(rg)) : rg;
+        }
+    }
     __device__ uint32_t eval(uint64_t row, State &, uint32_t &errs, uint64_t &) const {
         if (head.bad_of(head.raw(row))) errs |= 1u;
         if (k.tail.bad_of(k.tail.raw(row))) errs |= 2u;
@@ -179,8 +326,10 @@ int run_groupby(const ahead_col *head, const ahead_col *tail, const ahead_col *g
     }
     const uint64_t slots = (1ull << bits) + 1;
     const size_t o_flags = 0, o_keys = 256, o_first = o_keys + align_up(slots * 8, 256),
-                 o_gid = o_first + align_up(slots * 4, 256), o_end = o_gid + align_up(slots * 4, 256);
-    if ((st = ws_ensure(ws, o_end)) != AHEAD_OK) return st;
+                 o_gid = o_first + align_up(slots * 4, 256), o_list = o_gid + align_up(slots * 4, 256),
+                 o_end = o_list + align_up(kGrpFastMax * 8ull, 256);
+    // the streaming engine's tables (AssignOp) follow the group table
+    if ((st = ws_ensure(ws, o_end + map_scratch_bytes<AssignOp>(n))) != AHEAD_OK) return st;
     GrpTable t{};
     uint32_t *flags = reinterpret_cast<uint32_t *>(ws_region(ws, o_flags));
     t.ht.keys = reinterpret_cast<unsigned long long *>(ws_region(ws, o_keys));
@@ -207,15 +356,30 @@ int run_groupby(const ahead_col *head, const ahead_col *tail, const ahead_col *g
         hipLaunchKernelGGL(k_grp_insert, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, s, k, t, n);
         AHEAD_HIP_TRY(hipGetLastError());
     }
+    uint64_t *list = reinterpret_cast<uint64_t *>(ws_region(ws, o_list));
+    hipLaunchKernelGGL(k_grp_collect, dim3(1024), dim3(kBlock), 0, s, t, flags, list);
+    GrpSortArgs ga{};
+    ga.t = t;
+    ga.flags = flags;
+    ga.list = list;
+    ga.head = dcol(*head);
+    ga.binary = binary;
+    ga.a_oid = a_oid;
+    ga.groid = static_cast<uint64_t *>(vgidtoroid.data);
+    ga.groid_cap = vgidtoroid.data ? vgidtoroid.cap : 0;
+    ga.counts = d_counts;
+    hipLaunchKernelGGL(k_grp_sort, dim3(1), dim3(1024), 0, s, ga);
+    AHEAD_HIP_TRY(hipGetLastError());
     FirstsOp f{};
     f.n = n;
     f.k = k;
     f.t = t;
     f.head = dcol(*head);
     f.a_oid = a_oid;
-    f.groid = static_cast<uint64_t *>(vgidtoroid.data);
-    f.groid_cap = vgidtoroid.data ? vgidtoroid.cap : 0;
+    f.groid = ga.groid;
+    f.groid_cap = ga.groid_cap;
     f.counts = d_counts;
+    f.flags = flags;
     if ((st = launch_rowtile(f, n, 1, ws, s)) != AHEAD_OK) return st;
     AssignOp a{};
     a.n = n;
@@ -230,7 +394,7 @@ int run_groupby(const ahead_col *head, const ahead_col *tail, const ahead_col *g
         a.cap[i] = e[i].data ? e[i].cap : 0;
     }
     a.counts = d_counts;
-    if ((st = launch_rowtile(a, n, 4, ws, s)) != AHEAD_OK) return st;
+    if ((st = launch_map(a, ws, o_end, s)) != AHEAD_OK) return st;
     hipLaunchKernelGGL(k_grp_status, dim3(1), dim3(1), 0, s, flags, d_counts);
     AHEAD_HIP_TRY(hipGetLastError());
     return AHEAD_OK;

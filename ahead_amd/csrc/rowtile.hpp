@@ -23,6 +23,7 @@
 #pragma once
 
 #include <type_traits>
+#include <utility>
 
 #include "host_common.hpp"
 
@@ -41,8 +42,18 @@ struct has_eval_n : std::false_type {};
 template <class Op>
 struct has_eval_n<Op, std::void_t<decltype(Op::kBatched)>> : std::true_type {};
 
+// An operator may stand a whole launch down (skip(): true in every workgroup, decided from device
+// state before any tile is taken).
+template <class Op, class = void>
+struct has_skip : std::false_type {};
+template <class Op>
+struct has_skip<Op, std::void_t<decltype(std::declval<const Op &>().skip())>> : std::true_type {};
+
 template <class Op, int G>
 __global__ __launch_bounds__(kBlock) void k_rowtile(Op op, LbArgs lb) {
+    if constexpr (has_skip<Op>::value) {
+        if (op.skip()) return;
+    }
     constexpr int NS = 1 + Op::kErr;  // stream 0 = main
     constexpr int NW = G * kWaves;    // wave slots per tile, in row order
     static_assert(NW <= kBlock, "one thread per wave slot in the block scan");
