@@ -21,7 +21,7 @@
 //             every probe row in probe order, build rows in insertion order.
 // Both hash sides (hash_side_t::right / ::left) and the re-encoding variants are supported.
 #include "htable.hpp"
-#include "rowtile.hpp"
+#include "maptile.hpp"
 
 using namespace ahead_dev;
 
@@ -241,6 +241,61 @@ struct ProbeOp {
     uint64_t cap[2];
     uint64_t *counts;         // [0] pairs, error counts at [e_index[0]], [e_index[1]]
     uint32_t e_index[2];
+    uint32_t map_launched;    // the streaming-engine probe (ProbeMapOp) was launched before this one
+
+    // the tile engine runs this probe only when some build key has several rows (or the streaming
+    // probe could not be launched)
+    __device__ bool skip() const { return map_launched != 0u && t.flags[0] == 0u; }
+
+    static constexpr bool kBatched = true;
+    template <int G>
+    __device__ void eval_n(const uint64_t (&row)[G], const bool (&live)[G], State (&st)[G], uint32_t (&errs)[G],
+                           uint32_t (&cnt)[G], uint64_t &) const {
+        uint64_t h[G], tl[G], k[G];
+        bool ok[G];
+        phead.raw_n<G>(row, live, h);
+        ptail.raw_n<G>(row, live, tl);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            errs[g] = (live[g] && phead.bad_of(h[g]) ? 1u : 0u) | (live[g] && ptail.bad_of(tl[g]) ? 2u : 0u);
+            const uint64_t kv = key.dec_of(build_right ? tl[g] : h[g]);
+            st[g].pay = build_right ? h[g] : tl[g];
+            ok[g] = live[g] && (kv & guard_mask) <= build_max;
+            k[g] = kv & key_mask;
+            cnt[g] = 0;
+        }
+        if (t.flags[4] != 0u) {  // direct map: one gather per row, all G issued together
+            const uint64_t lo = t.flags[3], span = t.flags[5] - t.flags[3];
+            uint32_t e[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const uint64_t rel = k[g] - lo;
+                ok[g] = ok[g] && rel <= span;
+                e[g] = t.dmap[ok[g] ? rel : 0ull];
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (!ok[g] || e[g] == 0u) continue;
+                if ((e[g] & kDmapMulti) == 0u) {
+                    st[g].brow = e[g] - 1u;
+                    cnt[g] = 1;
+                } else {
+                    st[g].slot = e[g] & ~kDmapMulti;
+                    cnt[g] = t.cnt[st[g].slot];
+                }
+            }
+            return;
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (!ok[g]) continue;
+            const uint32_t sl = t.ht.find(k[g]);
+            if (sl == UINT32_MAX) continue;
+            st[g].slot = sl;
+            cnt[g] = t.cnt[sl];
+            if (cnt[g] == 1u) st[g].brow = t.first[sl];
+        }
+    }
 
     __device__ uint32_t eval(uint64_t row, State &st, uint32_t &errs, uint64_t &) const {
         const uint64_t h = phead.raw(row), tl = ptail.raw(row);
@@ -298,6 +353,20 @@ struct ProbeOp {
     }
 };
 
+// The probe on the streaming engine: when no build key has several rows, a probe row emits at
+// most one pair.
+struct ProbeMapOp : ProbeOp {
+    static constexpr bool kMulti = false;
+    static constexpr int kOuts = 2;
+    __device__ bool skip() const { return t.flags[0] != 0u; }
+    __device__ void out_arrays(void *(&p)[2], uint32_t (&w)[2]) const {
+        p[0] = out_h;
+        p[1] = out_t;
+        w[0] = build_right ? w_p : w_b;
+        w[1] = build_right ? w_b : w_p;
+    }
+};
+
 __global__ void k_hj_status(const uint32_t *flags, uint64_t *d_counts) {
     d_counts[5] = flags[1] ? 1u : (flags[2] ? 2u : 0u);
 }
@@ -341,7 +410,8 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
                  o_first = o_cnt + align_up(slots * 4, 256), o_off = o_first + align_up(slots * 4, 256),
                  o_cur = o_off + align_up(slots * 4, 256), o_rows = o_cur + align_up(slots * 4, 256),
                  o_dmap = o_rows + align_up(nb * 4, 256), o_end = o_dmap + align_up(map_cap * 4, 256);
-    if ((st = ws_ensure(ws, o_end)) != AHEAD_OK) return st;
+    // the streaming probe's tables follow the build's
+    if ((st = ws_ensure(ws, o_end + map_scratch_bytes<ProbeMapOp>(np))) != AHEAD_OK) return st;
 
     HjTable t{};
     t.flags = reinterpret_cast<uint32_t *>(ws_region(ws, o_flags));
@@ -433,6 +503,14 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     p.counts = d_counts;
     p.e_index[0] = right ? 1 : 3;
     p.e_index[1] = right ? 2 : 4;
+    // the streaming engine needs room for a staged pair per probe row
+    const bool map_ok = out_h.data && out_t.data && p.out_cap >= np;
+    if (map_ok) {
+        ProbeMapOp pm{};
+        static_cast<ProbeOp &>(pm) = p;
+        if ((st = launch_map(pm, ws, o_end, s)) != AHEAD_OK) return st;
+        p.map_launched = 1u;
+    }
     if ((st = launch_rowtile(p, np, 3, ws, s)) != AHEAD_OK) return st;
     hipLaunchKernelGGL(k_hj_status, dim3(1), dim3(1), 0, s, t.flags, d_counts);
     AHEAD_HIP_TRY(hipGetLastError());

@@ -96,6 +96,9 @@ __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
     static_assert(!Op::kMulti, "the map engine takes operators whose rows emit at most one entry");
     constexpr int NS = MapTraits<Op>::NS;
     __shared__ unsigned long long s_acc[SharedWords<Op>::value];
+    if constexpr (has_skip<Op>::value) {
+        if (op.skip()) return;
+    }
     if constexpr (has_shared<Op>::value) {
         op.shared_init(s_acc);
         __syncthreads();
@@ -167,6 +170,9 @@ __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
 template <class Op>
 __global__ __launch_bounds__(1024) void k_map_scan(Op op, MapArgs m) {
     __shared__ uint32_t s_w[16];
+    if constexpr (has_skip<Op>::value) {
+        if (op.skip()) return;
+    }
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const size_t s = blockIdx.x;
     uint32_t carry = 0;
@@ -221,6 +227,9 @@ __device__ __forceinline__ uint64_t map_load_entry(const void *p, uint32_t w, ui
 // most D slots left of its range).  side[(gw * 2 + array) * kMaxShift + k] holds entry k of them.
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_map_save(Op op, MapArgs m) {
+    if constexpr (has_skip<Op>::value) {
+        if (op.skip()) return;
+    }
     if constexpr (Op::kMain) {
         const uint32_t total = map_total_drops<Op>(m);
         if (total == 0u || total > kMaxShift) return;
@@ -249,6 +258,9 @@ __global__ __launch_bounds__(kBlock) void k_map_save(Op op, MapArgs m) {
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_map_fix(Op op, MapArgs m) {
     constexpr int NS = MapTraits<Op>::NS;
+    if constexpr (has_skip<Op>::value) {
+        if (op.skip()) return;
+    }
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
     if (gw >= m.waves) return;
