@@ -108,6 +108,9 @@ __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
     const uint64_t begin = static_cast<uint64_t>(gw) * m.rows_per_wave;
     const uint64_t end = begin + m.rows_per_wave < op.n ? begin + m.rows_per_wave : op.n;
     uint32_t kept = 0, err_rows = 0;  // wave-uniform
+    // A wave that alone drops more than kMaxShift rows knows the entries will be written again (the
+    // rewrite path): it stops staging them.
+    uint32_t dropped = 0;              // wave-uniform
     uint32_t ecnt[Op::kErr > 0 ? Op::kErr : 1] = {};
     uint64_t acc = 0;
     for (uint64_t seg = begin; seg < end; seg += 64u * kMapG) {
@@ -129,8 +132,9 @@ __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
         for (int g = 0; g < kMapG; ++g) {
             if constexpr (Op::kMain) {
                 const uint64_t b = __ballot(cnt[g] != 0u);
-                if (cnt[g] != 0u) op.emit_main(rows[g], st[g], begin + kept + map_below(b), 1u);
+                if (cnt[g] != 0u && dropped <= kMaxShift) op.emit_main(rows[g], st[g], begin + kept + map_below(b), 1u);
                 kept += static_cast<uint32_t>(__popcll(b));
+                dropped += static_cast<uint32_t>(__popcll(__ballot(live[g] && cnt[g] == 0u)));
             }
 #pragma unroll
             for (int e = 0; e < Op::kErr; ++e) ecnt[e] += (errs[g] >> e) & 1u;

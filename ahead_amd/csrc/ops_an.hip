@@ -126,6 +126,48 @@ struct MatchPosOp {
     uint64_t cap[3];
     uint64_t *d_counts;
 
+    static constexpr bool kBatched = true;
+    static constexpr int kOuts = 2;
+    __device__ void out_arrays(void *(&p)[2], uint32_t (&w)[2]) const {
+        p[0] = out_h;
+        p[1] = out_t;
+        w[0] = wh;
+        w[1] = wt;
+    }
+    template <int G>
+    __device__ void eval_n(const uint64_t (&row)[G], const bool (&live)[G], State (&st)[G], uint32_t (&errs)[G],
+                           uint32_t (&cnt)[G], uint64_t &) const {
+        uint64_t h[G], o[G], p2[G], v[G];
+        bool in[G];
+        h1.raw_n<G>(row, live, h);
+        t1.raw_n<G>(row, live, o);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            p2[g] = o[g] * t1.a_inv;
+            in[g] = live[g] && p2[g] < t2.n;
+        }
+        t2.raw_n<G>(p2, in, v);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            errs[g] = 0;
+            cnt[g] = 0;
+            if (!live[g]) continue;
+            if (h1.bad_of(h[g])) errs[g] |= 1u;
+            st[g].h = reenc_h ? (h[g] * f_h1) & mh : h[g];
+            if (!in[g]) {
+                errs[g] |= 2u;
+                continue;
+            }
+            if (t2.bad_of(v[g])) {
+                errs[g] |= 4u;
+                st[g].v = p2[g];
+                continue;
+            }
+            st[g].v = reenc_t ? (v[g] * f_t2) & mt : v[g];
+            cnt[g] = 1;
+        }
+    }
+
     __device__ uint32_t eval(uint64_t row, State &st, uint32_t &errs, uint64_t &) const {
         const uint64_t h = h1.raw(row);
         if (h1.bad_of(h)) errs |= 1u;
@@ -487,6 +529,9 @@ extern "C" int ahead_hip_matchjoin_pos_an(const ahead_col *h1, const ahead_col *
         op.cap[i] = e[i].data ? e[i].cap : 0;
     }
     op.d_counts = d_counts;
+    // the streaming engine stages pairs at their row's place: it needs room for every row
+    if (op.out_h != nullptr && op.out_t != nullptr && op.out_cap >= op.n)
+        return launch_map(op, ws, 0, static_cast<hipStream_t>(stream));
     return launch_rowtile(op, op.n, 4, ws, static_cast<hipStream_t>(stream));
 }
 
