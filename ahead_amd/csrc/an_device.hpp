@@ -75,8 +75,8 @@ __device__ __forceinline__ T wave_sum(T v) {
 }
 
 // Exclusive block scan of one value per thread; `total` receives the block sum.  `scratch`
-// must hold kWaves elements of LDS.  Contains two barriers.
-template <typename T>
+// must hold NW elements of LDS (NW: waves of the block).  Contains two barriers.
+template <typename T, int NW = kWaves>
 __device__ __forceinline__ T block_excl_scan(T v, T &total, T *scratch) {
     const T inc = wave_incl_scan(v);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -84,7 +84,7 @@ __device__ __forceinline__ T block_excl_scan(T v, T &total, T *scratch) {
     __syncthreads();
     T pre = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
+    for (int w = 0; w < NW; ++w) {
         const T x = scratch[w];
         if (w < wid) pre += x;
         tot += x;
@@ -94,7 +94,7 @@ __device__ __forceinline__ T block_excl_scan(T v, T &total, T *scratch) {
     return pre + inc - v;
 }
 
-template <typename T>
+template <typename T, int NW = kWaves>
 __device__ __forceinline__ T block_sum(T v, T *scratch) {
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -102,7 +102,7 @@ __device__ __forceinline__ T block_sum(T v, T *scratch) {
     __syncthreads();
     T tot = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) tot += scratch[w];
+    for (int w = 0; w < NW; ++w) tot += scratch[w];
     __syncthreads();
     return tot;
 }

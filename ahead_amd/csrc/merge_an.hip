@@ -9,25 +9,37 @@
 // A list-2 row is therefore checked at TOP and again when SKIP2 passes it, and a matched list-1
 // row is never checked; the error vectors record exactly these visits.
 //
-// The GPU splits list 1 into chunks of kMmRows rows, one chunk per thread.  For sorted inputs the
-// state in which the walk first reaches row i0 = c * kMmRows follows from three binary searches
-// (k_mm_start): with v = t1[i0-1], r its rank in its run of equal keys in list 1 and [lb2, ub2)
-// the run of v in list 2, the walk arrives by a match at (i0, lb2 + r + 1) in TOP if r < ub2 - lb2,
-// else by a skip at (i0, ub2) in SKIP1, or never (ub2 = |list 2|: the walk ended first).  Each
-// thread walks its chunk from that state (k_mm_count), counts what it emits into the five output
-// streams, and checks that it leaves the chunk in exactly the state computed for the next chunk.
-// Chunk 0 starts at (0, 0) in TOP, so if every chunk agrees the chunk walks are the reference's
-// walk, and k_mm_write repeats them, writing at offsets from a decoupled look-back scan.  If any
-// chunk disagrees (the inputs are not sorted in decoded value, e.g. a corrupted key),
-// k_mm_fallback walks the whole input with one thread instead: slower, same results.
+// The GPU cuts the walk into sub-tiles along the merge path of the two lists: sub-tile b starts at
+// the list-1 row i_b where the diagonal i + j = b * D of the merged order (ties: list 1 first)
+// crosses, so a sub-tile spans about D rows of both lists together whatever their length ratio.
+// For sorted inputs the state in which the walk first reaches row i follows from searches: with
+// v = t1[i-1], r its rank in its run of equal keys in list 1 and [lb2, ub2) the run of v in list 2,
+// the walk arrives by a match at (i, lb2 + r + 1) in TOP if r < ub2 - lb2, else by a skip at
+// (i, ub2) in SKIP1, or never (ub2 = |list 2|: the walk ended first).  k_mm_split computes i_b and
+// that state for every sub-tile.  k_mm_count stages a sub-tile's list-1 rows and the list-2 rows
+// its walk reads in LDS with coalesced loads (decoded keys and error bits), gives each thread a run
+// of the list-1 rows with the entry state computed the same way (searches in LDS), walks them,
+// checks that every thread leaves its rows in exactly the state the next one starts from (the last
+// thread: the next sub-tile's state) and writes the sub-tile's output pairs, coalesced, to a
+// provisional slot at min(i_b, j_b): no earlier sub-tile emits more pairs than it consumes rows of
+// either list, so the slots are disjoint and each lies at or after the pairs' final place.  A scan
+// of the per-sub-tile counts gives the final offsets and k_mm_move copies every slot there.
+// Sub-tiles with an error position to write (rare) or whose walk left its LDS window walk again in
+// k_mm_move, writing at the final offsets directly.  Sub-tile 0 starts at (0, 0) in TOP, so if
+// every check holds the pieces are the reference's walk.  If any fails (the inputs are not sorted
+// in decoded value, e.g. a corrupted key), k_mm_fallback walks the whole input with one thread
+// instead: slower, same results.
 #include "rowtile.hpp"
 
 namespace {
 
 using namespace ahead_dev;
 
-constexpr uint32_t kMmRows = 16;  // list-1 rows per chunk
-constexpr uint64_t kPhTop = 0, kPhSkip1 = 1, kPhEnd = 2;
+constexpr uint32_t kMmDiag = 2560;                   // merged rows per sub-tile
+constexpr uint32_t kMmRowsLds = 3072;                 // staged rows of both lists (slack for runs of equal keys)
+constexpr uint32_t kMmLdsBytes = kMmRowsLds * 9;      // decoded key + error bits per staged row
+constexpr uint32_t kMmGroup = 256;                    // sub-tiles per group of the offset scan
+constexpr uint64_t kPhTop = 0, kPhSkip1 = 1, kPhEnd = 2, kPhSkip2 = 3;  // kPhSkip2: inside a walk only
 constexpr uint64_t kJMask = (1ull << 62) - 1ull;
 constexpr int kMmStreams = 5;     // pairs, e1 (h1), e2 (t1), e3 (h2), e4 (t2)
 
@@ -36,7 +48,6 @@ __device__ __forceinline__ uint64_t st_pack(uint64_t j, uint64_t ph) { return ph
 struct MergeArgs {
     DCol h1, t1, h2, t2;
     uint64_t n1, n2;
-    uint64_t num_chunks;
     uint64_t f_h1, f_t2;
     uint32_t reenc_h, reenc_t, wout_h, wout_t;
     uint64_t a_oid;
@@ -44,17 +55,26 @@ struct MergeArgs {
     uint64_t out_cap;
     uint64_t *err[4];
     uint64_t err_cap[4];
-    uint64_t *start;    // [num_chunks] entry state of each chunk
-    uint32_t *offs;     // [kMmStreams][num_chunks] exclusive output offsets
-    uint32_t *flags;    // [0]: a chunk disagreed with its successor's entry state
+    const uint64_t *ti, *tstate, *tj0;  // per sub-tile: first list-1 row, entry state, first list-2 row read
+    uint64_t num_sub;   // sub-tiles
+    uint64_t cap1;      // list-1 rows a sub-tile may hold
+    uint32_t h1_is_t1, t2_is_h2;  // a list whose two sides are one column
+    uint32_t h1_from_t1, t2_from_h2;  // ... and whose codes re-encode exactly from the decoded keys
+    uint64_t a_t1, a_h2;  // A of the key columns (masked to the code width)
+    uint32_t *sub_cnt;  // [kMmStreams][num_sub] output of each sub-tile per stream
+    uint32_t *sub_slow; // [num_sub] 1: the sub-tile writes in k_mm_move by walking again
+    uint64_t *grp;      // [kMmStreams][groups] sums per group of kMmGroup sub-tiles, then their exclusive prefixes
+    uint64_t num_grp;
+    void *prov_h, *prov_t;  // provisional pair slots
+    uint64_t prov_cap;
+    uint32_t *flags;    // [0]: a piece of the walk disagreed with its successor's entry state
     uint64_t *d_counts;
 };
 
 __device__ __forceinline__ uint64_t dec_at(const DCol &c, uint64_t i) { return c.dec_of(c.raw(i)); }
 
-// first index in [0, hi) whose decoded value is >= v (strict = false) or > v (strict = true)
-__device__ uint64_t bound_dec(const DCol &c, uint64_t hi, uint64_t v, bool strict) {
-    uint64_t lo = 0;
+// first index in [lo, hi) whose decoded value is >= v (strict: > v); hi if there is none
+__device__ uint64_t first_ge(const DCol &c, uint64_t lo, uint64_t hi, uint64_t v, bool strict) {
     while (lo < hi) {
         const uint64_t mid = lo + ((hi - lo) >> 1);
         const uint64_t d = dec_at(c, mid);
@@ -63,35 +83,119 @@ __device__ uint64_t bound_dec(const DCol &c, uint64_t hi, uint64_t v, bool stric
     }
     return lo;
 }
-
-__global__ void k_mm_start(MergeArgs a) {
-    const uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= a.num_chunks) return;
-    if (c == 0) {
-        a.start[0] = st_pack(0, a.n1 && a.n2 ? kPhTop : kPhEnd);
-        return;
+// the same over decoded codes held in LDS
+__device__ uint32_t first_ge_lds(const DCol &c, const uint64_t *w, uint32_t n, uint64_t v, bool strict) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint64_t d = c.dec_of(w[mid]);
+        if (strict ? d <= v : d < v) lo = mid + 1;
+        else hi = mid;
     }
-    const uint64_t i0 = c * kMmRows;
-    const uint64_t v = dec_at(a.t1, i0 - 1);
-    const uint64_t lb1 = bound_dec(a.t1, i0, v, false);
-    const uint64_t r = i0 - 1 - lb1;
-    const uint64_t lb2 = bound_dec(a.h2, a.n2, v, false);
-    const uint64_t ub2 = bound_dec(a.h2, a.n2, v, true);
-    uint64_t s;
-    if (r < ub2 - lb2) s = st_pack(lb2 + r + 1, kPhTop);
-    else if (ub2 >= a.n2) s = st_pack(0, kPhEnd);
-    else s = st_pack(ub2, kPhSkip1);
-    a.start[c] = s;
+    return lo;
+}
+// First index in [0, p] whose value is >= v, for p whose value is >= v (or p = n): gallops back
+// from p, so a short run costs a load or two.
+__device__ uint64_t gallop_lb(const DCol &c, uint64_t p, uint64_t v) {
+    uint64_t hi = p, step = 1;
+    for (;;) {
+        if (hi < step) return first_ge(c, 0, hi, v, false);
+        const uint64_t q = hi - step;
+        if (dec_at(c, q) < v) return first_ge(c, q + 1, hi, v, false);
+        hi = q;
+        step <<= 1;
+    }
+}
+// First index in [p, n) whose value is > v, galloping forward from p.
+__device__ uint64_t gallop_ub(const DCol &c, uint64_t p, uint64_t n, uint64_t v) {
+    uint64_t lo = p, step = 1;
+    for (;;) {
+        if (lo >= n) return n;
+        const uint64_t q = lo + step - 1 < n ? lo + step - 1 : n - 1;
+        if (dec_at(c, q) > v) return first_ge(c, lo, q, v, true);
+        lo = q + 1;
+        step <<= 1;
+    }
+}
+// Entry state at list-1 row i >= 1 from the run of v = t1[i-1]: [lb1, i) in list 1, [lb2, ub2) in
+// list 2.
+__device__ __forceinline__ uint64_t entry_state(uint64_t i, uint64_t lb1, uint64_t lb2, uint64_t ub2, uint64_t n2) {
+    const uint64_t r = i - 1 - lb1;
+    if (r < ub2 - lb2) return st_pack(lb2 + r + 1, kPhTop);
+    if (ub2 >= n2) return st_pack(0, kPhEnd);
+    return st_pack(ub2, kPhSkip1);
 }
 
-// Walks list-1 rows [i, iend) from entry state `st`; returns the state in which the walk reaches
-// row iend (kPhEnd if it stops first or iend is the end of list 1).  WRITE: emits at base[] +
-// running counts; otherwise only counts.
-template <bool WRITE>
-__device__ uint64_t walk(const MergeArgs &a, uint64_t i, uint64_t iend, uint64_t st, uint32_t (&cnt)[kMmStreams],
-                         const uint64_t (&base)[kMmStreams]) {
+struct SplitArgs {
+    uint64_t diag;    // merged rows per tile
+    uint32_t tiles;
+    uint64_t *i;      // [tiles + 1] first list-1 row of each tile (i[tiles] = |list 1|)
+    uint64_t *state;  // [tiles + 1] entry state at that row (state[tiles]: end)
+    uint64_t *j0;     // [tiles + 1] lb2 of t1[i-1]: the first list-2 row the tile's walk can read
+};
+
+__global__ __launch_bounds__(kBlock) void k_mm_split(MergeArgs a, SplitArgs sp) {
+    const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b > sp.tiles) return;
+    uint64_t i, st, j0;
+    if (b == 0 || a.n1 == 0 || a.n2 == 0) {
+        i = b == 0 ? 0 : a.n1;
+        st = b == 0 && a.n1 && a.n2 ? st_pack(0, kPhTop) : st_pack(0, kPhEnd);
+        j0 = 0;
+        if (b == sp.tiles) st = st_pack(0, kPhEnd);
+    } else if (b == sp.tiles) {
+        i = a.n1;
+        st = st_pack(0, kPhEnd);
+        j0 = a.n2;
+    } else {
+        // merge path: the number of list-1 rows among the first d rows of the merged order
+        const uint64_t d = b * sp.diag;
+        uint64_t lo = d > a.n2 ? d - a.n2 : 0, hi = d < a.n1 ? d : a.n1;
+        while (lo < hi) {
+            const uint64_t mid = lo + ((hi - lo) >> 1);
+            if (dec_at(a.t1, mid) <= dec_at(a.h2, d - mid - 1)) lo = mid + 1;
+            else hi = mid;
+        }
+        i = lo;
+        if (i == 0) {
+            st = st_pack(0, kPhTop);
+            j0 = 0;
+        } else {
+            const uint64_t v = dec_at(a.t1, i - 1);
+            const uint64_t lb1 = gallop_lb(a.t1, i - 1, v);
+            const uint64_t jb = d - i;  // lb2(v) <= jb for sorted inputs
+            const uint64_t lb2 = gallop_lb(a.h2, jb, v);
+            const uint64_t ub2 = gallop_ub(a.h2, lb2, a.n2, v);
+            st = entry_state(i, lb1, lb2, ub2, a.n2);
+            j0 = lb2;
+        }
+    }
+    sp.i[b] = i;
+    sp.state[b] = st;
+    sp.j0[b] = j0;
+}
+
+// The walk straight from the columns: the single-thread walk, and the pieces of a sub-tile whose
+// list-2 rows leave its LDS window.
+struct GlobalSrc {
+    const MergeArgs &a;
+    __device__ uint64_t t1(uint64_t i) const { return a.t1.raw(i); }
+    __device__ uint64_t h1(uint64_t i) const { return a.h1.raw(i); }
+    __device__ uint64_t h2(uint64_t j) const { return a.h2.raw(j); }
+    __device__ uint64_t t2(uint64_t j) const { return a.t2.raw(j); }
+};
+// A walk piece from the columns with ORDERED set stops (returning kAbort, which matches no entry
+// state) at the first key that is smaller than the one read before it in the same list: on such
+// inputs the pieces cannot be the reference's walk, and without the stop a piece could wander
+// through most of list 2.
+constexpr uint64_t kAbort = ~0ull;
+
+template <bool WRITE, class Src, bool ORDERED = false>
+__device__ __forceinline__ uint64_t walk(const MergeArgs &a, const Src &src, uint64_t i, uint64_t iend, uint64_t st,
+                         uint32_t (&cnt)[kMmStreams], const uint64_t (&base)[kMmStreams]) {
     uint64_t ph = st >> 62, j = st & kJMask;
     if (ph == kPhEnd) return st_pack(0, kPhEnd);
+    uint64_t last1 = 0, last2 = 0;
     auto err = [&](int e, uint64_t pos) {
         if constexpr (WRITE) {
             const uint64_t p = base[1 + e] + cnt[1 + e];
@@ -101,40 +205,49 @@ __device__ uint64_t walk(const MergeArgs &a, uint64_t i, uint64_t iend, uint64_t
     };
     auto check2 = [&](uint64_t jj, uint64_t hcode) {
         if (a.h2.bad_of(hcode)) err(2, jj);
-        if (a.t2.encoded && a.t2.bad_of(a.t2.raw(jj))) err(3, jj);
+        if (a.t2.encoded && a.t2.bad_of(src.t2(jj))) err(3, jj);
     };
     uint64_t hcode = 0, h2v = 0;
     if (ph == kPhSkip1) {
-        hcode = a.h2.raw(j);
+        hcode = src.h2(j);
         h2v = a.h2.dec_of(hcode);
+        last2 = h2v;
     }
     for (;;) {
         if (ph == kPhTop) {
             if (j >= a.n2) return st_pack(0, kPhEnd);
-            hcode = a.h2.raw(j);
+            hcode = src.h2(j);
             h2v = a.h2.dec_of(hcode);
+            if (ORDERED && h2v < last2) return kAbort;
+            last2 = h2v;
             check2(j, hcode);
         }
-        uint64_t tcode = a.t1.raw(i);
+        uint64_t tcode = src.t1(i);
         uint64_t t1v = a.t1.dec_of(tcode);
+        if (ORDERED && t1v < last1) return kAbort;
+        last1 = t1v;
         while (t1v < h2v) {  // SKIP1
-            if (a.h1.encoded && a.h1.bad_of(a.h1.raw(i))) err(0, i);
+            if (a.h1.encoded && a.h1.bad_of(src.h1(i))) err(0, i);
             if (a.t1.bad_of(tcode)) err(1, i);
             if (++i == iend) return iend == a.n1 ? st_pack(0, kPhEnd) : st_pack(j, kPhSkip1);
-            tcode = a.t1.raw(i);
+            tcode = src.t1(i);
             t1v = a.t1.dec_of(tcode);
+            if (ORDERED && t1v < last1) return kAbort;
+            last1 = t1v;
         }
         while (t1v > h2v) {  // SKIP2
             check2(j, hcode);
             if (++j >= a.n2) return st_pack(0, kPhEnd);
-            hcode = a.h2.raw(j);
+            hcode = src.h2(j);
             h2v = a.h2.dec_of(hcode);
+            if (ORDERED && h2v < last2) return kAbort;
+            last2 = h2v;
         }
         if (t1v == h2v) {
             if constexpr (WRITE) {
                 const uint64_t p = base[0] + cnt[0];
                 if (p < a.out_cap) {
-                    const uint64_t h = a.h1.raw(i), t = a.t2.raw(j);
+                    const uint64_t h = src.h1(i), t = src.t2(j);
                     store_w(a.out_h, p, a.wout_h, a.reenc_h ? h * a.f_h1 : h);
                     store_w(a.out_t, p, a.wout_t, a.reenc_t ? t * a.f_t2 : t);
                 }
@@ -147,48 +260,564 @@ __device__ uint64_t walk(const MergeArgs &a, uint64_t i, uint64_t iend, uint64_t
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_mm_count(MergeArgs a, LbArgs lb) {
-    __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_excl[kMmStreams];
-    __shared__ uint32_t s_red[kWaves];
-    const uint32_t tile = take_tile(lb, &s_tile);
-    if (tile >= lb.num_tiles) return;
-    const uint64_t c = static_cast<uint64_t>(tile) * kBlock + threadIdx.x;
+constexpr uint32_t kMmThreads = 256;              // threads of the sub-tile kernels
+constexpr int kMmWaves = static_cast<int>(kMmThreads / 64);
+constexpr uint32_t kMmMoveSubs = 32;              // sub-tiles per block of k_mm_move (divides kMmGroup)
+
+// One sub-tile in LDS: decoded keys of its list-1 rows [I0, I1) and list-2 rows [J0, J0 + W), and
+// per row the error bits of both sides (list 1: bit 0 t1, bit 1 h1; list 2: bit 0 h2, bit 1 t2).
+struct SubTile {
+    uint64_t *L1, *L2;
+    uint8_t *F1, *F2;
+    uint64_t I0, I1, J0;
+    uint32_t n1t, W;
+    bool ok;
+};
+
+__device__ __forceinline__ void sub_view(const MergeArgs &a, uint64_t sub, uint8_t *lds, SubTile &T) {
+    T.I0 = a.ti[sub];
+    const uint64_t I1r = a.ti[sub + 1];
+    T.ok = I1r >= T.I0 && I1r - T.I0 <= a.cap1;  // false only for unsorted inputs
+    T.I1 = T.ok ? I1r : T.I0;
+    T.n1t = static_cast<uint32_t>(T.I1 - T.I0);
+    T.J0 = a.tj0[sub] < a.n2 ? a.tj0[sub] : a.n2;
+    const uint64_t room = kMmRowsLds - T.n1t;
+    T.W = static_cast<uint32_t>(room < a.n2 - T.J0 ? room : a.n2 - T.J0);
+    T.L1 = reinterpret_cast<uint64_t *>(lds);
+    T.L2 = T.L1 + T.n1t;
+    T.F1 = reinterpret_cast<uint8_t *>(T.L2 + T.W);
+    T.F2 = T.F1 + T.n1t;
+}
+
+// Stages n <= kMmRowsLds rows of a key column: decoded key and its error bit, plus the error bit of
+// the other side of the list when it is the same column (same code).  Each thread issues all its
+// loads (clamped to the last row, so unconditional) before the first LDS store waits on them.  The
+// stagers return what the calling thread saw: kSumErr if it set an error bit, kSumWide if a key
+// needs more than 32 bits.
+constexpr int kStageU = static_cast<int>((kMmRowsLds + kMmThreads - 1) / kMmThreads);
+constexpr uint32_t kSumErr = 1u, kSumWide = 2u;
+template <typename T>
+__device__ __forceinline__ uint32_t stage_keys_t(const DCol &key, const DCol &other, bool same, const T *p,
+                                                 uint64_t base, uint32_t n, uint64_t *L, uint8_t *F) {
+    T x[kStageU];
+#pragma unroll
+    for (int u = 0; u < kStageU; ++u) {
+        const uint32_t k = u * kMmThreads + threadIdx.x;
+        x[u] = p[base + (k < n ? k : n - 1)];
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int u = 0; u < kStageU; ++u) {
+        const uint32_t k = u * kMmThreads + threadIdx.x;
+        if (k < n) {
+            const uint64_t c = x[u];
+            const uint64_t d = key.dec_of(c);
+            const uint32_t f = (key.bad_of(c) ? 1u : 0u) | (same && other.bad_of(c) ? 2u : 0u);
+            L[k] = d;
+            F[k] = static_cast<uint8_t>(f);
+            sum |= (f ? kSumErr : 0u) | ((d >> 32) ? kSumWide : 0u);
+        }
+    }
+    return sum;
+}
+// ORs the error bit of the other side (a column of its own) into F.
+template <typename T>
+__device__ __forceinline__ uint32_t stage_other_t(const DCol &other, const T *p, uint64_t base, uint32_t n, uint8_t *F) {
+    T x[kStageU];
+#pragma unroll
+    for (int u = 0; u < kStageU; ++u) {
+        const uint32_t k = u * kMmThreads + threadIdx.x;
+        x[u] = p[base + (k < n ? k : n - 1)];
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int u = 0; u < kStageU; ++u) {
+        const uint32_t k = u * kMmThreads + threadIdx.x;
+        if (k < n && other.bad_of(x[u])) {
+            F[k] |= 2u;
+            sum = kSumErr;
+        }
+    }
+    return sum;
+}
+__device__ __forceinline__ uint32_t stage_list(const DCol &key, const DCol &other, bool same, uint64_t base,
+                                               uint32_t n, uint64_t *L, uint8_t *F) {
+    if (n == 0) return 0u;
+    uint32_t sum = 0;
+    if (key.data == nullptr) {  // void: seqbase + i, 32-bit oids, no error bit
+        for (uint32_t k = threadIdx.x; k < n; k += kMmThreads) {
+            L[k] = key.raw(base + k);
+            F[k] = 0;
+        }
+    } else {
+        switch (key.width) {
+            case 1: sum = stage_keys_t(key, other, same, static_cast<const uint8_t *>(key.data), base, n, L, F); break;
+            case 2: sum = stage_keys_t(key, other, same, static_cast<const uint16_t *>(key.data), base, n, L, F); break;
+            case 4: sum = stage_keys_t(key, other, same, static_cast<const uint32_t *>(key.data), base, n, L, F); break;
+            default: sum = stage_keys_t(key, other, same, static_cast<const uint64_t *>(key.data), base, n, L, F); break;
+        }
+    }
+    if (same || !other.encoded) return sum;  // a void or plain side has no error bit
+    switch (other.width) {
+        case 1: sum |= stage_other_t(other, static_cast<const uint8_t *>(other.data), base, n, F); break;
+        case 2: sum |= stage_other_t(other, static_cast<const uint16_t *>(other.data), base, n, F); break;
+        case 4: sum |= stage_other_t(other, static_cast<const uint32_t *>(other.data), base, n, F); break;
+        default: sum |= stage_other_t(other, static_cast<const uint64_t *>(other.data), base, n, F); break;
+    }
+    return sum;
+}
+__device__ __forceinline__ uint32_t stage_sub(const MergeArgs &a, const SubTile &T) {
+    return stage_list(a.t1, a.h1, a.h1_is_t1 != 0u, T.I0, T.n1t, T.L1, T.F1) |
+           stage_list(a.h2, a.t2, a.t2_is_h2 != 0u, T.J0, T.W, T.L2, T.F2);
+}
+
+// Number of keys among L[0, n) below v (STRICT: at most v), for keys in ascending order; n <= cap.
+// The step count depends on cap only, so lanes of a wave with different n stay in step.
+template <bool STRICT>
+__device__ __forceinline__ uint32_t lds_rank(const uint64_t *L, uint32_t n, uint32_t cap, uint64_t v) {
+    uint32_t pos = 0;
+    for (uint32_t step = cap ? (1u << (31 - __clz(cap))) : 0u; step != 0u; step >>= 1) {
+        const uint32_t k = pos + step - 1;
+        const uint64_t x = L[k < n ? k : 0u];
+        if (k < n && (STRICT ? x <= v : x < v)) pos += step;
+    }
+    return pos;
+}
+
+// Entry state at list-1 row I0 + li (1 <= li < n1t) from the staged keys; the columns are read only
+// when a run of equal keys reaches past the window.
+__device__ __forceinline__ uint64_t lds_state(const MergeArgs &a, const SubTile &T, uint32_t li) {
+    const uint64_t v = T.L1[li - 1];
+    uint64_t lb1 = T.I0 + li - 1;
+    if (li < 2 || !(T.L1[li - 2] < v)) {  // v repeats: the run's first row
+        const uint32_t k = lds_rank<false>(T.L1, li - 1, T.n1t, v);
+        lb1 = k == 0 && T.I0 > 0 ? gallop_lb(a.t1, T.I0, v) : T.I0 + k;
+    }
+    const uint32_t lo = lds_rank<false>(T.L2, T.W, T.W, v);
+    const bool past = T.J0 + T.W < a.n2;  // list 2 continues beyond the window
+    uint64_t lb2 = T.J0 + lo, ub2 = lb2;
+    const uint64_t at = lo < T.W ? T.L2[lo] : 0ull;
+    const uint64_t nx = lo + 1 < T.W ? T.L2[lo + 1] : 0ull;
+    if (lo == T.W) {
+        if (past) {
+            lb2 = first_ge(a.h2, T.J0 + T.W, a.n2, v, false);
+            ub2 = gallop_ub(a.h2, lb2, a.n2, v);
+        }
+    } else if (at == v) {
+        if (lo + 1 < T.W && nx > v) {
+            ub2 = lb2 + 1;
+        } else {
+            const uint32_t k = lds_rank<true>(T.L2, T.W, T.W, v);
+            ub2 = k == T.W && past ? first_ge(a.h2, T.J0 + T.W, a.n2, v, true) : T.J0 + k;
+        }
+    }
+    return entry_state(T.I0 + li, lb1, lb2, ub2, a.n2);
+}
+
+// Rows per thread and threads with rows, for a sub-tile of n1t list-1 rows.
+__device__ __forceinline__ void thread_rows(uint32_t n1t, uint32_t &q, uint32_t &active) {
+    q = (n1t + kMmThreads - 1) / kMmThreads;
+    active = q ? (n1t + q - 1) / q : 0u;
+}
+
+// Entry state of the calling thread's rows (st0) and of the next thread's (st1; the last thread
+// with rows: the next sub-tile's).  Contains a barrier.
+__device__ __forceinline__ void thread_states(const MergeArgs &a, const SubTile &T, uint64_t sub, uint32_t q,
+                                              uint32_t active, uint64_t *s_bound, uint64_t &st0, uint64_t &st1) {
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    st0 = 0;
+    if (t == 0) st0 = a.tstate[sub];
+    else if (t < active) st0 = lds_state(a, T, t * q);
+    st1 = __shfl_down(st0, 1, 64);
+    if (lane == 0) s_bound[w] = st0;
+    __syncthreads();
+    if (lane == 63 && w + 1 < static_cast<uint32_t>(kMmWaves)) st1 = s_bound[w + 1];
+    if (t + 1 == active) st1 = a.tstate[sub + 1];
+}
+
+__device__ __forceinline__ uint64_t enc_key(const DCol &c, uint64_t a_code, uint64_t v) {
+    return c.encoded ? (v * a_code) & c.mask : v;
+}
+
+constexpr int kWalkOk = 0, kWalkOutside = 1, kWalkUnsorted = 2;
+
+// The walk over staged rows, one step (an advance of i, of j, or of both) per iteration; the same
+// state machine as walk(), with SKIP2 as a phase of its own.  Every lane runs the same step code:
+// the comparison picks the advances and the next phase, error bits (rare) take the one branch, and
+// the loop has a single exit.  Returns kWalkOutside if the walk needs a list-2 row outside the
+// window (the caller redoes the piece from the columns, rewriting what this one wrote), and
+// kWalkUnsorted if a list-1 key is smaller than the one before it.  With REC, every match also
+// records its local list-2 row in mj[local list-1 row] (mj has a spare last slot that takes the
+// writes of the other steps) and sets the bit of its list-1 row (from the piece's first row) in
+// mbits.
+template <bool WRITE, bool REC = false>
+__device__ __forceinline__ int walk_lds(const MergeArgs &a, const SubTile &T, uint64_t i, uint64_t iend, uint64_t st,
+                                        uint32_t (&cnt)[kMmStreams], const uint64_t (&base)[kMmStreams], uint64_t &end,
+                                        uint16_t *mj = nullptr, uint32_t *mbits = nullptr) {
+    const uint64_t ph0 = st >> 62, j0 = st & kJMask;
+    if (ph0 == kPhEnd || (ph0 != kPhSkip1 && j0 >= a.n2)) {
+        end = st_pack(0, kPhEnd);
+        return kWalkOk;
+    }
+    if (j0 < T.J0 || j0 - T.J0 >= T.W) return kWalkOutside;
+    uint32_t c[kMmStreams];
+#pragma unroll
+    for (int s = 0; s < kMmStreams; ++s) c[s] = cnt[s];
+    auto err = [&](int e, uint64_t pos) {
+        if constexpr (WRITE) {
+            const uint64_t p = base[1 + e] + c[1 + e];
+            if (p < a.err_cap[e]) a.err[e][p] = pos * a.a_oid;
+        }
+        ++c[1 + e];
+    };
+    const uint64_t lim2 = a.n2 - T.J0;  // local index of the end of list 2
+    uint32_t lj = static_cast<uint32_t>(j0 - T.J0);
+    uint32_t li = static_cast<uint32_t>(i - T.I0);
+    const uint32_t li0 = li;
+    const uint32_t lend = static_cast<uint32_t>(iend - T.I0);
+    uint32_t ph = static_cast<uint32_t>(ph0);
+    uint32_t bits = 0;
+    uint64_t v1 = T.L1[li];
+    bool fin, uns, endn;
+    for (;;) {
+        const uint64_t v2 = T.L2[lj];
+        const uint32_t f1 = T.F1[li], f2 = T.F2[lj];
+        const bool lt = v1 < v2, gt = v2 < v1, eq = !lt && !gt;
+        const bool topchk = ph == kPhTop || (ph == kPhSkip2 && lt);
+        if (((topchk || gt) && (f2 & 3u)) || (lt && (f1 & 3u))) {  // error positions, in walk order
+            const uint64_t jg = T.J0 + lj, ig = T.I0 + li;
+            if (topchk) {
+                if (f2 & 1u) err(2, jg);
+                if (f2 & 2u) err(3, jg);
+            }
+            if (lt) {
+                if (f1 & 2u) err(0, ig);
+                if (f1 & 1u) err(1, ig);
+            } else if (gt) {
+                if (f2 & 1u) err(2, jg);
+                if (f2 & 2u) err(3, jg);
+            }
+        }
+        if constexpr (WRITE) {
+            if (eq) {
+                const uint64_t p = base[0] + c[0];
+                if (p < a.out_cap) {
+                    const uint64_t h = a.h1_from_t1 ? enc_key(a.t1, a.a_t1, v1) : a.h1.raw(T.I0 + li);
+                    const uint64_t t = a.t2_from_h2 ? enc_key(a.h2, a.a_h2, v2) : a.t2.raw(T.J0 + lj);
+                    store_w(a.out_h, p, a.wout_h, a.reenc_h ? h * a.f_h1 : h);
+                    store_w(a.out_t, p, a.wout_t, a.reenc_t ? t * a.f_t2 : t);
+                }
+            }
+        }
+        if constexpr (REC) {
+            mj[eq ? li : kMmDiag] = static_cast<uint16_t>(lj);
+            bits |= (eq ? 1u : 0u) << (li - li0);
+        }
+        c[0] += eq ? 1u : 0u;
+        li += gt ? 0u : 1u;
+        lj += lt ? 0u : 1u;
+        ph = eq ? static_cast<uint32_t>(kPhTop) : (lt ? static_cast<uint32_t>(kPhSkip1) : static_cast<uint32_t>(kPhSkip2));
+        fin = li == lend;
+        const uint64_t nv = T.L1[fin ? li - 1 : li];
+        uns = !fin && !gt && nv < v1;
+        v1 = gt ? v1 : nv;
+        endn = !fin && ph != kPhSkip1 && lj >= lim2;
+        if (fin || uns || endn || lj >= T.W) break;
+    }
+    if (uns) return kWalkUnsorted;
+    if (!fin && !endn) return kWalkOutside;
+    end = fin ? (iend == a.n1 ? st_pack(0, kPhEnd) : st_pack(T.J0 + lj, ph)) : st_pack(0, kPhEnd);
+#pragma unroll
+    for (int s = 0; s < kMmStreams; ++s) cnt[s] = c[s];
+    if constexpr (REC) *mbits = bits;
+    return kWalkOk;
+}
+
+// Hands a per-lane integer over in a vector register as it is: the compiler can no longer see that
+// it holds 0 or 1, so it does not turn it back into a lane mask (which costs scalar instructions
+// in every iteration of a loop it is carried through).
+__device__ __forceinline__ uint32_t vreg(uint32_t x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+// walk_lds<false, true> for a sub-tile whose staged rows carry no error bit (the common case): no
+// error can be recorded, so the step needs neither the error bits nor the phase, and keys of K bits
+// (the staged keys all fit) are compared.  Same results and return codes.
+template <typename K>
+__device__ __forceinline__ int walk_lean(const MergeArgs &a, const SubTile &T, uint64_t i, uint64_t iend, uint64_t st,
+                                         uint32_t &matches, uint64_t &end, uint16_t *mj, uint32_t &mbits) {
+    const uint64_t ph0 = st >> 62, j0 = st & kJMask;
+    if (ph0 == kPhEnd || (ph0 != kPhSkip1 && j0 >= a.n2)) {
+        end = st_pack(0, kPhEnd);
+        return kWalkOk;
+    }
+    if (j0 < T.J0 || j0 - T.J0 >= T.W) return kWalkOutside;
+    constexpr uint32_t S = sizeof(uint64_t) / sizeof(K);  // K words per staged key (low word first)
+    const K *L1 = reinterpret_cast<const K *>(T.L1);
+    const K *L2 = reinterpret_cast<const K *>(T.L2);
+    const uint64_t rest2 = a.n2 - T.J0;
+    const uint32_t lim2 = static_cast<uint32_t>(rest2 < 0xFFFFFFFFull ? rest2 : 0xFFFFFFFFull);
+    const uint32_t W = T.W;
+    uint32_t lj = static_cast<uint32_t>(j0 - T.J0);
+    uint32_t li = static_cast<uint32_t>(i - T.I0);
+    const uint32_t li0 = li;
+    const uint32_t lend = static_cast<uint32_t>(iend - T.I0);
+    uint32_t m = 0, bits = 0;
+    K v1 = L1[li * S];
+    // The loop state lives in integers: a per-lane flag that stays a lane mask across the loop
+    // costs scalar instructions every iteration.
+    uint32_t code, ltv;
+    do {
+        const K v2 = L2[lj * S];
+        const uint32_t lt = v1 < v2 ? 1u : 0u;
+        const uint32_t gt = v2 < v1 ? 1u : 0u;
+        const uint32_t eq = 1u - lt - gt;
+        mj[eq ? li : kMmDiag] = static_cast<uint16_t>(lj);
+        bits |= eq << (li - li0);
+        m += eq;
+        li += 1u - gt;
+        lj += 1u - lt;
+        const uint32_t fin = li == lend ? 1u : 0u;
+        const K nv = L1[(li - fin) * S];
+        const uint32_t uns = (fin | gt) == 0u && nv < v1 ? 1u : 0u;
+        v1 = gt ? v1 : nv;
+        const uint32_t endn = (fin | lt) == 0u && lj >= lim2 ? 1u : 0u;
+        code = vreg(fin | (uns << 1) | (endn << 2) | (lj >= W ? 8u : 0u));
+        ltv = vreg(lt);
+    } while (code == 0u);
+    if (code & 2u) return kWalkUnsorted;
+    if ((code & 5u) == 0u) return kWalkOutside;
+    end = (code & 1u) ? (iend == a.n1 ? st_pack(0, kPhEnd) : st_pack(T.J0 + lj, ltv ? kPhSkip1 : kPhTop))
+                      : st_pack(0, kPhEnd);
+    matches += m;
+    mbits = bits;
+    return kWalkOk;
+}
+
+// One thread's rows of a sub-tile, from the staged rows or (outside the window, or unsorted) the
+// columns, in order-checked form.
+template <bool WRITE>
+__device__ __forceinline__ uint64_t walk_piece(const MergeArgs &a, const SubTile &T, uint64_t i, uint64_t iend,
+                                               uint64_t st, uint32_t (&cnt)[kMmStreams],
+                                               const uint64_t (&base)[kMmStreams]) {
+    uint64_t end;
+    if (walk_lds<WRITE>(a, T, i, iend, st, cnt, base, end) == kWalkOk) return end;
+    return walk<WRITE, GlobalSrc, true>(a, GlobalSrc{a}, i, iend, st, cnt, base);
+}
+
+__device__ __forceinline__ uint64_t slot_of(uint64_t I0, uint64_t st) {
+    const uint64_t j = st & kJMask;
+    return I0 < j ? I0 : j;
+}
+
+// One block per sub-tile: stage, derive the entry states, walk (checking the order of the list-1
+// keys on the way), count, and write the pairs to the sub-tile's provisional slot.
+__global__ __launch_bounds__(kMmThreads) void k_mm_count(MergeArgs a) {
+    __shared__ __align__(16) uint8_t s_lds[kMmLdsBytes];
+    __shared__ uint16_t s_mj[kMmDiag + 1];  // local list-2 row matched by each local list-1 row (+ a spare)
+    __shared__ uint16_t s_os[kMmDiag];  // local list-1 row of each output pair
+    __shared__ uint64_t s_bound[kMmWaves];
+    __shared__ uint32_t s_red[kMmWaves];
+    const uint64_t sub = blockIdx.x;
+    const uint32_t t = threadIdx.x;
+    SubTile T;
+    sub_view(a, sub, s_lds, T);
+    uint32_t q, active;
+    thread_rows(T.n1t, q, active);
+    const uint32_t sum = stage_sub(a, T);
+    const bool dirty = __syncthreads_or((sum & kSumErr) != 0u) != 0;
+    const bool wide = __syncthreads_or((sum & kSumWide) != 0u) != 0;
+    uint64_t st0, st1;
+    thread_states(a, T, sub, q, active, s_bound, st0, st1);
+    bool bad = t == 0 && (!T.ok || (active == 0 && a.tstate[sub] != a.tstate[sub + 1]));
+    const uint64_t zero[kMmStreams] = {0, 0, 0, 0, 0};
     uint32_t cnt[kMmStreams] = {0, 0, 0, 0, 0};
-    if (c < a.num_chunks) {
-        const uint64_t zero[kMmStreams] = {0, 0, 0, 0, 0};
-        const uint64_t i = c * kMmRows;
-        const uint64_t iend = i + kMmRows < a.n1 ? i + kMmRows : a.n1;
-        const uint64_t end = walk<false>(a, i, iend, a.start[c], cnt, zero);
-        if (c + 1 < a.num_chunks && end != a.start[c + 1]) atomicOr(&a.flags[0], 1u);
+    uint32_t mbits = 0;
+    bool over = false;
+    if (t < active) {
+        const uint64_t i = T.I0 + static_cast<uint64_t>(t) * q;
+        const uint64_t iend = i + q < T.I1 ? i + q : T.I1;
+        const uint64_t prev = t > 0 ? T.L1[t * q - 1] : (T.I0 > 0 ? dec_at(a.t1, T.I0 - 1) : 0ull);
+        uint64_t end = kAbort;
+        int r;
+        if (T.L1[t * q] < prev) r = kWalkUnsorted;
+        else if (dirty) r = walk_lds<false, true>(a, T, i, iend, st0, cnt, zero, end, s_mj, &mbits);
+        else if (wide) r = walk_lean<uint64_t>(a, T, i, iend, st0, cnt[0], end, s_mj, mbits);
+        else r = walk_lean<uint32_t>(a, T, i, iend, st0, cnt[0], end, s_mj, mbits);
+        if (r == kWalkOutside) {
+            cnt[0] = 0;
+            over = true;
+            end = walk<false, GlobalSrc, true>(a, GlobalSrc{a}, i, iend, st0, cnt, zero);
+        }
+        if (r == kWalkUnsorted || end != st1) bad = true;
     }
-    uint32_t ex[kMmStreams], agg[kMmStreams];
+    if (bad) atomicOr(&a.flags[0], 1u);
+    uint32_t tot[kMmStreams] = {0, 0, 0, 0, 0};
+    const uint32_t ex0 = block_excl_scan<uint32_t, kMmWaves>(cnt[0], tot[0], s_red);
+    const bool slow = __syncthreads_or((cnt[1] | cnt[2] | cnt[3] | cnt[4]) != 0u || over) != 0;
+    if (slow) {
 #pragma unroll
-    for (int s = 0; s < kMmStreams; ++s) {
-        ex[s] = block_excl_scan<uint32_t>(cnt[s], agg[s], s_red);
-        __syncthreads();
+        for (int s = 1; s < kMmStreams; ++s) tot[s] = block_sum<uint32_t, kMmWaves>(cnt[s], s_red);
     }
-    lb_resolve<kMmStreams>(lb, tile, agg, s_excl);
-    if (c < a.num_chunks) {
+    if (t == 0) {
 #pragma unroll
-        for (int s = 0; s < kMmStreams; ++s) a.offs[static_cast<uint64_t>(s) * a.num_chunks + c] = s_excl[s] + ex[s];
+        for (int s = 0; s < kMmStreams; ++s) {
+            a.sub_cnt[static_cast<uint64_t>(s) * a.num_sub + sub] = tot[s];
+            if (tot[s]) atomicAdd(reinterpret_cast<unsigned long long *>(&a.grp[s * a.num_grp + sub / kMmGroup]),
+                                  static_cast<unsigned long long>(tot[s]));
+        }
+        a.sub_slow[sub] = slow ? 1u : 0u;
     }
-    if (tile == lb.num_tiles - 1 && threadIdx.x == 0) {
-#pragma unroll
-        for (int s = 0; s < kMmStreams; ++s) a.d_counts[s] = static_cast<uint64_t>(s_excl[s]) + agg[s];
+    if (slow || tot[0] == 0) return;
+    // pairs in walk order: the matched list-1 rows, ascending
+    for (uint32_t k = ex0, m = mbits; m != 0u; m &= m - 1u, ++k)
+        s_os[k] = static_cast<uint16_t>(t * q + __ffs(static_cast<int>(m)) - 1);
+    __syncthreads();
+    const uint64_t slot = slot_of(T.I0, a.tstate[sub]);
+    if (a.h1_from_t1 && a.t2_from_h2 && a.wout_h == 8 && a.wout_t == 8 && !a.reenc_h && !a.reenc_t) {
+        // both sides of both lists are their key columns: the pair is the matched key, encoded
+        uint64_t *ph = static_cast<uint64_t *>(a.prov_h), *pt = static_cast<uint64_t *>(a.prov_t);
+        for (uint32_t p = t; p < tot[0]; p += kMmThreads) {
+            const uint64_t pos = slot + p;
+            if (pos >= a.prov_cap) break;
+            const uint64_t v = T.L1[s_os[p]];
+            ph[pos] = enc_key(a.t1, a.a_t1, v);
+            pt[pos] = enc_key(a.h2, a.a_h2, v);
+        }
+        return;
+    }
+    for (uint32_t p = t; p < tot[0]; p += kMmThreads) {
+        const uint64_t pos = slot + p;
+        if (pos >= a.prov_cap) break;
+        const uint32_t li = s_os[p], lj = s_mj[li];
+        const uint64_t h = a.h1_from_t1 ? enc_key(a.t1, a.a_t1, T.L1[li]) : a.h1.raw(T.I0 + li);
+        const uint64_t v = a.t2_from_h2 ? enc_key(a.h2, a.a_h2, T.L2[lj]) : a.t2.raw(T.J0 + lj);
+        store_w(a.prov_h, pos, a.wout_h, a.reenc_h ? h * a.f_h1 : h);
+        store_w(a.prov_t, pos, a.wout_t, a.reenc_t ? v * a.f_t2 : v);
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_mm_write(MergeArgs a) {
-    const uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= a.num_chunks || a.flags[0] != 0u) return;
-    uint64_t base[kMmStreams];
+// Exclusive prefixes of the group sums, per stream (one block), and the totals.
+__global__ __launch_bounds__(kBlock) void k_mm_gscan(MergeArgs a) {
+    __shared__ uint64_t s_red[kWaves];
+    for (int s = 0; s < kMmStreams; ++s) {
+        uint64_t *g = a.grp + s * a.num_grp;
+        uint64_t carry = 0;
+        for (uint64_t r = 0; r < a.num_grp; r += kBlock) {
+            const uint64_t k = r + threadIdx.x;
+            const uint64_t v = k < a.num_grp ? g[k] : 0ull;
+            uint64_t tot;
+            const uint64_t ex = block_excl_scan<uint64_t>(v, tot, s_red);
+            if (k < a.num_grp) g[k] = carry + ex;
+            carry += tot;
+        }
+        if (threadIdx.x == 0 && a.flags[0] == 0u) a.d_counts[s] = carry;
+    }
+}
+
+// Copies m provisional pairs from slot src to the final offset dst (one wave, 4 pairs per lane in
+// flight).
+template <typename TH, typename TT>
+__device__ __forceinline__ void copy_pairs(const MergeArgs &a, uint64_t src, uint64_t dst, uint32_t m, uint32_t lane) {
+    const TH *ph = static_cast<const TH *>(a.prov_h);
+    const TT *pt = static_cast<const TT *>(a.prov_t);
+    TH *oh = static_cast<TH *>(a.out_h);
+    TT *ot = static_cast<TT *>(a.out_t);
+    for (uint32_t p0 = 0; p0 < m; p0 += 256) {
+        TH h[4];
+        TT v[4];
 #pragma unroll
-    for (int s = 0; s < kMmStreams; ++s) base[s] = a.offs[static_cast<uint64_t>(s) * a.num_chunks + c];
-    uint32_t cnt[kMmStreams] = {0, 0, 0, 0, 0};
-    const uint64_t i = c * kMmRows;
-    const uint64_t iend = i + kMmRows < a.n1 ? i + kMmRows : a.n1;
-    (void) walk<true>(a, i, iend, a.start[c], cnt, base);
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t p = p0 + u * 64 + lane;
+            const bool ok = p < m && src + p < a.prov_cap;
+            h[u] = ph[ok ? src + p : 0];
+            v[u] = pt[ok ? src + p : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t p = p0 + u * 64 + lane;
+            if (p < m && src + p < a.prov_cap && dst + p < a.out_cap) {
+                oh[dst + p] = h[u];
+                ot[dst + p] = v[u];
+            }
+        }
+    }
+}
+__device__ __forceinline__ void copy_pairs_w(const MergeArgs &a, uint64_t src, uint64_t dst, uint32_t m, uint32_t lane) {
+    if (a.wout_h == 8 && a.wout_t == 8) {
+        copy_pairs<uint64_t, uint64_t>(a, src, dst, m, lane);
+        return;
+    }
+    for (uint32_t p = lane; p < m; p += 64) {
+        if (src + p >= a.prov_cap || dst + p >= a.out_cap) break;
+        store_w(a.out_h, dst + p, a.wout_h, load_w(a.prov_h, src + p, a.wout_h));
+        store_w(a.out_t, dst + p, a.wout_t, load_w(a.prov_t, src + p, a.wout_t));
+    }
+}
+
+// One block per kMmMoveSubs sub-tiles: final offsets from the group prefixes, then each wave copies
+// whole provisional slots; sub-tiles marked slow are staged and walked again by the whole block,
+// writing pairs and error positions at their final offsets.
+__global__ __launch_bounds__(kMmThreads) void k_mm_move(MergeArgs a) {
+    __shared__ __align__(16) uint8_t s_lds[kMmLdsBytes];
+    __shared__ uint64_t s_off[kMmStreams][kMmMoveSubs];
+    __shared__ uint32_t s_cnt0[kMmMoveSubs], s_slow[kMmMoveSubs];
+    __shared__ uint64_t s_red[kMmWaves];
+    __shared__ uint32_t s_red32[kMmWaves];
+    __shared__ uint64_t s_bound[kMmWaves];
+    if (a.flags[0] != 0u) return;  // the one-thread walk writes everything
+    const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+    const uint64_t s0 = static_cast<uint64_t>(blockIdx.x) * kMmMoveSubs;
+    const uint32_t nsub = static_cast<uint32_t>(a.num_sub - s0 < kMmMoveSubs ? a.num_sub - s0 : kMmMoveSubs);
+    const uint64_t g = s0 / kMmGroup, gstart = g * kMmGroup;
+    for (int s = 0; s < kMmStreams; ++s) {
+        const uint32_t *cs = a.sub_cnt + static_cast<uint64_t>(s) * a.num_sub;
+        const uint64_t v = gstart + t < s0 ? cs[gstart + t] : 0ull;
+        const uint64_t pre = a.grp[s * a.num_grp + g] + block_sum<uint64_t, kMmWaves>(v, s_red);
+        if (t < 64) {
+            const uint64_t c = t < nsub ? cs[s0 + t] : 0ull;
+            const uint64_t inc = wave_incl_scan(c);
+            if (t < nsub) s_off[s][t] = pre + inc - c;
+        }
+    }
+    if (t < nsub) {
+        s_cnt0[t] = a.sub_cnt[s0 + t];
+        s_slow[t] = a.sub_slow[s0 + t];
+    }
+    __syncthreads();
+    for (uint32_t k = wv; k < nsub; k += kMmWaves) {
+        if (s_slow[k] || s_cnt0[k] == 0u) continue;
+        copy_pairs_w(a, slot_of(a.ti[s0 + k], a.tstate[s0 + k]), s_off[0][k], s_cnt0[k], lane);
+    }
+    for (uint32_t k = 0; k < nsub; ++k) {
+        if (!s_slow[k]) continue;
+        const uint64_t sub = s0 + k;
+        __syncthreads();  // the previous slow sub-tile's walks are done with the staged rows
+        SubTile T;
+        sub_view(a, sub, s_lds, T);
+        uint32_t q, active;
+        thread_rows(T.n1t, q, active);
+        stage_sub(a, T);
+        __syncthreads();
+        uint64_t st0, st1;
+        thread_states(a, T, sub, q, active, s_bound, st0, st1);
+        const uint64_t i = T.I0 + static_cast<uint64_t>(t) * q;
+        const uint64_t iend = i + q < T.I1 ? i + q : T.I1;
+        const uint64_t zero[kMmStreams] = {0, 0, 0, 0, 0};
+        uint32_t cnt[kMmStreams] = {0, 0, 0, 0, 0};
+        if (t < active) (void) walk_piece<false>(a, T, i, iend, st0, cnt, zero);
+        uint64_t base[kMmStreams];
+#pragma unroll
+        for (int s = 0; s < kMmStreams; ++s) {
+            uint32_t tot_s;
+            base[s] = s_off[s][k] + block_excl_scan<uint32_t, kMmWaves>(cnt[s], tot_s, s_red32);
+        }
+        if (t < active) {
+            uint32_t c2[kMmStreams] = {0, 0, 0, 0, 0};
+            (void) walk_piece<true>(a, T, i, iend, st0, c2, base);
+        }
+    }
 }
 
 // The whole walk on one thread, for inputs whose chunk entry states could not be derived.
@@ -196,7 +825,7 @@ __global__ void k_mm_fallback(MergeArgs a) {
     if (threadIdx.x != 0 || a.flags[0] == 0u) return;
     uint32_t cnt[kMmStreams] = {0, 0, 0, 0, 0};
     const uint64_t zero[kMmStreams] = {0, 0, 0, 0, 0};
-    if (a.n1 && a.n2) (void) walk<true>(a, 0, a.n1, st_pack(0, kPhTop), cnt, zero);
+    if (a.n1 && a.n2) (void) walk<true>(a, GlobalSrc{a}, 0, a.n1, st_pack(0, kPhTop), cnt, zero);
 #pragma unroll
     for (int s = 0; s < kMmStreams; ++s) a.d_counts[s] = cnt[s];
 }
@@ -232,7 +861,6 @@ extern "C" int ahead_hip_matchjoin_merge_an(const ahead_col *h1, const ahead_col
     a.t2 = dcol(*t2);
     a.n1 = t1->n;
     a.n2 = h2->n;
-    a.num_chunks = (a.n1 + kMmRows - 1) / kMmRows;
     a.f_h1 = f_h1;
     a.f_t2 = f_t2;
     a.reenc_h = reenc_h && h1->encoded && h1->data ? 1u : 0u;
@@ -250,22 +878,54 @@ extern "C" int ahead_hip_matchjoin_merge_an(const ahead_col *h1, const ahead_col
     }
     a.d_counts = d_counts;
 
-    const uint64_t K = a.num_chunks ? a.num_chunks : 1;
-    const size_t o_flags = 0, o_start = 256, o_offs = o_start + align_up(K * 8, 256);
-    const size_t o_end = o_offs + align_up(K * 4 * kMmStreams, 256);
-    const LbPlan plan = lb_plan(a.num_chunks, kBlock, kMmStreams);
+    // a list whose two sides are one column is staged once; its output codes are the keys
+    // re-encoded when A * A^-1 = 1 at the code width (always, for consistent metadata)
+    a.h1_is_t1 = h1->data != nullptr && h1->data == t1->data && a.h1.width == a.t1.width ? 1u : 0u;
+    a.t2_is_h2 = t2->data != nullptr && t2->data == h2->data && a.t2.width == a.h2.width ? 1u : 0u;
+    a.a_t1 = t1->a & a.t1.mask;
+    a.a_h2 = h2->a & a.h2.mask;
+    const bool inv1 = !a.t1.encoded || ((a.a_t1 * a.t1.a_inv) & a.t1.mask) == 1ull;
+    const bool inv2 = !a.h2.encoded || ((a.a_h2 * a.h2.a_inv) & a.h2.mask) == 1ull;
+    a.h1_from_t1 = a.h1_is_t1 && inv1 ? 1u : 0u;
+    a.t2_from_h2 = a.t2_is_h2 && inv2 ? 1u : 0u;
+    const uint64_t diag = kMmDiag;
+    a.cap1 = diag;
+    const uint64_t subs64 = (a.n1 + a.n2 + diag - 1) / diag;
+    const uint32_t subs = static_cast<uint32_t>(subs64 ? subs64 : 1);
+    a.num_sub = subs;
+    a.num_grp = (subs + kMmGroup - 1) / kMmGroup;
+    a.prov_cap = a.n1 < a.n2 ? a.n1 : a.n2;
+
+    const size_t arr = align_up((static_cast<size_t>(subs) + 1) * 8, 256);
+    const size_t o_flags = 0, o_i = 256, o_state = o_i + arr, o_j0 = o_state + arr, o_cnt = o_j0 + arr;
+    const size_t o_slow = o_cnt + align_up(static_cast<size_t>(subs) * 4 * kMmStreams, 256);
+    const size_t o_grp = o_slow + align_up(static_cast<size_t>(subs) * 4, 256);
+    const size_t o_ph = o_grp + align_up(a.num_grp * 8 * kMmStreams, 256);
+    const size_t o_pt = o_ph + align_up(a.prov_cap * wout_h, 256);
+    const size_t o_end = o_pt + align_up(a.prov_cap * wout_t, 256);
     if ((st = ws_ensure(ws, o_end)) != AHEAD_OK) return st;
     a.flags = reinterpret_cast<uint32_t *>(ws_region(ws, o_flags));
-    a.start = reinterpret_cast<uint64_t *>(ws_region(ws, o_start));
-    a.offs = reinterpret_cast<uint32_t *>(ws_region(ws, o_offs));
-    LbArgs lb{};
-    if ((st = lb_args(ws, plan, &lb)) != AHEAD_OK) return st;
+    SplitArgs sp{};
+    sp.diag = diag;
+    sp.tiles = subs;
+    sp.i = reinterpret_cast<uint64_t *>(ws_region(ws, o_i));
+    sp.state = reinterpret_cast<uint64_t *>(ws_region(ws, o_state));
+    sp.j0 = reinterpret_cast<uint64_t *>(ws_region(ws, o_j0));
+    a.ti = sp.i;
+    a.tstate = sp.state;
+    a.tj0 = sp.j0;
+    a.sub_cnt = reinterpret_cast<uint32_t *>(ws_region(ws, o_cnt));
+    a.sub_slow = reinterpret_cast<uint32_t *>(ws_region(ws, o_slow));
+    a.grp = reinterpret_cast<uint64_t *>(ws_region(ws, o_grp));
+    a.prov_h = ws_region(ws, o_ph);
+    a.prov_t = ws_region(ws, o_pt);
 
     if ((st = fill_u32(a.flags, 1, 0u, s)) != AHEAD_OK) return st;
-    const uint32_t grid = static_cast<uint32_t>((K + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_mm_start, dim3(grid), dim3(kBlock), 0, s, a);
-    hipLaunchKernelGGL(k_mm_count, dim3(plan.num_tiles), dim3(kBlock), 0, s, a, lb);
-    hipLaunchKernelGGL(k_mm_write, dim3(grid), dim3(kBlock), 0, s, a);
+    if ((st = fill_u64(a.grp, a.num_grp * kMmStreams, 0ull, s)) != AHEAD_OK) return st;
+    hipLaunchKernelGGL(k_mm_split, dim3((subs + kBlock) / kBlock), dim3(kBlock), 0, s, a, sp);
+    hipLaunchKernelGGL(k_mm_count, dim3(subs), dim3(kMmThreads), 0, s, a);
+    hipLaunchKernelGGL(k_mm_gscan, dim3(1), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_mm_move, dim3((subs + kMmMoveSubs - 1) / kMmMoveSubs), dim3(kMmThreads), 0, s, a);
     hipLaunchKernelGGL(k_mm_fallback, dim3(1), dim3(64), 0, s, a);
     AHEAD_HIP_TRY(hipGetLastError());
     return AHEAD_OK;

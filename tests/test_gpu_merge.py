@@ -1,6 +1,8 @@
 """GPU parity of the merge form of matchjoinAN (matchjoinAN.tcc:46-143) against the oracle,
-bit-exact, error vectors included: sorted inputs (the chunked path), sorted inputs with long runs
-of equal keys, and unsorted or corrupted inputs (the single-thread path)."""
+bit-exact, error vectors included: sorted inputs (the merge-path tiles), sorted inputs with long
+runs of equal keys, and unsorted or corrupted inputs (the single-thread path).  A list whose head
+and tail are one device column (a mirrored selection, as the star-schema queries pass them) is
+staged once; the tests run both that and separate columns."""
 import numpy as np
 import pytest
 
@@ -25,24 +27,29 @@ def oid_list(keys, seed, stream, flip_rate=0.0):
     return codes
 
 
-def run_both(h1, t1, h2, t2, m_h1, m_t1, m_h2, m_t2, *args):
+def run_both(h1, t1, h2, t2, m_h1, m_t1, m_h2, m_t2, *args, alias=True):
+    """alias: the same array passed for both sides of a list becomes one device column."""
     ref = O.matchjoin_merge_an(O.Col(h1, m_h1), O.Col(t1, m_t1), O.Col(h2, m_h2), O.Col(t2, m_t2), *args)
-    got = capi.matchjoin_merge_an(capi.Column.from_numpy(h1, m_h1), capi.Column.from_numpy(t1, m_t1),
-                                  capi.Column.from_numpy(h2, m_h2), capi.Column.from_numpy(t2, m_t2), *args, err_cap=2)
+    c_t1, c_h2 = capi.Column.from_numpy(t1, m_t1), capi.Column.from_numpy(h2, m_h2)
+    c_h1 = c_t1 if alias and h1 is t1 and m_h1 == m_t1 else capi.Column.from_numpy(h1, m_h1)
+    c_t2 = c_h2 if alias and t2 is h2 and m_t2 == m_h2 else capi.Column.from_numpy(t2, m_t2)
+    got = capi.matchjoin_merge_an(c_h1, c_t1, c_h2, c_t2, *args, err_cap=2)
     same(got, ref)
     return ref
 
 
 @pytest.mark.parametrize("n1,n2,hi", [(0, 10, 50), (10, 0, 50), (1, 1, 0), (15, 40, 60), (16, 16, 20), (17, 100, 300),
                                       (4095, 5000, 20_000), (4097, 300, 20_000), (100_003, 80_001, 400_000),
-                                      (200_000, 1000, 10_000_000), (1000, 200_000, 300_000)])
-def test_merge_sorted_oid_lists(n1, n2, hi):
+                                      (200_000, 1000, 10_000_000), (1000, 200_000, 300_000),
+                                      (20_000, 140_000, 200_000), (1_000_000, 600_000, 4_000_000)])
+@pytest.mark.parametrize("alias", [True, False])
+def test_merge_sorted_oid_lists(n1, n2, hi, alias):
     # two selections' OID lists (strictly ascending), the Q1.x shape: BAT<resoid, resoid> both
     seed = n1 + 3 * n2
     k1 = np.unique(datagen.uniform(n1, seed, 1, 0, hi))
     k2 = np.unique(datagen.uniform(n2, seed, 2, 0, hi))
     t1, t2 = oid_list(k1, seed, 3), oid_list(k2, seed, 4)
-    ref = run_both(t1, t1, t2, t2, RESOID, RESOID, RESOID, RESOID)
+    ref = run_both(t1, t1, t2, t2, RESOID, RESOID, RESOID, RESOID, alias=alias)
     assert ref[0].size == np.intersect1d(k1, k2).size
 
 
@@ -74,6 +81,27 @@ def test_merge_unsorted_and_corrupted(n1, n2):
     datagen.inject_flips(c1, seed, 7, datagen.flip_threshold(2e-3))
     datagen.inject_flips(c2, seed, 8, datagen.flip_threshold(2e-3))
     run_both(h1, c1, c2, t2, RESOID, meta, meta, RESOID)
+    # a mirrored list with flipped bits: both sides read from the one staged column
+    run_both(c1, c1, c2, c2, meta, meta, meta, meta)
+
+
+def test_merge_fetched_tail_with_errors():
+    # the shape of a fetched attribute matched against a later selection: list 2 (OIDs with an
+    # attribute) several times longer than list 1 (mirrored OIDs).  The keys stay sorted (clean),
+    # so the tiles run; the errors are in the columns that are not keys.
+    n2 = 400_000
+    k2 = np.unique(datagen.uniform(n2, 17, 1, 0, 3 * n2))
+    k1 = np.unique(k2[datagen.uniform(k2.size // 7, 17, 2, 0, k2.size - 1).astype(np.int64)])
+    k1 = np.union1d(k1, np.unique(datagen.uniform(k1.size // 3, 17, 7, 0, 3 * n2)))  # some unmatched
+    t1, h2 = oid_list(k1, 17, 3), oid_list(k2, 17, 4)
+    meta = an.an_meta(an.RESSHORT, 233)
+    t2 = datagen.encode(datagen.uniform(k2.size, 17, 5, 0, 0xFFFF), 233, 4)
+    datagen.inject_flips(t2, 17, 6, datagen.flip_threshold(1e-3))
+    ref = run_both(t1, t1, h2, t2, RESOID, RESOID, RESOID, meta)
+    assert ref[5].size > 0
+    h1 = oid_list(np.arange(k1.size, dtype=np.uint64), 17, 8, 1e-3)
+    ref = run_both(h1, t1, h2, t2, RESOID, RESOID, RESOID, meta)
+    assert ref[2].size > 0 and ref[5].size > 0
 
 
 def test_merge_reencode_and_widths():
