@@ -41,7 +41,8 @@ struct HjTable {
     uint32_t *cur;    // [size + 1] fill cursors
     uint32_t *rows;   // [n_build] row lists
     uint32_t *flags;  // [0] several rows share a key, [1] table overflow, [2] list too long to order,
-                      // [3] smallest build key, [5] largest build key, [4] direct map in use
+                      // [3] smallest build key, [5] largest build key, [4] direct map in use,
+                      // [6] under a quarter of the probe rows find a key (estimated)
 };
 
 struct BuildArgs {
@@ -243,9 +244,9 @@ struct ProbeOp {
     uint32_t e_index[2];
     uint32_t map_launched;    // the streaming-engine probe (ProbeMapOp) was launched before this one
 
-    // the tile engine runs this probe only when some build key has several rows (or the streaming
-    // probe could not be launched)
-    __device__ bool skip() const { return map_launched != 0u && t.flags[0] == 0u; }
+    // the tile engine runs this probe when some build key has several rows, when most probe rows
+    // find no key, or when the streaming probe could not be launched
+    __device__ bool skip() const { return map_launched != 0u && t.flags[0] == 0u && t.flags[6] == 0u; }
 
     static constexpr bool kBatched = true;
     template <int G>
@@ -354,11 +355,13 @@ struct ProbeOp {
 };
 
 // The probe on the streaming engine: when no build key has several rows, a probe row emits at
-// most one pair.
+// most one pair.  The engine stages each wave's pairs in place and moves them once the offsets are
+// known; when most rows emit nothing the waves leave gaps too wide to close in place and every row
+// is evaluated twice, so a probe that mostly misses runs on the tile engine instead.
 struct ProbeMapOp : ProbeOp {
     static constexpr bool kMulti = false;
     static constexpr int kOuts = 2;
-    __device__ bool skip() const { return t.flags[0] != 0u; }
+    __device__ bool skip() const { return t.flags[0] != 0u || t.flags[6] != 0u; }
     __device__ void out_arrays(void *(&p)[2], uint32_t (&w)[2]) const {
         p[0] = out_h;
         p[1] = out_t;
@@ -366,6 +369,23 @@ struct ProbeMapOp : ProbeOp {
         w[1] = build_right ? w_b : w_p;
     }
 };
+
+// Share of probe rows that find a build key, from kHjSamples evenly spaced rows: flags[6] = 1 when
+// under a quarter do.  It only picks the engine; both give the same results.
+constexpr uint32_t kHjSamples = 4096;
+__global__ __launch_bounds__(kBlock) void k_hj_sample(ProbeOp p, uint64_t np) {
+    __shared__ uint32_t s_red[kWaves];
+    uint32_t hits = 0;
+    for (uint32_t k = threadIdx.x; k < kHjSamples; k += kBlock) {
+        const uint64_t row = np * k / kHjSamples;
+        ProbeOp::State st{};
+        uint32_t errs = 0;
+        uint64_t aux = 0;
+        if (row < np && p.eval(row, st, errs, aux) != 0u) ++hits;
+    }
+    const uint32_t tot = block_sum<uint32_t>(hits, s_red);
+    if (threadIdx.x == 0) p.t.flags[6] = np > 0 && tot * 4u < kHjSamples ? 1u : 0u;
+}
 
 __global__ void k_hj_status(const uint32_t *flags, uint64_t *d_counts) {
     d_counts[5] = flags[1] ? 1u : (flags[2] ? 2u : 0u);
@@ -506,6 +526,7 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     // the streaming engine needs room for a staged pair per probe row
     const bool map_ok = out_h.data && out_t.data && p.out_cap >= np;
     if (map_ok) {
+        hipLaunchKernelGGL(k_hj_sample, dim3(1), dim3(kBlock), 0, s, p, np);
         ProbeMapOp pm{};
         static_cast<ProbeOp &>(pm) = p;
         if ((st = launch_map(pm, ws, o_end, s)) != AHEAD_OK) return st;
