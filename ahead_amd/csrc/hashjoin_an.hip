@@ -42,7 +42,7 @@ struct HjTable {
     uint32_t *rows;   // [n_build] row lists
     uint32_t *flags;  // [0] several rows share a key, [1] table overflow, [2] list too long to order,
                       // [3] smallest build key, [5] largest build key, [4] direct map in use,
-                      // [6] under a quarter of the probe rows find a key (estimated)
+                      // [6] some sampled probe row finds no key
 };
 
 struct BuildArgs {
@@ -244,8 +244,8 @@ struct ProbeOp {
     uint32_t e_index[2];
     uint32_t map_launched;    // the streaming-engine probe (ProbeMapOp) was launched before this one
 
-    // the tile engine runs this probe when some build key has several rows, when most probe rows
-    // find no key, or when the streaming probe could not be launched
+    // the tile engine runs this probe when some build key has several rows, when some sampled probe
+    // row finds no key, or when the streaming probe could not be launched
     __device__ bool skip() const { return map_launched != 0u && t.flags[0] == 0u && t.flags[6] == 0u; }
 
     static constexpr bool kBatched = true;
@@ -287,14 +287,14 @@ struct ProbeOp {
             }
             return;
         }
+        uint32_t sl[G];
+        t.ht.find_n<G>(k, ok, sl);
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            if (!ok[g]) continue;
-            const uint32_t sl = t.ht.find(k[g]);
-            if (sl == UINT32_MAX) continue;
-            st[g].slot = sl;
-            cnt[g] = t.cnt[sl];
-            if (cnt[g] == 1u) st[g].brow = t.first[sl];
+            if (sl[g] == UINT32_MAX) continue;
+            st[g].slot = sl[g];
+            cnt[g] = t.cnt[sl[g]];
+            if (cnt[g] == 1u) st[g].brow = t.first[sl[g]];
         }
     }
 
@@ -356,8 +356,8 @@ struct ProbeOp {
 
 // The probe on the streaming engine: when no build key has several rows, a probe row emits at
 // most one pair.  The engine stages each wave's pairs in place and moves them once the offsets are
-// known; when most rows emit nothing the waves leave gaps too wide to close in place and every row
-// is evaluated twice, so a probe that mostly misses runs on the tile engine instead.
+// known; when rows emit nothing the waves leave gaps that, beyond kMaxShift rows in all, cannot be
+// closed in place and every row is evaluated twice, so a probe that misses runs on the tile engine.
 struct ProbeMapOp : ProbeOp {
     static constexpr bool kMulti = false;
     static constexpr int kOuts = 2;
@@ -370,8 +370,10 @@ struct ProbeMapOp : ProbeOp {
     }
 };
 
-// Share of probe rows that find a build key, from kHjSamples evenly spaced rows: flags[6] = 1 when
-// under a quarter do.  It only picks the engine; both give the same results.
+// Whether every one of kHjSamples evenly spaced probe rows finds a build key: flags[6] = 1 when
+// some do not.  The streaming engine closes the gaps of dropped rows in place only while they total
+// at most kMaxShift rows, so it suits probes where (nearly) every row finds its key, as foreign
+// keys into a whole dimension do.  It only picks the engine; both give the same results.
 constexpr uint32_t kHjSamples = 4096;
 __global__ __launch_bounds__(kBlock) void k_hj_sample(ProbeOp p, uint64_t np) {
     __shared__ uint32_t s_red[kWaves];
@@ -384,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_hj_sample(ProbeOp p, uint64_t np) {
         if (row < np && p.eval(row, st, errs, aux) != 0u) ++hits;
     }
     const uint32_t tot = block_sum<uint32_t>(hits, s_red);
-    if (threadIdx.x == 0) p.t.flags[6] = np > 0 && tot * 4u < kHjSamples ? 1u : 0u;
+    if (threadIdx.x == 0) p.t.flags[6] = np > 0 && tot < kHjSamples ? 1u : 0u;
 }
 
 __global__ void k_hj_status(const uint32_t *flags, uint64_t *d_counts) {

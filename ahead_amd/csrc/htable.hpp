@@ -53,6 +53,46 @@ struct HTable {
         }
         return UINT32_MAX;
     }
+
+    // find() for G keys at once (ok[g] false: no lookup, UINT32_MAX).  Each round loads the current
+    // slot of every key still open in one batch, so a wave waits once per round, not once per key
+    // and slot.
+    template <int G>
+    __device__ __forceinline__ void find_n(const uint64_t (&key)[G], const bool (&ok)[G], uint32_t (&slot)[G]) const {
+        uint32_t s[G];
+        bool open[G];
+        const uint32_t mask = size() - 1u;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            slot[g] = UINT32_MAX;
+            open[g] = ok[g];
+            if (ok[g] && key[g] == ~0ull) {
+                slot[g] = keys[size()] != 0ull ? size() : UINT32_MAX;
+                open[g] = false;
+            }
+            s[g] = home(key[g]);
+        }
+        for (uint32_t round = 0; round < size(); ++round) {
+            unsigned long long k[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) k[g] = keys[open[g] ? s[g] : 0u];
+            bool more = false;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (!open[g]) continue;
+                if (k[g] == static_cast<unsigned long long>(key[g]) + 1ull) {
+                    slot[g] = s[g];
+                    open[g] = false;
+                } else if (k[g] == 0ull) {
+                    open[g] = false;
+                } else {
+                    s[g] = (s[g] + 1u) & mask;
+                    more = true;
+                }
+            }
+            if (!more) break;
+        }
+    }
 };
 
 }  // namespace ahead_dev
