@@ -374,19 +374,14 @@ struct ProbeMapOp : ProbeOp {
 // some do not.  The streaming engine closes the gaps of dropped rows in place only while they total
 // at most kMaxShift rows, so it suits probes where (nearly) every row finds its key, as foreign
 // keys into a whole dimension do.  It only picks the engine; both give the same results.
-constexpr uint32_t kHjSamples = 4096;
+constexpr uint32_t kHjSamples = 4096;  // one per thread of kHjSamples / kBlock workgroups
 __global__ __launch_bounds__(kBlock) void k_hj_sample(ProbeOp p, uint64_t np) {
-    __shared__ uint32_t s_red[kWaves];
-    uint32_t hits = 0;
-    for (uint32_t k = threadIdx.x; k < kHjSamples; k += kBlock) {
-        const uint64_t row = np * k / kHjSamples;
-        ProbeOp::State st{};
-        uint32_t errs = 0;
-        uint64_t aux = 0;
-        if (row < np && p.eval(row, st, errs, aux) != 0u) ++hits;
-    }
-    const uint32_t tot = block_sum<uint32_t>(hits, s_red);
-    if (threadIdx.x == 0) p.t.flags[6] = np > 0 && tot < kHjSamples ? 1u : 0u;
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    const uint64_t row = np * k / kHjSamples;
+    ProbeOp::State st{};
+    uint32_t errs = 0;
+    uint64_t aux = 0;
+    if (row < np && p.eval(row, st, errs, aux) == 0u) atomicOr(&p.t.flags[6], 1u);
 }
 
 __global__ void k_hj_status(const uint32_t *flags, uint64_t *d_counts) {
@@ -528,7 +523,7 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     // the streaming engine needs room for a staged pair per probe row
     const bool map_ok = out_h.data && out_t.data && p.out_cap >= np;
     if (map_ok) {
-        hipLaunchKernelGGL(k_hj_sample, dim3(1), dim3(kBlock), 0, s, p, np);
+        hipLaunchKernelGGL(k_hj_sample, dim3(kHjSamples / kBlock), dim3(kBlock), 0, s, p, np);
         ProbeMapOp pm{};
         static_cast<ProbeOp &>(pm) = p;
         if ((st = launch_map(pm, ws, o_end, s)) != AHEAD_OK) return st;
