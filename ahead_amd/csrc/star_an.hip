@@ -871,12 +871,14 @@ __global__ __launch_bounds__(kP2Block) void k_star_probe2(Probe2Args a) {
     uint32_t kept_before = 0;           // wave-uniform: kept rows of this wave so far
     uint32_t err_rows = 0;              // wave-uniform: rows with any error so far
     uint32_t unseen = a.flags[1];       // wave-uniform: combinations without a first row in this wave
-    uint32_t ecnt[4] = {0, 0, 0, 0};    // per lane
+    uint32_t ecnt[4] = {0, 0, 0, 0};    // wave-uniform: rows with each error kind (counted from ballots)
     uint32_t *wfirst = s_first[wave];
 
-    for (uint64_t seg = begin; seg < end; seg += kP2Seg) {
+    // one segment of kP2Seg rows; FULL: every row is in range (all but a wave's last segment), so
+    // the row-range tests compile away
+    auto segment = [&](uint64_t seg, auto full_tag) {
+        constexpr bool full = decltype(full_tag)::value;
         uint64_t fk[4][2], mv[4][2];
-        const bool full = seg + kP2Seg <= end;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const uint64_t r0 = seg + c * 128u + lane * 2u;
@@ -902,7 +904,6 @@ __global__ __launch_bounds__(kP2Block) void k_star_probe2(Probe2Args a) {
                 live[c][k] = full || seg + c * 128u + lane * 2u + k < end;
                 const uint64_t d = fk[c][k] * a.fk_inv;
                 efk[c][k] = live[c][k] && d > a.fk_maxu;
-                ecnt[0] += efk[c][k] ? 1u : 0u;
                 const uint32_t rel = static_cast<uint32_t>(d & a.key_mask) - kmin;
                 uint32_t cd;
                 if (rel < kl) {
@@ -944,9 +945,6 @@ __global__ __launch_bounds__(kP2Block) void k_star_probe2(Probe2Args a) {
                 }
                 const bool joined = cd != 0u;
                 const bool em = joined && (mv[c][k] * a.m_inv) > a.m_maxu;
-                ecnt[1] += e1 ? 1u : 0u;
-                ecnt[2] += e2 ? 1u : 0u;
-                ecnt[3] += em ? 1u : 0u;
                 eflags[k] = (efk[c][k] ? 1u : 0u) | (e1 ? 2u : 0u) | (e2 ? 4u : 0u) | (em ? 8u : 0u);
                 kept[k] = joined && !e1 && !e2 && !em;
                 cid[k] = cd - 1u;
@@ -979,6 +977,10 @@ __global__ __launch_bounds__(kP2Block) void k_star_probe2(Probe2Args a) {
             // rows with errors (rare): recorded in row order while the wave's list has room
             const uint64_t x0 = __ballot(eflags[0] != 0u), x1 = __ballot(eflags[1] != 0u);
             if ((x0 | x1) != 0ull) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    ecnt[e] += static_cast<uint32_t>(__popcll(__ballot((eflags[0] >> e) & 1u)) +
+                                                     __popcll(__ballot((eflags[1] >> e) & 1u)));
                 const uint32_t xb = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(x0 >> 32),
                                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(x0), 0u)) +
                                     __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(x1 >> 32),
@@ -994,10 +996,11 @@ __global__ __launch_bounds__(kP2Block) void k_star_probe2(Probe2Args a) {
                 err_rows += static_cast<uint32_t>(__popcll(x0) + __popcll(x1));
             }
         }
-    }
+    };
+    uint64_t seg = begin;
+    for (; seg + kP2Seg <= end; seg += kP2Seg) segment(seg, std::true_type{});
+    if (seg < end) segment(seg, std::false_type{});
     // per-wave counts
-#pragma unroll
-    for (int e = 0; e < 4; ++e) ecnt[e] = wave_sum(ecnt[e]);
     const uint32_t waves = gridDim.x * kP2Waves;
     if (lane == 0) {
         a.werows[gw] = err_rows;
