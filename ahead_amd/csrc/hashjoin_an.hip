@@ -43,7 +43,22 @@ struct HjTable {
     uint32_t *flags;  // [0] several rows share a key, [1] table overflow, [2] list too long to order,
                       // [3] smallest build key, [5] largest build key, [4] direct map in use,
                       // [6] some sampled probe row finds no key
+    uint64_t *filt;   // [kHjFiltWords] probe filter: the bit of every build key set (filt_bit)
 };
+
+// The probe filter: kHjFiltBits bits with the bit of every build key set, so a probe key whose bit
+// is clear has no build row.  Exact (bit = key - smallest) when the build keys are of 32 bits or
+// less and span fewer than kHjFiltBits values; otherwise the top bits of a multiplicative hash.
+// 32 KiB: a probe that misses keeps it in LDS and gathers only for keys it passes.
+constexpr uint32_t kHjFiltWords = 4096, kHjFiltBits = kHjFiltWords * 64;
+__device__ __forceinline__ uint32_t filt_bit(const uint32_t *flags, uint64_t k) {
+    const uint32_t lo = flags[3], hi = flags[5];
+    if (lo <= hi) {  // the range of 32-bit build keys is known
+        if (k < lo || k > hi) return UINT32_MAX;
+        if (hi - lo < kHjFiltBits) return static_cast<uint32_t>(k - lo);
+    }
+    return static_cast<uint32_t>(((k ^ (k >> 29)) * 0x9E3779B97F4A7C15ull) >> 46);
+}
 
 struct BuildArgs {
     HjTable t;
@@ -92,6 +107,17 @@ __global__ __launch_bounds__(kBlock) void k_hj_range(BuildArgs a) {
         }
         atomicMin(&a.t.flags[3], lo);
         atomicMax(&a.t.flags[5], hi);
+    }
+}
+
+// Sets the filter bit of every build key (the filter is zeroed before).
+__global__ __launch_bounds__(kBlock) void k_hj_filter(BuildArgs a) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    uint32_t *w = reinterpret_cast<uint32_t *>(a.t.filt);
+    for (uint64_t j = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; j < a.n; j += stride) {
+        const uint64_t k = a.key.dec_of(a.key.raw(j)) & a.key_mask;
+        const uint32_t b = filt_bit(a.t.flags, k);
+        if (b != UINT32_MAX) atomicOr(&w[b >> 5], 1u << (b & 31u));
     }
 }
 
@@ -242,11 +268,14 @@ struct ProbeOp {
     uint64_t cap[2];
     uint64_t *counts;         // [0] pairs, error counts at [e_index[0]], [e_index[1]]
     uint32_t e_index[2];
-    uint32_t map_launched;    // the streaming-engine probe (ProbeMapOp) was launched before this one
+    uint32_t map_launched;    // a streaming-engine probe (ProbeMapOp, or ProbeLeanOp when lean) was launched before
+    uint32_t lean;            // ... and it was the ProbeLeanOp pair, which takes probes that miss too
 
     // the tile engine runs this probe when some build key has several rows, when some sampled probe
     // row finds no key, or when the streaming probe could not be launched
-    __device__ bool skip() const { return map_launched != 0u && t.flags[0] == 0u && t.flags[6] == 0u; }
+    __device__ bool skip() const {
+        return map_launched != 0u && t.flags[0] == 0u && (lean != 0u || t.flags[6] == 0u);
+    }
 
     static constexpr bool kBatched = true;
     template <int G>
@@ -370,6 +399,186 @@ struct ProbeMapOp : ProbeOp {
     }
 };
 
+// The probe for its common shapes, on the streaming engine: every build key has one row, the probe
+// key column holds KW-byte codes, the probe payload is void (PV) or of 8-byte codes, the build
+// payload is void (BV) or of 8-byte codes, and both outputs are 8 bytes wide.  The same results as
+// ProbeOp (hashjoinAN.tcc:150-220 probe loop), with the widths fixed at compile time: a row costs
+// its loads, one or two products, the lookup and, when it joins, two stores.
+template <bool FILT>
+struct LeanFilter {};
+template <>
+struct LeanFilter<true> {
+    static constexpr int kFilterWords = static_cast<int>(kHjFiltWords);
+};
+
+template <int KW, bool PV, bool BV, bool FILT>
+struct ProbeLeanOp : LeanFilter<FILT> {
+    static constexpr int kErr = 2;  // probe head check, probe tail check
+    static constexpr bool kMulti = false, kMain = true, kAcc = false, kBatched = true;
+    static constexpr int kOuts = 2;
+    using State = ProbeState;
+    using KT = std::conditional_t<KW == 8, uint64_t, uint32_t>;
+    uint64_t n;
+    HjTable t;
+    const KT *key;                            // probe key codes
+    uint64_t key_inv, key_cmask, key_maxu;    // decode (key_enc != 0) and the code-width mask
+    uint32_t key_enc, key_err;                // error bit of the key side: 0 probe head, 1 probe tail
+    const uint64_t *pay;                      // probe payload codes (PV: void, pay_seq + row)
+    uint64_t pay_seq, pay_inv, pay_maxu;
+    uint32_t pay_enc;
+    const uint64_t *bpay;                     // build payload codes (BV: void, bpay_seq + row)
+    uint64_t bpay_seq;
+    uint64_t guard_mask, key_mask, build_max;
+    uint32_t reenc_p, reenc_b;
+    uint64_t f_p, f_b;
+    uint64_t *out_p, *out_b;                  // probe / build payload outputs
+    void *out_h, *out_t;
+    uint64_t out_cap;
+    uint64_t a_oid;
+    uint64_t *err[2];
+    uint64_t cap[2];
+    uint64_t *counts;
+    uint32_t e_index[2];
+    uint32_t pair_launched;  // !FILT: the FILT variant was launched too (it takes probes that miss)
+
+    // a build key with several rows: the tile engine's ProbeOp runs instead.  FILT: only probes in
+    // which some sampled row finds no key (flags[6]); its twin without the filter takes the others.
+    __device__ bool skip() const {
+        if (t.flags[0] != 0u) return true;
+        return FILT ? t.flags[6] == 0u : (pair_launched != 0u && t.flags[6] != 0u);
+    }
+    __device__ void filter_init(uint64_t *s) const {
+        for (uint32_t i = threadIdx.x; i < kHjFiltWords; i += kBlock) s[i] = t.filt[i];
+    }
+    template <int G>
+    __device__ __forceinline__ void eval_f(const uint64_t (&row)[G], const bool (&live)[G], State (&st)[G],
+                                           uint32_t (&errs)[G], uint32_t (&cnt)[G], uint64_t &acc,
+                                           const uint64_t *filt) const {
+        eval_core<G>(row, live, st, errs, cnt, filt);
+    }
+    __device__ void out_arrays(void *(&p)[2], uint32_t (&w)[2]) const {
+        p[0] = out_h;
+        p[1] = out_t;
+        w[0] = 8;
+        w[1] = 8;
+    }
+    template <int G>
+    __device__ __forceinline__ void eval_n(const uint64_t (&row)[G], const bool (&live)[G], State (&st)[G],
+                                           uint32_t (&errs)[G], uint32_t (&cnt)[G], uint64_t &) const {
+        eval_core<G>(row, live, st, errs, cnt, nullptr);
+    }
+    // filt: the probe filter in LDS, or NULL
+    template <int G>
+    __device__ __forceinline__ void eval_core(const uint64_t (&row)[G], const bool (&live)[G], State (&st)[G],
+                                              uint32_t (&errs)[G], uint32_t (&cnt)[G], const uint64_t *filt) const {
+        KT x[G];
+        uint64_t p[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            x[g] = key[live[g] ? row[g] : 0ull];
+            p[g] = PV ? static_cast<uint64_t>(static_cast<uint32_t>(pay_seq + row[g])) : pay[live[g] ? row[g] : 0ull];
+        }
+        bool ok[G];
+        uint64_t k[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const uint64_t c = x[g];
+            const uint64_t d = key_enc ? (c * key_inv) & key_cmask : c;
+            const bool kb = key_enc && d > key_maxu;
+            const bool pb = !PV && pay_enc && p[g] * pay_inv > pay_maxu;
+            errs[g] = live[g] ? ((kb ? 1u : 0u) << key_err) | ((pb ? 1u : 0u) << (1u - key_err)) : 0u;
+            st[g].pay = p[g];
+            ok[g] = live[g] && (d & guard_mask) <= build_max;
+            k[g] = d & key_mask;
+            cnt[g] = 0;
+        }
+        if (filt != nullptr) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const uint32_t b = filt_bit(t.flags, k[g]);
+                ok[g] = ok[g] && b != UINT32_MAX && ((filt[b >> 6] >> (b & 63u)) & 1ull) != 0ull;
+            }
+        }
+        if (t.flags[4] != 0u) {  // direct map: one gather per row (one row per key: no flag bit)
+            const uint64_t lo = t.flags[3], span = t.flags[5] - t.flags[3];
+            uint32_t e[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const uint64_t rel = k[g] - lo;
+                ok[g] = ok[g] && rel <= span;
+                e[g] = t.dmap[ok[g] ? rel : 0ull];
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (ok[g] && e[g] != 0u) {
+                    st[g].brow = e[g] - 1u;
+                    cnt[g] = 1;
+                }
+            }
+            return;
+        }
+        uint32_t sl[G];
+        t.ht.find_n<G>(k, ok, sl);
+        uint32_t f[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) f[g] = t.first[sl[g] != UINT32_MAX ? sl[g] : 0u];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (sl[g] != UINT32_MAX) {
+                st[g].brow = f[g];
+                cnt[g] = 1;
+            }
+        }
+    }
+    __device__ uint32_t eval(uint64_t row, State &st, uint32_t &errs, uint64_t &acc) const {
+        const uint64_t r[1] = {row};
+        const bool l[1] = {row < n};
+        State s1[1];
+        uint32_t e1[1], c1[1];
+        eval_n<1>(r, l, s1, e1, c1, acc);
+        st = s1[0];
+        errs = e1[0];
+        return c1[0];
+    }
+    __device__ void emit_main(uint64_t, const State &st, uint64_t pos, uint32_t) const {
+        if (pos >= out_cap) return;
+        const uint64_t b = BV ? static_cast<uint64_t>(static_cast<uint32_t>(bpay_seq + st.brow)) : bpay[st.brow];
+        out_p[pos] = reenc_p ? st.pay * f_p : st.pay;
+        out_b[pos] = reenc_b ? b * f_b : b;
+    }
+    __device__ void emit_err(int e, uint64_t row, const State &, uint64_t pos) const {
+        if (pos < cap[e]) err[e][pos] = row * a_oid;
+    }
+    __device__ void tile_done(uint32_t, uint64_t) const {}
+    __device__ void finish(const uint64_t *tot) const {
+        counts[0] = tot[0];
+        counts[e_index[0]] = tot[1];
+        counts[e_index[1]] = tot[2];
+    }
+};
+
+// Calls f with each ProbeLeanOp variant of the given shape: without the filter, and (8-byte keys)
+// with it.  Returns whether the shape has a variant with the filter.
+template <class F>
+bool lean_shape(uint32_t kw, bool pv, bool bv, F &&f) {
+#define AHEAD_LEAN(K, P, B, FL)                     \
+    if (kw == K && pv == P && bv == B) {            \
+        f(ProbeLeanOp<K, P, B, false>{});           \
+        if (FL) f(ProbeLeanOp<K, P, B, FL>{});      \
+        return FL;                                  \
+    }
+    AHEAD_LEAN(8, true, true, true)
+    AHEAD_LEAN(8, true, false, true)
+    AHEAD_LEAN(8, false, true, true)
+    AHEAD_LEAN(8, false, false, true)
+    AHEAD_LEAN(4, true, true, false)
+    AHEAD_LEAN(4, true, false, false)
+    AHEAD_LEAN(4, false, true, false)
+    AHEAD_LEAN(4, false, false, false)
+#undef AHEAD_LEAN
+    return false;
+}
+
 // Whether every one of kHjSamples evenly spaced probe rows finds a build key: flags[6] = 1 when
 // some do not.  The streaming engine closes the gaps of dropped rows in place only while they total
 // at most kMaxShift rows, so it suits probes where (nearly) every row finds its key, as foreign
@@ -422,13 +631,29 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     const uint64_t slots = (1ull << bits) + 1;
     // layout: [flags 256][keys][cnt][first][off][cur][rows][direct map]
     const uint32_t key_bits = bkey.data ? bkey.unenc_bits : 32;
-    const uint64_t map_cap = key_bits <= 32 ? 4ull * nb + 1024 : 0;  // direct map only for <= 32-bit keys
+    // direct map only for <= 32-bit keys; at least 4 Mi entries (16 MiB), so a filtered dimension
+    // of a few million keys (the probe then gathers one entry per row, never a probe chain) fits
+    const uint64_t map_cap = key_bits <= 32 ? (4ull * nb + 1024 > (1ull << 22) ? 4ull * nb + 1024 : 1ull << 22) : 0;
     const size_t o_flags = 0, o_keys = 256, o_cnt = o_keys + align_up(slots * 8, 256),
                  o_first = o_cnt + align_up(slots * 4, 256), o_off = o_first + align_up(slots * 4, 256),
                  o_cur = o_off + align_up(slots * 4, 256), o_rows = o_cur + align_up(slots * 4, 256),
-                 o_dmap = o_rows + align_up(nb * 4, 256), o_end = o_dmap + align_up(map_cap * 4, 256);
+                 o_dmap = o_rows + align_up(nb * 4, 256), o_filt = o_dmap + align_up(map_cap * 4, 256),
+                 o_end = o_filt + kHjFiltWords * 8;
+    // the lean probe's shapes (ProbeLeanOp)
+    const ahead_col &ppay = right ? phead : ptail;
+    const uint32_t lean_kw = pkey.data ? pkey.width : 0u;
+    const bool lean = (lean_kw == 8 || lean_kw == 4) && (!ppay.data || ppay.width == 8) &&
+                      (!bpay.data || bpay.width == 8) && wout_h == 8 && wout_t == 8 && out_h.data && out_t.data &&
+                      (out_h.cap < out_t.cap ? out_h.cap : out_t.cap) >= np;
     // the streaming probe's tables follow the build's
-    if ((st = ws_ensure(ws, o_end + map_scratch_bytes<ProbeMapOp>(np))) != AHEAD_OK) return st;
+    size_t need = o_end + map_scratch_bytes<ProbeMapOp>(np);
+    if (lean) {
+        lean_shape(lean_kw, !ppay.data, !bpay.data, [&](auto proto) {
+            const size_t b = o_end + map_scratch_bytes<decltype(proto)>(np);
+            if (b > need) need = b;
+        });
+    }
+    if ((st = ws_ensure(ws, need)) != AHEAD_OK) return st;
 
     HjTable t{};
     t.flags = reinterpret_cast<uint32_t *>(ws_region(ws, o_flags));
@@ -442,7 +667,9 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     t.rows = reinterpret_cast<uint32_t *>(ws_region(ws, o_rows));
     t.dmap = reinterpret_cast<uint32_t *>(ws_region(ws, o_dmap));
     t.map_cap = static_cast<uint32_t>(map_cap);
+    t.filt = reinterpret_cast<uint64_t *>(ws_region(ws, o_filt));
     if ((st = fill_u32(t.flags, 64, 0, s)) != AHEAD_OK) return st;
+    if ((st = fill_u64(t.filt, kHjFiltWords, 0, s)) != AHEAD_OK) return st;
     if ((st = fill_u32(t.flags + 3, 1, 0xFFFFFFFFu, s)) != AHEAD_OK) return st;  // smallest key
     if ((st = fill_u64(reinterpret_cast<uint64_t *>(t.ht.keys), slots, 0, s)) != AHEAD_OK) return st;
     if ((st = fill_u32(t.cnt, slots, 0, s)) != AHEAD_OK) return st;
@@ -462,6 +689,7 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
         hipLaunchKernelGGL(k_hj_dmap_clear, dim3(512), dim3(kBlock), 0, s, t);
         hipLaunchKernelGGL(k_hj_dmap_fill, grid, dim3(kBlock), 0, s, b);
     }
+    hipLaunchKernelGGL(k_hj_filter, grid, dim3(kBlock), 0, s, b);
     AHEAD_HIP_TRY(hipGetLastError());
     OffsetOp off{};
     off.n = slots;
@@ -522,7 +750,56 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     p.e_index[1] = right ? 2 : 4;
     // the streaming engine needs room for a staged pair per probe row
     const bool map_ok = out_h.data && out_t.data && p.out_cap >= np;
-    if (map_ok) {
+    if (lean) {
+        int lst = AHEAD_OK;
+        // the variant with the filter exists for 8-byte keys: the sample picks between the two
+        const bool pair = lean_kw == 8;
+        if (pair) hipLaunchKernelGGL(k_hj_sample, dim3(kHjSamples / kBlock), dim3(kBlock), 0, s, p, np);
+        lean_shape(lean_kw, !ppay.data, !bpay.data, [&](auto proto) {
+            if (lst != AHEAD_OK) return;
+            auto q = proto;
+            q.pair_launched = pair ? 1u : 0u;
+            q.n = np;
+            q.t = t;
+            q.key = static_cast<const typename decltype(q)::KT *>(pkey.data);
+            q.key_enc = p.key.encoded;
+            q.key_inv = p.key.a_inv;
+            q.key_cmask = p.key.mask;
+            q.key_maxu = p.key.max_u;
+            q.key_err = right ? 1u : 0u;
+            q.pay = static_cast<const uint64_t *>(ppay.data);
+            q.pay_seq = ppay.seqbase;
+            q.pay_enc = p.pay.encoded;
+            q.pay_inv = p.pay.a_inv;
+            q.pay_maxu = p.pay.max_u;
+            q.bpay = static_cast<const uint64_t *>(bpay.data);
+            q.bpay_seq = bpay.seqbase;
+            q.guard_mask = p.guard_mask;
+            q.key_mask = p.key_mask;
+            q.build_max = p.build_max;
+            q.reenc_p = p.reenc_p;
+            q.reenc_b = p.reenc_b;
+            q.f_p = p.f_p;
+            q.f_b = p.f_b;
+            q.out_h = out_h.data;
+            q.out_t = out_t.data;
+            q.out_p = static_cast<uint64_t *>(right ? out_h.data : out_t.data);
+            q.out_b = static_cast<uint64_t *>(right ? out_t.data : out_h.data);
+            q.out_cap = p.out_cap;
+            q.a_oid = a_oid;
+            q.err[0] = p.err[0];
+            q.err[1] = p.err[1];
+            q.cap[0] = p.cap[0];
+            q.cap[1] = p.cap[1];
+            q.counts = d_counts;
+            q.e_index[0] = p.e_index[0];
+            q.e_index[1] = p.e_index[1];
+            lst = launch_map(q, ws, o_end, s);
+        });
+        if (lst != AHEAD_OK) return lst;
+        p.map_launched = 1u;
+        p.lean = 1u;
+    } else if (map_ok) {
         hipLaunchKernelGGL(k_hj_sample, dim3(kHjSamples / kBlock), dim3(kBlock), 0, s, p, np);
         ProbeMapOp pm{};
         static_cast<ProbeOp &>(pm) = p;

@@ -59,11 +59,28 @@ __device__ __forceinline__ uint32_t map_below(uint64_t ballot) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(ballot), 0u));
 }
 
+// An operator may bring a read-only table to LDS (kFilterWords 64-bit words): filter_init copies it
+// at the start of k_map and k_map_fix, and eval_f (eval_n with the table) evaluates the rows.
+template <class Op, class = void>
+struct has_filter : std::false_type {};
+template <class Op>
+struct has_filter<Op, std::void_t<decltype(Op::kFilterWords)>> : std::true_type {};
+template <class Op, bool = has_filter<Op>::value>
+struct FilterWords {
+    static constexpr int value = 1;
+};
+template <class Op>
+struct FilterWords<Op, true> {
+    static constexpr int value = Op::kFilterWords;
+};
+
 template <class Op>
 __device__ __forceinline__ void map_eval(const Op &op, const uint64_t (&rows)[kMapG], const bool (&live)[kMapG],
                                          typename Op::State (&st)[kMapG], uint32_t (&errs)[kMapG],
-                                         uint32_t (&cnt)[kMapG], uint64_t &acc) {
-    if constexpr (has_eval_n<Op>::value) {
+                                         uint32_t (&cnt)[kMapG], uint64_t &acc, const uint64_t *filt = nullptr) {
+    if constexpr (has_filter<Op>::value) {
+        op.template eval_f<kMapG>(rows, live, st, errs, cnt, acc, filt);
+    } else if constexpr (has_eval_n<Op>::value) {
         op.template eval_n<kMapG>(rows, live, st, errs, cnt, acc);
     } else {
 #pragma unroll
@@ -96,11 +113,16 @@ __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
     static_assert(!Op::kMulti, "the map engine takes operators whose rows emit at most one entry");
     constexpr int NS = MapTraits<Op>::NS;
     __shared__ unsigned long long s_acc[SharedWords<Op>::value];
+    __shared__ uint64_t s_filt[FilterWords<Op>::value];
     if constexpr (has_skip<Op>::value) {
         if (op.skip()) return;
     }
     if constexpr (has_shared<Op>::value) {
         op.shared_init(s_acc);
+        __syncthreads();
+    }
+    if constexpr (has_filter<Op>::value) {
+        op.filter_init(s_filt);
         __syncthreads();
     }
     const uint32_t lane = threadIdx.x & 63u;
@@ -126,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
         if constexpr (has_shared<Op>::value) {
             op.template eval_shared<kMapG>(rows, live, st, errs, cnt, acc, s_acc);
         } else {
-            map_eval(op, rows, live, st, errs, cnt, acc);
+            map_eval(op, rows, live, st, errs, cnt, acc, s_filt);
         }
 #pragma unroll
         for (int g = 0; g < kMapG; ++g) {
@@ -262,8 +284,13 @@ __global__ __launch_bounds__(kBlock) void k_map_save(Op op, MapArgs m) {
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_map_fix(Op op, MapArgs m) {
     constexpr int NS = MapTraits<Op>::NS;
+    __shared__ uint64_t s_filt[FilterWords<Op>::value];
     if constexpr (has_skip<Op>::value) {
         if (op.skip()) return;
+    }
+    if constexpr (has_filter<Op>::value) {  // before any wave leaves
+        op.filter_init(s_filt);
+        __syncthreads();
     }
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
@@ -325,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_map_fix(Op op, MapArgs m) {
                 }
                 typename Op::State st[kMapG];
                 uint32_t errs[kMapG], cnt[kMapG];
-                map_eval(op, rows, live, st, errs, cnt, acc);
+                map_eval(op, rows, live, st, errs, cnt, acc, s_filt);
 #pragma unroll
                 for (int g = 0; g < kMapG; ++g) {
                     const uint64_t b = __ballot(cnt[g] != 0u);
