@@ -401,7 +401,7 @@ struct ProbeMapOp : ProbeOp {
 
 // The probe for its common shapes, on the streaming engine: every build key has one row, the probe
 // key column holds KW-byte codes, the probe payload is void (PV) or of 8-byte codes, the build
-// payload is void (BV) or of 8-byte codes, and both outputs are 8 bytes wide.  The same results as
+// payload is void (BV) or of 8-byte codes, and both outputs are 4 or 8 bytes wide.  The same results as
 // ProbeOp (hashjoinAN.tcc:150-220 probe loop), with the widths fixed at compile time: a row costs
 // its loads, one or two products, the lookup and, when it joins, two stores.
 template <bool FILT>
@@ -431,7 +431,8 @@ struct ProbeLeanOp : LeanFilter<FILT> {
     uint64_t guard_mask, key_mask, build_max;
     uint32_t reenc_p, reenc_b;
     uint64_t f_p, f_b;
-    uint64_t *out_p, *out_b;                  // probe / build payload outputs
+    void *out_p, *out_b;                      // probe / build payload outputs
+    uint32_t ow;                              // their width: 4 or 8 bytes
     void *out_h, *out_t;
     uint64_t out_cap;
     uint64_t a_oid;
@@ -459,8 +460,8 @@ struct ProbeLeanOp : LeanFilter<FILT> {
     __device__ void out_arrays(void *(&p)[2], uint32_t (&w)[2]) const {
         p[0] = out_h;
         p[1] = out_t;
-        w[0] = 8;
-        w[1] = 8;
+        w[0] = ow;
+        w[1] = ow;
     }
     template <int G>
     __device__ __forceinline__ void eval_n(const uint64_t (&row)[G], const bool (&live)[G], State (&st)[G],
@@ -543,8 +544,14 @@ struct ProbeLeanOp : LeanFilter<FILT> {
     __device__ void emit_main(uint64_t, const State &st, uint64_t pos, uint32_t) const {
         if (pos >= out_cap) return;
         const uint64_t b = BV ? static_cast<uint64_t>(static_cast<uint32_t>(bpay_seq + st.brow)) : bpay[st.brow];
-        out_p[pos] = reenc_p ? st.pay * f_p : st.pay;
-        out_b[pos] = reenc_b ? b * f_b : b;
+        const uint64_t pv = reenc_p ? st.pay * f_p : st.pay, bv = reenc_b ? b * f_b : b;
+        if (ow == 8) {
+            static_cast<uint64_t *>(out_p)[pos] = pv;
+            static_cast<uint64_t *>(out_b)[pos] = bv;
+        } else {  // 4-byte outputs keep the low half, as store_w does
+            static_cast<uint32_t *>(out_p)[pos] = static_cast<uint32_t>(pv);
+            static_cast<uint32_t *>(out_b)[pos] = static_cast<uint32_t>(bv);
+        }
     }
     __device__ void emit_err(int e, uint64_t row, const State &, uint64_t pos) const {
         if (pos < cap[e]) err[e][pos] = row * a_oid;
@@ -643,7 +650,7 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     const ahead_col &ppay = right ? phead : ptail;
     const uint32_t lean_kw = pkey.data ? pkey.width : 0u;
     const bool lean = (lean_kw == 8 || lean_kw == 4) && (!ppay.data || ppay.width == 8) &&
-                      (!bpay.data || bpay.width == 8) && wout_h == 8 && wout_t == 8 && out_h.data && out_t.data &&
+                      (!bpay.data || bpay.width == 8) && wout_h == wout_t && (wout_h == 8 || wout_h == 4) && out_h.data && out_t.data &&
                       (out_h.cap < out_t.cap ? out_h.cap : out_t.cap) >= np;
     // the streaming probe's tables follow the build's
     size_t need = o_end + map_scratch_bytes<ProbeMapOp>(np);
@@ -783,8 +790,9 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
             q.f_b = p.f_b;
             q.out_h = out_h.data;
             q.out_t = out_t.data;
-            q.out_p = static_cast<uint64_t *>(right ? out_h.data : out_t.data);
-            q.out_b = static_cast<uint64_t *>(right ? out_t.data : out_h.data);
+            q.out_p = right ? out_h.data : out_t.data;
+            q.out_b = right ? out_t.data : out_h.data;
+            q.ow = wout_h;
             q.out_cap = p.out_cap;
             q.a_oid = a_oid;
             q.err[0] = p.err[0];

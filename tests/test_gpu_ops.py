@@ -101,6 +101,48 @@ def test_hashjoin_void_sides_and_dense_keys():
     same(got, ref)
 
 
+# The probe for the common shapes (8- or 4-byte key codes, void or 8-byte payloads, 4- or 8-byte
+# outputs, room for a pair per probe row): all rows joining (one pass), some missing (the variant
+# with the key filter in LDS and the second, writing pass), both build sides, re-encoding.
+@pytest.mark.parametrize("key_w", [8, 4])
+@pytest.mark.parametrize("void_pay", [True, False])
+@pytest.mark.parametrize("build_right", [1, 0])
+@pytest.mark.parametrize("wout", [8, 4])
+@pytest.mark.parametrize("hit", ["all", "some"])
+def test_hashjoin_lean_shapes(key_w, void_pay, build_right, wout, hit):
+    seed = key_w + 3 * void_pay + 7 * build_right + wout + (11 if hit == "all" else 0)
+    np_, nb = 150_000, 20_000
+    a_k = 63877 if key_w == 8 else 233
+    meta_k = an.an_meta(an.RESINT if key_w == 8 else an.RESSHORT, a_k)
+    bkeys = np.arange(nb, dtype=np.uint64) * (1 if hit == "all" else 3)
+    pk = datagen.uniform(np_, seed, 1, 0, nb - 1) if hit == "all" else datagen.uniform(np_, seed, 1, 0, 3 * nb)
+    kc, bc = datagen.encode(pk, a_k, key_w), datagen.encode(bkeys, a_k, key_w)
+    datagen.inject_flips(kc, seed, 2, datagen.flip_threshold(1e-3))
+    pay_p = None if void_pay else datagen.encode(datagen.uniform(np_, seed, 3, 0, 10**6), an.A_OID, 8)
+    pay_b = None if void_pay else datagen.encode(datagen.uniform(nb, seed, 4, 0, 10**6), an.A_OID, 8)
+    if pay_p is not None:
+        datagen.inject_flips(pay_p, seed, 5, datagen.flip_threshold(1e-3))
+    f = (an.A_OID * an.inverse(an.A_OID, 64)) & an.MASK64
+    # the output width applies to re-encoded values (hashjoinAN.tcc): 8-byte payload codes go to
+    # 4-byte outputs only re-encoded
+    for reenc in ((0, 1) if wout == 8 or void_pay else (1,)):
+        args = (build_right, reenc, f, wout, reenc, f, wout)
+        if build_right:  # probe <pay_p, key>, build <key, pay_b>
+            o = (O.void(np_) if void_pay else O.Col(pay_p, RESOID), O.Col(kc, meta_k), O.Col(bc, meta_k),
+                 O.void(nb) if void_pay else O.Col(pay_b, RESOID))
+            g = (capi.Column.void(np_) if void_pay else gcol(pay_p, RESOID), gcol(kc, meta_k), gcol(bc, meta_k),
+                 capi.Column.void(nb) if void_pay else gcol(pay_b, RESOID))
+        else:  # build <pay_b, key>, probe <key, pay_p>
+            o = (O.void(nb) if void_pay else O.Col(pay_b, RESOID), O.Col(bc, meta_k), O.Col(kc, meta_k),
+                 O.void(np_) if void_pay else O.Col(pay_p, RESOID))
+            g = (capi.Column.void(nb) if void_pay else gcol(pay_b, RESOID), gcol(bc, meta_k), gcol(kc, meta_k),
+                 capi.Column.void(np_) if void_pay else gcol(pay_p, RESOID))
+        ref = O.hashjoin_an(*o, *args)
+        got = capi.hashjoin_an(*g, *args, err_cap=2)
+        same(got, ref)
+        assert ref[0].size > 0 and (hit == "all") == (ref[0].size >= 0.99 * np_)
+
+
 @pytest.mark.parametrize("n,card", [(0, 5), (1, 5), (2049, 3), (100_003, 25), (300_000, 70_000)])
 def test_groupby_parity(n, card):
     d = datagen.uniform(n, n + card, 1, 0, card - 1)
