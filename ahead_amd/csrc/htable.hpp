@@ -23,16 +23,19 @@ struct HTable {
         return static_cast<uint32_t>(x) & (size() - 1u);
     }
 
-    // Inserts key (if absent) and returns its slot, or UINT32_MAX if the table is full.
-    __device__ __forceinline__ uint32_t insert(uint64_t key) const {
+    // Inserts key (if absent) and returns its slot, or UINT32_MAX if the table is full.  *fresh
+    // (if given) tells whether this call claimed the slot.
+    __device__ __forceinline__ uint32_t insert(uint64_t key, bool *fresh = nullptr) const {
         if (key == ~0ull) {
-            atomicCAS(&keys[size()], 0ull, 1ull);
+            const unsigned long long prev = atomicCAS(&keys[size()], 0ull, 1ull);
+            if (fresh) *fresh = prev == 0ull;
             return size();
         }
         const unsigned long long word = static_cast<unsigned long long>(key) + 1ull;
         uint32_t s = home(key);
         for (uint32_t probes = 0; probes < size(); ++probes) {
             const unsigned long long prev = atomicCAS(&keys[s], 0ull, word);
+            if (fresh) *fresh = prev == 0ull;
             if (prev == 0ull || prev == word) return s;
             s = (s + 1u) & (size() - 1u);
         }

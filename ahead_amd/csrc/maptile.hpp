@@ -35,7 +35,7 @@ namespace ahead_dev {
 
 constexpr uint32_t kMapErrList = 16;  // error rows a wave records in row order
 constexpr uint32_t kMaxShift = 64;    // dropped rows the move path handles (else: rewrite)
-constexpr int kMapG = 4;              // rows per lane per iteration
+constexpr int kMapG = 4;              // rows per lane per iteration (unless the operator sets kMapRows)
 
 struct MapArgs {
     uint64_t rows_per_wave;
@@ -74,17 +74,28 @@ struct FilterWords<Op, true> {
     static constexpr int value = Op::kFilterWords;
 };
 
+// Rows per lane and iteration: kMapG, or the operator's kMapRows (an operator whose rows bring
+// few bytes keeps more loads in flight with more rows).
+template <class Op, class = void>
+struct MapG {
+    static constexpr int value = kMapG;
+};
 template <class Op>
-__device__ __forceinline__ void map_eval(const Op &op, const uint64_t (&rows)[kMapG], const bool (&live)[kMapG],
-                                         typename Op::State (&st)[kMapG], uint32_t (&errs)[kMapG],
-                                         uint32_t (&cnt)[kMapG], uint64_t &acc, const uint64_t *filt = nullptr) {
+struct MapG<Op, std::void_t<decltype(Op::kMapRows)>> {
+    static constexpr int value = Op::kMapRows;
+};
+
+template <class Op, int G>
+__device__ __forceinline__ void map_eval(const Op &op, const uint64_t (&rows)[G], const bool (&live)[G],
+                                         typename Op::State (&st)[G], uint32_t (&errs)[G],
+                                         uint32_t (&cnt)[G], uint64_t &acc, const uint64_t *filt = nullptr) {
     if constexpr (has_filter<Op>::value) {
-        op.template eval_f<kMapG>(rows, live, st, errs, cnt, acc, filt);
+        op.template eval_f<G>(rows, live, st, errs, cnt, acc, filt);
     } else if constexpr (has_eval_n<Op>::value) {
-        op.template eval_n<kMapG>(rows, live, st, errs, cnt, acc);
+        op.template eval_n<G>(rows, live, st, errs, cnt, acc);
     } else {
 #pragma unroll
-        for (int g = 0; g < kMapG; ++g) {
+        for (int g = 0; g < G; ++g) {
             cnt[g] = 0;
             errs[g] = 0;
             if (live[g]) cnt[g] = op.eval(rows[g], st[g], errs[g], acc);
@@ -112,6 +123,7 @@ template <class Op>
 __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
     static_assert(!Op::kMulti, "the map engine takes operators whose rows emit at most one entry");
     constexpr int NS = MapTraits<Op>::NS;
+    constexpr int G = MapG<Op>::value;
     __shared__ unsigned long long s_acc[SharedWords<Op>::value];
     __shared__ uint64_t s_filt[FilterWords<Op>::value];
     if constexpr (has_skip<Op>::value) {
@@ -135,23 +147,23 @@ __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
     uint32_t dropped = 0;              // wave-uniform
     uint32_t ecnt[Op::kErr > 0 ? Op::kErr : 1] = {};
     uint64_t acc = 0;
-    for (uint64_t seg = begin; seg < end; seg += 64u * kMapG) {
-        uint64_t rows[kMapG];
-        bool live[kMapG];
+    for (uint64_t seg = begin; seg < end; seg += 64u * G) {
+        uint64_t rows[G];
+        bool live[G];
 #pragma unroll
-        for (int g = 0; g < kMapG; ++g) {
+        for (int g = 0; g < G; ++g) {
             rows[g] = seg + static_cast<uint64_t>(g) * 64u + lane;
             live[g] = rows[g] < end;
         }
-        typename Op::State st[kMapG];
-        uint32_t errs[kMapG], cnt[kMapG];
+        typename Op::State st[G];
+        uint32_t errs[G], cnt[G];
         if constexpr (has_shared<Op>::value) {
-            op.template eval_shared<kMapG>(rows, live, st, errs, cnt, acc, s_acc);
+            op.template eval_shared<G>(rows, live, st, errs, cnt, acc, s_acc);
         } else {
             map_eval(op, rows, live, st, errs, cnt, acc, s_filt);
         }
 #pragma unroll
-        for (int g = 0; g < kMapG; ++g) {
+        for (int g = 0; g < G; ++g) {
             if constexpr (Op::kMain) {
                 const uint64_t b = __ballot(cnt[g] != 0u);
                 if (cnt[g] != 0u && dropped <= kMaxShift) op.emit_main(rows[g], st[g], begin + kept + map_below(b), 1u);
@@ -284,6 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_map_save(Op op, MapArgs m) {
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_map_fix(Op op, MapArgs m) {
     constexpr int NS = MapTraits<Op>::NS;
+    constexpr int G = MapG<Op>::value;
     __shared__ uint64_t s_filt[FilterWords<Op>::value];
     if constexpr (has_skip<Op>::value) {
         if (op.skip()) return;
@@ -342,19 +355,19 @@ __global__ __launch_bounds__(kBlock) void k_map_fix(Op op, MapArgs m) {
             // too many drops to move: evaluate the range again and write every entry in place
             uint32_t kept = 0;
             uint64_t acc = 0;
-            for (uint64_t seg = begin; seg < end; seg += 64u * kMapG) {
-                uint64_t rows[kMapG];
-                bool live[kMapG];
+            for (uint64_t seg = begin; seg < end; seg += 64u * G) {
+                uint64_t rows[G];
+                bool live[G];
 #pragma unroll
-                for (int g = 0; g < kMapG; ++g) {
+                for (int g = 0; g < G; ++g) {
                     rows[g] = seg + static_cast<uint64_t>(g) * 64u + lane;
                     live[g] = rows[g] < end;
                 }
-                typename Op::State st[kMapG];
-                uint32_t errs[kMapG], cnt[kMapG];
+                typename Op::State st[G];
+                uint32_t errs[G], cnt[G];
                 map_eval(op, rows, live, st, errs, cnt, acc, s_filt);
 #pragma unroll
-                for (int g = 0; g < kMapG; ++g) {
+                for (int g = 0; g < G; ++g) {
                     const uint64_t b = __ballot(cnt[g] != 0u);
                     if (cnt[g] != 0u) op.emit_main(rows[g], st[g], first + kept + map_below(b), 1u);
                     kept += static_cast<uint32_t>(__popcll(b));
@@ -452,7 +465,8 @@ int launch_map(const Op &op, ahead_hip_ws *ws, size_t offset, hipStream_t s) {
     const uint32_t grid = map_grid<Op>(op.n);
     const uint32_t waves = grid * kWaves;
     uint64_t per = (op.n + waves - 1) / waves;
-    per = (per + 64u * kMapG - 1) / (64u * kMapG) * (64u * kMapG);
+    constexpr uint64_t step = 64u * MapG<Op>::value;
+    per = (per + step - 1) / step * step;
     const size_t o_cnt = offset, o_pre = o_cnt + align_up(static_cast<size_t>(NS + 1) * waves * 4, 256),
                  o_erows = o_pre + align_up(static_cast<size_t>(NS + 1) * waves * 4, 256),
                  o_elist = o_erows + align_up(static_cast<size_t>(waves) * 4, 256),
