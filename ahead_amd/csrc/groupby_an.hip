@@ -142,12 +142,12 @@ struct Run {
     }
 };
 
-// Key word of the per-wave set: 32 bits for unary keys of codes up to 4 bytes.  Runs: 16 rows per
-// lane, 8 where a row brings 8 bytes or more.
+// Key word of the per-wave set: 32 bits for unary keys of codes up to 4 bytes.  Runs: 64 bytes of
+// codes per lane (at most 32 rows), 8 rows where a row brings 8 bytes or more.
 template <class T, bool BIN>
 struct GfTraits {
     using K = std::conditional_t<(!BIN && sizeof(T) <= 4), uint32_t, uint64_t>;
-    static constexpr int R = (BIN || sizeof(T) == 8) ? 8 : 16;
+    static constexpr int R = (BIN || sizeof(T) == 8) ? 8 : (sizeof(T) == 4 ? 16 : 32);
 };
 
 __device__ __forceinline__ uint32_t gf_slot(uint64_t key) {
@@ -213,32 +213,71 @@ __global__ __launch_bounds__(kBlock) void k_grp_first(GrpKey k, GrpTable t, uint
     bool ones = false;              // wave-uniform: kOnes is listed
     K my_key = 0;                   // lane i holds the wave's i-th listed key and its first row
     uint32_t my_row = 0;
+    auto keys_of = [&](const Run<T, R> &c, const Run<uint64_t, R> &g, K (&key)[R]) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            if constexpr (!BIN && sizeof(T) <= 2) {  // both factors below 2^16: the 24-bit multiply is exact
+                key[j] = static_cast<K>(__umul24(static_cast<uint32_t>(c.get(j)), ka) & km);
+            } else if constexpr (!BIN && sizeof(T) == 4) {
+                key[j] = static_cast<K>((static_cast<uint32_t>(c.get(j)) * ka) & km);
+            } else if constexpr (BIN) {
+                key[j] = static_cast<K>(k.key_t<T, BIN>(c.get(j), g.get(j)));
+            } else {
+                key[j] = static_cast<K>(k.key_t<T, BIN>(c.get(j), 0ull));
+            }
+        }
+    };
     // one batch: the lanes' keys against the set, then the keys new to the wave
     auto examine = [&](const K (&key)[R], uint64_t base, uint64_t r0) {
-        // bit j: row r0 + j; every set word is read before any is tested (no per-row branches)
-        const uint32_t live = r0 >= end ? 0u : (end - r0 >= static_cast<uint64_t>(R) ? ~0u >> (32 - R)
-                                                                                       : (1u << (end - r0)) - 1u);
-        uint32_t pend = 0, probe = 0;
+        // the common case first: a whole run whose keys the wave has all listed.  Every set word is
+        // read before any is tested (no per-row branches); nothing of the keys outlives this test.
+        bool all = r0 + R <= end;
         if constexpr (DIRECT) {
             uint32_t w[R];
 #pragma unroll
             for (int j = 0; j < R; ++j) w[j] = set[static_cast<uint32_t>(key[j]) >> 5];
-            // the common case first: every key of a whole run already listed (bit 0 of acc stays set)
-            uint32_t acc = 1u;
+            uint32_t acc = 1u;  // bit 0 stays set while every key's bit is
 #pragma unroll
             for (int j = 0; j < R; ++j) acc &= w[j] >> (static_cast<uint32_t>(key[j]) & 31u);
-            if (__ballot((acc & 1u) == 0u || r0 + R > end) == 0ull) return;
-#pragma unroll
-            for (int j = 0; j < R; ++j) pend |= (~(w[j] >> (static_cast<uint32_t>(key[j]) & 31u)) & 1u) << j;
+            all = all && (acc & 1u) != 0u;
         } else {
             K w[R];
 #pragma unroll
             for (int j = 0; j < R; ++j) w[j] = set[gf_slot(key[j])];
 #pragma unroll
+            for (int j = 0; j < R; ++j) all = all && (key[j] == kOnes ? ones : w[j] == static_cast<K>(key[j] + 1));
+        }
+        if (__ballot(!all) == 0ull) return;
+        // exact pass: the batch's codes again (cached) and its keys; bit j: row r0 + j
+        K k2[R];
+        {
+            Run<T, R> c;
+            Run<uint64_t, R> g;
+            if (vec != 0u && r0 + R <= end) {
+                c.load_vec(tail, r0);
+                if constexpr (BIN) g.load_vec(grp, r0);
+            } else {
+                c.load_elem(tail, r0, end);
+                if constexpr (BIN) g.load_elem(grp, r0, end);
+            }
+            keys_of(c, g, k2);
+        }
+        const uint32_t live = r0 >= end ? 0u : (end - r0 >= static_cast<uint64_t>(R) ? ~0u >> (32 - R)
+                                                                                       : (1u << (end - r0)) - 1u);
+        uint32_t pend = 0, probe = 0;
+        if constexpr (DIRECT) {
+#pragma unroll
             for (int j = 0; j < R; ++j) {
-                const bool hit = key[j] == kOnes ? ones : w[j] == static_cast<K>(key[j] + 1);
+                const uint32_t w = set[static_cast<uint32_t>(k2[j]) >> 5];
+                pend |= (~(w >> (static_cast<uint32_t>(k2[j]) & 31u)) & 1u) << j;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const K w = set[gf_slot(k2[j])];
+                const bool hit = k2[j] == kOnes ? ones : w == static_cast<K>(k2[j] + 1);
                 pend |= static_cast<uint32_t>(!hit) << j;
-                probe |= static_cast<uint32_t>(key[j] != kOnes && w[j] != 0) << j;
+                probe |= static_cast<uint32_t>(k2[j] != kOnes && w != 0) << j;
             }
         }
         pend &= live;
@@ -248,11 +287,11 @@ __global__ __launch_bounds__(kBlock) void k_grp_first(GrpKey k, GrpTable t, uint
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             if (((probe >> j) & 1u) == 0u) continue;
-            uint32_t s = gf_slot(key[j]);
+            uint32_t s = gf_slot(k2[j]);
             for (uint32_t p = 1; p < kGfSlots; ++p) {
                 s = (s + 1u) & (kGfSlots - 1u);
                 const K w = set[s];
-                if (w == static_cast<K>(key[j] + 1)) {
+                if (w == static_cast<K>(k2[j] + 1)) {
                     pend &= ~(1u << j);
                     break;
                 }
@@ -269,12 +308,12 @@ __global__ __launch_bounds__(kBlock) void k_grp_first(GrpKey k, GrpTable t, uint
             const uint32_t mj = pend != 0u ? static_cast<uint32_t>(__builtin_ctz(pend)) : 0u;
             K mk = key[0];
 #pragma unroll
-            for (int j = 1; j < R; ++j) mk = mj == static_cast<uint32_t>(j) ? key[j] : mk;
+            for (int j = 1; j < R; ++j) mk = mj == static_cast<uint32_t>(j) ? k2[j] : mk;
             const K k0 = readlane_k<K>(mk, L);
             const uint32_t row0 = static_cast<uint32_t>(base) + L * static_cast<uint32_t>(R) +
                                   static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mj), L));
 #pragma unroll
-            for (int j = 0; j < R; ++j) pend &= key[j] == k0 ? ~(1u << j) : ~0u;
+            for (int j = 0; j < R; ++j) pend &= k2[j] == k0 ? ~(1u << j) : ~0u;
             if constexpr (DIRECT) {
                 if (lane == 0) set[static_cast<uint32_t>(k0) >> 5] |= 1u << (static_cast<uint32_t>(k0) & 31u);
             } else if (k0 == kOnes) {
@@ -301,54 +340,25 @@ __global__ __launch_bounds__(kBlock) void k_grp_first(GrpKey k, GrpTable t, uint
             }
         }
     };
-    auto keys_of = [&](const Run<T, R> &c, const Run<uint64_t, R> &g, K (&key)[R]) {
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            if constexpr (!BIN && sizeof(T) <= 2) {  // both factors below 2^16: the 24-bit multiply is exact
-                key[j] = static_cast<K>(__umul24(static_cast<uint32_t>(c.get(j)), ka) & km);
-            } else if constexpr (!BIN && sizeof(T) == 4) {
-                key[j] = static_cast<K>((static_cast<uint32_t>(c.get(j)) * ka) & km);
-            } else if constexpr (BIN) {
-                key[j] = static_cast<K>(k.key_t<T, BIN>(c.get(j), g.get(j)));
-            } else {
-                key[j] = static_cast<K>(k.key_t<T, BIN>(c.get(j), 0ull));
-            }
-        }
-    };
     const uint64_t rl = begin + static_cast<uint64_t>(lane) * R;  // this lane's first row
-    // whole batches of an aligned column: vector loads, two batches in flight (buffers a and b,
-    // each reloaded right after its keys are taken, so no register waits on a load in flight)
+    // whole batches of an aligned column: vector loads, the next batch's issued as soon as this
+    // batch's keys are taken.  The reload is unconditional (past the last whole batch it reads that
+    // batch again), so that no path through the loop leaves a different set of loads in flight.
     const uint64_t nfull = (vec != 0u && end > begin) ? (end - begin) / kStep : 0u;
-    Run<T, R> ca, cb;
-    Run<uint64_t, R> ga, gb;
+    Run<T, R> ca;
+    Run<uint64_t, R> ga;
     if (nfull > 0) {
         ca.load_vec(tail, rl);
         if constexpr (BIN) ga.load_vec(grp, rl);
     }
-    if (nfull > 1) {
-        cb.load_vec(tail, rl + kStep);
-        if constexpr (BIN) gb.load_vec(grp, rl + kStep);
-    }
     uint64_t b = 0;
     K key[R];
-    for (; b + 2 <= nfull; b += 2) {
-        // the reloads are unconditional (past the last whole batch they read that batch again), so
-        // that the wait before each buffer's keys covers that buffer's loads alone
+    for (; b < nfull; ++b) {
         keys_of(ca, ga, key);
-        const uint64_t na = b + 2 < nfull ? b + 2 : nfull - 1;
-        ca.load_vec(tail, rl + na * kStep);
-        if constexpr (BIN) ga.load_vec(grp, rl + na * kStep);
+        const uint64_t nb = b + 1 < nfull ? b + 1 : b;
+        ca.load_vec(tail, rl + nb * kStep);
+        if constexpr (BIN) ga.load_vec(grp, rl + nb * kStep);
         examine(key, begin + b * kStep, rl + b * kStep);
-        keys_of(cb, gb, key);
-        const uint64_t nb = b + 3 < nfull ? b + 3 : nfull - 1;
-        cb.load_vec(tail, rl + nb * kStep);
-        if constexpr (BIN) gb.load_vec(grp, rl + nb * kStep);
-        examine(key, begin + (b + 1) * kStep, rl + (b + 1) * kStep);
-    }
-    if (b < nfull) {
-        keys_of(ca, ga, key);
-        examine(key, begin + b * kStep, rl + b * kStep);
-        ++b;
     }
     // the last, partial batch (every batch of an unaligned column): element loads
     for (uint64_t base = begin + b * kStep; base < end; base += kStep) {
