@@ -110,8 +110,27 @@ __global__ __launch_bounds__(kBlock) void k_hj_range(BuildArgs a) {
     }
 }
 
-// Sets the filter bit of every build key (the filter is zeroed before).
+// The flags, the filter and the table's per-slot words, cleared in one launch: flags 0 except [3]
+// (smallest build key: all ones), keys / counts / cursors 0, first rows all ones.
+__global__ __launch_bounds__(kBlock) void k_hj_clear(HjTable t, uint64_t slots) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; i < slots || i < kHjFiltWords;
+         i += stride) {
+        if (i < 64) t.flags[i] = i == 3 ? 0xFFFFFFFFu : 0u;
+        if (i < kHjFiltWords) t.filt[i] = 0ull;
+        if (i < slots) {
+            t.ht.keys[i] = 0ull;
+            t.cnt[i] = 0u;
+            t.first[i] = 0xFFFFFFFFu;
+            t.cur[i] = 0u;
+        }
+    }
+}
+
+// Sets the filter bit of every build key (the filter is zeroed before).  Only the probe variant with
+// the filter reads it, and that one runs when some sampled probe row found no key (flags[6]).
 __global__ __launch_bounds__(kBlock) void k_hj_filter(BuildArgs a) {
+    if (a.t.flags[6] == 0u) return;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
     uint32_t *w = reinterpret_cast<uint32_t *>(a.t.filt);
     for (uint64_t j = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; j < a.n; j += stride) {
@@ -675,13 +694,12 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     t.dmap = reinterpret_cast<uint32_t *>(ws_region(ws, o_dmap));
     t.map_cap = static_cast<uint32_t>(map_cap);
     t.filt = reinterpret_cast<uint64_t *>(ws_region(ws, o_filt));
-    if ((st = fill_u32(t.flags, 64, 0, s)) != AHEAD_OK) return st;
-    if ((st = fill_u64(t.filt, kHjFiltWords, 0, s)) != AHEAD_OK) return st;
-    if ((st = fill_u32(t.flags + 3, 1, 0xFFFFFFFFu, s)) != AHEAD_OK) return st;  // smallest key
-    if ((st = fill_u64(reinterpret_cast<uint64_t *>(t.ht.keys), slots, 0, s)) != AHEAD_OK) return st;
-    if ((st = fill_u32(t.cnt, slots, 0, s)) != AHEAD_OK) return st;
-    if ((st = fill_u32(t.first, slots, 0xFFFFFFFFu, s)) != AHEAD_OK) return st;
-    if ((st = fill_u32(t.cur, slots, 0, s)) != AHEAD_OK) return st;
+    {
+        const uint64_t span = slots > kHjFiltWords ? slots : kHjFiltWords;
+        const uint64_t blocks = (span + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(k_hj_clear, dim3(static_cast<uint32_t>(blocks < 4096 ? blocks : 4096)), dim3(kBlock), 0, s,
+                           t, slots);
+    }
 
     BuildArgs b{};
     b.t = t;
@@ -696,7 +714,6 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
         hipLaunchKernelGGL(k_hj_dmap_clear, dim3(512), dim3(kBlock), 0, s, t);
         hipLaunchKernelGGL(k_hj_dmap_fill, grid, dim3(kBlock), 0, s, b);
     }
-    hipLaunchKernelGGL(k_hj_filter, grid, dim3(kBlock), 0, s, b);
     AHEAD_HIP_TRY(hipGetLastError());
     OffsetOp off{};
     off.n = slots;
@@ -761,7 +778,10 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
         int lst = AHEAD_OK;
         // the variant with the filter exists for 8-byte keys: the sample picks between the two
         const bool pair = lean_kw == 8;
-        if (pair) hipLaunchKernelGGL(k_hj_sample, dim3(kHjSamples / kBlock), dim3(kBlock), 0, s, p, np);
+        if (pair) {  // the sample, then the filter (built only when the sample found a missing key)
+            hipLaunchKernelGGL(k_hj_sample, dim3(kHjSamples / kBlock), dim3(kBlock), 0, s, p, np);
+            hipLaunchKernelGGL(k_hj_filter, grid, dim3(kBlock), 0, s, b);
+        }
         lean_shape(lean_kw, !ppay.data, !bpay.data, [&](auto proto) {
             if (lst != AHEAD_OK) return;
             auto q = proto;
