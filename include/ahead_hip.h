@@ -215,7 +215,8 @@ int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, const ahead_
 /* ---- grouping and aggregation -----------------------------------------------------------------
  * groupbyAN unary (OperatorsAN.hpp:108-113; groupbyAN.tcc:51-94): voidtorgid[i] = gid * a_oid
  * (n entries), vgidtoroid[g] = head(first row of g) * a_oid, ids by first occurrence.
- * table_bits: log2 of the dictionary slots (0 = automatic, up to 2^22).
+ * table_bits: log2 of the dictionary slots (0 = automatic: twice the rows, at most 2^22 slots
+ * and at most twice the number of keys the key's width allows).
  * d_counts[0..5] = n, #groups, #e_head, #e_tail, 0, status (0 ok, 1 = dictionary overflow:
  * call again with a larger table_bits).                                                      */
 int ahead_hip_groupby_an(const ahead_col *head, const ahead_col *tail, uint64_t a_oid,
