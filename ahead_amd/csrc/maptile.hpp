@@ -18,8 +18,9 @@
 //  k_map_fix  per wave: the error positions (from the recorded rows, or by re-evaluating the range
 //             when they did not fit), and the move of the staged entries to their final place
 //             (from the side buffer where a neighbour overwrote them).  When more than kMaxShift
-//             rows were dropped the waves instead re-evaluate their range and write every entry
-//             at its final place.
+//             rows were dropped the waves instead copy their staged entries to their final places
+//             in a scratch array (no staged entry is overwritten before it is read) ...
+//  k_map_back ... which this launch copies back over the main outputs (only then).
 //
 // Op interface: that of rowtile.hpp (kMulti must be false), plus, for kMain operators,
 //   static constexpr int kOuts;                                  // main output arrays (1 or 2)
@@ -34,7 +35,7 @@
 namespace ahead_dev {
 
 constexpr uint32_t kMapErrList = 16;  // error rows a wave records in row order
-constexpr uint32_t kMaxShift = 64;    // dropped rows the move path handles (else: rewrite)
+constexpr uint32_t kMaxShift = 64;    // dropped rows the move path handles (else: compaction through scratch)
 constexpr int kMapG = 4;              // rows per lane per iteration (unless the operator sets kMapRows)
 
 struct MapArgs {
@@ -47,6 +48,7 @@ struct MapArgs {
     uint64_t *side;    // [waves][kMaxShift] staged entries saved before a move
     uint32_t *flags;   // [0] dropped rows in all (k_map_scan)
     uint64_t *totals;  // [NS + 1] per stream (k_map_scan)
+    uint8_t *scratch[2];  // kMain: per main output array, n entries of 8 bytes (the compaction)
 };
 
 template <class Op>
@@ -142,9 +144,6 @@ __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
     const uint64_t begin = static_cast<uint64_t>(gw) * m.rows_per_wave;
     const uint64_t end = begin + m.rows_per_wave < op.n ? begin + m.rows_per_wave : op.n;
     uint32_t kept = 0, err_rows = 0;  // wave-uniform
-    // A wave that alone drops more than kMaxShift rows knows the entries will be written again (the
-    // rewrite path): it stops staging them.
-    uint32_t dropped = 0;              // wave-uniform
     uint32_t ecnt[Op::kErr > 0 ? Op::kErr : 1] = {};
     uint64_t acc = 0;
     for (uint64_t seg = begin; seg < end; seg += 64u * G) {
@@ -166,9 +165,8 @@ __global__ __launch_bounds__(kBlock) void k_map(Op op, MapArgs m) {
         for (int g = 0; g < G; ++g) {
             if constexpr (Op::kMain) {
                 const uint64_t b = __ballot(cnt[g] != 0u);
-                if (cnt[g] != 0u && dropped <= kMaxShift) op.emit_main(rows[g], st[g], begin + kept + map_below(b), 1u);
+                if (cnt[g] != 0u) op.emit_main(rows[g], st[g], begin + kept + map_below(b), 1u);
                 kept += static_cast<uint32_t>(__popcll(b));
-                dropped += static_cast<uint32_t>(__popcll(__ballot(live[g] && cnt[g] == 0u)));
             }
 #pragma unroll
             for (int e = 0; e < Op::kErr; ++e) ecnt[e] += (errs[g] >> e) & 1u;
@@ -292,18 +290,12 @@ __global__ __launch_bounds__(kBlock) void k_map_save(Op op, MapArgs m) {
     }
 }
 
-// Error positions, then the move (or the rewrite) of the staged entries.
+// Error positions, then the move (or the compaction) of the staged entries.
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_map_fix(Op op, MapArgs m) {
     constexpr int NS = MapTraits<Op>::NS;
-    constexpr int G = MapG<Op>::value;
-    __shared__ uint64_t s_filt[FilterWords<Op>::value];
     if constexpr (has_skip<Op>::value) {
         if (op.skip()) return;
-    }
-    if constexpr (has_filter<Op>::value) {  // before any wave leaves
-        op.filter_init(s_filt);
-        __syncthreads();
     }
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
@@ -311,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_map_fix(Op op, MapArgs m) {
     const uint64_t begin = static_cast<uint64_t>(gw) * m.rows_per_wave;
     const uint64_t end = begin + m.rows_per_wave < op.n ? begin + m.rows_per_wave : op.n;
     const uint32_t total_drops = map_total_drops<Op>(m);
-    const bool rewrite = total_drops > kMaxShift;
+    const bool compact = total_drops > kMaxShift;
     if (gw == 0 && lane == 0) op.finish(m.totals);
 
     // error positions
@@ -351,26 +343,30 @@ __global__ __launch_bounds__(kBlock) void k_map_fix(Op op, MapArgs m) {
         if (total_drops == 0u) return;
         const uint32_t d = map_drops_before<Op>(m, gw);
         const uint64_t first = m.wpre[gw];  // final position of the wave's first entry
-        if (rewrite) {
-            // too many drops to move: evaluate the range again and write every entry in place
-            uint32_t kept = 0;
-            uint64_t acc = 0;
-            for (uint64_t seg = begin; seg < end; seg += 64u * G) {
-                uint64_t rows[G];
-                bool live[G];
+        if (compact) {
+            // too many drops to move in place: the staged entries [begin, begin + kept) go to
+            // scratch[first, first + kept); k_map_back copies the scratch over the outputs
+            const uint32_t kept = m.wcnt[gw];
+            void *p[2];
+            uint32_t w[2];
+            op.out_arrays(p, w);
+            constexpr int kPer = 8;
+            for (uint32_t k0 = 0; k0 < kept; k0 += 64u * kPer) {
 #pragma unroll
-                for (int g = 0; g < G; ++g) {
-                    rows[g] = seg + static_cast<uint64_t>(g) * 64u + lane;
-                    live[g] = rows[g] < end;
-                }
-                typename Op::State st[G];
-                uint32_t errs[G], cnt[G];
-                map_eval(op, rows, live, st, errs, cnt, acc, s_filt);
+                for (int a = 0; a < Op::kOuts; ++a) {
+                    uint64_t idx[kPer], v[kPer];
+                    bool in[kPer];
 #pragma unroll
-                for (int g = 0; g < G; ++g) {
-                    const uint64_t b = __ballot(cnt[g] != 0u);
-                    if (cnt[g] != 0u) op.emit_main(rows[g], st[g], first + kept + map_below(b), 1u);
-                    kept += static_cast<uint32_t>(__popcll(b));
+                    for (int q = 0; q < kPer; ++q) {
+                        const uint32_t k = k0 + q * 64u + lane;
+                        in[q] = k < kept;
+                        idx[q] = begin + k;
+                    }
+                    load_rows<kPer>(p[a], w[a], idx, in, v);
+#pragma unroll
+                    for (int q = 0; q < kPer; ++q) {
+                        if (in[q]) store_w(m.scratch[a], first + k0 + q * 64u + lane, w[a], v[q]);
+                    }
                 }
             }
             return;
@@ -420,9 +416,40 @@ __global__ __launch_bounds__(kBlock) void k_map_fix(Op op, MapArgs m) {
     }
 }
 
+// After a compaction (more than kMaxShift rows dropped): the main entries [0, kept) from the
+// scratch arrays back over the outputs.
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_map_back(Op op, MapArgs m) {
+    if constexpr (has_skip<Op>::value) {
+        if (op.skip()) return;
+    }
+    if constexpr (Op::kMain) {
+        if (map_total_drops<Op>(m) <= kMaxShift) return;
+        const uint64_t kept = m.totals[0];
+        void *p[2];
+        uint32_t w[2];
+        op.out_arrays(p, w);
+        const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+        for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < kept; i += stride) {
+#pragma unroll
+            for (int a = 0; a < Op::kOuts; ++a) store_w(p[a], i, w[a], load_w(m.scratch[a], i, w[a]));
+        }
+    }
+}
+
 }  // namespace ahead_dev
 
 namespace ahead_host {
+
+// Main output arrays of an operator (0 without a main output).
+template <class Op>
+constexpr int map_outs() {
+    if constexpr (Op::kMain) {
+        return Op::kOuts;
+    } else {
+        return 0;
+    }
+}
 
 // Persistent grid of the map engine: as many workgroups as can be resident (registers decide),
 // at most one wave per 64 rows.
@@ -451,8 +478,9 @@ size_t map_scratch_bytes(uint64_t n) {
     using namespace ahead_dev;
     constexpr int NS = MapTraits<Op>::NS;
     const size_t waves = static_cast<size_t>(map_grid<Op>(n)) * kWaves;
+    const size_t compact = map_outs<Op>() * align_up(static_cast<size_t>(n) * 8, 256);
     return 2 * align_up((NS + 1) * waves * 4, 256) + align_up(waves * 4, 256) + align_up(waves * kMapErrList * 8, 256) +
-           align_up(waves * 2 * kMaxShift * 8, 256) + 256;
+           align_up(waves * 2 * kMaxShift * 8, 256) + 256 + compact;
 }
 
 // Launches the map engine for op over op.n rows.  Scratch: per-wave tables in the workspace region
@@ -472,7 +500,9 @@ int launch_map(const Op &op, ahead_hip_ws *ws, size_t offset, hipStream_t s) {
                  o_elist = o_erows + align_up(static_cast<size_t>(waves) * 4, 256),
                  o_side = o_elist + align_up(static_cast<size_t>(waves) * kMapErrList * 8, 256),
                  o_flags = o_side + align_up(static_cast<size_t>(waves) * 2 * kMaxShift * 8, 256),
-                 o_end = o_flags + 256;  // flags, then totals
+                 o_scratch = o_flags + 256,  // flags, then totals; then the compaction's scratch arrays
+                 scratch_each = Op::kMain ? align_up(static_cast<size_t>(op.n) * 8, 256) : 0,
+                 o_end = o_scratch + map_outs<Op>() * scratch_each;
     int st;
     if (offset > 0 && ws->bytes < kHeaderBytes + o_end)
         return fail(AHEAD_ERR_INVALID, "map engine: the workspace was not grown before its regions were placed");
@@ -487,11 +517,14 @@ int launch_map(const Op &op, ahead_hip_ws *ws, size_t offset, hipStream_t s) {
     m.side = reinterpret_cast<uint64_t *>(ws_region(ws, o_side));
     m.flags = reinterpret_cast<uint32_t *>(ws_region(ws, o_flags));
     m.totals = reinterpret_cast<uint64_t *>(ws_region(ws, o_flags + 64));
+    m.scratch[0] = Op::kMain ? reinterpret_cast<uint8_t *>(ws_region(ws, o_scratch)) : nullptr;
+    m.scratch[1] = map_outs<Op>() > 1 ? reinterpret_cast<uint8_t *>(ws_region(ws, o_scratch + scratch_each)) : nullptr;
     static_assert(64 + 8 * (NS + 1) <= 256, "totals fit the flags block");
     hipLaunchKernelGGL(k_map<Op>, dim3(grid), dim3(kBlock), 0, s, op, m);
     hipLaunchKernelGGL(k_map_scan<Op>, dim3(NS + 1), dim3(1024), 0, s, op, m);
     if constexpr (Op::kMain) hipLaunchKernelGGL(k_map_save<Op>, dim3(grid), dim3(kBlock), 0, s, op, m);
     hipLaunchKernelGGL(k_map_fix<Op>, dim3(grid), dim3(kBlock), 0, s, op, m);
+    if constexpr (Op::kMain) hipLaunchKernelGGL(k_map_back<Op>, dim3(grid), dim3(kBlock), 0, s, op, m);
     AHEAD_HIP_TRY(hipGetLastError());
     return AHEAD_OK;
 }

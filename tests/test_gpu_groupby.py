@@ -120,3 +120,22 @@ def test_groupby_many_new_keys_per_wave():
         ref = O.groupby_an(O.void(n), O.Col(codes, meta))
         got = capi.groupby_an(capi.Column.void(n), gcol(codes, meta), err_cap=2)
         same(got, ref)
+
+
+@pytest.mark.parametrize("card", [25, 300])
+def test_groupby_dictionary_overflow_retry(card):
+    """A dictionary of 16 slots cannot hold the keys: the call reports status 1 (overflow) and the
+    wrapper calls again with a larger table; the second result is the reference's."""
+    n = 50_003
+    vals = datagen.uniform(n, 500 + card, 1, 0, card - 1)
+    codes = datagen.encode(vals, 233, 4)
+    datagen.inject_flips(codes, 500 + card, 2, datagen.flip_threshold(1e-3))
+    meta = an.an_meta(an.RESSHORT, 233)
+    ref = O.groupby_an(O.void(n), O.Col(codes, meta))
+    got = capi.groupby_an(capi.Column.void(n), gcol(codes, meta), err_cap=4, table_bits=4)
+    same(got, ref)
+    ng = ref[1].size
+    ref2 = O.groupby2_an(O.void(n), O.Col(codes, meta), O.Col(ref[0], RESOID), ng, an.RESSHORT.larger_bits)
+    got2 = capi.groupby2_an(capi.Column.void(n), gcol(codes, meta), gcol(ref[0], RESOID), ng,
+                            an.RESSHORT.larger_bits, err_cap=4, table_bits=4)
+    same(got2, ref2)
