@@ -435,6 +435,9 @@ struct ProbeLeanOp : LeanFilter<FILT> {
     static constexpr int kErr = 2;  // probe head check, probe tail check
     static constexpr bool kMulti = false, kMain = true, kAcc = false, kBatched = true;
     static constexpr int kOuts = 2;
+    // 8 rows per lane and step: half the steps, so half the waits of a step's loads for the
+    // previous step's stores (one counter covers both)
+    static constexpr int kMapRows = 8;
     using State = ProbeState;
     using KT = std::conditional_t<KW == 8, uint64_t, uint32_t>;
     uint64_t n;
