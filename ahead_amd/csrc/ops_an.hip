@@ -34,6 +34,7 @@ struct FetchOp {
 
     static constexpr bool kBatched = true;
     static constexpr int kOuts = 1;
+    static constexpr int kMapRows = 8;  // half the steps whose loads wait for the previous step's stores
     __device__ void out_arrays(void *(&p)[2], uint32_t (&w)[2]) const {
         p[0] = out;
         w[0] = ow;
@@ -128,6 +129,7 @@ struct MatchPosOp {
 
     static constexpr bool kBatched = true;
     static constexpr int kOuts = 2;
+    static constexpr int kMapRows = 8;  // half the steps whose loads wait for the previous step's stores
     __device__ void out_arrays(void *(&p)[2], uint32_t (&w)[2]) const {
         p[0] = out_h;
         p[1] = out_t;
@@ -206,6 +208,7 @@ struct MatchPosOp {
 struct ArithOp {
     static constexpr int kErr = 4;  // head1, tail1, head2, tail2
     static constexpr bool kMulti = false, kMain = false, kAcc = false;
+    static constexpr int kMapRows = 8;  // half the steps whose loads wait for the previous step's stores
     using State = uint8_t;
     uint64_t n;
     DCol h1, t1, h2, t2;
