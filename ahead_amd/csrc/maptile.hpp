@@ -429,10 +429,22 @@ __global__ __launch_bounds__(kBlock) void k_map_back(Op op, MapArgs m) {
         void *p[2];
         uint32_t w[2];
         op.out_arrays(p, w);
+        const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
         const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
-        for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < kept; i += stride) {
 #pragma unroll
-            for (int a = 0; a < Op::kOuts; ++a) store_w(p[a], i, w[a], load_w(m.scratch[a], i, w[a]));
+        for (int a = 0; a < Op::kOuts; ++a) {
+            // the first kept * w bytes: 16 per thread and step where the output is 16-byte aligned
+            // (the scratch always is), the tail and unaligned outputs entry by entry
+            const uint64_t bytes = kept * w[a];
+            uint64_t done = 0;
+            if ((reinterpret_cast<uintptr_t>(p[a]) & 15u) == 0u) {
+                const uint64_t n16 = bytes / 16u;
+                const uint4 *src = reinterpret_cast<const uint4 *>(m.scratch[a]);
+                uint4 *dst = reinterpret_cast<uint4 *>(p[a]);
+                for (uint64_t i = t0; i < n16; i += stride) dst[i] = src[i];
+                done = n16 * 16u / w[a];
+            }
+            for (uint64_t i = done + t0; i < kept; i += stride) store_w(p[a], i, w[a], load_w(m.scratch[a], i, w[a]));
         }
     }
 }
