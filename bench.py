@@ -629,6 +629,7 @@ def run_serve(args):
     sat = {}
     for w, srv in servers.items():
         sat[w] = serve.summarize(srv.run(serve.random_queries(n_q, 0, 7), realtime=False))
+        print(f"serve: {w} slot(s), all at once: {sat[w]['qps']:.1f} queries/s", file=sys.stderr, flush=True)
     # Poisson arrivals at fractions of the unbatched capacity
     sweep = []
     base = sat[1]["qps"]
@@ -637,6 +638,8 @@ def run_serve(args):
         row = {"rate_qps": rate}
         for w, srv in servers.items():
             row[f"workers{w}"] = serve.summarize(srv.run(serve.random_queries(n_q, rate, 11), realtime=True))
+            print(f"serve: {w} slot(s), Poisson {rate:.1f} queries/s: p50 {row[f'workers{w}']['p50_ms']:.0f} ms",
+                  file=sys.stderr, flush=True)
         sweep.append(row)
     t0 = time.perf_counter()
     final = servers[4].run(serve.random_queries(n_q, 0, 13), realtime=False)
