@@ -265,6 +265,14 @@ struct ProbeState {
     uint32_t brow;  // the build row (keys with one row)
 };
 
+// The lean probe's row: the build row and, for 8-byte build payloads, the payload code itself,
+// gathered while the row is evaluated (before any store of the step).
+struct LeanState {
+    uint64_t pay;   // probe-side payload (raw)
+    uint64_t bval;  // build-side payload code (8-byte build payloads)
+    uint32_t brow;  // the build row
+};
+
 struct ProbeOp {
     static constexpr int kErr = 2;  // probe head check, probe tail check
     static constexpr bool kMulti = true, kMain = true, kAcc = false;
@@ -438,7 +446,7 @@ struct ProbeLeanOp : LeanFilter<FILT> {
     // 8 rows per lane and step: half the steps, so half the waits of a step's loads for the
     // previous step's stores (one counter covers both)
     static constexpr int kMapRows = 8;
-    using State = ProbeState;
+    using State = LeanState;
     using KT = std::conditional_t<KW == 8, uint64_t, uint32_t>;
     uint64_t n;
     HjTable t;
@@ -538,18 +546,25 @@ struct ProbeLeanOp : LeanFilter<FILT> {
                     cnt[g] = 1;
                 }
             }
-            return;
+        } else {
+            uint32_t sl[G];
+            t.ht.find_n<G>(k, ok, sl);
+            uint32_t f[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) f[g] = t.first[sl[g] != UINT32_MAX ? sl[g] : 0u];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (sl[g] != UINT32_MAX) {
+                    st[g].brow = f[g];
+                    cnt[g] = 1;
+                }
+            }
         }
-        uint32_t sl[G];
-        t.ht.find_n<G>(k, ok, sl);
-        uint32_t f[G];
+        if constexpr (!BV) {  // the build payloads of the joined rows, all before the step's stores
 #pragma unroll
-        for (int g = 0; g < G; ++g) f[g] = t.first[sl[g] != UINT32_MAX ? sl[g] : 0u];
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            if (sl[g] != UINT32_MAX) {
-                st[g].brow = f[g];
-                cnt[g] = 1;
+            for (int g = 0; g < G; ++g) {
+                st[g].bval = 0;
+                if (cnt[g] != 0u) st[g].bval = bpay[st[g].brow];
             }
         }
     }
@@ -565,7 +580,7 @@ struct ProbeLeanOp : LeanFilter<FILT> {
     }
     __device__ void emit_main(uint64_t, const State &st, uint64_t pos, uint32_t) const {
         if (pos >= out_cap) return;
-        const uint64_t b = BV ? static_cast<uint64_t>(static_cast<uint32_t>(bpay_seq + st.brow)) : bpay[st.brow];
+        const uint64_t b = BV ? static_cast<uint64_t>(static_cast<uint32_t>(bpay_seq + st.brow)) : st.bval;
         const uint64_t pv = reenc_p ? st.pay * f_p : st.pay, bv = reenc_b ? b * f_b : b;
         if (ow == 8) {
             static_cast<uint64_t *>(out_p)[pos] = pv;
