@@ -874,20 +874,25 @@ __global__ __launch_bounds__(kP2Block) void k_star_probe2(Probe2Args a) {
     uint32_t ecnt[4] = {0, 0, 0, 0};    // wave-uniform: rows with each error kind (counted from ballots)
     uint32_t *wfirst = s_first[wave];
 
+    // The codes of one segment: 4 chunks of 128 rows, 2 rows per lane.  A full segment finds its
+    // chunks already loaded and, as it finishes with chunk c, loads chunk c of the next full
+    // segment into the same registers, so the wave's next loads are in flight while it computes.
+    uint64_t fk[4][2], mv[4][2];
+    auto load_chunk = [&](uint64_t seg, int c) {
+        const uint64_t r0 = seg + c * 128u + lane * 2u;
+        const u32x4 q = load16_nt(a.fk + r0);
+        const u32x4 w = load16_nt(a.m + r0);
+        __builtin_memcpy(&fk[c][0], &q, 16);
+        __builtin_memcpy(&mv[c][0], &w, 16);
+    };
     // one segment of kP2Seg rows; FULL: every row is in range (all but a wave's last segment), so
-    // the row-range tests compile away
-    auto segment = [&](uint64_t seg, auto full_tag) {
+    // the row-range tests compile away; NEXT (full segments only): the next segment is full too
+    auto segment = [&](uint64_t seg, auto full_tag, bool next) {
         constexpr bool full = decltype(full_tag)::value;
-        uint64_t fk[4][2], mv[4][2];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const uint64_t r0 = seg + c * 128u + lane * 2u;
-            if (full) {
-                const u32x4 q = load16_nt(a.fk + r0);
-                const u32x4 w = load16_nt(a.m + r0);
-                __builtin_memcpy(&fk[c][0], &q, 16);
-                __builtin_memcpy(&mv[c][0], &w, 16);
-            } else {
+            if (!full) {
 #pragma unroll
                 for (int k = 0; k < 2; ++k) {
                     fk[c][k] = r0 + k < end ? a.fk[r0 + k] : 0ull;
@@ -995,11 +1000,16 @@ __global__ __launch_bounds__(kP2Block) void k_star_probe2(Probe2Args a) {
                 }
                 err_rows += static_cast<uint32_t>(__popcll(x0) + __popcll(x1));
             }
+            if (full && next) load_chunk(seg + kP2Seg, c);
         }
     };
     uint64_t seg = begin;
-    for (; seg + kP2Seg <= end; seg += kP2Seg) segment(seg, std::true_type{});
-    if (seg < end) segment(seg, std::false_type{});
+    if (seg + kP2Seg <= end) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) load_chunk(seg, c);
+    }
+    for (; seg + kP2Seg <= end; seg += kP2Seg) segment(seg, std::true_type{}, seg + 2 * kP2Seg <= end);
+    if (seg < end) segment(seg, std::false_type{}, false);
     // per-wave counts
     const uint32_t waves = gridDim.x * kP2Waves;
     if (lane == 0) {
