@@ -18,7 +18,8 @@
 //                 (attr1, attr2) combinations of the valid rows;
 //  k_star_cid     numbers the marked combinations densely (combination id, cid);
 //  k_star_info    stores per dimension row its cid and its two attribute-error bits.
-//  k_star_range / k_star_map_clear / k_star_map_fill
+//  k_star_reset   clears the build's workspace (flags, tables, bitmap, infos, key map) in one launch.
+//  k_star_range / k_star_map_fill
 //                 when the valid keys span a range of at most 4 x |dim| + 1024 values, a direct
 //                 map of 16-bit entries (attr-error bits | cid + 1) indexed by key - kmin, so a
 //                 fact row costs one 2-byte gather; keys outside the range use the table.
@@ -147,22 +148,14 @@ __global__ __launch_bounds__(kBlock) void k_star_range(DimArgs a) {
 }
 
 // Decides whether the valid keys form a range the direct map can hold and clears that range.
-__global__ __launch_bounds__(kBlock) void k_star_map_clear(DimArgs a) {
+// The direct key map: applicable (flags[4]) when the valid keys span at most map_cap values and the
+// combinations fit 14-bit entries.  The map starts zeroed (k_star_reset).
+__global__ __launch_bounds__(kBlock) void k_star_map_fill(DimArgs a) {
     const uint32_t lo = a.flags[2], hi = a.flags[3];
     const bool dense = lo <= hi && static_cast<uint64_t>(hi) - lo < a.map_cap && a.flags[1] < 0x3FFEu;
     if (blockIdx.x == 0 && threadIdx.x == 0) a.flags[4] = dense ? 1u : 0u;
     if (!dense) return;
     const uint32_t range = hi - lo + 1u;
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < range; i += stride) a.map[i] = 0;
-}
-
-// map[key - kmin] = attr1 error << 15 | attr2 error << 14 | (cid + 1, or 0x3FFF if an attribute is
-// corrupt); 0 = no dimension row under that key.  Every dimension row whose truncated key lies in
-// the range enters, corrupt keys included (they are build entries of the reference's map too).
-__global__ __launch_bounds__(kBlock) void k_star_map_fill(DimArgs a) {
-    if (a.flags[4] == 0u) return;
-    const uint32_t lo = a.flags[2], range = a.flags[3] - lo + 1u;
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < a.n; j += stride) {
         const uint32_t k = static_cast<uint32_t>((a.key[j] * a.key_inv) & a.key_mask);
@@ -1184,9 +1177,13 @@ __global__ __launch_bounds__(1024) void k_star_final(FinalArgs a) {
     }
 }
 
-__global__ void k_clear_u64(uint64_t *p, uint64_t n) {
+// The build's starting state in one launch: the workspace from its start (flags, group tables,
+// dimension hash table, combination bitmap, combination ids, row infos, direct key map) is
+// zeroed, except flags[2] (the smallest key seen), which starts at 0xFFFFFFFF.
+__global__ void k_star_reset(uint64_t *ws, uint64_t words) {
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) p[i] = 0ull;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < words; i += stride)
+        ws[i] = i == 1u ? 0xFFFFFFFFull : 0ull;  // word 1 = flags[2] | flags[3] << 32
 }
 
 }  // namespace
@@ -1328,11 +1325,11 @@ extern "C" int ahead_hip_star1_build(const ahead_star1_args *in, const ahead_sta
     b.table = reinterpret_cast<uint64_t *>(ws_region(ws, L.off_table));
     b.table_bits = L.tbits;
     b.flags = reinterpret_cast<uint32_t *>(ws_region(ws, L.off_flags));
-    hipLaunchKernelGGL(k_clear_u64, dim3(256), dim3(256), 0, s, b.table, 1ull << L.tbits);
-    hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(32), 0, s, reinterpret_cast<uint64_t *>(b.flags), 32ull);
-    // the group tables start empty; every finalize leaves them empty again
-    hipLaunchKernelGGL(k_clear_u64, dim3(256), dim3(256), 0, s, reinterpret_cast<uint64_t *>(ws_region(ws, L.off_gsum)),
-                       (L.off_table - L.off_gsum) / 8);
+    // flags, the group tables, the hash table, the combination bitmap and ids, the row infos and
+    // the direct key map lie one after another from the workspace's start: one launch clears them
+    // all and sets the smallest-key flag
+    hipLaunchKernelGGL(k_star_reset, dim3(256), dim3(256), 0, s, reinterpret_cast<uint64_t *>(ws_region(ws, 0)),
+                       align_up(L.off_map + L.map_cap * 2ull, 8) / 8);
     AHEAD_HIP_TRY(hipGetLastError());
     DimArgs d{};
     d.a1 = static_cast<const uint16_t *>(in->dim_a1.data);
@@ -1353,8 +1350,6 @@ extern "C" int ahead_hip_star1_build(const ahead_star1_args *in, const ahead_sta
     d.key_mask = b.key_mask;
     d.map = reinterpret_cast<uint16_t *>(ws_region(ws, L.off_map));
     d.map_cap = L.map_cap;
-    hipLaunchKernelGGL(k_clear_u64, dim3(8), dim3(256), 0, s, reinterpret_cast<uint64_t *>(d.present), kCombos / 64ull);
-    if ((st = fill_u32(b.flags + 2, 1, 0xFFFFFFFFu, s)) != AHEAD_OK) return st;  // kmin
     const uint32_t g = b.n ? (b.n + kBlock - 1) / kBlock : 1;
     const dim3 gd(g < 4096 ? g : 4096);
     if (b.n) {
@@ -1366,7 +1361,6 @@ extern "C" int ahead_hip_star1_build(const ahead_star1_args *in, const ahead_sta
         hipLaunchKernelGGL(k_star_info, gd, dim3(kBlock), 0, s, d);
         hipLaunchKernelGGL(k_star_range, dim3(g < 128 ? g : 128), dim3(kBlock), 0, s, d);
     }
-    hipLaunchKernelGGL(k_star_map_clear, dim3(512), dim3(kBlock), 0, s, d);
     if (b.n) hipLaunchKernelGGL(k_star_map_fill, gd, dim3(kBlock), 0, s, d);
     PackArgs pk{};
     pk.map = d.map;
