@@ -668,7 +668,8 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     const ahead_col &phead = right ? *h1 : *h2, &ptail = right ? *t1 : *t2;
     const ahead_col &pkey = right ? *t1 : *h2;
     const uint64_t nb = bkey.n, np = phead.n;
-    if (nb > 0x7FFFFFFFull) return fail(AHEAD_ERR_TOO_LARGE, "hashjoin: build side above 2^31 rows");
+    // the table holds 2^bits >= 2 x build rows slots with bits <= 31 (32-bit slot arithmetic)
+    if (nb > (1ull << 30)) return fail(AHEAD_ERR_TOO_LARGE, "hashjoin: build side above 2^30 rows");
 
     uint32_t bits = 4;
     while ((1ull << bits) < 2ull * nb) ++bits;

@@ -21,6 +21,8 @@ import numpy as np
 
 from . import an
 
+MAXLEN_STRING = 64
+
 # header type name -> (column type, bytes per value); None marks types this loader does not load
 _TYPES = {
     "INTEGER": (an.INT, 4), "INT": (an.INT, 4), "TINYINT": (an.TINYINT, 1), "SHORTINT": (an.SHORTINT, 2),
@@ -42,7 +44,9 @@ def _type_of(name: str):
     if name.startswith("INT"):
         return _TYPES["INT"]
     if name.startswith("STRING"):
-        n = int(name[7:]) if len(name) > 6 and name[6] == ":" else 0
+        # STRING[:len]: len + 1 bytes per row, len defaults to MAXLEN_STRING = 64
+        # (Transaction.cpp:249-255, MetaRepositoryManager.cpp:41)
+        n = int(name[7:]) if len(name) > 6 and name[6] == ":" else MAXLEN_STRING
         return (None, n + 1)
     raise ValueError(f"data type {name} in header unknown")
 
@@ -94,6 +98,9 @@ def load_table(prefix: str | Path, columns: list[str] | None = None, min_bfw: di
     for name, typ, t in read_header(prefix):
         if columns is not None and name not in columns:
             continue
+        if t is not None and t.name == "resbigint":
+            # Transaction.cpp:378-381: RESBIGINT columns are "currently unsupported for load"
+            raise ValueError(f"column {name}: data type {typ} in header currently unsupported for load")
         if t is None:
             if columns is not None:
                 raise ValueError(f"column {name} has type {typ}, which the hardened integer path does not load")

@@ -61,3 +61,23 @@ def test_string_columns_are_written_but_not_loaded(tmp_path):
     assert (tmp_path / "t_name.ahead").stat().st_size == 6 * n
     cols = {c[0]: c[2] for c in loader.read_header(tmp_path / "t")}
     assert cols["name"] is None and cols["k"] is an.RESINT
+
+
+def test_bare_string_defaults_to_maxlen_64(tmp_path):
+    # Transaction.cpp:249-255: STRING without ":len" takes MAXLEN_STRING = 64 (+1 terminator) bytes per row
+    n = 7
+    s = np.zeros(65 * n, dtype=np.uint8)
+    loader.write_table(tmp_path / "t", {"name": ("STRING", s), "k": ("RESINT", np.arange(n, dtype=np.uint64) * 65117)})
+    assert (tmp_path / "t_name.ahead").stat().st_size == 65 * n
+    assert loader._type_of("STRING") == (None, 65)
+    assert loader._type_of("STRING:25") == (None, 26)
+
+
+def test_resbigint_is_refused_at_load(tmp_path):
+    # Transaction.cpp:378-381: RESBIGINT columns are listed by the header but not loaded
+    n = 4
+    loader.write_table(tmp_path / "t", {"v": ("RESBIGINT", np.arange(n, dtype=np.uint64) * 65117)})
+    cols = loader.read_header(tmp_path / "t")
+    assert cols[0][2] is an.RESBIGINT
+    with pytest.raises(ValueError, match="unsupported for load"):
+        loader.load_table(tmp_path / "t")
