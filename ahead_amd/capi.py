@@ -88,6 +88,8 @@ _SIGS = {
     "ahead_hip_get_tiling": ([ctypes.POINTER(_u32), ctypes.POINTER(_u32)], _int),
     "ahead_hip_select_an": ([_colp, _int, _u64, _int, _u64, _int, _u64, _u64, _vp, _u64, _vp, _u64,
                              _vp, _vp, _vp, _vp], _int),
+    "ahead_hip_select_an_noalloc": ([_colp, _int, _u64, _int, _u64, _int, _u64, _u64, _vp, _u64, _vp, _u64,
+                                     _vp, _vp, _vp], _int),
     "ahead_hip_encode_an": ([_colp, _u64, _u32, _vp, _vp], _int),
     "ahead_hip_check_an": ([_colp, _u64, _vp, _u64, _vp, _vp, _vp], _int),
     "ahead_hip_decode_an": ([_colp, _u64, _vp, _vp, _u64, _vp, _vp, _vp], _int),

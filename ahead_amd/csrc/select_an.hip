@@ -21,6 +21,9 @@
 //           re-read to rank their error positions, which are scattered.
 // The fused decoded sum leaves per-tile partials that k_sum_partials adds up; the kernel
 // boundary orders them without any device-wide fence.
+#include <mutex>
+#include <string>
+
 #include "select_tile.hpp"
 
 using namespace ahead_dev;
@@ -87,11 +90,19 @@ void dispatch(const SelectArgs &a, int comb, bool sum, hipStream_t s) {
 
 using namespace ahead_host;
 
-extern "C" int ahead_hip_select_an(const ahead_col *col, int op1, uint64_t th1, int op2, uint64_t th2,
-                                   int comb, uint64_t a_head, uint64_t a_oid, uint64_t *oids,
-                                   uint64_t oids_cap, uint64_t *err, uint64_t err_cap,
-                                   uint64_t *d_counts, uint64_t *d_sum, ahead_hip_ws *ws,
-                                   void *stream) {
+namespace {
+
+// Arguments common to both select entry points, validated; the caller supplies the partial-sum
+// array and the look-back state.
+struct SelectPlan {
+    SelectArgs a{};
+    LbPlan plan{};
+    int comb = 0;
+};
+
+int select_prepare(const ahead_col *col, int op1, uint64_t th1, int op2, uint64_t th2, int comb, uint64_t a_head,
+                   uint64_t a_oid, uint64_t *oids, uint64_t oids_cap, uint64_t *err, uint64_t err_cap,
+                   uint64_t *d_counts, SelectPlan *p) {
     int st = check_col(col, "select.col", false, true);
     if (st != AHEAD_OK) return st;
     if (d_counts == nullptr) return fail(AHEAD_ERR_INVALID, "select: d_counts is NULL");
@@ -108,10 +119,9 @@ extern "C" int ahead_hip_select_an(const ahead_col *col, int op1, uint64_t th1, 
     const uint64_t sub_rows = 4ull * kBlock * (16u / col->width);
     const uint64_t subs = (col->n + sub_rows - 1) / sub_rows;
     const uint32_t iters = subtiles_per_tile(subs);
-    const LbPlan plan = lb_plan(col->n, static_cast<uint32_t>(sub_rows * iters), 2);
-    if ((st = ws_ensure(ws, static_cast<size_t>(plan.num_tiles) * 8)) != AHEAD_OK) return st;
-
-    SelectArgs a{};
+    p->plan = lb_plan(col->n, static_cast<uint32_t>(sub_rows * iters), 2);
+    p->comb = comb;
+    SelectArgs &a = p->a;
     a.codes = col->data;
     a.n = col->n;
     a.seqbase = col->seqbase;
@@ -126,21 +136,112 @@ extern "C" int ahead_hip_select_an(const ahead_col *col, int op1, uint64_t th1, 
     a.err = err;
     a.err_cap = err == nullptr ? 0 : err_cap;
     a.d_counts = d_counts;
-    a.partials = reinterpret_cast<uint64_t *>(ws_region(ws, 0));
     a.iters = iters;
-    if ((st = lb_args(ws, plan, &a.lb)) != AHEAD_OK) return st;
+    return AHEAD_OK;
+}
 
-    hipStream_t s = static_cast<hipStream_t>(stream);
+int select_launch(const SelectPlan &p, uint64_t width, uint64_t *d_sum, hipStream_t s) {
     const bool sum = d_sum != nullptr;
-    switch (col->width) {
-        case 2: dispatch<uint16_t>(a, comb, sum, s); break;
-        case 4: dispatch<uint32_t>(a, comb, sum, s); break;
-        default: dispatch<uint64_t>(a, comb, sum, s); break;
+    switch (width) {
+        case 2: dispatch<uint16_t>(p.a, p.comb, sum, s); break;
+        case 4: dispatch<uint32_t>(p.a, p.comb, sum, s); break;
+        default: dispatch<uint64_t>(p.a, p.comb, sum, s); break;
     }
     AHEAD_HIP_TRY(hipGetLastError());
     if (sum) {
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, a.partials, plan.num_tiles, d_sum);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, p.a.partials, p.plan.num_tiles, d_sum);
         AHEAD_HIP_TRY(hipGetLastError());
     }
     return AHEAD_OK;
+}
+
+}  // namespace
+
+// ---- the allocation-free entry point's scratch ------------------------------------------------
+// One array per GPU the library runs on, held until the process ends: the workspace header (tile
+// ticket), then the per-tile partial sums, then the look-back status words of the two output
+// streams.  Host code never passes `scratch` itself to a kernel: it is a host-side stand-in for the
+// symbol.  Its device address, per device, comes from hipGetSymbolAddress at the first call on that
+// device.
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+struct NoAllocState {
+    uint8_t *ptr = nullptr;  // device address of `scratch` on this device
+    uint32_t epoch = 0;      // last look-back epoch handed out on this device
+};
+NoAllocState g_noalloc[kNoAllocMaxDevices];
+std::mutex g_noalloc_mu;
+
+// Layout of the scratch for a plan of `tiles` look-back tiles: header, partials, status words.
+size_t noalloc_partials_off() { return kHeaderBytes; }
+size_t noalloc_status_off(uint32_t tiles) { return align_up(kHeaderBytes + size_t(tiles) * 8, 256); }
+size_t noalloc_bytes(uint32_t tiles) { return noalloc_status_off(tiles) + size_t(tiles) * 2 * 8; }
+
+}  // namespace
+
+extern "C" int ahead_hip_select_an(const ahead_col *col, int op1, uint64_t th1, int op2, uint64_t th2,
+                                   int comb, uint64_t a_head, uint64_t a_oid, uint64_t *oids,
+                                   uint64_t oids_cap, uint64_t *err, uint64_t err_cap,
+                                   uint64_t *d_counts, uint64_t *d_sum, ahead_hip_ws *ws,
+                                   void *stream) {
+    SelectPlan p;
+    int st = select_prepare(col, op1, th1, op2, th2, comb, a_head, a_oid, oids, oids_cap, err, err_cap, d_counts, &p);
+    if (st != AHEAD_OK) return st;
+    if ((st = ws_ensure(ws, static_cast<size_t>(p.plan.num_tiles) * 8)) != AHEAD_OK) return st;
+    p.a.partials = reinterpret_cast<uint64_t *>(ws_region(ws, 0));
+    if ((st = lb_args(ws, p.plan, &p.a.lb)) != AHEAD_OK) return st;
+    return select_launch(p, col->width, d_sum, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int ahead_hip_select_an_noalloc(const ahead_col *col, int op1, uint64_t th1, int op2, uint64_t th2,
+                                           int comb, uint64_t a_head, uint64_t a_oid, uint64_t *oids,
+                                           uint64_t oids_cap, uint64_t *err, uint64_t err_cap,
+                                           uint64_t *d_counts, uint64_t *d_sum, void *stream) {
+    SelectPlan p;
+    int st = select_prepare(col, op1, th1, op2, th2, comb, a_head, a_oid, oids, oids_cap, err, err_cap, d_counts, &p);
+    if (st != AHEAD_OK) return st;
+    const size_t need = noalloc_bytes(p.plan.num_tiles);
+    if (need > kNoAllocScratchBytes)
+        return fail(AHEAD_ERR_TOO_LARGE, "select_an_noalloc: " + std::to_string(col->n) + " rows need " +
+                                             std::to_string(need) + " bytes of scan state; the static scratch holds " +
+                                             std::to_string(kNoAllocScratchBytes));
+    int dev = 0;
+    AHEAD_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(AHEAD_ERR_INVALID, "select_an_noalloc: device index out of range");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    uint32_t epoch = 0;
+    uint8_t *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        NoAllocState &g = g_noalloc[dev];
+        if (g.ptr == nullptr) {
+            // set-up, once per device with that device current: the array's device address, then
+            // a clear of the whole array (ticket counter and status words must start at zero)
+            void *ptr = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || ptr == nullptr) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+            if ((st = fill_u64(static_cast<uint64_t *>(ptr), kNoAllocScratchBytes / 8, 0, s)) != AHEAD_OK) return st;
+            g.ptr = static_cast<uint8_t *>(ptr);
+            g.epoch = 0;
+        }
+        if (g.epoch + 1 >= (1u << 30)) {
+            // status words of every earlier call carry an epoch below 2^30: clear them (stream-ordered)
+            if ((st = fill_u64(reinterpret_cast<uint64_t *>(g.ptr + kHeaderBytes), (kNoAllocScratchBytes - kHeaderBytes) / 8, 0,
+                               s)) != AHEAD_OK)
+                return st;
+            g.epoch = 0;
+        }
+        epoch = ++g.epoch;
+        base = g.ptr;
+    }
+    p.a.partials = reinterpret_cast<uint64_t *>(base + noalloc_partials_off());
+    p.a.lb.hdr = reinterpret_cast<WsHeader *>(base);
+    p.a.lb.status = reinterpret_cast<uint64_t *>(base + noalloc_status_off(p.plan.num_tiles));
+    p.a.lb.num_tiles = p.plan.num_tiles;
+    p.a.lb.epoch = epoch;
+    return select_launch(p, col->width, d_sum, s);
 }

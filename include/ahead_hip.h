@@ -99,6 +99,20 @@ int ahead_hip_select_an(const ahead_col *col, int op1, uint64_t th1, int op2, ui
                         uint64_t oids_cap, uint64_t *err, uint64_t err_cap, uint64_t *d_counts,
                         uint64_t *d_sum, ahead_hip_ws *ws, void *stream);
 
+/* ---- selectAN without allocation ---------------------------------------------------------------
+ * No counterpart in the reference's API: for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * semantics and results as ahead_hip_select_an, without a workspace: the scan state (tile ticket,
+ * per-tile partial sums, look-back status words) lives in a fixed 4 MiB device array of the
+ * library, one per GPU, held until the process ends.  The call allocates nothing.  A column whose
+ * look-back tiles do not fit the array is rejected with AHEAD_ERR_TOO_LARGE (with the default
+ * tiling that never happens below 2^32 rows).  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two streams, or two host threads) race and must not be issued. */
+int ahead_hip_select_an_noalloc(const ahead_col *col, int op1, uint64_t th1, int op2, uint64_t th2,
+                                int comb, uint64_t a_head, uint64_t a_oid, uint64_t *oids,
+                                uint64_t oids_cap, uint64_t *err, uint64_t err_cap,
+                                uint64_t *d_counts, uint64_t *d_sum, void *stream);
+
 /* ---- batched selectAN ---------------------------------------------------------------------------
  * No counterpart in the reference's API: many independent selectAN (+ fused decoded sum)
  * instances, each with the semantics of ahead_hip_select_an, run by one persistent launch.  A

@@ -3,7 +3,7 @@ import ctypes
 import re
 from pathlib import Path
 
-from ahead_amd import capi
+from ahead_amd import an, capi
 
 ROOT = Path(__file__).resolve().parents[1]
 
@@ -45,3 +45,21 @@ def test_status_and_version_without_gpu():
 
 def test_column_struct_layout_matches_header():
     assert ctypes.sizeof(capi.AheadCol) == 64
+
+
+def test_noalloc_select_rejects_columns_beyond_its_scratch():
+    # one 2048-row tile per 16 KiB sub-tile (tiling 1 x 4096): 400M 64-bit codes need ~195k tiles,
+    # more than the 4 MiB static scratch holds; the call is refused on the host, before any device
+    # work, so the column pointer is never dereferenced
+    lib = capi.load()
+    old = capi.get_tiling()
+    capi.set_tiling(1, 4096)
+    try:
+        meta = an.an_meta(an.RESINT, 65117)
+        d = capi.AheadCol(0x1000, 400_000_000, 8, 1, 32, 0, meta.a, meta.a_inv, meta.type.max_u, 0)
+        st = lib.ahead_hip_select_an_noalloc(ctypes.byref(d), capi.GE, 0, capi.LE, 1, capi.COMB_AND, 1, 1, 0x1000,
+                                             0, None, 0, 0x1000, None, None)
+        assert st == 5  # AHEAD_ERR_TOO_LARGE
+        assert b"static scratch" in lib.ahead_hip_last_error()
+    finally:
+        capi.set_tiling(*old)

@@ -166,11 +166,7 @@ def test_aggregate_sum_grouped_matches_add_at():
     np.add.at(ref, gid.astype(np.int64), contrib.astype(object))
     assert [int(x) for x in sums] == [int(x) & M64 for x in ref]
     assert np.array_equal(et, flips * np.uint64(an.A_OID))
-    # decoded clean sums
-    clean = np.ones(n, bool)
-    clean[flips.astype(np.int64)] = False
-    assert eh.size == 0 and eg.size == 0
-    assert int(sum(int(x) for x in m[clean])) <= sum((int(s) * an.inverse(an.A_RESULT, 64)) & M64 for s in sums)
+    assert eh.size == 0 and eg.size == 0  # exact decoded sums: test_oracle_flip_pins.py
 
 
 def test_aggregate_sum_grouped_out_of_range_group_is_reported():
