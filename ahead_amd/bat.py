@@ -356,3 +356,24 @@ def arithmeticAN(op: int, b1: BAT, b2: BAT, a_result: int = an.A_RESULT, a_oid: 
     e = _errs(bufs[1:], c[1:], [h1.meta.type.encoded and h1.buf is not None, t1.meta.type.encoded,
                                 h2.meta.type.encoded and h2.buf is not None, t2.meta.type.encoded])
     return (BAT(Column.void(n), Column(bufs[0], n, meta)),) + tuple(e)
+
+
+def star_query(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_mask: int = 0,
+               a_result: int = an.A_RESULT, a_oid: int = A_OID_ERR) -> dict:
+    """The fused star query (ahead_hip_star_query) over device columns.  dims: dicts in the
+    chain's order with fk, key, attr (Column or None), sel (Column of encoded OIDs or None), flags;
+    preds: (Column, op1, th1, op2, th2, comb) with encoded thresholds; meas: Columns.  Returns the
+    entry point's outputs as numpy (status, groups, kept, sums, first_row, keys by dimension, the
+    16 error vectors by kind)."""
+    cdims = []
+    for d in dims:
+        x = dict(d)
+        sel = d.get("sel")
+        if sel is not None:
+            x["sel"] = sel.buf if sel.buf is not None else DeviceBuffer(8)
+            x["sel_count"] = DeviceBuffer.from_numpy(np.array([sel.n], dtype=np.uint64))
+            x["sel_a_inv"] = sel.meta.a_inv
+        cdims.append(x)
+    q = capi.StarQuery(n_fact, cdims, preds, meas, kind, pre_mask, a_result, a_oid)
+    q.launch()
+    return q.results()

@@ -72,6 +72,37 @@ class Star1Out(ctypes.Structure):
                 ("e_a2_cap", ctypes.c_uint64), ("e_m_cap", ctypes.c_uint64), ("d_counts", ctypes.c_void_p)]
 
 
+SQ_MAX_DIMS, SQ_MAX_PREDS, SQ_MAX_MEAS, SQ_KINDS = 4, 2, 2, 16
+SQ_SUM, SQ_SUB, SQ_MULSUM = range(3)
+SQ_PROBE, SQ_REFETCH = 1, 2
+SQ_E_PRED, SQ_E_MEAS, SQ_E_FK, SQ_E_REFETCH, SQ_E_ATTR = 0, 2, 4, 8, 12
+SQ_STATUS = {0: "OK", 1: "BAD_OID", 2: "BAD_KEY", 3: "DUP_KEY", 4: "EVENTS", 5: "KEY_SPAN", 6: "GROUPS"}
+
+
+class SqDim(ctypes.Structure):
+    _fields_ = [("fk", AheadCol), ("key", AheadCol), ("attr", AheadCol), ("sel", ctypes.c_void_p),
+                ("sel_count", ctypes.c_void_p), ("sel_a_inv", ctypes.c_uint64), ("flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class SqPred(ctypes.Structure):
+    _fields_ = [("col", AheadCol), ("op1", ctypes.c_int32), ("op2", ctypes.c_int32), ("comb", ctypes.c_int32),
+                ("reserved", ctypes.c_uint32), ("th1", ctypes.c_uint64), ("th2", ctypes.c_uint64)]
+
+
+class SqArgs(ctypes.Structure):
+    _fields_ = [("n_fact", ctypes.c_uint64), ("n_dims", ctypes.c_uint32), ("n_preds", ctypes.c_uint32),
+                ("meas_kind", ctypes.c_uint32), ("meas_pre_mask", ctypes.c_uint32), ("dims", SqDim * SQ_MAX_DIMS),
+                ("preds", SqPred * SQ_MAX_PREDS), ("meas", AheadCol * SQ_MAX_MEAS), ("a_result", ctypes.c_uint64),
+                ("a_oid_err", ctypes.c_uint64)]
+
+
+class SqOut(ctypes.Structure):
+    _fields_ = [("sums", ctypes.c_void_p), ("first_row", ctypes.c_void_p), ("keys", ctypes.c_void_p * SQ_MAX_DIMS),
+                ("groups_cap", ctypes.c_uint64), ("err", ctypes.c_void_p * SQ_KINDS),
+                ("err_cap", ctypes.c_uint64 * SQ_KINDS), ("d_counts", ctypes.c_void_p)]
+
+
 _u64 = ctypes.c_uint64
 _u32 = ctypes.c_uint32
 _int = ctypes.c_int
@@ -120,6 +151,7 @@ _SIGS = {
     "ahead_hip_select_batch_launch": ([_vp, _vp], _int),
     "ahead_hip_select_batch_destroy": ([_vp], _int),
     "ahead_hip_select_batch_info": ([_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)], _int),
+    "ahead_hip_star_query": ([ctypes.POINTER(SqArgs), ctypes.POINTER(SqOut), _vp, _vp], _int),
     "ahead_hip_gen_uniform": ([_vp, _u32, _u64, _u64, _u64, _u64, _u64, _u64, _vp], _int),
     "ahead_hip_inject_flips": ([_vp, _u32, _u64, _u64, _u64, _u64, _vp], _int),
 }
@@ -548,6 +580,89 @@ def star1_pipeline(fact_fk: Column, fact_m: Column, dim_key: Column, dim_a1: Col
     st = Star1(fact_fk, fact_m, dim_key, dim_a1, dim_a2, **kw)
     st.launch()
     return st.results()
+
+
+class StarQuery:
+    """Argument and output blocks of one fused star query (ahead_hip_star_query), reusable.
+
+    dims: list of dicts in the chain's order with keys fk, key (Columns), attr (Column or None),
+    sel (DeviceBuffer of encoded OIDs or None), sel_count (DeviceBuffer holding the count at offset
+    0), sel_a_inv, flags.  preds: list of (Column, op1, th1, op2, th2, comb).  meas: list of Columns.
+    """
+
+    def __init__(self, n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_mask: int = 0,
+                 a_result: int = an.A_RESULT, a_oid_err: int = an.A_OID_ERR, groups_cap: int = 4096,
+                 err_cap: int = 1024):
+        self.keep = (dims, preds, meas)  # the columns and selections stay alive with the query
+        a = SqArgs()
+        a.n_fact = n_fact
+        a.n_dims = len(dims)
+        a.n_preds = len(preds)
+        a.meas_kind = kind
+        a.meas_pre_mask = pre_mask
+        for i, d in enumerate(dims):
+            x = a.dims[i]
+            x.fk = d["fk"].desc()
+            x.key = d["key"].desc()
+            x.attr = d["attr"].desc() if d.get("attr") is not None else AheadCol()
+            x.sel = d["sel"].ptr if d.get("sel") is not None else None
+            x.sel_count = d["sel_count"].ptr if d.get("sel") is not None else None
+            x.sel_a_inv = d.get("sel_a_inv", an.A_OID_INV) & an.MASK64
+            x.flags = d.get("flags", 0)
+        for i, (col, op1, th1, op2, th2, comb) in enumerate(preds):
+            a.preds[i] = SqPred(col.desc(), op1, op2, comb, 0, th1 & an.MASK64, th2 & an.MASK64)
+        for i, m in enumerate(meas):
+            a.meas[i] = m.desc()
+        a.a_result = a_result
+        a.a_oid_err = a_oid_err
+        self.args = a
+        self.attr_dims = [i for i, d in enumerate(dims) if d.get("attr") is not None]
+        self.groups_cap = groups_cap
+        self.err_cap = err_cap
+        self._alloc()
+
+    def _alloc(self):
+        g, e = self.groups_cap, self.err_cap
+        self.sums = DeviceBuffer(8 * g)
+        self.first = DeviceBuffer(8 * g)
+        self.keys = {d: DeviceBuffer(8 * g) for d in self.attr_dims}
+        self.errs = [DeviceBuffer(8 * e) for _ in range(SQ_KINDS)]
+        self.counts = DeviceBuffer(8 * (4 + SQ_KINDS))
+        o = SqOut()
+        o.sums = self.sums.ptr
+        o.first_row = self.first.ptr
+        for d, b in self.keys.items():
+            o.keys[d] = b.ptr
+        o.groups_cap = g
+        for k in range(SQ_KINDS):
+            o.err[k] = self.errs[k].ptr
+            o.err_cap[k] = e
+        o.d_counts = self.counts.ptr
+        self.out = o
+
+    def launch(self, stream: int | None = None) -> None:
+        s = current_stream() if stream is None else stream
+        st = load().ahead_hip_star_query(ctypes.byref(self.args), ctypes.byref(self.out), workspace().h, s)
+        if st:
+            check(st)
+
+    def results(self) -> dict:
+        """Synchronises, re-runs with larger buffers if an output was truncated, returns numpy."""
+        while True:
+            synchronize()
+            c = [int(x) for x in self.counts.to_numpy(np.uint64, 4 + SQ_KINDS)]
+            need_e = max(c[4:])
+            if c[0] <= self.groups_cap and need_e <= self.err_cap:
+                break
+            self.groups_cap = max(self.groups_cap, c[0])
+            self.err_cap = max(self.err_cap, need_e)
+            self._alloc()
+            self.launch()
+        ng = c[0]
+        return {"status": c[2], "groups": ng, "kept": c[1], "events": c[3],
+                "sums": self.sums.to_numpy(np.uint64, ng), "first_row": self.first.to_numpy(np.uint64, ng),
+                "keys": {d: b.to_numpy(np.uint64, ng) for d, b in self.keys.items()},
+                "err": [self.errs[k].to_numpy(np.uint64, c[4 + k]) for k in range(SQ_KINDS)]}
 
 
 # ------------------------------------------------------------------------------------------------

@@ -317,6 +317,84 @@ int ahead_hip_star1_build(const ahead_star1_args *in, const ahead_star1_out *out
 int ahead_hip_star1_probe(const ahead_star1_args *in, const ahead_star1_out *out,
                           ahead_hip_ws *ws, void *stream);
 
+/* ---- fused star query ---------------------------------------------------------------------------
+ * Replaces the fact side of one hardened SSB operator chain (tools/ssbm/ssbm-q11 .. q43_continuous.cpp)
+ * by one pass over the fact rows: the fact selectAN calls (Q1.x), per dimension the hashjoinAN probe
+ * (ssbm-q21_continuous.cpp:95) or positional matchjoinAN + hashjoinAN (:102-115), the fetchjoinAN
+ * calls of measures and group attributes, arithmeticAN<sub> (ssbm-q41_continuous.cpp:195),
+ * groupbyAN over the attributes and aggregate_sum_groupedAN / aggregate_mul_sumAN
+ * (OperatorsAN.hpp:79-134, 280-298).  The dimension-side selectAN calls (and their error vectors)
+ * stay with the caller: `sel` is their OID list.
+ * Row semantics, in fact order, with the row-consistent policy for corrupt fetched codes
+ * (reported, row dropped from the sums; the reference's chain throws there, groupbyAN.tcc:105-107):
+ * fact predicates on raw codes (every row checked); measures flagged in meas_pre_mask checked at
+ * the rows that pass them; dimensions in chain order (a corrupt FK is reported and ends the row,
+ * except for AHEAD_SQ_PROBE dimensions, which probe the truncated key anyway, hashjoinAN.tcc:199);
+ * joined rows check their other measures, re-fetch AHEAD_SQ_REFETCH keys and fetch their group
+ * attributes; kept rows add SUM d0 | SUB d0-d1 | MULSUM d0*d1 (decoded, mod 2^64) to their group.
+ * Groups are numbered by their first kept fact row.  Outputs: sums (x a_result), the first row
+ * and the attribute codes of every group; error vectors by kind (positions x a_oid_err: fact rows,
+ * dimension rows for attribute fetches).  d_counts: [0] groups, [1] kept rows, [2] status,
+ * [3] error events, [4 + kind] entries of each error vector.  A status other than OK means the
+ * fused form does not apply (a corrupt or repeated dimension key, a selected OID out of range, a
+ * key span above 4 x rows + 1024, more than AHEAD_SQ_MAX_GROUPS groups, more than
+ * AHEAD_SQ_MAX_EVENTS error events) and the caller runs the operator chain.  The call waits for
+ * the dimension statistics (a few hundred bytes) before it plans and launches the fact scan. */
+#define AHEAD_SQ_MAX_DIMS 4
+#define AHEAD_SQ_MAX_PREDS 2
+#define AHEAD_SQ_MAX_MEAS 2
+#define AHEAD_SQ_MAX_GROUPS 4096
+#define AHEAD_SQ_MAX_EVENTS 8192
+#define AHEAD_SQ_KINDS 16
+enum { AHEAD_SQ_SUM = 0, AHEAD_SQ_SUB = 1, AHEAD_SQ_MULSUM = 2 };
+enum { AHEAD_SQ_PROBE = 1, AHEAD_SQ_REFETCH = 2 };
+/* error vector kinds: + predicate / measure / dimension index (chain order) */
+enum { AHEAD_SQ_E_PRED = 0, AHEAD_SQ_E_MEAS = 2, AHEAD_SQ_E_FK = 4, AHEAD_SQ_E_REFETCH = 8,
+       AHEAD_SQ_E_ATTR = 12 };
+enum { AHEAD_SQ_ST_OK = 0, AHEAD_SQ_ST_BAD_OID = 1, AHEAD_SQ_ST_BAD_KEY = 2, AHEAD_SQ_ST_DUP_KEY = 3,
+       AHEAD_SQ_ST_EVENTS = 4, AHEAD_SQ_ST_KEY_SPAN = 5, AHEAD_SQ_ST_GROUPS = 6 };
+
+typedef struct {
+    ahead_col fk;               /* fact FK codes (8-byte codes, keys of at most 32 bits)        */
+    ahead_col key;              /* dimension key codes (8-byte codes)                           */
+    ahead_col attr;             /* group attribute codes; data NULL: no group attribute         */
+    const uint64_t *sel;        /* selected dimension OIDs (encoded); NULL: every row           */
+    const uint64_t *sel_count;  /* device: number of OIDs in sel (a selectAN's d_counts[0])     */
+    uint64_t sel_a_inv;         /* A^-1 of the OIDs in sel                                      */
+    uint32_t flags;             /* AHEAD_SQ_PROBE | AHEAD_SQ_REFETCH                            */
+    uint32_t reserved;
+} ahead_sq_dim;
+
+typedef struct {
+    ahead_col col;              /* fact column; th1 / th2 are ENCODED thresholds, as selectAN's */
+    int op1, op2, comb;
+    uint32_t reserved;
+    uint64_t th1, th2;
+} ahead_sq_pred;
+
+typedef struct {
+    uint64_t n_fact;
+    uint32_t n_dims, n_preds;
+    uint32_t meas_kind;         /* AHEAD_SQ_SUM (meas[0]), _SUB, _MULSUM (meas[0], meas[1])     */
+    uint32_t meas_pre_mask;     /* bit m: measure m is fetched right after the fact predicates  */
+    ahead_sq_dim dims[AHEAD_SQ_MAX_DIMS];  /* in the chain's order                               */
+    ahead_sq_pred preds[AHEAD_SQ_MAX_PREDS];
+    ahead_col meas[AHEAD_SQ_MAX_MEAS];
+    uint64_t a_result;          /* A of the sums (65117: v2_resbigint_t)                        */
+    uint64_t a_oid_err;         /* A of error positions (16089)                                 */
+} ahead_sq_args;
+
+typedef struct {
+    uint64_t *sums, *first_row;             /* per group (NULL: not written)                    */
+    uint64_t *keys[AHEAD_SQ_MAX_DIMS];      /* per group, attribute codes of dimension d        */
+    uint64_t groups_cap;
+    uint64_t *err[AHEAD_SQ_KINDS];
+    uint64_t err_cap[AHEAD_SQ_KINDS];
+    uint64_t *d_counts;                     /* device, 4 + AHEAD_SQ_KINDS entries               */
+} ahead_sq_out;
+
+int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *out, ahead_hip_ws *ws, void *stream);
+
 /* ---- data generation (seeded, counter-based; the same streams as ahead_amd/datagen.py) ------
  * Not part of the reference's operator API: the device-side generator for the synthetic
  * columns of the benchmark workloads (the reference loads SSB tables from dbgen instead).

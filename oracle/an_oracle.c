@@ -742,3 +742,203 @@ int64_t or_star1_pipeline(const or_col *fact_fk, const or_col *fact_m, const or_
     free(a2w4);
     return (int64_t)ng;
 }
+
+/* ---------------------------------------------------------------------------------------------- */
+/* fused star query: one SSB operator chain evaluated row by row                                    */
+/* ---------------------------------------------------------------------------------------------- */
+
+/* One selected dimension row, found by its truncated key. */
+typedef struct {
+    uint64_t key;
+    uint64_t row;
+} sq_entry;
+
+static int sq_entry_cmp(const void *x, const void *y) {
+    const sq_entry *a = (const sq_entry *)x, *b = (const sq_entry *)y;
+    return a->key < b->key ? -1 : (a->key > b->key ? 1 : (a->row < b->row ? -1 : (a->row > b->row)));
+}
+
+/* The build side of the dimension's hash join (hashjoinAN.tcc:175-187): the selected rows keyed on
+ * their decoded key truncated to the key's unencoded width, sorted for binary search.  The fused
+ * form applies only when every key of the dimension (selected or not: the chains also join the
+ * whole key column, e.g. ssbm-q42_continuous.cpp:226-231) is valid and unique. */
+static int sq_build(const or_sq_dim *d, sq_entry **out, uint64_t *n_out) {
+    const uint64_t nk = d->key.n;
+    sq_entry *all = (sq_entry *)malloc((nk + 1) * sizeof(sq_entry));
+    if (all == NULL) abort();
+    for (uint64_t j = 0; j < nk; ++j) {
+        if (or_is_corrupt(&d->key, j)) {
+            free(all);
+            return 2;
+        }
+        all[j].key = or_dec(&d->key, j) & mask_bits(d->key.unenc_bits);
+        all[j].row = j;
+    }
+    qsort(all, nk, sizeof(sq_entry), sq_entry_cmp);
+    for (uint64_t j = 1; j < nk; ++j) {
+        if (all[j].key == all[j - 1].key) {
+            free(all);
+            return 3;
+        }
+    }
+    if (d->sel == NULL) {
+        *out = all;
+        *n_out = nk;
+        return 0;
+    }
+    free(all);
+    sq_entry *e = (sq_entry *)malloc((d->n_sel + 1) * sizeof(sq_entry));
+    if (e == NULL) abort();
+    for (uint64_t j = 0; j < d->n_sel; ++j) {
+        const uint64_t row = d->sel[j] * d->sel_a_inv;
+        if (row >= nk) {
+            free(e);
+            return 1;
+        }
+        e[j].key = or_dec(&d->key, row) & mask_bits(d->key.unenc_bits);
+        e[j].row = row;
+    }
+    qsort(e, d->n_sel, sizeof(sq_entry), sq_entry_cmp);
+    *out = e;
+    *n_out = d->n_sel;
+    return 0;
+}
+
+static int64_t sq_find(const sq_entry *e, uint64_t n, uint64_t key) {
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (e[mid].key < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < n && e[lo].key == key) ? (int64_t)e[lo].row : -1;
+}
+
+static int sq_pred_pass(const or_sq_pred *p, uint64_t code) {
+    const uint64_t m = or_mask(p->col.width);
+    const uint64_t th1 = p->th1 & m, th2 = p->th2 & m;
+    if (p->comb == OR_COMB_NONE) return pred(p->op1, code, th1);
+    if (p->comb == OR_COMB_AND) return pred(p->op1, code, th1) && pred(p->op2, code, th2);
+    return pred(p->op1, code, th1) || pred(p->op2, code, th2);
+}
+
+void or_sq_out_free(or_sq_out *o) {
+    or_vec_free(&o->sums);
+    or_vec_free(&o->first_row);
+    for (int d = 0; d < 4; ++d) or_vec_free(&o->keys[d]);
+    for (int k = 0; k < OR_SQ_KINDS; ++k) or_vec_free(&o->err[k]);
+}
+
+int or_star_query(const or_sq_args *a, or_sq_out *o) {
+    memset(o, 0, sizeof *o);
+    sq_entry *build[4] = {NULL, NULL, NULL, NULL};
+    uint64_t nb[4] = {0, 0, 0, 0};
+    for (int d = 0; d < a->n_dims; ++d) {
+        const int st = sq_build(&a->dims[d], &build[d], &nb[d]);
+        if (st != 0) {
+            for (int k = 0; k < d; ++k) free(build[k]);
+            o->status = st;
+            return st;
+        }
+    }
+    /* groups: attribute-value tuples in first-occurrence order (linear dictionary; small) */
+    uint64_t *gkey = NULL; /* [groups][4] */
+    uint64_t gcap = 0, ng = 0;
+    uint64_t *gsum = NULL, *gfirst = NULL;
+    const uint64_t aoe = a->a_oid_err;
+    for (uint64_t i = 0; i < a->n_fact; ++i) {
+        /* 1. fact predicates (selectAN over the whole column: every row checked) */
+        int alive = 1;
+        for (int p = 0; p < a->n_preds; ++p) {
+            const or_sq_pred *pr = &a->preds[p];
+            if (or_is_corrupt(&pr->col, i)) vec_push(&o->err[OR_SQ_E_PRED + p], i * aoe);
+            if (!sq_pred_pass(pr, or_get(&pr->col, i))) alive = 0;
+        }
+        if (!alive) continue;
+        int dropped = 0;
+        /* 2. measures fetched right after the predicates */
+        for (int m = 0; m < a->n_meas; ++m) {
+            if (((a->meas_pre_mask >> m) & 1) && or_is_corrupt(&a->meas[m], i)) {
+                vec_push(&o->err[OR_SQ_E_MEAS + m], i * aoe);
+                dropped = 1;
+            }
+        }
+        /* 3. the dimensions in chain order */
+        int64_t drow[4] = {-1, -1, -1, -1};
+        for (int d = 0; d < a->n_dims && alive; ++d) {
+            const or_sq_dim *dm = &a->dims[d];
+            if (or_is_corrupt(&dm->fk, i)) {
+                vec_push(&o->err[OR_SQ_E_FK + d], i * aoe);
+                if (!(dm->flags & OR_SQ_PROBE)) {
+                    alive = 0;
+                    break;
+                }
+            }
+            const uint64_t key = or_dec(&dm->fk, i) & mask_bits(dm->fk.unenc_bits);
+            drow[d] = sq_find(build[d], nb[d], key);
+            if (drow[d] < 0) alive = 0;
+        }
+        if (!alive) continue;
+        /* 4. final fetches: measures, re-fetched FKs, group attributes */
+        for (int m = 0; m < a->n_meas; ++m) {
+            if (!((a->meas_pre_mask >> m) & 1) && or_is_corrupt(&a->meas[m], i)) {
+                vec_push(&o->err[OR_SQ_E_MEAS + m], i * aoe);
+                dropped = 1;
+            }
+        }
+        for (int d = 0; d < a->n_dims; ++d) {
+            const or_sq_dim *dm = &a->dims[d];
+            if ((dm->flags & OR_SQ_REFETCH) && or_is_corrupt(&dm->fk, i)) {
+                vec_push(&o->err[OR_SQ_E_REFETCH + d], i * aoe);
+                dropped = 1;
+            }
+        }
+        for (int d = 0; d < a->n_dims; ++d) {
+            const or_sq_dim *dm = &a->dims[d];
+            if (dm->attr.n > 0 && or_is_corrupt(&dm->attr, (uint64_t)drow[d])) {
+                vec_push(&o->err[OR_SQ_E_ATTR + d], (uint64_t)drow[d] * aoe);
+                dropped = 1;
+            }
+        }
+        if (dropped) continue;
+        /* 5. group and measure */
+        uint64_t k[4] = {0, 0, 0, 0};
+        for (int d = 0; d < a->n_dims; ++d)
+            if (a->dims[d].attr.n > 0) k[d] = or_dec(&a->dims[d].attr, (uint64_t)drow[d]) &
+                                              mask_bits(a->dims[d].attr.unenc_bits);
+        uint64_t g = 0;
+        for (; g < ng; ++g)
+            if (gkey[4 * g] == k[0] && gkey[4 * g + 1] == k[1] && gkey[4 * g + 2] == k[2] && gkey[4 * g + 3] == k[3])
+                break;
+        if (g == ng) {
+            if (ng == gcap) {
+                gcap = gcap ? 2 * gcap : 64;
+                gkey = (uint64_t *)realloc(gkey, gcap * 4 * sizeof(uint64_t));
+                gsum = (uint64_t *)realloc(gsum, gcap * sizeof(uint64_t));
+                gfirst = (uint64_t *)realloc(gfirst, gcap * sizeof(uint64_t));
+                if (gkey == NULL || gsum == NULL || gfirst == NULL) abort();
+            }
+            for (int d = 0; d < 4; ++d) gkey[4 * g + d] = k[d];
+            gsum[g] = 0;
+            gfirst[g] = i;
+            ++ng;
+            for (int d = 0; d < a->n_dims; ++d)
+                if (a->dims[d].attr.n > 0) vec_push(&o->keys[d], or_get(&a->dims[d].attr, (uint64_t)drow[d]));
+        }
+        const uint64_t d0 = or_dec(&a->meas[0], i);
+        uint64_t v = d0;
+        if (a->meas_kind == OR_SQ_SUB) v = d0 - or_dec(&a->meas[1], i);
+        if (a->meas_kind == OR_SQ_MULSUM) v = d0 * or_dec(&a->meas[1], i);
+        gsum[g] += v;
+        ++o->kept;
+    }
+    for (uint64_t g = 0; g < ng; ++g) {
+        vec_push(&o->sums, gsum[g] * a->a_result);
+        vec_push(&o->first_row, gfirst[g]);
+    }
+    free(gkey);
+    free(gsum);
+    free(gfirst);
+    for (int d = 0; d < a->n_dims; ++d) free(build[d]);
+    return 0;
+}

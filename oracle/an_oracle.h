@@ -171,6 +171,64 @@ int64_t or_star1_pipeline(const or_col *fact_fk, const or_col *fact_m, const or_
                           or_vec *e_fk, or_vec *e_dk, or_vec *e_a1, or_vec *e_a2, or_vec *e_m,
                           or_vec *e_grp);
 
+/* ---- fused star query (one SSB operator chain as one pass over the fact rows) -------------------
+ * The chains of tools/ssbm/ssbm-q{11..43}_continuous.cpp evaluated row by row in fact order, with
+ * the row-consistent policy for corrupt fetched codes (reported, row dropped from the sums; the
+ * reference's chain throws there, groupbyAN.tcc:105-107).  Per fact row i:
+ *   1. every fact predicate column p is checked (corrupt: E[PRED p] += i) and its predicate is
+ *      evaluated on the raw code (selectAN_scalar.tcc:84-90); the row goes on if all pass;
+ *   2. measures fetched right after the predicates (flag bit m of meas_pre_mask; the discount
+ *      fetch of ssbm-q11_continuous.cpp:113) are checked: corrupt -> E[MEAS m] += i, dropped;
+ *   3. the dimensions in chain order: the FK code is checked; a corrupt code is reported
+ *      (E[FK d] += i) and, unless the dimension is probed straight on the fact column (PROBE:
+ *      hashjoinAN keeps probing with the truncated key, hashjoinAN.tcc:199), ends the row (the
+ *      positional fetch drops it, matchjoinAN.tcc:287-295).  The key truncated to the key's
+ *      unencoded width must be a selected dimension row's key, or the row ends;
+ *   4. rows that joined every dimension fetch their final-scope measures (corrupt: E[MEAS m] += i),
+ *      re-fetch the FK of REFETCH dimensions (corrupt: E[REFETCH d] += i) and fetch the group
+ *      attribute of every dimension that has one (corrupt: E[ATTR d] += dimension row); any of
+ *      these drops the row;
+ *   5. kept rows add their measure to their group: SUM d0, SUB d0 - d1, MULSUM d0 * d1 (decoded
+ *      values, mod 2^64); groups are numbered by their first kept fact row.
+ * Sums are written encoded (x a_result); keys are the attribute codes; error vectors hold
+ * position * a_oid_err.  Status: 0 done; 1 a selected OID is out of range, 2 a dimension key is
+ * corrupt, 3 two dimension rows share a key (the fused form does not apply; the caller runs the
+ * operator chain). */
+enum { OR_SQ_SUM = 0, OR_SQ_SUB = 1, OR_SQ_MULSUM = 2 };
+enum { OR_SQ_PROBE = 1, OR_SQ_REFETCH = 2 };
+enum { OR_SQ_E_PRED = 0, OR_SQ_E_MEAS = 2, OR_SQ_E_FK = 4, OR_SQ_E_REFETCH = 8, OR_SQ_E_ATTR = 12,
+       OR_SQ_KINDS = 16 };
+typedef struct {
+    or_col fk;            /* fact FK codes                                        */
+    or_col key;           /* dimension key codes                                  */
+    or_col attr;          /* group attribute codes; data == NULL and n == 0: none */
+    const uint64_t *sel;  /* selected dimension OIDs (encoded), NULL: every row   */
+    uint64_t n_sel;
+    uint64_t sel_a_inv;
+    int flags;            /* OR_SQ_PROBE | OR_SQ_REFETCH                          */
+} or_sq_dim;
+typedef struct {
+    or_col col;
+    int op1, op2, comb;
+    uint64_t th1, th2;    /* encoded thresholds                                   */
+} or_sq_pred;
+typedef struct {
+    uint64_t n_fact;
+    int n_dims, n_preds, n_meas, meas_kind, meas_pre_mask;
+    or_sq_dim dims[4];
+    or_sq_pred preds[2];
+    or_col meas[2];
+    uint64_t a_result, a_oid_err;
+} or_sq_args;
+typedef struct {
+    or_vec sums, first_row, keys[4];  /* per group, first-occurrence order */
+    or_vec err[OR_SQ_KINDS];
+    uint64_t kept;
+    int status;
+} or_sq_out;
+int or_star_query(const or_sq_args *a, or_sq_out *o);
+void or_sq_out_free(or_sq_out *o);
+
 #ifdef __cplusplus
 }
 #endif

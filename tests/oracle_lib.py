@@ -41,6 +41,28 @@ class OrVec(ctypes.Structure):
         return np.ctypeslib.as_array(self.v, shape=(self.n,)).copy()
 
 
+class OrSqDim(ctypes.Structure):
+    _fields_ = [("fk", OrCol), ("key", OrCol), ("attr", OrCol), ("sel", ctypes.c_void_p), ("n_sel", ctypes.c_uint64),
+                ("sel_a_inv", ctypes.c_uint64), ("flags", ctypes.c_int)]
+
+
+class OrSqPred(ctypes.Structure):
+    _fields_ = [("col", OrCol), ("op1", ctypes.c_int), ("op2", ctypes.c_int), ("comb", ctypes.c_int),
+                ("th1", ctypes.c_uint64), ("th2", ctypes.c_uint64)]
+
+
+class OrSqArgs(ctypes.Structure):
+    _fields_ = [("n_fact", ctypes.c_uint64), ("n_dims", ctypes.c_int), ("n_preds", ctypes.c_int),
+                ("n_meas", ctypes.c_int), ("meas_kind", ctypes.c_int), ("meas_pre_mask", ctypes.c_int),
+                ("dims", OrSqDim * 4), ("preds", OrSqPred * 2), ("meas", OrCol * 2), ("a_result", ctypes.c_uint64),
+                ("a_oid_err", ctypes.c_uint64)]
+
+
+class OrSqOut(ctypes.Structure):
+    _fields_ = [("sums", OrVec), ("first_row", OrVec), ("keys", OrVec * 4), ("err", OrVec * 16),
+                ("kept", ctypes.c_uint64), ("status", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -78,6 +100,8 @@ def lib():
             "or_arithmetic_an": ([c, c, c, c, i, u64, i, u64, v, v, v, v, v], i),
             "or_star1_pipeline": ([c, c, c, c, c, u64, u64, u64, u64] + [v] * 10, ctypes.c_int64),
             "or_vec_free": ([v], None),
+            "or_star_query": ([P(OrSqArgs), P(OrSqOut)], i),
+            "or_sq_out_free": ([P(OrSqOut)], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -260,3 +284,49 @@ def star1_pipeline(fact_fk: Col, fact_m: Col, dim_key: Col, dim_a1: Col, dim_a2:
         raise RuntimeError("pipeline failed")
     names = ["sums", "groid", "key1", "key2", "e_fk", "e_dk", "e_a1", "e_a2", "e_m", "e_grp"]
     return dict(zip(names, out))
+
+
+SQ_SUM, SQ_SUB, SQ_MULSUM = 0, 1, 2
+SQ_PROBE, SQ_REFETCH = 1, 2
+
+
+def star_query(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_mask: int = 0,
+               a_result=an.A_RESULT, a_oid_err=an.A_OID_ERR) -> dict:
+    """or_star_query.  dims: dicts with fk, key (Col), attr (Col or None), sel (uint64 numpy array of
+    encoded OIDs or None), sel_a_inv, flags; preds: (Col, op1, th1, op2, th2, comb); meas: Cols."""
+    a = OrSqArgs()
+    a.n_fact = n_fact
+    a.n_dims = len(dims)
+    a.n_preds = len(preds)
+    a.n_meas = len(meas)
+    a.meas_kind = kind
+    a.meas_pre_mask = pre_mask
+    keep = []
+    for i, d in enumerate(dims):
+        x = a.dims[i]
+        x.fk = d["fk"].c
+        x.key = d["key"].c
+        if d.get("attr") is not None:
+            x.attr = d["attr"].c
+        sel = d.get("sel")
+        if sel is not None:
+            sel = np.ascontiguousarray(sel, dtype=np.uint64)
+            keep.append(sel)
+            x.sel = sel.ctypes.data
+            x.n_sel = sel.size
+        x.sel_a_inv = d.get("sel_a_inv", an.A_OID_INV) & an.MASK64
+        x.flags = d.get("flags", 0)
+    for i, (col, op1, th1, op2, th2, comb) in enumerate(preds):
+        a.preds[i] = OrSqPred(col.c, op1, op2, comb, th1 & an.MASK64, th2 & an.MASK64)
+    for i, m in enumerate(meas):
+        a.meas[i] = m.c
+    a.a_result = a_result
+    a.a_oid_err = a_oid_err
+    o = OrSqOut()
+    st = lib().or_star_query(ctypes.byref(a), ctypes.byref(o))
+    res = {"status": st, "kept": int(o.kept), "sums": o.sums.numpy(), "first_row": o.first_row.numpy(),
+           "keys": {i: o.keys[i].numpy() for i, d in enumerate(dims) if d.get("attr") is not None},
+           "err": [o.err[k].numpy() for k in range(16)]}
+    res["groups"] = res["sums"].size
+    lib().or_sq_out_free(ctypes.byref(o))
+    return res

@@ -1,0 +1,79 @@
+"""The fused star query on the GPU (ahead_hip_star_query) against the oracle's (or_star_query): all
+thirteen SSB queries, bit for bit in results, group order, first rows and every error vector, with
+flips in every fact column and dimension attribute; against the GPU operator chains on data the
+chains process (selection flips); and the plan-time statuses."""
+import numpy as np
+import pytest
+
+import oracle_bat as OB
+from ahead_amd import ssb
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(n_fact=300_000, n_date=2556, n_cust=30_000, n_supp=2_000, n_part=14_000)
+KEY_COLS = {"d_datekey", "c_custkey", "s_suppkey", "p_partkey"}
+ALL_FLIPS = tuple(c for c in ssb.COLUMNS if c not in KEY_COLS)
+
+
+def same(got, ref):
+    assert got["rows"] == ref["rows"]
+    if "result" in ref:
+        assert got["result"] == ref["result"]
+    else:
+        assert got["groups"] == ref["groups"]
+        assert np.array_equal(got["sums"], ref["sums"])
+        for a, b in zip(got["keys"], ref["keys"]):
+            assert np.array_equal(a, b)
+    ge, re = dict(got["errors"].items), dict(ref["errors"].items)
+    assert sorted(ge) == sorted(re)
+    for k in re:
+        assert np.array_equal(ge[k], re[k]), k
+
+
+@pytest.mark.parametrize("name", list(ssb.QUERIES))
+def test_star_query_matches_oracle_with_flips_everywhere(name):
+    spec = ssb.SsbSpec(rate=1e-3, seed=31, flip_cols=ALL_FLIPS, **SMALL)
+    q = ssb.QUERIES[name]
+    print(f"star_query {name} vs oracle, {spec.n_fact} rows", flush=True)
+    got = ssb.fused(ssb.device_tables(spec), q, fallback=False)
+    ref = ssb.fused(OB.tables(spec), q, OB, fallback=False)
+    assert got["status"] == 0 and ref["status"] == 0
+    same(got, ref)
+    assert np.array_equal(got["first_row"], ref["first_row"])
+    assert got["errors"].total() > 0
+
+
+@pytest.mark.parametrize("name", list(ssb.QUERIES))
+def test_star_query_matches_gpu_chain(name):
+    q = ssb.QUERIES[name]
+    attrs = {d.attr for d in q.dims}
+    flips = tuple(c for c, v in ssb.COLUMNS.items() if v[4] and c not in attrs)
+    spec = ssb.SsbSpec(rate=1e-3, seed=32, flip_cols=flips, **SMALL)
+    print(f"star_query {name} vs chain, {spec.n_fact} rows", flush=True)
+    t = ssb.device_tables(spec)
+    got = ssb.fused(t, q, fallback=False)
+    ref = ssb.chain(t, q)
+    assert got["status"] == 0
+    same(got, ref)
+
+
+def test_star_query_falls_back_to_the_chain_on_corrupt_keys():
+    spec = ssb.SsbSpec(rate=5e-3, seed=33, flip_cols=("s_suppkey",), **SMALL)
+    q = ssb.QUERIES["Q2.1"]
+    print("star_query corrupt supplier keys", flush=True)
+    t = ssb.device_tables(spec)
+    assert ssb.fused(t, q, fallback=False)["status"] == 2
+    got = ssb.fused(t, q)
+    assert got["status"] == 2
+    same(got, ssb.chain(t, q))
+
+
+def test_star_query_empty_and_tiny_fact_tables():
+    for n in (0, 1, 511, 513):
+        spec = ssb.SsbSpec(rate=0.0, seed=34, n_fact=n, n_date=2556, n_cust=300, n_supp=50, n_part=140)
+        print(f"star_query tiny fact {n}", flush=True)
+        for name in ("Q1.1", "Q3.1", "Q4.1"):
+            q = ssb.QUERIES[name]
+            got = ssb.fused(ssb.device_tables(spec), q, fallback=False)
+            ref = ssb.fused(OB.tables(spec), q, OB, fallback=False)
+            same(got, ref)

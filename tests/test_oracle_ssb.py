@@ -33,4 +33,4 @@ def test_oracle_pipeline_reports_flipped_quantities():
     e = dict(got["errors"].items)
     flipped = host["lo_quantity"][2]
     assert flipped.size > 0
-    assert np.array_equal(e["select_quantity.e1"], flipped.astype(np.uint64) * np.uint64(16089))
+    assert np.array_equal(e["select_lo_quantity.e1"], flipped.astype(np.uint64) * np.uint64(16089))
