@@ -359,7 +359,7 @@ def arithmeticAN(op: int, b1: BAT, b2: BAT, a_result: int = an.A_RESULT, a_oid: 
 
 
 def star_query(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_mask: int = 0,
-               a_result: int = an.A_RESULT, a_oid: int = A_OID_ERR) -> dict:
+               a_result: int = an.A_RESULT, a_oid: int = A_OID_ERR, ev_scan: tuple | None = None) -> dict:
     """The fused star query (ahead_hip_star_query) over device columns.  dims: dicts in the
     chain's order with fk, key, attr (Column or None), sel (Column of encoded OIDs or None), flags;
     preds: (Column, op1, th1, op2, th2, comb) with encoded thresholds; meas: Columns.  Returns the
@@ -374,6 +374,6 @@ def star_query(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_
             x["sel_count"] = DeviceBuffer.from_numpy(np.array([sel.n], dtype=np.uint64))
             x["sel_a_inv"] = sel.meta.a_inv
         cdims.append(x)
-    q = capi.StarQuery(n_fact, cdims, preds, meas, kind, pre_mask, a_result, a_oid)
+    q = capi.StarQuery(n_fact, cdims, preds, meas, kind, pre_mask, a_result, a_oid, ev_scan=ev_scan)
     q.launch()
     return q.results()

@@ -94,7 +94,7 @@ class SqArgs(ctypes.Structure):
     _fields_ = [("n_fact", ctypes.c_uint64), ("n_dims", ctypes.c_uint32), ("n_preds", ctypes.c_uint32),
                 ("meas_kind", ctypes.c_uint32), ("meas_pre_mask", ctypes.c_uint32), ("dims", SqDim * SQ_MAX_DIMS),
                 ("preds", SqPred * SQ_MAX_PREDS), ("meas", AheadCol * SQ_MAX_MEAS), ("a_result", ctypes.c_uint64),
-                ("a_oid_err", ctypes.c_uint64)]
+                ("a_oid_err", ctypes.c_uint64), ("ev_scan", ctypes.c_void_p * 2)]
 
 
 class SqOut(ctypes.Structure):
@@ -592,7 +592,7 @@ class StarQuery:
 
     def __init__(self, n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_mask: int = 0,
                  a_result: int = an.A_RESULT, a_oid_err: int = an.A_OID_ERR, groups_cap: int = 4096,
-                 err_cap: int = 1024):
+                 err_cap: int = 1024, ev_scan: tuple | None = None):
         self.keep = (dims, preds, meas)  # the columns and selections stay alive with the query
         a = SqArgs()
         a.n_fact = n_fact
@@ -615,6 +615,8 @@ class StarQuery:
             a.meas[i] = m.desc()
         a.a_result = a_result
         a.a_oid_err = a_oid_err
+        if ev_scan is not None:
+            a.ev_scan[0], a.ev_scan[1] = ev_scan
         self.args = a
         self.attr_dims = [i for i, d in enumerate(dims) if d.get("attr") is not None]
         self.groups_cap = groups_cap
@@ -879,3 +881,34 @@ def arithmetic_an(h1: Column, t1: Column, h2: Column, t2: Column, op: int, a_res
 
     b, c = _run(call, outs, 5, [None, 1, 2, 3, 4])
     return (b[0].to_numpy(np.dtype(f"u{result_width}"), c[0]), *(b[i].to_numpy(np.uint64, c[i]) for i in range(1, 5)))
+
+
+class HipEvents:
+    """Raw hipEvent_t pairs (libamdhip64) for timing one kernel inside an entry point."""
+
+    def __init__(self, count: int):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), _vp, _vp]
+        self.hip.hipEventSynchronize.argtypes = [_vp]
+        self.hip.hipEventDestroy.argtypes = [_vp]
+        self.pairs = []
+        for _ in range(count):
+            e = [_vp(), _vp()]
+            for x in e:
+                if self.hip.hipEventCreate(ctypes.byref(x)) != 0:
+                    raise AheadError(3, "hipEventCreate failed")
+            self.pairs.append((e[0].value, e[1].value))
+
+    def ms(self, i: int) -> float:
+        a, b = self.pairs[i]
+        self.hip.hipEventSynchronize(b)
+        out = ctypes.c_float()
+        if self.hip.hipEventElapsedTime(ctypes.byref(out), a, b) != 0:
+            raise AheadError(3, "hipEventElapsedTime failed")
+        return float(out.value)
+
+    def close(self) -> None:
+        for a, b in self.pairs:
+            self.hip.hipEventDestroy(a)
+            self.hip.hipEventDestroy(b)
+        self.pairs = []

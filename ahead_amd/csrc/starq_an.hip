@@ -43,6 +43,7 @@
 
 #include <algorithm>
 #include <numeric>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -52,10 +53,12 @@ namespace {
 
 constexpr int kSqBlock = 1024;
 constexpr int kSqWaves = kSqBlock / 64;
-constexpr int kSqSeg = 512;  // rows per wave step: 4 chunks of 128 rows, 2 rows per lane
+constexpr int kSqChunks = 2;                // chunks of 128 rows per wave step
+constexpr int kSqSeg = kSqChunks * 128;     // rows per wave step
+constexpr int kSqRows = 2 * kSqChunks;      // rows per lane per step
 constexpr uint32_t kSqMaxGroups = AHEAD_SQ_MAX_GROUPS;
 constexpr uint32_t kSqMaxEvents = AHEAD_SQ_MAX_EVENTS;
-constexpr uint32_t kSqMapWords = 28416;  // LDS words for maps (with the group tables: < 160 KiB)
+constexpr uint32_t kSqMapWords = 28416;  // LDS words for maps (with group tables and staging: < 160 KiB)
 constexpr uint32_t kNoLds = 0xFFFFFFFFu;
 constexpr int kKinds = AHEAD_SQ_KINDS;
 // flags[] words
@@ -83,18 +86,24 @@ struct SqPredK {
     uint32_t width;
     int op1, op2, comb;
     uint64_t inv, maxu, wmask, th1, th2;
+    // the set of passing codes as at most three intervals [lo, lo + len] (unused ones repeat the
+    // first; `never`: no code passes), so that the scan tests (code - lo) <= len per interval
+    uint64_t lo[3], len[3];
+    uint32_t never;
 };
 
 struct SqMeasK {
     const void *col;
     uint32_t width;
     uint32_t pre;  // fetched right after the predicates
+    int pred;      // index of a fact predicate over the same column (checked with it), or -1
     uint64_t inv, maxu, wmask;
 };
 
 struct SqArgs {
     uint64_t n, rows_per_wave;
     uint32_t nd, np, nm, kind, has_pre;
+    uint32_t pre_load;  // a pre-predicate measure over a column no predicate reads
     SqDimK d[AHEAD_SQ_MAX_DIMS];  // in evaluation order
     SqPredK p[AHEAD_SQ_MAX_PREDS];
     SqMeasK m[AHEAD_SQ_MAX_MEAS];
@@ -109,21 +118,22 @@ struct SqArgs {
     uint64_t *events;  // (row << 32) | (kind << 27) | dimension row
 };
 
-__device__ __forceinline__ bool cmp_op(int op, uint64_t t, uint64_t th) {
-    switch (op) {
-        case AHEAD_LT: return t < th;
-        case AHEAD_LE: return t <= th;
-        case AHEAD_GT: return t > th;
-        case AHEAD_GE: return t >= th;
-        case AHEAD_EQ: return t == th;
-        default: return t != th;
-    }
+__host__ __device__ __forceinline__ bool pred_pass(const SqPredK &p, uint64_t code) {
+    auto cmp = [](int op, uint64_t t, uint64_t th) {
+        return op == AHEAD_LT ? t < th : op == AHEAD_LE ? t <= th : op == AHEAD_GT ? t > th
+             : op == AHEAD_GE ? t >= th : op == AHEAD_EQ ? t == th : t != th;
+    };
+    if (p.comb == AHEAD_COMB_NONE) return cmp(p.op1, code, p.th1);
+    if (p.comb == AHEAD_COMB_AND) return cmp(p.op1, code, p.th1) && cmp(p.op2, code, p.th2);
+    return cmp(p.op1, code, p.th1) || cmp(p.op2, code, p.th2);
 }
 
-__device__ __forceinline__ bool pred_pass(const SqPredK &p, uint64_t code) {
-    if (p.comb == AHEAD_COMB_NONE) return cmp_op(p.op1, code, p.th1);
-    if (p.comb == AHEAD_COMB_AND) return cmp_op(p.op1, code, p.th1) && cmp_op(p.op2, code, p.th2);
-    return cmp_op(p.op1, code, p.th1) || cmp_op(p.op2, code, p.th2);
+template <typename U>
+__device__ __forceinline__ bool pred_fast(const SqPredK &p, U code) {
+    bool in = false;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) in = in || static_cast<U>(code - static_cast<U>(p.lo[i])) <= static_cast<U>(p.len[i]);
+    return in && !p.never;
 }
 
 __device__ __forceinline__ uint64_t load_code(const void *col, uint32_t width, uint64_t i) {
@@ -136,7 +146,8 @@ __device__ __forceinline__ uint64_t load_code(const void *col, uint32_t width, u
 }
 
 // Two consecutive codes (rows r0, r0 + 1; r0 even) of a column of any width, one load.
-__device__ __forceinline__ void load_pair(const void *col, uint32_t width, uint64_t r0, uint64_t &x0, uint64_t &x1) {
+template <typename T>
+__device__ __forceinline__ void load_pair(const void *col, uint32_t width, uint64_t r0, T &x0, T &x1) {
     switch (width) {
         case 1: {
             const uint16_t w = static_cast<const uint16_t *>(col)[r0 >> 1];
@@ -158,8 +169,8 @@ __device__ __forceinline__ void load_pair(const void *col, uint32_t width, uint6
         }
         default: {
             const u32x4 q = load16_nt(static_cast<const uint64_t *>(col) + r0);
-            x0 = (static_cast<uint64_t>(q[1]) << 32) | q[0];
-            x1 = (static_cast<uint64_t>(q[3]) << 32) | q[2];
+            x0 = static_cast<T>((static_cast<uint64_t>(q[1]) << 32) | q[0]);
+            x1 = static_cast<T>((static_cast<uint64_t>(q[3]) << 32) | q[2]);
             break;
         }
     }
@@ -261,12 +272,23 @@ __device__ void exact_row(const SqArgs &a, uint64_t row, unsigned long long *s_s
     ++kept;
 }
 
-template <int ND, int NP>
+// One wave step covers kSqSeg rows: kSqChunks chunks of 128 rows, each lane holding 2 consecutive
+// rows of every chunk (kSqRows rows per lane).  Its FK codes (and predicate codes) are loaded into
+// registers one step ahead: as soon as a step has decoded its keys, the wave issues the next step's
+// loads into the same registers, so they are in flight while the step's dependent reads run (the
+// L2 maps, the measures).  Dependent reads are issued for all of a lane's rows at once and waited
+// for once: a row whose value is not needed reads a fixed address of the step instead of
+// branching, so no read waits behind another row's branch.
+template <int ND, int NP, bool PW8>
 __global__ __launch_bounds__(kSqBlock) void k_sq_scan(SqArgs a) {
     __shared__ uint32_t s_map[kSqMapWords];
     __shared__ unsigned long long s_sum[kSqMaxGroups];
     __shared__ uint32_t s_first[kSqMaxGroups];
-    static_assert(sizeof(s_map) + sizeof(s_sum) + sizeof(s_first) <= 160 * 1024 - 1024, "LDS budget");
+    __shared__ uint32_t s_stage[kSqWaves * kSqSeg];  // per wave: the step's joined rows (offset | group << 8)
+    static_assert(sizeof(s_map) + sizeof(s_sum) + sizeof(s_first) + sizeof(s_stage) <= 160 * 1024 - 1024,
+                  "LDS budget");
+    static_assert(kSqSeg <= 256 && kSqMaxGroups < (1u << 24), "staged entry layout");
+    constexpr int R = kSqRows;
     if (a.flags[kFlagDup] != 0u) return;
     for (uint32_t i = threadIdx.x; i < a.lds_words; i += kSqBlock) s_map[i] = a.maps[i];
     for (uint32_t g = threadIdx.x; g < a.groups; g += kSqBlock) {
@@ -281,110 +303,204 @@ __global__ __launch_bounds__(kSqBlock) void k_sq_scan(SqArgs a) {
     const uint64_t end = begin + a.rows_per_wave < a.n ? begin + a.rows_per_wave : a.n;
     uint32_t kept = 0;
 
-    for (uint64_t seg = begin; seg < end; seg += kSqSeg) {
-        const bool full = seg + kSqSeg <= end;
-        uint64_t fk[ND][4][2];
-        uint64_t pc[NP > 0 ? NP : 1][4][2];
+    using PT = typename std::conditional<PW8, uint64_t, uint32_t>::type;  // predicate code registers
+    uint64_t fk[ND][R];
+    PT pc[NP > 0 ? NP : 1][R];
+    // row of register slot r in the step at seg: seg + (r / 2) * 128 + lane * 2 + r % 2
+    auto load_step = [&](uint64_t seg) {
+        if (seg + kSqSeg <= end) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const uint64_t r0 = seg + c * 128u + lane * 2u;
-            if (full) {
+            for (int c = 0; c < kSqChunks; ++c) {
+                const uint64_t r0 = seg + c * 128u + lane * 2u;
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
                     const u32x4 q = load16_nt(a.d[d].fk + r0);
-                    fk[d][c][0] = (static_cast<uint64_t>(q[1]) << 32) | q[0];
-                    fk[d][c][1] = (static_cast<uint64_t>(q[3]) << 32) | q[2];
+                    fk[d][2 * c] = (static_cast<uint64_t>(q[1]) << 32) | q[0];
+                    fk[d][2 * c + 1] = (static_cast<uint64_t>(q[3]) << 32) | q[2];
                 }
 #pragma unroll
-                for (int p = 0; p < NP; ++p) load_pair(a.p[p].col, a.p[p].width, r0, pc[p][c][0], pc[p][c][1]);
+                for (int p = 0; p < NP; ++p) load_pair(a.p[p].col, a.p[p].width, r0, pc[p][2 * c], pc[p][2 * c + 1]);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint64_t row = seg + (r >> 1) * 128u + lane * 2u + (r & 1);
+                const bool in = row < end;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) fk[d][r] = in ? a.d[d].fk[row] : 0ull;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) pc[p][r] = in ? static_cast<PT>(load_code(a.p[p].col, a.p[p].width, row)) : PT(0);
+            }
+        }
+    };
+
+    // the pending batch: one joined row per lane whose measure reads are in flight
+    bool pend_act = false;
+    uint64_t pend_row = 0;
+    uint32_t pend_g = 0;
+    uint64_t pend_code[AHEAD_SQ_MAX_MEAS] = {0, 0};
+    // finishes the pending batch: kept rows into the group table; returns whether this lane's row
+    // holds a corrupt measure (left for the exact path)
+    auto finish_pending = [&]() -> bool {
+        if (!pend_act) return false;
+        pend_act = false;
+        uint64_t dm[AHEAD_SQ_MAX_MEAS] = {0, 0};
+        bool bad = pend_g >= a.groups;
+#pragma unroll
+        for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m) {
+            if (m >= static_cast<int>(a.nm)) continue;
+            const SqMeasK &mk = a.m[m];
+            dm[m] = (pend_code[m] * mk.inv) & mk.wmask;
+            bad |= dm[m] > mk.maxu;
+        }
+        if (bad) return true;
+        atomicAdd(&s_sum[pend_g], static_cast<unsigned long long>(measure_value(a, dm[0], dm[1])));
+        atomicMin(&s_first[pend_g], static_cast<uint32_t>(pend_row));
+        ++kept;
+        return false;
+    };
+
+    if (begin < end) load_step(begin);
+    for (uint64_t seg = begin; seg < end; seg += kSqSeg) {
+        // 1. decode and check the codes in registers; predicates on raw codes
+        uint32_t key[ND][R];
+        uint32_t alive = 0, err = 0;  // bit r: row of slot r
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint64_t row = seg + (r >> 1) * 128u + lane * 2u + (r & 1);
+            bool e = false, al = row < end;
+            const bool live = al;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                const SqPredK &pr = a.p[p];
+                if constexpr (PW8) {
+                    e |= ((pc[p][r] * pr.inv) & pr.wmask) > pr.maxu;
+                    al = al && pred_fast<uint64_t>(pr, pc[p][r]);
+                } else {  // codes of at most 4 bytes: the low 32 bits of the product decide
+                    const uint32_t c32 = static_cast<uint32_t>(pc[p][r]);
+                    e |= ((c32 * static_cast<uint32_t>(pr.inv)) & static_cast<uint32_t>(pr.wmask)) >
+                         static_cast<uint32_t>(pr.maxu);
+                    al = al && pred_fast<uint32_t>(pr, c32);
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                const uint64_t dec = fk[d][r] * a.d[d].fk_inv;
+                e |= dec > a.d[d].fk_maxu;
+                key[d][r] = static_cast<uint32_t>(dec) & a.d[d].kmask;
+            }
+            if (live && e) err |= 1u << r;
+            if (al) alive |= 1u << r;
+        }
+        // 2. the next step's codes go in flight now
+        if (seg + kSqSeg < end) load_step(seg + kSqSeg);
+        alive &= ~err;
+        const uint64_t dummy = seg;  // a row of this step: reads for rows not needed go here
+        // 3. measures fetched right after the predicates: a measure over a predicate's column is
+        // already checked with it; any other one is checked here at every row that passed them
+        if (a.pre_load) {
+#pragma unroll
+            for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m) {
+                const SqMeasK &mk = a.m[m];
+                if (m >= static_cast<int>(a.nm) || !mk.pre || mk.pred >= 0) continue;
+                uint64_t code[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint64_t row = seg + (r >> 1) * 128u + lane * 2u + (r & 1);
+                    code[r] = load_code(mk.col, mk.width, ((alive >> r) & 1u) ? row : dummy);
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (((alive >> r) & 1u) && ((code[r] * mk.inv) & mk.wmask) > mk.maxu) err |= 1u << r;
+            }
+            alive &= ~err;
+        }
+        // 4. dimensions in evaluation order (LDS maps first); an L2 map's reads go out together
+        uint32_t g[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) g[r] = 0;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const SqDimK &dd = a.d[d];
+            const uint32_t sh = 5u - dd.lb;
+            uint32_t w[R], idx[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                idx[r] = key[d][r] - dd.kmin;
+                const bool use = ((alive >> r) & 1u) && idx[r] < dd.span;
+                if (!use) idx[r] = 0;
+                if (!use) alive &= ~(1u << r);
+            }
+            if (dd.lds_off != kNoLds) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) w[r] = s_map[dd.lds_off + (idx[r] >> sh)];
             } else {
 #pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const bool in = r0 + k < end;
+                for (int r = 0; r < R; ++r) w[r] = a.maps[dd.map_off + (idx[r] >> sh)];
+            }
 #pragma unroll
-                    for (int d = 0; d < ND; ++d) fk[d][c][k] = in ? a.d[d].fk[r0 + k] : 0ull;
-#pragma unroll
-                    for (int p = 0; p < NP; ++p) pc[p][c][k] = in ? load_code(a.p[p].col, a.p[p].width, r0 + k) : 0ull;
+            for (int r = 0; r < R; ++r) {
+                const uint32_t e = map_entry(w[r], idx[r], dd.lb);
+                if (e == 0u) alive &= ~(1u << r);
+                if (dd.errv != 0u && ((alive >> r) & 1u)) {
+                    if (e == dd.errv) err |= 1u << r;
+                    g[r] += (e - 1u) * dd.stride;
                 }
             }
+            alive &= ~err;
         }
-        uint32_t errbits = 0;
+        // 5. the rows that joined every dimension (a few per step) are compacted into the wave's
+        // staging area; one lane per row reads its measures.  The last 64 rows' reads are left in
+        // flight until the next step has decoded its codes (the pending batch); earlier batches
+        // (more than 64 rows, rare) are finished at once.  Rows with a corrupt measure go to the
+        // exact path.
+        const bool prev_bad = finish_pending();
+        const uint64_t prev_row = pend_row;
+        uint32_t extra_bad = 0;  // bit b: this lane's row of staged batch b holds a corrupt measure
+        uint32_t *stage = s_stage + wave * kSqSeg;
+        uint32_t total = 0;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int r = 0; r < R; ++r) {
+            const bool al = (alive >> r) & 1u;
+            const uint64_t b = __ballot(al);
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
+            if (al) stage[total + below] = ((r >> 1) * 128u + lane * 2u + (r & 1)) | (g[r] << 8);
+            total += static_cast<uint32_t>(__popcll(b));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t i0 = 0; i0 < total; i0 += 64u) {
+            const bool act = i0 + lane < total;
+            const uint32_t ent = act ? stage[i0 + lane] : 0u;
+            pend_act = act;
+            pend_row = seg + (ent & 0xFFu);
+            pend_g = ent >> 8;
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const uint64_t row = seg + c * 128u + lane * 2u + k;
-                const bool live = full || row < end;
-                bool err = false, alive = live;
-#pragma unroll
-                for (int p = 0; p < NP; ++p) {
-                    const SqPredK &pr = a.p[p];
-                    const uint64_t code = pc[p][c][k];
-                    err |= ((code * pr.inv) & pr.wmask) > pr.maxu;
-                    alive = alive && pred_pass(pr, code);
-                }
-                uint32_t key[ND];
-#pragma unroll
-                for (int d = 0; d < ND; ++d) {
-                    const uint64_t dec = fk[d][c][k] * a.d[d].fk_inv;
-                    err |= dec > a.d[d].fk_maxu;
-                    key[d] = static_cast<uint32_t>(dec) & a.d[d].kmask;
-                }
-                err = err && live;
-                // measures fetched right after the predicates: checked at every row that passes them
-                uint64_t dm[AHEAD_SQ_MAX_MEAS] = {0, 0};
-                if (alive && a.has_pre) {
-                    for (uint32_t m = 0; m < a.nm; ++m) {
-                        const SqMeasK &mk = a.m[m];
-                        if (!mk.pre) continue;
-                        dm[m] = (load_code(mk.col, mk.width, row) * mk.inv) & mk.wmask;
-                        err |= dm[m] > mk.maxu;
-                    }
-                }
-                alive = alive && !err;
-                uint32_t g = 0;
-#pragma unroll
-                for (int d = 0; d < ND; ++d) {
-                    const SqDimK &dd = a.d[d];
-                    if (alive) {
-                        const uint32_t idx = key[d] - dd.kmin;
-                        uint32_t e = 0u;
-                        if (idx < dd.span) {
-                            const uint32_t wi = idx >> (5u - dd.lb);
-                            const uint32_t w = dd.lds_off != kNoLds ? s_map[dd.lds_off + wi] : a.maps[dd.map_off + wi];
-                            e = map_entry(w, idx, dd.lb);
-                        }
-                        if (e == 0u) {
-                            alive = false;
-                        } else if (dd.errv != 0u) {
-                            if (e == dd.errv) err = true;
-                            g += (e - 1u) * dd.stride;
-                        }
-                    }
-                }
-                alive = alive && !err;
-                if (alive) {
-                    for (uint32_t m = 0; m < a.nm; ++m) {
-                        const SqMeasK &mk = a.m[m];
-                        if (mk.pre) continue;
-                        dm[m] = (load_code(mk.col, mk.width, row) * mk.inv) & mk.wmask;
-                        err |= dm[m] > mk.maxu;
-                    }
-                    if (!err && g < a.groups) {
-                        atomicAdd(&s_sum[g], static_cast<unsigned long long>(measure_value(a, dm[0], dm[1])));
-                        atomicMin(&s_first[g], static_cast<uint32_t>(row));
-                        ++kept;
-                    }
-                }
-                if (err) errbits |= 1u << (c * 2 + k);
+            for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m)
+                if (m < static_cast<int>(a.nm)) pend_code[m] = load_code(a.m[m].col, a.m[m].width, act ? pend_row : dummy);
+            if (i0 + 64u < total && finish_pending()) extra_bad |= 1u << (i0 >> 6);
+        }
+        // 6. the exact path (rare), one call site: rows with a corrupt code, the previous step's
+        // pending row, rows of extra batches with a corrupt measure
+        uint32_t todo = err | (prev_bad ? 1u << R : 0u) | (extra_bad << (R + 1));
+        while (__ballot(todo != 0u) != 0ull) {
+            if (todo != 0u) {
+                const int b = __ffs(todo) - 1;
+                todo &= todo - 1u;
+                uint64_t row;
+                if (b < R) row = seg + (b >> 1) * 128u + lane * 2u + (b & 1);
+                else if (b == R) row = prev_row;
+                else row = seg + (stage[(b - R - 1) * 64 + lane] & 0xFFu);
+                exact_row(a, row, s_sum, s_first, kept);
             }
         }
-        if (__ballot(errbits != 0u) != 0ull) {
-            for (int b = 0; b < 8; ++b)
-                if ((errbits >> b) & 1u)
-                    exact_row(a, seg + (b >> 1) * 128u + lane * 2u + (b & 1), s_sum, s_first, kept);
-        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    if (finish_pending()) exact_row(a, pend_row, s_sum, s_first, kept);
     __syncthreads();
     for (uint32_t g = threadIdx.x; g < a.groups; g += kSqBlock) {
         if (s_first[g] != 0xFFFFFFFFu) {
@@ -686,15 +802,65 @@ int check_fact_col(const ahead_col *c, uint64_t n, const char *what, bool need64
     return AHEAD_OK;
 }
 
+// The passing codes of a predicate as at most three intervals: its thresholds cut [0, max] into at
+// most five segments, each passing or failing as a whole; adjacent passing segments merge.
+void pred_intervals(SqPredK &k) {
+    const uint64_t mx = k.wmask;
+    uint64_t t[2] = {k.th1, k.th2};
+    int nt = k.comb == AHEAD_COMB_NONE ? 1 : 2;
+    if (nt == 2 && t[1] < t[0]) std::swap(t[0], t[1]);
+    if (nt == 2 && t[0] == t[1]) nt = 1;
+    struct Seg {
+        uint64_t lo, hi;
+    };
+    Seg seg[5];
+    int ns = 0;
+    uint64_t next = 0;
+    bool more = true;  // next <= mx still to cover
+    for (int i = 0; i < nt; ++i) {
+        if (t[i] > next) seg[ns++] = {next, t[i] - 1};
+        seg[ns++] = {t[i], t[i]};
+        if (t[i] == mx) more = false;
+        else next = t[i] + 1;
+    }
+    if (more) seg[ns++] = {next, mx};
+    int n = 0;
+    for (int i = 0; i < ns; ++i) {
+        if (!pred_pass(k, seg[i].lo)) continue;
+        if (n > 0 && k.lo[n - 1] + k.len[n - 1] + 1 == seg[i].lo) {
+            k.len[n - 1] += seg[i].hi - seg[i].lo + 1;
+        } else {
+            k.lo[n] = seg[i].lo;
+            k.len[n] = seg[i].hi - seg[i].lo;
+            ++n;
+        }
+    }
+    k.never = n == 0 ? 1u : 0u;
+    if (n == 0) {
+        k.lo[0] = 0;
+        k.len[0] = 0;
+        n = 1;
+    }
+    for (int i = n; i < 3; ++i) {
+        k.lo[i] = k.lo[0];
+        k.len[i] = k.len[0];
+    }
+}
+
 template <int ND, int NP>
 void launch_scan(const SqArgs &a, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_sq_scan<ND, NP>), dim3(grid), dim3(kSqBlock), 0, s, a);
+    bool wide = false;
+    for (uint32_t p = 0; p < a.np; ++p) wide = wide || a.p[p].width == 8;
+    if (wide)
+        hipLaunchKernelGGL((k_sq_scan<ND, NP, true>), dim3(grid), dim3(kSqBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_sq_scan<ND, NP, false>), dim3(grid), dim3(kSqBlock), 0, s, a);
 }
 
 template <int ND>
 void launch_scan_np(const SqArgs &a, uint32_t grid, hipStream_t s) {
     switch (a.np) {
-        case 0: launch_scan<ND, 0>(a, grid, s); break;
+        case 0: hipLaunchKernelGGL((k_sq_scan<ND, 0, false>), dim3(grid), dim3(kSqBlock), 0, s, a); break;
         case 1: launch_scan<ND, 1>(a, grid, s); break;
         default: launch_scan<ND, 2>(a, grid, s); break;
     }
@@ -740,8 +906,13 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         if (dm.sel != nullptr && dm.sel_count == nullptr)
             return fail(AHEAD_ERR_INVALID, "star_query: a selection needs its device count");
     }
-    for (uint32_t p = 0; p < q->n_preds; ++p)
-        if ((st = check_fact_col(&q->preds[p].col, n, "a fact predicate column", false)) != AHEAD_OK) return st;
+    for (uint32_t p = 0; p < q->n_preds; ++p) {
+        const ahead_sq_pred &pr = q->preds[p];
+        if ((st = check_fact_col(&pr.col, n, "a fact predicate column", false)) != AHEAD_OK) return st;
+        if (pr.op1 < AHEAD_LT || pr.op1 > AHEAD_NE || pr.op2 < AHEAD_LT || pr.op2 > AHEAD_NE ||
+            pr.comb < AHEAD_COMB_NONE || pr.comb > AHEAD_COMB_OR)
+            return fail(AHEAD_ERR_INVALID, "star_query: unknown predicate or combiner code");
+    }
     for (uint32_t m = 0; m < nm; ++m)
         if ((st = check_fact_col(&q->meas[m], n, "a measure column", false)) != AHEAD_OK) return st;
 
@@ -942,6 +1113,7 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         k.comb = pr.comb;
         k.th1 = pr.th1 & k.wmask;
         k.th2 = pr.th2 & k.wmask;
+        pred_intervals(k);
     }
     for (uint32_t m = 0; m < nm; ++m) {
         SqMeasK &k = a.m[m];
@@ -951,7 +1123,13 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         k.inv = q->meas[m].a_inv;
         k.maxu = q->meas[m].max_u & k.wmask;
         k.pre = (q->meas_pre_mask >> m) & 1u;
+        k.pred = -1;
+        for (uint32_t p = 0; p < q->n_preds; ++p)
+            if (q->preds[p].col.data == k.col && q->preds[p].col.width == k.width && q->preds[p].col.a_inv == k.inv &&
+                (q->preds[p].col.max_u & k.wmask) == k.maxu)
+                k.pred = static_cast<int>(p);
         a.has_pre |= k.pre;
+        if (k.pre && k.pred < 0) a.pre_load = 1;
     }
     a.maps = maps;
     a.lds_words = lds_used;
@@ -966,6 +1144,7 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
     const uint64_t waves = static_cast<uint64_t>(grid) * kSqWaves;
     const uint64_t per = (n + waves - 1) / waves;
     a.rows_per_wave = (per + kSqSeg - 1) / kSqSeg * kSqSeg;
+    if (q->ev_scan[0] != nullptr) AHEAD_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(q->ev_scan[0]), s));
     if (n > 0) {
         switch (nd) {
             case 1: launch_scan_np<1>(a, grid, s); break;
@@ -975,6 +1154,7 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         }
         AHEAD_HIP_TRY(hipGetLastError());
     }
+    if (q->ev_scan[1] != nullptr) AHEAD_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(q->ev_scan[1]), s));
     f.groups = a.groups;
     f.nd = nd;
     f.gsum = a.gsum;

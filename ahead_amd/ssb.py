@@ -616,7 +616,7 @@ FUSED_KINDS = ["pred0", "pred1", "meas0", "meas1", "fk0", "fk1", "fk2", "fk3", "
                "refetch3", "attr0", "attr1", "attr2", "attr3"]
 
 
-def fused(t: dict, q: Query, ops=B, fallback: bool = True) -> dict:
+def fused(t: dict, q: Query, ops=B, fallback: bool = True, ev_scan: tuple | None = None) -> dict:
     """The query with one fact scan (ops.star_query) after the chain's dimension selections.  The
     result has the chain's form; `status` is the fused entry point's (0: it applied).  With a
     status other than 0 and `fallback`, the chain's result is returned (with that status)."""
@@ -643,7 +643,7 @@ def fused(t: dict, q: Query, ops=B, fallback: bool = True) -> dict:
             preds.append((c.tail, p[0], _enc(c, p[1]), p[2], _enc(c, p[3]), p[4]))
     meas = [t[m].tail for m in q.meas]
     n_fact = t[q.meas[0]].tail.n
-    r = ops.star_query(n_fact, dims, preds, meas, q.kind, q.pre_mask)
+    r = ops.star_query(n_fact, dims, preds, meas, q.kind, q.pre_mask, ev_scan=ev_scan)
     if r["status"] != 0:
         if not fallback:
             return {"status": r["status"], "errors": E}
@@ -694,72 +694,133 @@ def _pred_mask(v: np.ndarray, p: tuple) -> np.ndarray:
     return (m & m2) if p[4] == B.AND else (m | m2)
 
 
-def textbook(spec: SsbSpec, q: Query | str, chunk: int = 50_000_000, **kw) -> dict:
-    """GROUP BY / SUM of the query over decoded values, with numpy, the fact table taken in chunks
-    of `chunk` rows.  q: a Query, a QUERIES name, or a SHAPES name with keyword arguments.  Groups
-    come in first-occurrence order over the qualifying fact rows."""
+def codes_chunk(spec: SsbSpec, name: str, s0: int, cnt: int) -> np.ndarray:
+    """Stored codes of fact rows [s0, s0 + cnt) of one column: encoded values with the generator's
+    flips applied (numpy twin of the device columns)."""
+    _, kind, stream, _, _ = COLUMNS[name]
+    meta = META[kind]
+    codes = datagen.encode(host_values(spec, name, {}, s0, cnt), meta.a, meta.width)
+    th = datagen.flip_threshold(spec.rate)
+    if spec.flipped(name) and th:
+        local = np.nonzero(datagen.rand_u64(spec.seed, 100 + stream, cnt, s0) < np.uint64(th))[0]
+        if local.size:
+            r = datagen.mix(local.astype(np.uint64) + np.uint64(s0) +
+                            np.uint64(datagen.stream_key(spec.seed, 100 + stream + 1)))
+            bits = ((r >> np.uint64(32)) * np.uint64(8 * meta.width)) >> np.uint64(32)
+            codes[local] ^= np.ones(local.size, dtype=codes.dtype) << bits.astype(codes.dtype)
+    return codes
+
+
+def _pred_codes(codes: np.ndarray, meta, p: tuple) -> np.ndarray:
+    """A selectAN predicate on stored codes against encoded thresholds (selectAN_scalar.tcc:89)."""
+    m = (1 << (8 * meta.width)) - 1
+    enc = [((v * meta.a) & m) for v in (p[1],) + ((p[3],) if len(p) > 2 else ())]
+    q = (p[0], enc[0]) + ((p[2], enc[1], p[4]) if len(p) > 2 else ())
+    return _pred_mask(codes.astype(np.uint64), q)
+
+
+def _textbook_chunk(spec: SsbSpec, q: Query, dsel: list, dattr: list, s0: int, cnt: int):
+    """One fact row range of the textbook evaluation: (rows, total) for Q1.x, else (rows, {key tuple:
+    [sum, first row]})."""
+    fv = lambda c: host_values(spec, c, {}, s0, cnt)  # noqa: E731
+    m = np.ones(cnt, dtype=bool)
+    for col, p in q.preds:
+        m &= _pred_codes(codes_chunk(spec, col, s0, cnt), META[COLUMNS[col][1]], p)
+    fks = []
+    for i, dq in enumerate(q.dims):
+        fk = fv(DIM_KEYS[dq.name][0]).astype(np.int64)
+        m &= dsel[i][fk]
+        fks.append(fk)
+    idx = np.nonzero(m)[0]
+    m0 = fv(q.meas[0])[idx]
+    with np.errstate(over="ignore"):
+        if q.kind == SUM:
+            val = m0
+        elif q.kind == SUBTRACT:
+            val = m0 - fv(q.meas[1])[idx]
+        else:
+            val = m0 * fv(q.meas[1])[idx]
+        if q.family == "q1":
+            return idx.size, int(val.sum(dtype=np.uint64))
+    if idx.size == 0:
+        return 0, {}
+    keys = np.stack([dattr[d][fks[d][idx]] for d in q.groups], axis=1)
+    uk, firsts, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
+    acc = np.zeros(uk.shape[0], dtype=np.uint64)
+    np.add.at(acc, inv.reshape(-1), val)
+    return idx.size, {tuple(int(x) for x in uk[j]): [int(acc[j]), s0 + int(idx[firsts[j]])] for j in range(uk.shape[0])}
+
+
+def textbook(spec: SsbSpec, q: Query | str, chunk: int = 50_000_000, workers: int = 1, **kw) -> dict:
+    """GROUP BY / SUM of the query with numpy, the fact table taken in chunks of `chunk` rows
+    (evaluated by `workers` threads): predicates on the stored codes (flips included, as selectAN
+    evaluates them), group attributes and measures on decoded values (columns the chains fetch
+    must be fault-free).  q: a Query, a QUERIES name, or a SHAPES name with keyword arguments.
+    Groups come in first-occurrence order over the qualifying fact rows."""
     if isinstance(q, str):
         q = QUERIES[q] if q in QUERIES else SHAPES[q](**kw)
     cache = {}
     v = lambda c: host_values(spec, c, cache)  # noqa: E731
+    th = datagen.flip_threshold(spec.rate)
     # per dimension: selected rows and the group attribute per key (keys are row ids)
     dsel, dattr = [], []
     for dq in q.dims:
         table = COLUMNS[DIM_KEYS[dq.name][1]][0]
         m = np.ones(_rows(spec, table), dtype=bool)
         for col, p in dq.sels:
-            m &= _pred_mask(v(col), p)
+            meta = META[COLUMNS[col][1]]
+            codes = datagen.encode(v(col), meta.a, meta.width)
+            if spec.flipped(col):
+                datagen.inject_flips(codes, spec.seed, 100 + COLUMNS[col][2], th)
+            m &= _pred_codes(codes, meta, p)
         dsel.append(m)
         dattr.append(v(dq.attr) if dq.attr else None)
-    total = 0
-    rows = 0
-    sums: dict = {}
-    first: dict = {}
-    for s0 in range(0, spec.n_fact, chunk):
-        cnt = min(chunk, spec.n_fact - s0)
-        fv = lambda c: host_values(spec, c, {}, s0, cnt)  # noqa: E731
-        m = np.ones(cnt, dtype=bool)
-        for col, p in q.preds:
-            m &= _pred_mask(fv(col), p)
-        fks = []
-        for i, dq in enumerate(q.dims):
-            fk = fv(DIM_KEYS[dq.name][0]).astype(np.int64)
-            m &= dsel[i][fk]
-            fks.append(fk)
-        idx = np.nonzero(m)[0]
-        rows += idx.size
-        m0 = fv(q.meas[0])[idx]
-        if q.kind == SUM:
-            val = m0
-        elif q.kind == SUBTRACT:
-            with np.errstate(over="ignore"):
-                val = m0 - fv(q.meas[1])[idx]
-        else:
-            with np.errstate(over="ignore"):
-                val = m0 * fv(q.meas[1])[idx]
-        if q.family == "q1":
-            with np.errstate(over="ignore"):
-                total = (total + int(val.sum(dtype=np.uint64))) & an.MASK64
-            continue
-        keys = np.stack([dattr[d][fks[d][idx]] for d in q.groups], axis=1) if idx.size else np.zeros((0, len(q.groups)),
-                                                                                                    np.uint64)
-        uk, firsts, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
-        inv = inv.reshape(-1)
-        acc = np.zeros(uk.shape[0], dtype=np.uint64)
-        np.add.at(acc, inv, val)
-        for j in range(uk.shape[0]):
-            k = tuple(int(x) for x in uk[j])
-            if k not in sums:
-                sums[k] = 0
-                first[k] = s0 + int(idx[firsts[j]])
-            sums[k] = (sums[k] + int(acc[j])) & an.MASK64
+    starts = list(range(0, spec.n_fact, chunk))
+    job = lambda s0: _textbook_chunk(spec, q, dsel, dattr, s0, min(chunk, spec.n_fact - s0))  # noqa: E731
+    if workers > 1 and len(starts) > 1:
+        from concurrent.futures import ThreadPoolExecutor
+
+        with ThreadPoolExecutor(workers) as ex:
+            parts = list(ex.map(job, starts))
+    else:
+        parts = [job(s0) for s0 in starts]
+    rows = sum(p[0] for p in parts)
     if q.family == "q1":
-        return {"result": total, "rows": rows}
-    order = sorted(sums, key=lambda k: first[k])
-    return {"sums": np.array([sums[k] for k in order], dtype=np.uint64),
+        return {"result": sum(p[1] for p in parts) & an.MASK64, "rows": rows}
+    sums: dict = {}
+    for _, d in parts:  # chunk order: the first chunk that holds a key holds its first row
+        for k, (sm, f) in d.items():
+            if k not in sums:
+                sums[k] = [0, f]
+            sums[k][0] = (sums[k][0] + sm) & an.MASK64
+    order = sorted(sums, key=lambda k: sums[k][1])
+    return {"sums": np.array([sums[k][0] for k in order], dtype=np.uint64),
             "keys": [np.array([k[j] for k in order], dtype=np.uint64) for j in range(len(q.groups))],
-            "first_row": np.array([first[k] for k in order], dtype=np.uint64),
+            "first_row": np.array([sums[k][1] for k in order], dtype=np.uint64),
             "groups": len(order), "rows": rows}
+
+
+def flip_positions(spec: SsbSpec, name: str, chunk: int = 50_000_000, workers: int = 1) -> np.ndarray:
+    """The rows of column `name` that the generator flips (sorted), computed in chunks."""
+    table, _, stream, _, _ = COLUMNS[name]
+    n = _rows(spec, table)
+    th = np.uint64(datagen.flip_threshold(spec.rate))
+    if not spec.flipped(name) or th == 0:
+        return np.zeros(0, dtype=np.uint64)
+
+    def job(s0):
+        r = datagen.rand_u64(spec.seed, 100 + stream, min(chunk, n - s0), s0)
+        return np.nonzero(r < th)[0].astype(np.uint64) + np.uint64(s0)
+
+    starts = list(range(0, n, chunk))
+    if workers > 1 and len(starts) > 1:
+        from concurrent.futures import ThreadPoolExecutor
+
+        with ThreadPoolExecutor(workers) as ex:
+            parts = list(ex.map(job, starts))
+    else:
+        parts = [job(s0) for s0 in starts]
+    return np.concatenate(parts) if parts else np.zeros(0, dtype=np.uint64)
 
 
 def with_spec(spec: SsbSpec, **kw) -> SsbSpec:

@@ -5,7 +5,7 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
 A step is one pass of the hardened hot path over one batch of synthetic input already resident in
 HBM.  Workloads:
-  star1       : (default) hardened hash join + group-by: 60,000,000 fact rows with a foreign key
+  star1       : hardened hash join + group-by: 60,000,000 fact rows with a foreign key
                 uniform over a dense dimension of 200,000 keys and a measure uniform in
                 [90000, 10500000]; the dimension has two 8-bit attributes with 25 and 5 distinct
                 values coded to uint16 (A=29); keys are 32-bit values coded to uint64 (A=63877),
@@ -20,10 +20,14 @@ HBM.  Workloads:
                 predicates of selectivity 1-50%, 1e-6 flips; one persistent launch (autotuned
                 tiling and workgroup count, replayed from a hipGraph) against one launch per
                 instance.  A step is the whole batch.
-  ssb         : the four star-schema pipelines (Q1.1, Q2.1, Q3.1, Q4.1 operator chains) over a
-                600,000,000-row fact table with 4 foreign keys into date (2,556 rows), customer
-                (3M), supplier (200k) and part (1.4M), integer-coded attributes; one operator
-                launch after another, as the reference's drivers run them.  A step runs all four.
+  ssb         : (default) the star-schema queries Q1.1, Q2.1, Q3.1 and Q4.1 of the reference's SSB
+                drivers over a 600,000,000-row fact table with 4 foreign keys into date (2,556
+                rows), customer (3M), supplier (200k) and part (1.4M), integer-coded attributes,
+                each run as a fused star query (the dimension selections, then one fact scan).  A
+                step runs all four.  Before timing, each query's result is compared with its
+                operator chain (one operator launch after another, as the reference's drivers run
+                them), and after timing Q1.1 with a host textbook evaluation and every selection
+                error vector with the generator's flip positions.
   serve       : the same schema resident in HBM, queries of the four shapes with seeded random
                 predicate constants arriving on Poisson clocks; continuous batching on 4 GPU
                 slots (own streams, admission by HBM) against the unbatched path (one query at a
@@ -57,10 +61,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="star1", choices=["star1", "select_sum", "ops", "batch", "ssb", "serve",
+    ap.add_argument("--workload", default="ssb", choices=["star1", "select_sum", "ops", "batch", "ssb", "serve",
                                                                  "hbm_copy"])
     ap.add_argument("--rows", type=int, default=None, help="override the row count (scaling studies)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true", help="skip the host-side checks of the ssb outputs")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     return ap.parse_args()
 
@@ -673,95 +678,196 @@ def run_serve(args):
     }
 
 
-SSB_BYTES_PER_ROW = {"P1": 20, "P2": 32, "P3": 32, "P4": 48}  # fact-column bytes each pipeline reads
+SSB_SHAPES = ("P1", "P2", "P3", "P4")  # Q1.1, Q2.1, Q3.1, Q4.1 of the reference's SSB drivers
+SSB_BYTES_PER_ROW = {"P1": 20, "P2": 32, "P3": 32, "P4": 48}  # fact-column bytes each query reads once
+SSB_SCAN_KERNEL = {"P1": "k_sq_scan<1, 2>", "P2": "k_sq_scan<3, 0>", "P3": "k_sq_scan<3, 0>", "P4": "k_sq_scan<4, 0>"}
+
+
+def host_cpu() -> dict:
+    """This host's CPU: model name, logical CPUs and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def _same_query(a: dict, b: dict) -> bool:
+    import numpy as np
+
+    if a["rows"] != b["rows"]:
+        return False
+    if "result" in b:
+        if a["result"] != b["result"]:
+            return False
+    elif a["groups"] != b["groups"] or not np.array_equal(a["sums"], b["sums"]) or \
+            not all(np.array_equal(x, y) for x, y in zip(a["keys"], b["keys"])):
+        return False
+    if "errors" in a and "errors" in b:
+        ea, eb = dict(a["errors"].items), dict(b["errors"].items)
+        return sorted(ea) == sorted(eb) and all(np.array_equal(ea[k], eb[k]) for k in eb)
+    return True
 
 
 def run_ssb(args):
+    import numpy as np
     import torch
 
-    from ahead_amd import ssb
+    from ahead_amd import an, capi, ssb
 
     spec = ssb.SsbSpec()
     if args.rows:
         spec.n_fact = args.rows
+    n = spec.n_fact
+    print(f"ssb: generating the schema ({n} fact rows)", file=sys.stderr, flush=True)
     t = ssb.device_tables(spec)
     torch.cuda.synchronize()
-    results, peak = {}, {}
-    for name, fn in ssb.PIPELINES.items():
+    queries = {k: ssb.SHAPES[k]() for k in SSB_SHAPES}
+    # the operator chains (the per-operator drop-in path) once, then the fused form: bit-exact
+    print("ssb: operator chains vs fused star queries", file=sys.stderr, flush=True)
+    chain_ms, chain_res, fused_res = {}, {}, {}
+    for k, q in queries.items():
         torch.cuda.synchronize()
-        base = torch.cuda.memory_allocated()
-        torch.cuda.reset_peak_memory_stats()
-        results[name] = fn(t)
+        q0 = time.perf_counter()
+        chain_res[k] = ssb.chain(t, q)
         torch.cuda.synchronize()
-        peak[name] = (torch.cuda.max_memory_allocated() - base) / spec.n_fact
-    per = {name: 0.0 for name in ssb.PIPELINES}
+        chain_ms[k] = (time.perf_counter() - q0) * 1e3
+        fused_res[k] = ssb.fused(t, q, fallback=False)
+        if fused_res[k]["status"] != 0 or not _same_query(fused_res[k], chain_res[k]):
+            raise RuntimeError(f"{k}: fused star query differs from the operator chain")
+    for _ in range(args.warmup):
+        for q in queries.values():
+            ssb.fused(t, q)
+    torch.cuda.synchronize()
+    ev = capi.HipEvents(len(queries) * args.steps)
+    wall = {k: 0.0 for k in queries}
+    scan_ms = {k: [] for k in queries}
+    print(f"ssb: timing {args.steps} steps of {len(queries)} fused queries", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
+    i = 0
     for _ in range(args.steps):
-        for name, fn in ssb.PIPELINES.items():
+        for k, q in queries.items():
             q0 = time.perf_counter()
-            r = fn(t)
-            torch.cuda.synchronize()
-            per[name] += time.perf_counter() - q0
-            if r["rows"] != results[name]["rows"]:
-                raise RuntimeError(f"{name}: result changed between runs")
+            r = ssb.fused(t, q, ev_scan=ev.pairs[i])  # returns after the query's results are on the host
+            wall[k] += time.perf_counter() - q0
+            if r["rows"] != fused_res[k]["rows"]:
+                raise RuntimeError(f"{k}: result changed between runs")
+            i += 1
+    torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    nq = len(ssb.PIPELINES) * args.steps
-    algo = sum(SSB_BYTES_PER_ROW[k] for k in ssb.PIPELINES) * spec.n_fact * args.steps
-    achieved = algo / el / 1e9
-    summary = {k: {"ms": per[k] / args.steps * 1e3, "rows": results[k]["rows"],
-                   "groups": results[k].get("groups", 1), "errors": results[k]["errors"].total(),
-                   "peak_device_bytes_per_fact_row": round(peak[k], 2)}
-               for k in ssb.PIPELINES}
+    i = 0
+    for _ in range(args.steps):
+        for k in queries:
+            scan_ms[k].append(ev.ms(i))
+            i += 1
+    ev.close()
+    # host checks of the step's outputs: Q1.1 against a textbook evaluation over the whole fact
+    # table, and every selection error vector against the generator's flip positions (closed form:
+    # every single-bit flip is detected)
+    print("ssb: host textbook and flip-position checks", file=sys.stderr, flush=True)
+    workers = min(16, len(os.sched_getaffinity(0)))
+    host_ok = flips_ok = None
+    if not args.no_check:
+        tb = ssb.textbook(spec, queries["P1"], chunk=20_000_000, workers=workers)
+        host_ok = tb["rows"] == fused_res["P1"]["rows"] and tb["result"] == fused_res["P1"]["result"]
+        flips_ok = True
+    aoe = np.uint64(an.A_OID_ERR)
+    for k, q in queries.items():
+        if args.no_check:
+            break
+        e = dict(fused_res[k]["errors"].items)
+        cols = [c for c, _ in q.preds] + [c for d in q.dims for c, _ in d.sels]
+        for c in cols:
+            exp = ssb.flip_positions(spec, c, chunk=20_000_000, workers=workers) * aoe
+            got = e.get(f"select_{c}.e1", np.zeros(0, np.uint64))
+            flips_ok = flips_ok and np.array_equal(got, exp)
+    if not args.no_check and not (host_ok and flips_ok):
+        raise RuntimeError(f"host checks failed: textbook {host_ok}, flip positions {flips_ok}")
+    per = {}
+    for k in queries:
+        sm = float(np.mean(scan_ms[k]))
+        ab = SSB_BYTES_PER_ROW[k] * n
+        per[k] = {"wall_ms": wall[k] / args.steps * 1e3, "scan_ms": sm, "scan_kernel": SSB_SCAN_KERNEL[k],
+                  "scan_algorithmic_bytes": ab, "scan_gbs": ab / (sm * 1e-3) / 1e9,
+                  "scan_frac": ab / (sm * 1e-3) / 1e9 / HBM_PEAK_GBS, "chain_ms": chain_ms[k],
+                  "rows": fused_res[k]["rows"], "groups": fused_res[k].get("groups", 1),
+                  "errors": fused_res[k]["errors"].total()}
+    dom = max(per, key=lambda k: per[k]["scan_ms"])
+    nq = len(queries) * args.steps
     res = {
-        "metric": "hardened star-schema queries/s (Q1.1, Q2.1, Q3.1, Q4.1 operator chains)",
-        "value": nq / el,
-        "unit": "queries/s",
+        "metric": "AN-hardened tuples/s and % HBM peak per op; batched queries/s + p99 vs plain",
+        "value": n * nq / el,
+        "unit": "tuples/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": 1,
+        "warmup": args.warmup,
         "ms_per_step": el / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u16/u32/u64",
+        "dtype": "u16/u64",
         "data": "synthetic (seeded; fact columns generated on device)",
-        "config": {"workload": f"ssb: fact {spec.n_fact} rows, date {spec.n_date}, customer {spec.n_cust}, supplier "
-                               f"{spec.n_supp}, part {spec.n_part}; flips {spec.rate} in selected columns",
-                   "pipelines": summary, "fact_rows_per_s": spec.n_fact * nq / el},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "P1-P4 operator chains (every launch, host waits between operators)",
-                     "kernel_ms": el / nq * 1e3, "algorithmic_bytes": algo // max(1, nq)},
+        "config": {"workload": f"ssb: star schema, fact {n} rows (4 FKs, quantity/discount u8, price/revenue/"
+                               f"supplycost int32, AN-coded per width), date {spec.n_date}, customer {spec.n_cust}, "
+                               f"supplier {spec.n_supp}, part {spec.n_part}; flips {spec.rate} in the selected "
+                               f"columns; a step runs Q1.1, Q2.1, Q3.1, Q4.1 as fused star queries",
+                   "queries_per_s": nq / el, "queries": per,
+                   "chain_queries_per_s": len(queries) / (sum(chain_ms.values()) * 1e-3),
+                   "checks": {"fused_equals_operator_chain": True, "q11_equals_host_textbook": host_ok,
+                              "selection_errors_equal_flip_positions": flips_ok}},
+        "roofline": {"bound": "hbm", "achieved": per[dom]["scan_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": per[dom]["scan_frac"],
+                     "traffic": profiled_traffic("ssb_scan_traffic_600M") if n == 600_000_000 else None,
+                     "kernel": f"{SSB_SCAN_KERNEL[dom]} ({dom})", "kernel_ms": per[dom]["scan_ms"],
+                     "algorithmic_bytes": per[dom]["scan_algorithmic_bytes"]},
     }
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline_ssb(spec, args.cpu_seconds)
+        res["cpu_baseline"] = cpu_baseline_ssb(spec, queries, args.cpu_seconds)
     return res
 
 
-def cpu_baseline_ssb(spec_full, seconds):
-    """The same four operator chains on the oracle's operators (tests/oracle_bat.py), single thread,
-    on a fact sample scaled down until one round fits the time budget."""
+def cpu_baseline_ssb(spec_full, queries, seconds):
+    """The same four queries on the all-core vectorised CPU restatement of the fused star query
+    (oracle/an_simd.cpp: OpenMP over every usable core, AVX-512 / AVX2 build picked by the CPU), on a
+    fact sample; the sample's results are asserted equal to the GPU's on the same sample."""
     sys.path.insert(0, str(ROOT / "tests"))
     import dataclasses
 
     import oracle_bat as OB
+    import oracle_lib as O
     from ahead_amd import ssb
 
-    n = min(spec_full.n_fact, 3_000_000)
+    n = min(spec_full.n_fact, 30_000_000)
     spec = dataclasses.replace(spec_full, n_fact=n)
-    t = OB.tables(spec)
-    done, el = 0, 0.0
-    while el < seconds:
-        for fn in ssb.PIPELINES.values():
-            t0 = time.perf_counter()
-            fn(t, ops=OB)
-            el += time.perf_counter() - t0
-            done += 1
-    return {"value": done / el * (n / spec_full.n_fact), "unit": "queries/s (scaled to the full fact table)",
-            "cores": 1, "kind": "port",
-            "sample": f"{done} pipeline runs on a {n}-row fact sample ({el:.1f} s), single thread, oracle operators "
-                      f"(oracle/an_oracle.c -O3 -fno-tree-vectorize) chained as on the GPU; rate scaled by "
-                      f"{n}/{spec_full.n_fact} rows"}
+    print(f"ssb cpu baseline: {n}-row sample", file=sys.stderr, flush=True)
+    host = OB.tables(spec)
+    dev = ssb.device_tables(spec)
+    threads = len(os.sched_getaffinity(0))
+    OB.SIMD_THREADS = threads
+    try:
+        exact = all(_same_query(ssb.fused(dev, q), ssb.fused(host, q, OB)) for q in queries.values())
+        if not exact:
+            raise RuntimeError("the CPU baseline's results differ from the GPU's on the sample")
+        done, el = 0, 0.0
+        while el < seconds:
+            for q in queries.values():
+                t0 = time.perf_counter()
+                ssb.fused(host, q, OB)
+                el += time.perf_counter() - t0
+                done += 1
+    finally:
+        OB.SIMD_THREADS = None
+    cpu = host_cpu()
+    isa = {4: "AVX-512 (x86-64-v4)", 3: "AVX2 (x86-64-v3)", 0: "x86-64 baseline"}[O.simd_level()]
+    return {"value": n * done / el, "unit": "tuples/s", "cores": threads, "kind": "port",
+            "sample": f"{done} fused star queries (Q1.1/Q2.1/Q3.1/Q4.1) on a {n}-row fact sample of the same "
+                      f"generator ({el:.1f} s); oracle/an_simd.cpp or_star_query_simd, {isa} build, OpenMP on "
+                      f"{threads} threads of {cpu['model']} ({cpu['nproc']} logical CPUs); dimension selections "
+                      f"by the scalar oracle; results bit-exact vs the GPU on the sample: {exact}"}
 
 
 def main():

@@ -343,7 +343,7 @@ int ahead_hip_star1_probe(const ahead_star1_args *in, const ahead_star1_out *out
 #define AHEAD_SQ_MAX_DIMS 4
 #define AHEAD_SQ_MAX_PREDS 2
 #define AHEAD_SQ_MAX_MEAS 2
-#define AHEAD_SQ_MAX_GROUPS 4096
+#define AHEAD_SQ_MAX_GROUPS 2048
 #define AHEAD_SQ_MAX_EVENTS 8192
 #define AHEAD_SQ_KINDS 16
 enum { AHEAD_SQ_SUM = 0, AHEAD_SQ_SUB = 1, AHEAD_SQ_MULSUM = 2 };
@@ -382,6 +382,8 @@ typedef struct {
     ahead_col meas[AHEAD_SQ_MAX_MEAS];
     uint64_t a_result;          /* A of the sums (65117: v2_resbigint_t)                        */
     uint64_t a_oid_err;         /* A of error positions (16089)                                 */
+    void *ev_scan[2];           /* optional hipEvent_t pair recorded on `stream` around the fact
+                                   scan kernel (NULL: none), for timing that kernel alone        */
 } ahead_sq_args;
 
 typedef struct {

@@ -174,8 +174,12 @@ def arithmeticAN(op, b1, b2, a_result=an.A_RESULT, a_oid=A_OID_ERR, result_type=
             _e(e4, b2.tail.meta.type.encoded))
 
 
-def star_query(n_fact, dims, preds, meas, kind, pre_mask=0, a_result=an.A_RESULT, a_oid=A_OID_ERR):
-    """The oracle's fused star query (or_star_query) with the argument form of bat.star_query."""
+SIMD_THREADS = None  # set to a thread count (0: all) to run star queries on the SIMD restatement
+
+
+def star_query(n_fact, dims, preds, meas, kind, pre_mask=0, a_result=an.A_RESULT, a_oid=A_OID_ERR, ev_scan=None):
+    """The oracle's fused star query (or_star_query, or or_star_query_simd when SIMD_THREADS is set)
+    with the argument form of bat.star_query."""
     odims = []
     for d in dims:
         sel = d.get("sel")
@@ -183,7 +187,8 @@ def star_query(n_fact, dims, preds, meas, kind, pre_mask=0, a_result=an.A_RESULT
                       "sel": sel.arr if sel is not None else None,
                       "sel_a_inv": sel.meta.a_inv if sel is not None else an.A_OID_INV, "flags": d.get("flags", 0)})
     opreds = [(c.oc(), op1, th1, op2, th2, comb) for c, op1, th1, op2, th2, comb in preds]
-    return O.star_query(n_fact, odims, opreds, [m.oc() for m in meas], kind, pre_mask, a_result, a_oid)
+    return O.star_query(n_fact, odims, opreds, [m.oc() for m in meas], kind, pre_mask, a_result, a_oid,
+                        simd_threads=SIMD_THREADS)
 
 
 def tables(spec) -> dict:

@@ -16,6 +16,7 @@ from ahead_amd import an
 ROOT = Path(__file__).resolve().parents[1]
 ORACLE_DIR = ROOT / "oracle"
 ORACLE_SO = ORACLE_DIR / "_build" / "liboracle.so"
+SIMD_SO = ORACLE_DIR / "_build" / "liboracle_simd.so"
 
 
 class OrCol(ctypes.Structure):
@@ -290,10 +291,47 @@ SQ_SUM, SQ_SUB, SQ_MULSUM = 0, 1, 2
 SQ_PROBE, SQ_REFETCH = 1, 2
 
 
+_simd = None
+
+
+def simd_lib():
+    """The all-core vectorised restatement (oracle/an_simd.cpp), the CPU baseline."""
+    global _simd
+    if _simd is None:
+        if not SIMD_SO.exists():
+            build()
+        L = ctypes.CDLL(str(SIMD_SO))
+        P = ctypes.POINTER
+        c = P(OrCol)
+        v = P(OrVec)
+        u64, i = ctypes.c_uint64, ctypes.c_int
+        sig = {"or_simd_level": ([], i),
+               "or_select_an_simd": ([c, i, u64, i, u64, i, u64, u64, v, v, i], u64),
+               "or_star_query_simd": ([P(OrSqArgs), P(OrSqOut), i], i)}
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _simd = L
+    return _simd
+
+
+def simd_level() -> int:
+    return int(simd_lib().or_simd_level())
+
+
+def select_an_simd(col: Col, op1, th1, op2=0, th2=0, comb=0, a_head=an.A_OID, a_oid=an.A_OID_ERR, threads=0):
+    vs = Vecs(2)
+    s = simd_lib().or_select_an_simd(col.ref, op1, th1, op2, th2, comb, a_head, a_oid, *vs.refs(), threads)
+    out = vs.numpy()
+    return out[0], out[1], int(s)
+
+
 def star_query(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_mask: int = 0,
-               a_result=an.A_RESULT, a_oid_err=an.A_OID_ERR) -> dict:
-    """or_star_query.  dims: dicts with fk, key (Col), attr (Col or None), sel (uint64 numpy array of
-    encoded OIDs or None), sel_a_inv, flags; preds: (Col, op1, th1, op2, th2, comb); meas: Cols."""
+               a_result=an.A_RESULT, a_oid_err=an.A_OID_ERR, simd_threads: int | None = None) -> dict:
+    """or_star_query (or, with simd_threads set, or_star_query_simd on that many threads, 0: all).
+    dims: dicts with fk, key (Col), attr (Col or None), sel (uint64 numpy array of encoded OIDs or
+    None), sel_a_inv, flags; preds: (Col, op1, th1, op2, th2, comb); meas: Cols."""
     a = OrSqArgs()
     a.n_fact = n_fact
     a.n_dims = len(dims)
@@ -323,7 +361,10 @@ def star_query(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_
     a.a_result = a_result
     a.a_oid_err = a_oid_err
     o = OrSqOut()
-    st = lib().or_star_query(ctypes.byref(a), ctypes.byref(o))
+    if simd_threads is None:
+        st = lib().or_star_query(ctypes.byref(a), ctypes.byref(o))
+    else:
+        st = simd_lib().or_star_query_simd(ctypes.byref(a), ctypes.byref(o), simd_threads)
     res = {"status": st, "kept": int(o.kept), "sums": o.sums.numpy(), "first_row": o.first_row.numpy(),
            "keys": {i: o.keys[i].numpy() for i, d in enumerate(dims) if d.get("attr") is not None},
            "err": [o.err[k].numpy() for k in range(16)]}

@@ -77,3 +77,21 @@ def test_star_query_empty_and_tiny_fact_tables():
             got = ssb.fused(ssb.device_tables(spec), q, fallback=False)
             ref = ssb.fused(OB.tables(spec), q, OB, fallback=False)
             same(got, ref)
+
+
+def test_star_query_where_every_row_joins():
+    """No dimension selection: every fact row joins, so each wave step stages more rows than one
+    batch of 64 and finishes the earlier batches at once; flips in the measures, a key and a group
+    attribute send rows of every batch to the exact path."""
+    q = ssb.Query("q41", [ssb.Dim("supp", [], "s_nation"), ssb.Dim("date", [], "d_year", ssb.REFETCH)],
+                  ["lo_revenue", "lo_supplycost"], ssb.SUBTRACT, groups=[1, 0])
+    spec = ssb.SsbSpec(rate=4e-3, seed=35, flip_cols=("lo_revenue", "lo_supplycost", "lo_orderdate", "s_nation"),
+                       **SMALL)
+    print("star_query every row joins", flush=True)
+    got = ssb.fused(ssb.device_tables(spec), q, fallback=False)
+    ref = ssb.fused(OB.tables(spec), q, OB, fallback=False)
+    assert got["status"] == 0 and ref["status"] == 0
+    same(got, ref)
+    assert np.array_equal(got["first_row"], ref["first_row"])
+    e = dict(ref["errors"].items)
+    assert e["fused_meas0.e1"].size > 100 and e["fused_attr0.e1"].size > 100

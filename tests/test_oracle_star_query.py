@@ -65,6 +65,7 @@ def test_fused_equals_chain_with_selection_flips(name):
     f = ssb.fused(t, q, OB, fallback=False)
     c = ssb.chain(t, q, OB)
     same_result(f, c)
+    same_result(f, ssb.textbook(spec, q))  # predicates on the stored (flipped) codes
     fe, ce = errs(f), errs(c)
     assert {k for k in ce if not k.startswith("select_")} == set()
     assert set(fe) == set(ce)
