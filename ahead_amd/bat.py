@@ -368,6 +368,7 @@ def star_query(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_
     cdims = []
     for d in dims:
         x = dict(d)
+        x["key_index"] = key_index(d["key"])
         sel = d.get("sel")
         if sel is not None:
             x["sel"] = sel.buf if sel.buf is not None else DeviceBuffer(8)
@@ -377,3 +378,13 @@ def star_query(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_
     q = capi.StarQuery(n_fact, cdims, preds, meas, kind, pre_mask, a_result, a_oid, ev_scan=ev_scan)
     q.launch()
     return q.results()
+
+
+def key_index(key: Column) -> "capi.StarKeyIndex":
+    """The star-query key index of a resident dimension key column, built at its first use and kept
+    with the column (the column's codes must not change afterwards)."""
+    ix = getattr(key, "_star_key_index", None)
+    if ix is None:
+        ix = capi.StarKeyIndex(key)
+        key._star_key_index = ix
+    return ix

@@ -82,7 +82,7 @@ SQ_STATUS = {0: "OK", 1: "BAD_OID", 2: "BAD_KEY", 3: "DUP_KEY", 4: "EVENTS", 5: 
 class SqDim(ctypes.Structure):
     _fields_ = [("fk", AheadCol), ("key", AheadCol), ("attr", AheadCol), ("sel", ctypes.c_void_p),
                 ("sel_count", ctypes.c_void_p), ("sel_a_inv", ctypes.c_uint64), ("flags", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("reserved", ctypes.c_uint32), ("key_index", ctypes.c_void_p)]
 
 
 class SqPred(ctypes.Structure):
@@ -152,6 +152,9 @@ _SIGS = {
     "ahead_hip_select_batch_destroy": ([_vp], _int),
     "ahead_hip_select_batch_info": ([_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)], _int),
     "ahead_hip_star_query": ([ctypes.POINTER(SqArgs), ctypes.POINTER(SqOut), _vp, _vp], _int),
+    "ahead_hip_star_key_index_create": ([_colp, ctypes.POINTER(_vp), _vp], _int),
+    "ahead_hip_star_key_index_destroy": ([_vp], _int),
+    "ahead_hip_star_key_index_status": ([_vp], _int),
     "ahead_hip_gen_uniform": ([_vp, _u32, _u64, _u64, _u64, _u64, _u64, _u64, _vp], _int),
     "ahead_hip_inject_flips": ([_vp, _u32, _u64, _u64, _u64, _u64, _vp], _int),
 }
@@ -609,6 +612,8 @@ class StarQuery:
             x.sel_count = d["sel_count"].ptr if d.get("sel") is not None else None
             x.sel_a_inv = d.get("sel_a_inv", an.A_OID_INV) & an.MASK64
             x.flags = d.get("flags", 0)
+            ki = d.get("key_index")
+            x.key_index = ki.h if ki is not None else None
         for i, (col, op1, th1, op2, th2, comb) in enumerate(preds):
             a.preds[i] = SqPred(col.desc(), op1, op2, comb, 0, th1 & an.MASK64, th2 & an.MASK64)
         for i, m in enumerate(meas):
@@ -881,6 +886,33 @@ def arithmetic_an(h1: Column, t1: Column, h2: Column, t2: Column, op: int, a_res
 
     b, c = _run(call, outs, 5, [None, 1, 2, 3, 4])
     return (b[0].to_numpy(np.dtype(f"u{result_width}"), c[0]), *(b[i].to_numpy(np.uint64, c[i]) for i in range(1, 5)))
+
+
+class StarKeyIndex:
+    """ahead_hip_star_key_index_create over a resident key column (kept alive with the index)."""
+
+    def __init__(self, key: Column, stream: int | None = None):
+        self.key = key
+        h = ctypes.c_void_p()
+        d = key.desc()
+        check(load().ahead_hip_star_key_index_create(ctypes.byref(d), ctypes.byref(h),
+                                                     current_stream() if stream is None else stream))
+        self.h = h
+
+    @property
+    def status(self) -> int:
+        return int(load().ahead_hip_star_key_index_status(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            check(load().ahead_hip_star_key_index_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class HipEvents:

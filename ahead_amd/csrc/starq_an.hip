@@ -53,12 +53,14 @@ namespace {
 
 constexpr int kSqBlock = 1024;
 constexpr int kSqWaves = kSqBlock / 64;
-constexpr int kSqChunks = 2;                // chunks of 128 rows per wave step
-constexpr int kSqSeg = kSqChunks * 128;     // rows per wave step
-constexpr int kSqRows = 2 * kSqChunks;      // rows per lane per step
+// A wave step covers CH chunks of 128 rows (the registers of a step's codes and of its prefetched
+// successor decide; 4 chunks spill even at one dimension with two fact predicates).
+constexpr int sq_chunks(int) { return 2; }
 constexpr uint32_t kSqMaxGroups = AHEAD_SQ_MAX_GROUPS;
 constexpr uint32_t kSqMaxEvents = AHEAD_SQ_MAX_EVENTS;
-constexpr uint32_t kSqMapWords = 28416;  // LDS words for maps (with group tables and staging: < 160 KiB)
+constexpr uint32_t kSqMapWords = 28416;  // LDS words for maps with 2-chunk steps (with group tables and staging: < 160 KiB)
+// map words that fit in LDS beside the group tables and a staging area of 128 * ch rows per wave
+constexpr uint32_t sq_map_words(int ch) { return kSqMapWords - static_cast<uint32_t>((ch - 2) * 128 * 16); }
 constexpr uint32_t kNoLds = 0xFFFFFFFFu;
 constexpr int kKinds = AHEAD_SQ_KINDS;
 // flags[] words
@@ -176,6 +178,36 @@ __device__ __forceinline__ void load_pair(const void *col, uint32_t width, uint6
     }
 }
 
+// load_pair for a column width known at compile time (PWB bytes)
+template <int PWB, typename T>
+__device__ __forceinline__ void load_pair_w(const void *col, uint64_t r0, T &x0, T &x1) {
+    if constexpr (PWB == 1) {
+        const uint16_t w = static_cast<const uint16_t *>(col)[r0 >> 1];
+        x0 = static_cast<T>(w & 0xFFu);
+        x1 = static_cast<T>(w >> 8);
+    } else if constexpr (PWB == 2) {
+        const uint32_t w = static_cast<const uint32_t *>(col)[r0 >> 1];
+        x0 = static_cast<T>(w & 0xFFFFu);
+        x1 = static_cast<T>(w >> 16);
+    } else if constexpr (PWB == 4) {
+        const uint64_t w = static_cast<const uint64_t *>(col)[r0 >> 1];
+        x0 = static_cast<T>(w & 0xFFFFFFFFull);
+        x1 = static_cast<T>(w >> 32);
+    } else {
+        const u32x4 q = load16_nt(static_cast<const uint64_t *>(col) + r0);
+        x0 = static_cast<T>((static_cast<uint64_t>(q[1]) << 32) | q[0]);
+        x1 = static_cast<T>((static_cast<uint64_t>(q[3]) << 32) | q[2]);
+    }
+}
+
+template <int PWB>
+__device__ __forceinline__ uint64_t load_code_w(const void *col, uint64_t i) {
+    if constexpr (PWB == 1) return static_cast<const uint8_t *>(col)[i];
+    else if constexpr (PWB == 2) return static_cast<const uint16_t *>(col)[i];
+    else if constexpr (PWB == 4) return static_cast<const uint32_t *>(col)[i];
+    else return static_cast<const uint64_t *>(col)[i];
+}
+
 __device__ __forceinline__ uint32_t map_entry(uint32_t w, uint32_t idx, uint32_t lb) {
     const uint32_t sh = 5u - lb;
     const uint32_t bits = 1u << lb;
@@ -272,23 +304,28 @@ __device__ void exact_row(const SqArgs &a, uint64_t row, unsigned long long *s_s
     ++kept;
 }
 
-// One wave step covers kSqSeg rows: kSqChunks chunks of 128 rows, each lane holding 2 consecutive
-// rows of every chunk (kSqRows rows per lane).  Its FK codes (and predicate codes) are loaded into
+// One wave step covers CH chunks of 128 rows, each lane holding 2 consecutive rows of every chunk
+// (R rows per lane).  Its FK codes (and predicate codes) are loaded into
 // registers one step ahead: as soon as a step has decoded its keys, the wave issues the next step's
 // loads into the same registers, so they are in flight while the step's dependent reads run (the
 // L2 maps, the measures).  Dependent reads are issued for all of a lane's rows at once and waited
 // for once: a row whose value is not needed reads a fixed address of the step instead of
 // branching, so no read waits behind another row's branch.
-template <int ND, int NP, bool PW8>
+// PWB: the byte width shared by the fact predicate columns (1, 2, 4, 8), or 0 when they differ
+template <int ND, int NP, int PWB>
 __global__ __launch_bounds__(kSqBlock) void k_sq_scan(SqArgs a) {
-    __shared__ uint32_t s_map[kSqMapWords];
+    constexpr bool PW8 = PWB == 0 || PWB == 8;
+    constexpr int CH = sq_chunks(ND);
+    constexpr int kSqSeg = CH * 128;  // rows per wave step
+    constexpr int R = 2 * CH;         // rows per lane per step
+    constexpr uint32_t OFFB = CH == 4 ? 9u : 8u;  // bits of a row's offset in its step
+    __shared__ uint32_t s_map[sq_map_words(CH)];
     __shared__ unsigned long long s_sum[kSqMaxGroups];
     __shared__ uint32_t s_first[kSqMaxGroups];
-    __shared__ uint32_t s_stage[kSqWaves * kSqSeg];  // per wave: the step's joined rows (offset | group << 8)
+    __shared__ uint32_t s_stage[kSqWaves * kSqSeg];  // per wave: the step's joined rows (offset | group << OFFB)
     static_assert(sizeof(s_map) + sizeof(s_sum) + sizeof(s_first) + sizeof(s_stage) <= 160 * 1024 - 1024,
                   "LDS budget");
-    static_assert(kSqSeg <= 256 && kSqMaxGroups < (1u << 24), "staged entry layout");
-    constexpr int R = kSqRows;
+    static_assert((1u << OFFB) == kSqSeg && kSqMaxGroups <= (1u << (32 - OFFB)), "staged entry layout");
     if (a.flags[kFlagDup] != 0u) return;
     for (uint32_t i = threadIdx.x; i < a.lds_words; i += kSqBlock) s_map[i] = a.maps[i];
     for (uint32_t g = threadIdx.x; g < a.groups; g += kSqBlock) {
@@ -310,7 +347,7 @@ __global__ __launch_bounds__(kSqBlock) void k_sq_scan(SqArgs a) {
     auto load_step = [&](uint64_t seg) {
         if (seg + kSqSeg <= end) {
 #pragma unroll
-            for (int c = 0; c < kSqChunks; ++c) {
+            for (int c = 0; c < CH; ++c) {
                 const uint64_t r0 = seg + c * 128u + lane * 2u;
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
@@ -319,7 +356,10 @@ __global__ __launch_bounds__(kSqBlock) void k_sq_scan(SqArgs a) {
                     fk[d][2 * c + 1] = (static_cast<uint64_t>(q[3]) << 32) | q[2];
                 }
 #pragma unroll
-                for (int p = 0; p < NP; ++p) load_pair(a.p[p].col, a.p[p].width, r0, pc[p][2 * c], pc[p][2 * c + 1]);
+                for (int p = 0; p < NP; ++p) {
+                    if constexpr (PWB > 0) load_pair_w<PWB>(a.p[p].col, r0, pc[p][2 * c], pc[p][2 * c + 1]);
+                    else load_pair(a.p[p].col, a.p[p].width, r0, pc[p][2 * c], pc[p][2 * c + 1]);
+                }
             }
         } else {
 #pragma unroll
@@ -329,7 +369,10 @@ __global__ __launch_bounds__(kSqBlock) void k_sq_scan(SqArgs a) {
 #pragma unroll
                 for (int d = 0; d < ND; ++d) fk[d][r] = in ? a.d[d].fk[row] : 0ull;
 #pragma unroll
-                for (int p = 0; p < NP; ++p) pc[p][r] = in ? static_cast<PT>(load_code(a.p[p].col, a.p[p].width, row)) : PT(0);
+                for (int p = 0; p < NP; ++p) {
+                    if constexpr (PWB > 0) pc[p][r] = in ? static_cast<PT>(load_code_w<PWB>(a.p[p].col, row)) : PT(0);
+                    else pc[p][r] = in ? static_cast<PT>(load_code(a.p[p].col, a.p[p].width, row)) : PT(0);
+                }
             }
         }
     };
@@ -465,7 +508,7 @@ __global__ __launch_bounds__(kSqBlock) void k_sq_scan(SqArgs a) {
             const uint64_t b = __ballot(al);
             const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
                                                             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
-            if (al) stage[total + below] = ((r >> 1) * 128u + lane * 2u + (r & 1)) | (g[r] << 8);
+            if (al) stage[total + below] = ((r >> 1) * 128u + lane * 2u + (r & 1)) | (g[r] << OFFB);
             total += static_cast<uint32_t>(__popcll(b));
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -475,8 +518,8 @@ __global__ __launch_bounds__(kSqBlock) void k_sq_scan(SqArgs a) {
             const bool act = i0 + lane < total;
             const uint32_t ent = act ? stage[i0 + lane] : 0u;
             pend_act = act;
-            pend_row = seg + (ent & 0xFFu);
-            pend_g = ent >> 8;
+            pend_row = seg + (ent & ((1u << OFFB) - 1u));
+            pend_g = ent >> OFFB;
 #pragma unroll
             for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m)
                 if (m < static_cast<int>(a.nm)) pend_code[m] = load_code(a.m[m].col, a.m[m].width, act ? pend_row : dummy);
@@ -492,7 +535,7 @@ __global__ __launch_bounds__(kSqBlock) void k_sq_scan(SqArgs a) {
                 uint64_t row;
                 if (b < R) row = seg + (b >> 1) * 128u + lane * 2u + (b & 1);
                 else if (b == R) row = prev_row;
-                else row = seg + (stage[(b - R - 1) * 64 + lane] & 0xFFu);
+                else row = seg + (stage[(b - R - 1) * 64 + lane] & ((1u << OFFB) - 1u));
                 exact_row(a, row, s_sum, s_first, kept);
             }
         }
@@ -849,18 +892,20 @@ void pred_intervals(SqPredK &k) {
 
 template <int ND, int NP>
 void launch_scan(const SqArgs &a, uint32_t grid, hipStream_t s) {
-    bool wide = false;
-    for (uint32_t p = 0; p < a.np; ++p) wide = wide || a.p[p].width == 8;
-    if (wide)
-        hipLaunchKernelGGL((k_sq_scan<ND, NP, true>), dim3(grid), dim3(kSqBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_sq_scan<ND, NP, false>), dim3(grid), dim3(kSqBlock), 0, s, a);
+    uint32_t w = a.p[0].width;
+    for (uint32_t p = 1; p < a.np; ++p)
+        if (a.p[p].width != w) w = 0;
+    switch (w) {
+        case 2: hipLaunchKernelGGL((k_sq_scan<ND, NP, 2>), dim3(grid), dim3(kSqBlock), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_sq_scan<ND, NP, 4>), dim3(grid), dim3(kSqBlock), 0, s, a); break;
+        default: hipLaunchKernelGGL((k_sq_scan<ND, NP, 0>), dim3(grid), dim3(kSqBlock), 0, s, a); break;
+    }
 }
 
 template <int ND>
 void launch_scan_np(const SqArgs &a, uint32_t grid, hipStream_t s) {
     switch (a.np) {
-        case 0: hipLaunchKernelGGL((k_sq_scan<ND, 0, false>), dim3(grid), dim3(kSqBlock), 0, s, a); break;
+        case 0: hipLaunchKernelGGL((k_sq_scan<ND, 0, 0>), dim3(grid), dim3(kSqBlock), 0, s, a); break;
         case 1: launch_scan<ND, 1>(a, grid, s); break;
         default: launch_scan<ND, 2>(a, grid, s); break;
     }
@@ -877,6 +922,100 @@ struct DimPlan {
 };
 
 }  // namespace
+
+struct ahead_sq_key_index {
+    int device = 0;
+    const void *key = nullptr;  // the key column it was built from
+    uint64_t n = 0;
+    uint64_t a_inv = 0;
+    uint32_t unenc_bits = 0;
+    uint32_t kmin = 0, span = 0;
+    int status = AHEAD_SQ_ST_OK;
+    uint32_t *row_of = nullptr;  // [span] row of every key (device)
+};
+
+extern "C" int ahead_hip_star_key_index_create(const ahead_col *key, ahead_sq_key_index **out, void *stream) {
+    if (key == nullptr || out == nullptr) return fail(AHEAD_ERR_INVALID, "star_key_index: NULL argument");
+    *out = nullptr;
+    int st = check_col(key, "star_key_index.key", false, true);
+    if (st != AHEAD_OK) return st;
+    if (key->width != 8) return fail(AHEAD_ERR_INVALID, "star_key_index: keys must be 8-byte codes");
+    if (key->n == 0 || key->n >= (1ull << 27)) return fail(AHEAD_ERR_TOO_LARGE, "star_key_index: 1 .. 2^27-1 rows");
+    if (key->unenc_bits > 32) return fail(AHEAD_ERR_INVALID, "star_key_index: keys wider than 32 bits");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto *ix = new ahead_sq_key_index();
+    AHEAD_HIP_TRY(hipGetDevice(&ix->device));
+    ix->key = key->data;
+    ix->n = key->n;
+    ix->a_inv = key->a_inv;
+    ix->unenc_bits = key->unenc_bits;
+    const uint64_t span_cap = 4ull * key->n + 1024;
+    uint32_t *tmp = nullptr;  // flags (4 words), stats (kStatWords words), seen bitmap
+    const size_t seen_words = span_cap / 32 + 1;
+    if (hipMalloc(reinterpret_cast<void **>(&tmp), (4 + kStatWords + seen_words) * 4) != hipSuccess) {
+        delete ix;
+        return fail(AHEAD_ERR_NOMEM, "star_key_index: device allocation failed");
+    }
+    auto done = [&](int rc) {
+        (void)hipFree(tmp);
+        if (rc != AHEAD_OK) {
+            if (ix->row_of) (void)hipFree(ix->row_of);
+            delete ix;
+        } else {
+            *out = ix;
+        }
+        return rc;
+    };
+    uint32_t *flags = tmp, *stats = tmp + 4, *seen = tmp + 4 + kStatWords;
+    if ((st = fill_u32(flags, 4, 0u, s)) != AHEAD_OK) return done(st);
+    if ((st = fill_u32(stats, kStatWords, 0xFFFFFFFFu, s)) != AHEAD_OK) return done(st);
+    DimPrepArgs a{};
+    a.key = static_cast<const uint64_t *>(key->data);
+    a.n = key->n;
+    a.key_inv = key->a_inv;
+    a.key_maxu = key->max_u;
+    a.kmask = static_cast<uint32_t>(bits_mask(key->unenc_bits));
+    a.stats = stats;
+    a.flags = flags;
+    hipLaunchKernelGGL(k_sq_keystats, dim3(prep_grid(a.n)), dim3(kBlock), 0, s, a);
+    uint32_t h[4 + kStatWords];
+    if (hipMemcpyAsync(h, tmp, sizeof h, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return done(fail(AHEAD_ERR_HIP, "star_key_index: key statistics"));
+    if (h[kFlagBadKey]) {
+        ix->status = AHEAD_SQ_ST_BAD_KEY;
+        return done(AHEAD_OK);
+    }
+    ix->kmin = h[4];
+    const uint64_t span = static_cast<uint64_t>(~h[5]) - ix->kmin + 1;
+    if (span > span_cap) {
+        ix->status = AHEAD_SQ_ST_KEY_SPAN;
+        return done(AHEAD_OK);
+    }
+    ix->span = static_cast<uint32_t>(span);
+    if (hipMalloc(reinterpret_cast<void **>(&ix->row_of), static_cast<size_t>(ix->span) * 4) != hipSuccess)
+        return done(fail(AHEAD_ERR_NOMEM, "star_key_index: device allocation failed"));
+    if ((st = fill_u32(seen, ix->span / 32 + 1, 0u, s)) != AHEAD_OK) return done(st);
+    a.kmin = ix->kmin;
+    a.span = ix->span;
+    a.seen = seen;
+    a.row_of = ix->row_of;
+    hipLaunchKernelGGL(k_sq_keys, dim3(prep_grid(a.n)), dim3(kBlock), 0, s, a);
+    if (hipMemcpyAsync(h, flags, 4 * 4, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return done(fail(AHEAD_ERR_HIP, "star_key_index: key check"));
+    if (h[kFlagDup]) ix->status = AHEAD_SQ_ST_DUP_KEY;
+    return done(AHEAD_OK);
+}
+
+extern "C" int ahead_hip_star_key_index_destroy(ahead_sq_key_index *ix) {
+    if (ix == nullptr) return AHEAD_OK;
+    if (ix->row_of) AHEAD_HIP_TRY(hipFree(ix->row_of));
+    delete ix;
+    return AHEAD_OK;
+}
+
+extern "C" int ahead_hip_star_key_index_status(const ahead_sq_key_index *ix) {
+    return ix == nullptr ? -1 : ix->status;
+}
 
 extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *out, ahead_hip_ws *ws,
                                     void *stream) {
@@ -905,6 +1044,10 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         }
         if (dm.sel != nullptr && dm.sel_count == nullptr)
             return fail(AHEAD_ERR_INVALID, "star_query: a selection needs its device count");
+        if (dm.key_index != nullptr &&
+            (dm.key_index->key != dm.key.data || dm.key_index->n != dm.key.n || dm.key_index->a_inv != dm.key.a_inv ||
+             dm.key_index->unenc_bits != dm.key.unenc_bits))
+            return fail(AHEAD_ERR_INVALID, "star_query: a key index built from another key column");
     }
     for (uint32_t p = 0; p < q->n_preds; ++p) {
         const ahead_sq_pred &pr = q->preds[p];
@@ -942,6 +1085,7 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
     }
     off = off_maps + align_up(maps_words_cap * 4, 256);
     for (uint32_t d = 0; d < nd; ++d) {
+        if (q->dims[d].key_index != nullptr) continue;
         P[d].off_seen = off;
         off += align_up((P[d].span_cap / 32 + 1) * 4, 256);
         P[d].off_row = off;
@@ -975,7 +1119,7 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         a.amask = static_cast<uint32_t>(bits_mask(dm.attr.unenc_bits));
         a.stats = stats + d * kStatWords;
         a.flags = flags;
-        hipLaunchKernelGGL(k_sq_keystats, dim3(prep_grid(a.n)), dim3(kBlock), 0, s, a);
+        if (dm.key_index == nullptr) hipLaunchKernelGGL(k_sq_keystats, dim3(prep_grid(a.n)), dim3(kBlock), 0, s, a);
         hipLaunchKernelGGL(k_sq_selstats, dim3(prep_grid(a.n)), dim3(kBlock), 0, s, a);
     }
     AHEAD_HIP_TRY(hipGetLastError());
@@ -991,6 +1135,11 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         AHEAD_HIP_TRY(hipGetLastError());
         return AHEAD_OK;
     };
+    // statuses in dimension order, as the oracle reports them
+    for (uint32_t d = 0; d < nd; ++d) {
+        const ahead_sq_key_index *ix = q->dims[d].key_index;
+        if (ix != nullptr && ix->status != AHEAD_SQ_ST_OK) return not_fused(ix->status);
+    }
     if (h_flags[kFlagBadKey]) return not_fused(AHEAD_SQ_ST_BAD_KEY);
     if (h_flags[kFlagBadOid]) return not_fused(AHEAD_SQ_ST_BAD_OID);
     uint64_t groups = 1;
@@ -998,11 +1147,17 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
     for (uint32_t d = 0; d < nd; ++d) {
         const uint32_t *sd = h_stats + d * kStatWords;
         DimPlan &pl = P[d];
-        pl.kmin = sd[0];
-        const uint32_t kmax = ~sd[1];
-        const uint64_t span = static_cast<uint64_t>(kmax) - pl.kmin + 1;
-        if (span > pl.span_cap) return not_fused(AHEAD_SQ_ST_KEY_SPAN);
-        pl.span = static_cast<uint32_t>(span);
+        const ahead_sq_key_index *ix = q->dims[d].key_index;
+        if (ix != nullptr) {
+            pl.kmin = ix->kmin;
+            pl.span = ix->span;
+        } else {
+            pl.kmin = sd[0];
+            const uint32_t kmax = ~sd[1];
+            const uint64_t span = static_cast<uint64_t>(kmax) - pl.kmin + 1;
+            if (span > pl.span_cap) return not_fused(AHEAD_SQ_ST_KEY_SPAN);
+            pl.span = static_cast<uint32_t>(span);
+        }
         const uint32_t nsel = sd[4];
         pl.frac = static_cast<double>(nsel) / static_cast<double>(q->dims[d].key.n);
         if (q->dims[d].attr.data != nullptr) {
@@ -1029,8 +1184,9 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
     std::iota(by_frac.begin(), by_frac.end(), 0u);
     std::stable_sort(by_frac.begin(), by_frac.end(), [&](uint32_t x, uint32_t y) { return P[x].frac < P[y].frac; });
     uint32_t lds_used = 0;
+    const uint32_t lds_cap = sq_map_words(sq_chunks(static_cast<int>(nd)));
     for (uint32_t d : by_frac) {
-        if (lds_used + P[d].map_words <= kSqMapWords) {
+        if (lds_used + P[d].map_words <= lds_cap) {
             P[d].in_lds = true;
             lds_used += P[d].map_words;
         }
@@ -1055,17 +1211,22 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         const uint32_t d = order[i];
         DimPlan &pl = P[d];
         DimPrepArgs &pa = pl.prep;
-        uint32_t *seen = reinterpret_cast<uint32_t *>(base + pl.off_seen);
-        if ((st = fill_u32(seen, pl.span / 32 + 1, 0u, s)) != AHEAD_OK) return st;
+        const ahead_sq_key_index *ix = q->dims[d].key_index;
         pa.kmin = pl.kmin;
         pa.span = pl.span;
         pa.lb = pl.lb;
         pa.amin = pl.amin;
         pa.errv = q->dims[d].attr.data != nullptr ? static_cast<uint32_t>((1ull << (1u << pl.lb)) - 1ull) : 0u;
         pa.map = maps + pl.off_map;
-        pa.seen = seen;
-        pa.row_of = reinterpret_cast<uint32_t *>(base + pl.off_row);
-        hipLaunchKernelGGL(k_sq_keys, dim3(prep_grid(pa.n)), dim3(kBlock), 0, s, pa);
+        if (ix != nullptr) {
+            pa.row_of = ix->row_of;
+        } else {
+            uint32_t *seen = reinterpret_cast<uint32_t *>(base + pl.off_seen);
+            if ((st = fill_u32(seen, pl.span / 32 + 1, 0u, s)) != AHEAD_OK) return st;
+            pa.seen = seen;
+            pa.row_of = reinterpret_cast<uint32_t *>(base + pl.off_row);
+            hipLaunchKernelGGL(k_sq_keys, dim3(prep_grid(pa.n)), dim3(kBlock), 0, s, pa);
+        }
         hipLaunchKernelGGL(k_sq_sel, dim3(prep_grid(pa.n)), dim3(kBlock), 0, s, pa);
 
         const ahead_sq_dim &dm = q->dims[d];
@@ -1143,7 +1304,8 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
     const uint32_t grid = sq_grid();
     const uint64_t waves = static_cast<uint64_t>(grid) * kSqWaves;
     const uint64_t per = (n + waves - 1) / waves;
-    a.rows_per_wave = (per + kSqSeg - 1) / kSqSeg * kSqSeg;
+    const uint64_t seg_rows = 128ull * sq_chunks(static_cast<int>(nd));
+    a.rows_per_wave = (per + seg_rows - 1) / seg_rows * seg_rows;
     if (q->ev_scan[0] != nullptr) AHEAD_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(q->ev_scan[0]), s));
     if (n > 0) {
         switch (nd) {

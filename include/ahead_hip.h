@@ -354,6 +354,17 @@ enum { AHEAD_SQ_E_PRED = 0, AHEAD_SQ_E_MEAS = 2, AHEAD_SQ_E_FK = 4, AHEAD_SQ_E_R
 enum { AHEAD_SQ_ST_OK = 0, AHEAD_SQ_ST_BAD_OID = 1, AHEAD_SQ_ST_BAD_KEY = 2, AHEAD_SQ_ST_DUP_KEY = 3,
        AHEAD_SQ_ST_EVENTS = 4, AHEAD_SQ_ST_KEY_SPAN = 5, AHEAD_SQ_ST_GROUPS = 6 };
 
+/* Key index of a dimension: the validity, uniqueness and range of the keys of its key column and
+ * the row of every key, built once for a resident dimension and then passed to any number of star
+ * queries over it (which then skip those checks).  The key column must not change while an index
+ * built from it is in use; create/destroy allocate and free device memory and synchronise. */
+typedef struct ahead_sq_key_index ahead_sq_key_index;
+int ahead_hip_star_key_index_create(const ahead_col *key, ahead_sq_key_index **out, void *stream);
+int ahead_hip_star_key_index_destroy(ahead_sq_key_index *ix);
+/* the index's status: AHEAD_SQ_ST_OK, _BAD_KEY, _DUP_KEY or _KEY_SPAN (the fused form does not
+ * apply to queries over this dimension) */
+int ahead_hip_star_key_index_status(const ahead_sq_key_index *ix);
+
 typedef struct {
     ahead_col fk;               /* fact FK codes (8-byte codes, keys of at most 32 bits)        */
     ahead_col key;              /* dimension key codes (8-byte codes)                           */
@@ -363,6 +374,7 @@ typedef struct {
     uint64_t sel_a_inv;         /* A^-1 of the OIDs in sel                                      */
     uint32_t flags;             /* AHEAD_SQ_PROBE | AHEAD_SQ_REFETCH                            */
     uint32_t reserved;
+    const ahead_sq_key_index *key_index; /* optional index of `key` (NULL: checked per call)   */
 } ahead_sq_dim;
 
 typedef struct {

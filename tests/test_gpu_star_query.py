@@ -95,3 +95,32 @@ def test_star_query_where_every_row_joins():
     assert np.array_equal(got["first_row"], ref["first_row"])
     e = dict(ref["errors"].items)
     assert e["fused_meas0.e1"].size > 100 and e["fused_attr0.e1"].size > 100
+
+
+def test_star_query_key_index_statuses():
+    """A key index built over a dimension with a repeated or a corrupt key reports it, and the
+    queries over that dimension fall back to the chain with that status."""
+    from ahead_amd import an, bat as B, capi
+
+    spec = ssb.SsbSpec(rate=0.0, seed=36, **SMALL)
+    t = ssb.device_tables(spec)
+    q = ssb.QUERIES["Q2.1"]
+    keys = t["s_suppkey"].tail.to_numpy().copy()
+    meta = t["s_suppkey"].tail.meta
+    print("star_query key index statuses", flush=True)
+    assert B.key_index(t["s_suppkey"].tail).status == 0
+    dup = keys.copy()
+    dup[11] = dup[5]
+    t2 = dict(t)
+    t2["s_suppkey"] = B.BAT(capi.Column.void(dup.size), capi.Column.from_numpy(dup, meta))
+    assert B.key_index(t2["s_suppkey"].tail).status == 3
+    got = ssb.fused(t2, q)
+    assert got["status"] == 3
+    same(got, ssb.chain(t2, q))
+    bad = keys.copy()
+    bad[7] ^= np.uint64(1 << 45)
+    t3 = dict(t)
+    t3["s_suppkey"] = B.BAT(capi.Column.void(bad.size), capi.Column.from_numpy(bad, meta))
+    assert B.key_index(t3["s_suppkey"].tail).status == 2
+    assert ssb.fused(t3, q, fallback=False)["status"] == 2
+    assert an.A_OID_ERR == 16089
