@@ -825,6 +825,10 @@ def run_ssb(args):
                      "kernel": f"{SSB_SCAN_KERNEL[dom]} ({dom})", "kernel_ms": per[dom]["scan_ms"],
                      "algorithmic_bytes": per[dom]["scan_algorithmic_bytes"]},
     }
+    tr = res["roofline"]["traffic"]
+    if tr:  # the bytes the kernel really moves (it skips measure lines of rows that do not join)
+        res["roofline"]["traffic_gbs"] = tr / (per[dom]["scan_ms"] * 1e-3) / 1e9
+        res["roofline"]["traffic_frac"] = res["roofline"]["traffic_gbs"] / HBM_PEAK_GBS
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_ssb(spec, queries, args.cpu_seconds)
     return res
