@@ -71,28 +71,35 @@ def parse():
 
 
 def cpu_baseline_select(codes_np, meta, lo, hi, seconds, expect):
-    """Oracle (scalar restatement of selectAN + check-and-decode + sum) on this host's cores."""
+    """The all-core vectorised CPU restatement of selectAN + check-and-decode + sum
+    (oracle/an_simd.cpp) on this host's usable cores; its outputs are asserted equal to the GPU's."""
     sys.path.insert(0, str(ROOT / "tests"))
     import numpy as np
 
     import oracle_lib as O
 
     col = O.Col(codes_np, meta)
+    threads = len(os.sched_getaffinity(0))
     reps, t0 = 0, time.perf_counter()
     while True:
-        oids, err, s = O.select_an(col, 3, lo, 1, hi, 1)
+        oids, err, s = O.select_an_simd(col, 3, lo, 1, hi, 1, threads=threads)
         reps += 1
         el = time.perf_counter() - t0
         if el >= seconds:
             break
     ok = (oids.size == expect[0] and np.array_equal(err, expect[1]) and s == expect[2])
+    if not ok:
+        raise RuntimeError("the CPU baseline's select differs from the GPU's")
+    cpu = host_cpu()
+    isa = {4: "AVX-512 (x86-64-v4)", 3: "AVX2 (x86-64-v3)", 0: "x86-64 baseline"}[O.simd_level()]
     return {
         "value": codes_np.size * reps / el,
         "unit": "tuples/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"{codes_np.size}-row column x {reps} repetitions ({el:.1f} s), single thread, "
-                  f"oracle/an_oracle.c built -O3 -fno-tree-vectorize; bit-exact vs GPU: {ok}",
+        "sample": f"{codes_np.size}-row column x {reps} repetitions ({el:.1f} s); oracle/an_simd.cpp "
+                  f"or_select_an_simd, {isa} build, OpenMP on {threads} threads of {cpu['model']} "
+                  f"({cpu['nproc']} logical CPUs); bit-exact vs GPU: {ok}",
     }
 
 
@@ -411,29 +418,34 @@ def run_batch(args):
 
 
 def cpu_baseline_batch(spec, jobs, res_b, seconds):
-    """The oracle's select on instances of the batch (host-built columns), in order, until the
-    time budget is spent; bit-exact check against the batched GPU results of the same instances."""
+    """The all-core vectorised CPU select (oracle/an_simd.cpp) on instances of the batch (host-built
+    columns), in order, until the time budget is spent; each result is asserted equal to the batched
+    GPU result of the same instance."""
     sys.path.insert(0, str(ROOT / "tests"))
     import numpy as np
 
     import oracle_lib as O
     from ahead_amd import workloads as W
 
-    rows, done, el, ok = 0, 0, 0.0, True
+    threads = len(os.sched_getaffinity(0))
+    rows, done, el = 0, 0, 0.0
     for i, j in enumerate(jobs):
         codes, meta = W.batch_host_column(spec, i)
         t0 = time.perf_counter()
-        oids, err, s = O.select_an(O.Col(codes, meta), j["op1"], j["th1"], j["op2"], j["th2"], j["comb"])
+        oids, err, s = O.select_an_simd(O.Col(codes, meta), j["op1"], j["th1"], j["op2"], j["th2"], j["comb"],
+                                        threads=threads)
         el += time.perf_counter() - t0
-        ok &= bool(np.array_equal(oids, res_b[i][0]) and np.array_equal(err, res_b[i][1]) and s == res_b[i][2])
+        if not (np.array_equal(oids, res_b[i][0]) and np.array_equal(err, res_b[i][1]) and s == res_b[i][2]):
+            raise RuntimeError(f"the CPU baseline's instance {i} differs from the GPU's")
         rows += codes.size
         done += 1
         if el >= seconds:
             break
-    return {"value": done / el, "unit": "instances/s", "cores": 1, "kind": "port",
-            "sample": f"first {done} of {len(jobs)} instances ({rows} rows, {el:.1f} s of select time), single "
-                      f"thread, oracle/an_oracle.c -O3 -fno-tree-vectorize; {rows / el:.3e} tuples/s; "
-                      f"bit-exact vs GPU: {ok}"}
+    cpu = host_cpu()
+    return {"value": done / el, "unit": "instances/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} of {len(jobs)} instances ({rows} rows, {el:.1f} s of select time); "
+                      f"oracle/an_simd.cpp or_select_an_simd on {threads} threads of {cpu['model']}; "
+                      f"{rows / el:.3e} tuples/s; bit-exact vs GPU: True"}
 
 
 def run_ops(args):
