@@ -380,6 +380,14 @@ def star_query(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_
     return q.results()
 
 
+def star_query_ws_bytes(n_fact: int, dims: list, meas: list, kind: int) -> int:
+    """Device workspace bytes of a fused star query over these columns
+    (ahead_hip_star_query_ws_bytes): it depends on the dimensions' row counts and on which of them
+    have a key index, not on the selections or predicates."""
+    cdims = [dict(d, key_index=key_index(d["key"]), sel=None) for d in dims]
+    return capi.StarQuery(n_fact, cdims, [], meas, kind).ws_bytes()
+
+
 def key_index(key: Column) -> "capi.StarKeyIndex":
     """The star-query key index of a resident dimension key column, built at its first use and kept
     with the column (the column's codes must not change afterwards)."""

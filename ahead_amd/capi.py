@@ -152,6 +152,7 @@ _SIGS = {
     "ahead_hip_select_batch_destroy": ([_vp], _int),
     "ahead_hip_select_batch_info": ([_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)], _int),
     "ahead_hip_star_query": ([ctypes.POINTER(SqArgs), ctypes.POINTER(SqOut), _vp, _vp], _int),
+    "ahead_hip_star_query_ws_bytes": ([ctypes.POINTER(SqArgs)], ctypes.c_size_t),
     "ahead_hip_star_key_index_create": ([_colp, ctypes.POINTER(_vp), _vp], _int),
     "ahead_hip_star_key_index_destroy": ([_vp], _int),
     "ahead_hip_star_key_index_status": ([_vp], _int),
@@ -237,7 +238,9 @@ def current_stream() -> int:
 
 
 def synchronize() -> None:
-    _torch().cuda.synchronize()
+    """Waits for the calling thread's current stream (every wrapper issues its work there), so
+    threads serving queries on their own streams do not wait for each other."""
+    _torch().cuda.current_stream().synchronize()
 
 
 @dataclass
@@ -646,6 +649,10 @@ class StarQuery:
             o.err_cap[k] = e
         o.d_counts = self.counts.ptr
         self.out = o
+
+    def ws_bytes(self) -> int:
+        """Workspace bytes this query uses (ahead_hip_star_query_ws_bytes)."""
+        return int(load().ahead_hip_star_query_ws_bytes(ctypes.byref(self.args)))
 
     def launch(self, stream: int | None = None) -> None:
         s = current_stream() if stream is None else stream

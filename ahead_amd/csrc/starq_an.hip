@@ -1017,6 +1017,36 @@ extern "C" int ahead_hip_star_key_index_status(const ahead_sq_key_index *ix) {
     return ix == nullptr ? -1 : ix->status;
 }
 
+namespace {
+
+// Workspace bytes of one star query (the layout of ahead_hip_star_query below, computed alike).
+size_t sq_ws_bytes(const ahead_sq_args *q) {
+    size_t off = 256;
+    off += align_up(static_cast<size_t>(AHEAD_SQ_MAX_DIMS) * kStatWords * 4, 256);
+    off += static_cast<size_t>(kSqMaxGroups) * 8;
+    off += static_cast<size_t>(kSqMaxGroups) * 4;
+    off += 256;
+    off += static_cast<size_t>(kSqMaxEvents) * 8;
+    const size_t off_maps = align_up(off, 256);
+    size_t maps_words_cap = 0;
+    for (uint32_t d = 0; d < q->n_dims; ++d) maps_words_cap += static_cast<size_t>((4ull * q->dims[d].key.n + 1024) / 2 + 1);
+    off = off_maps + align_up(maps_words_cap * 4, 256);
+    for (uint32_t d = 0; d < q->n_dims; ++d) {
+        if (q->dims[d].key_index != nullptr) continue;
+        const uint64_t span_cap = 4ull * q->dims[d].key.n + 1024;
+        off += align_up((span_cap / 32 + 1) * 4, 256);
+        off += align_up(span_cap * 4, 256);
+    }
+    return off;
+}
+
+}  // namespace
+
+extern "C" size_t ahead_hip_star_query_ws_bytes(const ahead_sq_args *q) {
+    if (q == nullptr || q->n_dims < 1 || q->n_dims > AHEAD_SQ_MAX_DIMS) return 0;
+    return kHeaderBytes + sq_ws_bytes(q);
+}
+
 extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *out, ahead_hip_ws *ws,
                                     void *stream) {
     if (q == nullptr || out == nullptr || ws == nullptr) return fail(AHEAD_ERR_INVALID, "star_query: NULL argument");
@@ -1064,6 +1094,7 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
     const uint32_t nd = q->n_dims;
     std::vector<DimPlan> P(nd);
     size_t off = 0;
+    const size_t ws_need = sq_ws_bytes(q);
     const size_t off_flags = off;
     off += 256;
     const size_t off_stats = off;
@@ -1091,6 +1122,7 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         P[d].off_row = off;
         off += align_up(P[d].span_cap * 4, 256);
     }
+    if (off != ws_need) return fail(AHEAD_ERR_DEVICE, "star_query: workspace layout mismatch");
     if ((st = ws_ensure(ws, off)) != AHEAD_OK) return st;
     char *base = ws_region(ws, 0);
     uint32_t *flags = reinterpret_cast<uint32_t *>(base + off_flags);

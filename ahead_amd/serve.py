@@ -1,13 +1,21 @@
 """Served star-schema queries: a device-resident schema, Poisson arrivals, continuous batching.
 
-The tables of ahead_amd.ssb stay in HBM for the server's lifetime.  Queries (the four pipeline
-shapes with seeded random predicate constants) arrive on a Poisson clock.  `workers` GPU slots
-each run one query's operator chain on their own HIP stream, so the operators of concurrent
-queries overlap on the device.  A query is admitted as soon as a slot is free and the HBM its
-intermediates need (an upper bound per shape and fact row) fits the budget beside those running.
-Queries that arrive meanwhile wait in arrival order.  Latency runs from arrival to the query's
-last result on the host.  With workers=1 the same loop is the unbatched path: one query on the
-GPU at a time.
+The tables of ahead_amd.ssb stay in HBM for the server's lifetime.  Queries (the four query shapes
+Q1.1 / Q2.1 / Q3.1 / Q4.1 with seeded random predicate constants) arrive on a Poisson clock.
+`workers` GPU slots each run one query at a time on their own HIP stream and workspace, so
+concurrent queries overlap on the device: one query's dimension selections, planning and result
+copies run beside another's fact scan.  Admission is first come, first served: the query at the
+head of the arrival queue starts as soon as a slot is free and the device memory it needs fits the
+budget beside the queries running; later arrivals wait behind it even if they would fit.
+Latency runs from arrival to the query's results on the host.
+
+Two execution forms:
+  fused  each query is a fused star query (ssb.fused): the dimension selections, then one pass over
+         the fact rows; it needs its workspace (dimension maps, group tables, as
+         ahead_hip_star_query_ws_bytes reports) and its output buffers;
+  chain  each query is its operator chain (ssb.chain), one launch per operator, every operator
+         waiting for its counts: the plain path; it needs its materialised intermediates (a bound
+         per fact row measured at 600 M rows).
 """
 from __future__ import annotations
 
@@ -20,9 +28,11 @@ import numpy as np
 
 from . import ssb
 
-# upper bound of the device bytes a query's intermediates hold at once, per fact row: the peaks
-# measured at 600M fact rows (P1 14.0, P2 17.6, P3 17.6, P4 19.8) plus about 15%
+# upper bound of the device bytes an operator chain's intermediates hold at once, per fact row: the
+# peaks measured at 600M fact rows (P1 14.0, P2 17.6, P3 17.6, P4 19.8) plus about 15%
 HBM_PER_ROW = {"P1": 16, "P2": 20, "P3": 20, "P4": 23}
+# the fused form's output buffers (4096 groups x 6 arrays, 16 error vectors of 1024 positions)
+FUSED_OUT_BYTES = 4096 * 8 * 6 + 16 * 1024 * 8 + 4096
 
 
 @dataclass
@@ -39,10 +49,17 @@ class Query:
     def latency(self) -> float:
         return self.finish - self.arrival
 
+    def spec(self) -> ssb.Query:
+        return ssb.SHAPES[self.shape](**self.params)
+
+
+# a query of each shape with its default constants (the workspace depends on the shape only)
+SHAPE_DEFAULTS = {k: (lambda k=k: ssb.SHAPES[k]()) for k in ("P1", "P2", "P3", "P4")}
+
 
 def random_queries(n: int, rate: float, seed: int) -> list[Query]:
-    """n queries with exponential inter-arrival gaps of mean 1/rate, shapes uniform over P1-P4,
-    predicate constants uniform over the schema's value ranges (seeded)."""
+    """n queries with exponential inter-arrival gaps of mean 1/rate (0: all at once), shapes uniform
+    over P1-P4, predicate constants uniform over the schema's value ranges (seeded)."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, n)) if rate > 0 else np.zeros(n)
     out = []
@@ -66,18 +83,54 @@ def random_queries(n: int, rate: float, seed: int) -> list[Query]:
     return out
 
 
+class Admission:
+    """First-come-first-served admission of device memory: callers take tickets in arrival order;
+    the holder of the oldest unserved ticket is admitted once its bytes fit the budget beside the
+    bytes in use, and no later ticket is admitted before it (no overtaking, so a large query is not
+    starved by a stream of small ones)."""
+
+    def __init__(self, budget: float):
+        self.budget = budget
+        self.in_use = 0.0
+        self._cv = threading.Condition()
+        self._next = 0     # next ticket handed out
+        self._serving = 0  # oldest ticket not yet admitted
+
+    def ticket(self) -> int:
+        with self._cv:
+            t = self._next
+            self._next += 1
+            return t
+
+    def acquire(self, ticket: int, need: float) -> None:
+        need = min(need, self.budget)
+        with self._cv:
+            while ticket != self._serving or self.in_use + need > self.budget:
+                self._cv.wait()
+            self._serving += 1
+            self.in_use += need
+            self._cv.notify_all()
+
+    def release(self, need: float) -> None:
+        need = min(need, self.budget)
+        with self._cv:
+            self.in_use -= need
+            self._cv.notify_all()
+
+
 class Server:
     """A long-lived query server: its worker threads, their streams and their workspaces persist
     across `run` calls (close() ends them), so device memory cached for one stream's queries is
     reused by that stream's later queries."""
 
-    def __init__(self, tables: dict, n_fact: int, workers: int = 4, hbm_budget: float = 64e9):
+    def __init__(self, tables: dict, n_fact: int, workers: int = 4, hbm_budget: float = 64e9, mode: str = "fused"):
+        assert mode in ("fused", "chain")
         self.tables = tables
         self.n_fact = n_fact
         self.workers = workers
-        self.budget = hbm_budget
-        self._in_use = 0.0
-        self._cv = threading.Condition()
+        self.mode = mode
+        self.admission = Admission(hbm_budget)
+        self._need_cache: dict = {}
         self._local = threading.local()
         self._ex = ThreadPoolExecutor(max_workers=workers)
 
@@ -98,32 +151,38 @@ class Server:
             s = self._local.stream = torch.cuda.Stream()
         return s
 
-    def _need(self, q: Query) -> float:
-        return min(self.budget, HBM_PER_ROW[q.shape] * self.n_fact)
+    def need(self, shape: str) -> float:
+        """Device bytes a query of this shape holds while it runs: for the fused form its workspace
+        (as the library computes it from the dimension sizes) and outputs, for the chain the bound
+        of its intermediates."""
+        if self.mode == "chain":
+            return HBM_PER_ROW[shape] * self.n_fact
+        if shape not in self._need_cache:
+            spec = SHAPE_DEFAULTS[shape]()
+            self._need_cache[shape] = ssb.fused_ws_bytes(self.tables, spec) + FUSED_OUT_BYTES
+        return self._need_cache[shape]
 
-    def _run_one(self, q: Query, t0: float) -> Query:
+    def _run_one(self, q: Query, ticket: int, t0: float) -> Query:
         import torch
 
-        need = self._need(q)
-        with self._cv:
-            while self._in_use + need > self.budget:
-                self._cv.wait()
-            self._in_use += need
+        need = self.need(q.shape)
+        self.admission.acquire(ticket, need)
         try:
             q.start = time.perf_counter() - t0
             s = self._stream()
             with torch.cuda.stream(s):
-                q.result = ssb.PIPELINES[q.shape](self.tables, **q.params)
+                spec = q.spec()
+                q.result = ssb.fused(self.tables, spec) if self.mode == "fused" else ssb.chain(self.tables, spec)
             s.synchronize()
             q.finish = time.perf_counter() - t0
         finally:
-            with self._cv:
-                self._in_use -= need
-                self._cv.notify_all()
+            self.admission.release(need)
         return q
 
     def run(self, queries: list[Query], realtime: bool = True) -> list[Query]:
         """Replays the arrivals (in real time, or all at once) and returns the queries, completed."""
+        for shape in {q.shape for q in queries}:
+            self.need(shape)  # computed once per shape, before the clock starts
         t0 = time.perf_counter()
         futs = []
         for q in queries:
@@ -133,7 +192,7 @@ class Server:
                     time.sleep(delay)
             else:
                 q.arrival = time.perf_counter() - t0
-            futs.append(self._ex.submit(self._run_one, q, t0))
+            futs.append(self._ex.submit(self._run_one, q, self.admission.ticket(), t0))
         for f in futs:
             f.result()
         return queries
@@ -148,11 +207,11 @@ def summarize(queries: list[Query]) -> dict:
 
 
 def same_result(a: dict, b: dict) -> bool:
-    """Bit-exact equality of two pipeline results, error vectors included."""
+    """Bit-exact equality of two query results, error vectors included."""
     if a["rows"] != b["rows"] or a.get("result") != b.get("result") or a.get("groups") != b.get("groups"):
         return False
     if "sums" in a and not (np.array_equal(a["sums"], b["sums"]) and
                             all(np.array_equal(x, y) for x, y in zip(a["keys"], b["keys"]))):
         return False
-    ea, eb = a["errors"].items, b["errors"].items
-    return [k for k, _ in ea] == [k for k, _ in eb] and all(np.array_equal(x, y) for (_, x), (_, y) in zip(ea, eb))
+    ea, eb = dict(a["errors"].items), dict(b["errors"].items)
+    return sorted(ea) == sorted(eb) and all(np.array_equal(ea[k], eb[k]) for k in eb)

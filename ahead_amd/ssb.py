@@ -667,6 +667,13 @@ def fused(t: dict, q: Query, ops=B, fallback: bool = True, ev_scan: tuple | None
     return out
 
 
+def fused_ws_bytes(t: dict, q: Query, ops=B) -> int:
+    """Device workspace bytes of the query's fused form (its dimension maps and tables)."""
+    dims = [{"fk": t[DIM_KEYS[d.name][0]].tail, "key": t[DIM_KEYS[d.name][1]].tail,
+             "attr": t[d.attr].tail if d.attr else None, "flags": d.flags} for d in q.dims]
+    return ops.star_query_ws_bytes(t[q.meas[0]].tail.n, dims, [t[m].tail for m in q.meas], q.kind)
+
+
 def _p(shape: str):
     def run(t: dict, ops=B, **kw) -> dict:
         return chain(t, SHAPES[shape](**kw), ops)

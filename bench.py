@@ -29,9 +29,10 @@ HBM.  Workloads:
                 them), and after timing Q1.1 with a host textbook evaluation and every selection
                 error vector with the generator's flip positions.
   serve       : the same schema resident in HBM, queries of the four shapes with seeded random
-                predicate constants arriving on Poisson clocks; continuous batching on 4 GPU
-                slots (own streams, admission by HBM) against the unbatched path (one query at a
-                time); reports queries/s with p50/p99 latency per arrival rate.
+                predicate constants arriving on Poisson clocks; fused star queries on 4 GPU slots
+                (own streams and workspaces, first-come-first-served admission by the bytes each
+                query needs) against the plain operator chains (one launch per operator) on 1 and
+                4 slots; reports queries/s with p50/p99 latency per arrival rate.
   hbm_copy    : a plain device-to-device copy of 4 GiB (read 4 GiB, write 4 GiB): the calibration
                 kernel for the FETCH_SIZE / WRITE_SIZE counters behind `roofline.traffic`.
 
@@ -635,41 +636,47 @@ def run_serve(args):
         spec.n_fact = args.rows
     tables = ssb.device_tables(spec)
     torch.cuda.synchronize()
-    n_q = max(8, args.steps * 8)
-    # one long-lived server per form; each is warmed up on every shape before it is measured
-    servers = {w: serve.Server(tables, spec.n_fact, workers=w) for w in (1, 4)}
-    for w, srv in servers.items():
-        srv.run(serve.random_queries(4 * w, 0, 99), realtime=False)
+    n_q = max(16, args.steps * 16)
+    # long-lived servers: the fused form on 1 and 4 slots, the plain operator chains on 1 and 4
+    forms = {"fused4": ("fused", 4), "fused1": ("fused", 1), "chain1": ("chain", 1), "chain4": ("chain", 4)}
+    servers = {k: serve.Server(tables, spec.n_fact, workers=w, mode=m) for k, (m, w) in forms.items()}
+    for k, srv in servers.items():   # warm-up on every shape
+        srv.run(serve.random_queries(4 * srv.workers, 0, 99), realtime=False)
     torch.cuda.reset_peak_memory_stats()
     retries0 = torch.cuda.memory_stats().get("num_alloc_retries", 0)
     # capacity of each form: everything arrives at once
     sat = {}
-    for w, srv in servers.items():
-        sat[w] = serve.summarize(srv.run(serve.random_queries(n_q, 0, 7), realtime=False))
-        print(f"serve: {w} slot(s), all at once: {sat[w]['qps']:.1f} queries/s", file=sys.stderr, flush=True)
-    # Poisson arrivals at fractions of the unbatched capacity
+    for k, srv in servers.items():
+        sat[k] = serve.summarize(srv.run(serve.random_queries(n_q, 0, 7), realtime=False))
+        print(f"serve: {k}, all at once: {sat[k]['qps']:.1f} queries/s, p99 {sat[k]['p99_ms']:.1f} ms",
+              file=sys.stderr, flush=True)
+    # Poisson arrivals at multiples of the plain path's capacity; every form sees the same arrivals
     sweep = []
-    base = sat[1]["qps"]
-    for frac in (0.5, 0.9, 1.5):
+    base = sat["chain1"]["qps"]
+    for frac in (0.5, 0.9, 2.0, 4.0):
         rate = base * frac
         row = {"rate_qps": rate}
-        for w, srv in servers.items():
-            row[f"workers{w}"] = serve.summarize(srv.run(serve.random_queries(n_q, rate, 11), realtime=True))
-            print(f"serve: {w} slot(s), Poisson {rate:.1f} queries/s: p50 {row[f'workers{w}']['p50_ms']:.0f} ms",
-                  file=sys.stderr, flush=True)
+        for k, srv in servers.items():
+            if k.startswith("chain") and frac > 1.5:
+                continue   # beyond the chains' capacity: their queue only grows
+            row[k] = serve.summarize(srv.run(serve.random_queries(n_q, rate, 11), realtime=True))
+            print(f"serve: {k}, Poisson {rate:.1f} queries/s: p50 {row[k]['p50_ms']:.1f} ms, "
+                  f"p99 {row[k]['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
         sweep.append(row)
     t0 = time.perf_counter()
-    final = servers[4].run(serve.random_queries(n_q, 0, 13), realtime=False)
+    final = servers["fused4"].run(serve.random_queries(n_q, 0, 13), realtime=False)
     el = time.perf_counter() - t0
     s4 = serve.summarize(final)
+    needs = {k: {sh: servers[k].need(sh) for sh in SSB_SHAPES} for k in ("fused4", "chain1")}
     for srv in servers.values():
         srv.close()
     mem = torch.cuda.memory_stats()
     memory = {"alloc_retries": mem.get("num_alloc_retries", 0) - retries0,
               "peak_allocated_gb": mem.get("allocated_bytes.all.peak", 0) / 1e9,
-              "peak_reserved_gb": mem.get("reserved_bytes.all.peak", 0) / 1e9}
+              "peak_reserved_gb": mem.get("reserved_bytes.all.peak", 0) / 1e9,
+              "admission_bytes_fused": needs["fused4"], "admission_bytes_chain": needs["chain1"]}
     return {
-        "metric": "served star-schema queries/s (continuous batching, 4 slots) with p50/p99 latency",
+        "metric": "served star-schema queries/s (fused star queries, 4 slots) with p50/p99 latency",
         "value": s4["qps"],
         "unit": "queries/s",
         "n_gpus": 1,
@@ -683,9 +690,9 @@ def run_serve(args):
         "data": "synthetic (seeded; fact columns generated on device)",
         "config": {"workload": f"serve: ssb schema with {spec.n_fact} fact rows resident in HBM; {n_q} queries "
                                f"per run, shapes P1-P4 with seeded random predicates",
-                   "saturated": {"unbatched": sat[1], "batched_4_slots": sat[4]},
-                   "speedup_saturated": sat[4]["qps"] / sat[1]["qps"] if sat[1]["qps"] else None,
-                   "poisson_sweep": sweep, "final_batched": s4, "device_memory": memory},
+                   "saturated": sat,
+                   "speedup_saturated_vs_plain": sat["fused4"]["qps"] / base if base else None,
+                   "poisson_sweep": sweep, "final_fused_4_slots": s4, "device_memory": memory},
         "roofline": None,
     }
 

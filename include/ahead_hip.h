@@ -408,6 +408,9 @@ typedef struct {
 } ahead_sq_out;
 
 int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *out, ahead_hip_ws *ws, void *stream);
+/* Device bytes of workspace a star query with these arguments uses (dimension maps sized for the
+ * largest key span the fused form accepts), for admission by memory; 0 for invalid arguments.  */
+size_t ahead_hip_star_query_ws_bytes(const ahead_sq_args *q);
 
 /* ---- data generation (seeded, counter-based; the same streams as ahead_amd/datagen.py) ------
  * Not part of the reference's operator API: the device-side generator for the synthetic

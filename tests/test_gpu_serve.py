@@ -1,6 +1,7 @@
-"""Served queries (ahead_amd.serve): every query's result, error vectors included, is the same when
-it runs alone (one query on the GPU at a time) and in the mix (four queries overlapping on four
-streams, all arriving at once), and both equal the oracle's operator chain for the query."""
+"""Served queries (ahead_amd.serve), in the fused form and as operator chains: every query's result,
+error vectors included, is the same when it runs alone (one query on the GPU at a time) and in the
+mix (four queries overlapping on four streams, all arriving at once), and both equal the oracle's
+operator chain for the query; the admission budget serialises queries that do not fit together."""
 import pytest
 
 import oracle_bat as OB
@@ -11,12 +12,14 @@ pytestmark = pytest.mark.gpu
 SMALL = dict(n_fact=250_000, n_date=2556, n_cust=3_000, n_supp=200, n_part=1_400)
 
 
-def test_results_alone_equal_results_in_the_mix():
+@pytest.mark.parametrize("mode", ["fused", "chain"])
+def test_results_alone_equal_results_in_the_mix(mode):
     spec = ssb.SsbSpec(rate=2e-3, seed=21, **SMALL)
     tables = ssb.device_tables(spec)
-    with serve.Server(tables, spec.n_fact, workers=1) as srv:
+    print(f"serve {mode}: 24 queries, 1 then 4 slots", flush=True)
+    with serve.Server(tables, spec.n_fact, workers=1, mode=mode) as srv:
         alone = srv.run(serve.random_queries(24, 0, 5), realtime=False)
-    with serve.Server(tables, spec.n_fact, workers=4) as srv:
+    with serve.Server(tables, spec.n_fact, workers=4, mode=mode) as srv:
         mixed = srv.run(serve.random_queries(24, 0, 5), realtime=False)
         again = srv.run(serve.random_queries(24, 0, 5), realtime=False)  # the same threads, warm
     host = OB.tables(spec)
@@ -28,18 +31,26 @@ def test_results_alone_equal_results_in_the_mix():
         assert serve.same_result(a.result, ref), ("alone", a.qid, a.shape, a.params)
         assert serve.same_result(m.result, ref), ("mixed", a.qid, a.shape, a.params)
         assert serve.same_result(again[a.qid].result, ref), ("mixed again", a.qid, a.shape, a.params)
+        if mode == "fused":
+            assert a.result["status"] == 0 and m.result["status"] == 0
         shapes.add(a.shape)
     assert shapes == {"P1", "P2", "P3", "P4"}
     s = serve.summarize(mixed)
     assert s["queries"] == 24 and s["p99_ms"] >= s["p50_ms"] > 0
 
 
-def test_admission_budget_serializes_when_memory_is_short():
+@pytest.mark.parametrize("mode", ["fused", "chain"])
+def test_admission_budget_serializes_when_memory_is_short(mode):
     spec = ssb.SsbSpec(rate=0.0, seed=22, **SMALL)
     tables = ssb.device_tables(spec)
-    # a budget that fits one query at a time: queries must not overlap
-    with serve.Server(tables, spec.n_fact, workers=4, hbm_budget=28 * spec.n_fact) as srv:
+    print(f"serve {mode}: admission budget", flush=True)
+    with serve.Server(tables, spec.n_fact, workers=4, mode=mode) as srv:
+        needs = [srv.need(s) for s in ("P1", "P2", "P3", "P4")]
+        assert all(n > 0 for n in needs)
+        # a budget that fits one query at a time: queries must not overlap
+        srv.admission.budget = max(needs) + min(needs) - 1
         qs = srv.run(serve.random_queries(8, 0, 6), realtime=False)
+    assert [q.start for q in qs] == sorted(q.start for q in qs)   # admitted in arrival order
     spans = sorted((q.start, q.finish) for q in qs)
     for (s0, f0), (s1, f1) in zip(spans, spans[1:]):
         assert s1 >= f0 - 1e-4

@@ -24,3 +24,41 @@ def test_summary_percentiles():
         q.arrival, q.finish = 0.0, (i + 1) / 1000.0
     s = serve.summarize(qs)
     assert s["queries"] == 100 and abs(s["p50_ms"] - 50.5) < 1e-6 and s["p99_ms"] > 99
+
+
+def test_admission_is_first_come_first_served():
+    """A query that does not fit yet holds back the later ones even when they would fit: with a
+    budget of 10 and 6 in use, ticket 1 (needs 8) waits, and ticket 2 (needs 1) waits behind it."""
+    import threading
+    import time
+
+    adm = serve.Admission(10.0)
+    order = []
+    t0, t1, t2 = adm.ticket(), adm.ticket(), adm.ticket()
+    adm.acquire(t0, 6.0)
+
+    def take(t, need):
+        adm.acquire(t, need)
+        order.append(t)
+
+    th = [threading.Thread(target=take, args=(t2, 1.0)), threading.Thread(target=take, args=(t1, 8.0))]
+    for x in th:
+        x.start()
+    time.sleep(0.2)
+    assert order == [] and adm.in_use == 6.0
+    adm.release(6.0)
+    for x in th:
+        x.join(5)
+    assert order == [t1, t2] and adm.in_use == 9.0
+    adm.release(8.0)
+    adm.release(1.0)
+    assert adm.in_use == 0.0
+
+
+def test_admission_caps_a_need_at_the_budget():
+    adm = serve.Admission(4.0)
+    t = adm.ticket()
+    adm.acquire(t, 100.0)   # larger than the budget: runs alone
+    assert adm.in_use == 4.0
+    adm.release(100.0)
+    assert adm.in_use == 0.0
