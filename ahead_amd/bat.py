@@ -358,13 +358,8 @@ def arithmeticAN(op: int, b1: BAT, b2: BAT, a_result: int = an.A_RESULT, a_oid: 
     return (BAT(Column.void(n), Column(bufs[0], n, meta)),) + tuple(e)
 
 
-def star_query(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_mask: int = 0,
-               a_result: int = an.A_RESULT, a_oid: int = A_OID_ERR, ev_scan: tuple | None = None) -> dict:
-    """The fused star query (ahead_hip_star_query) over device columns.  dims: dicts in the
-    chain's order with fk, key, attr (Column or None), sel (Column of encoded OIDs or None), flags;
-    preds: (Column, op1, th1, op2, th2, comb) with encoded thresholds; meas: Columns.  Returns the
-    entry point's outputs as numpy (status, groups, kept, sums, first_row, keys by dimension, the
-    16 error vectors by kind)."""
+def _star_query_obj(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_mask: int = 0,
+                    a_result: int = an.A_RESULT, a_oid: int = A_OID_ERR, ev_scan: tuple | None = None):
     cdims = []
     for d in dims:
         x = dict(d)
@@ -375,9 +370,30 @@ def star_query(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_
             x["sel_count"] = DeviceBuffer.from_numpy(np.array([sel.n], dtype=np.uint64))
             x["sel_a_inv"] = sel.meta.a_inv
         cdims.append(x)
-    q = capi.StarQuery(n_fact, cdims, preds, meas, kind, pre_mask, a_result, a_oid, ev_scan=ev_scan)
+    return capi.StarQuery(n_fact, cdims, preds, meas, kind, pre_mask, a_result, a_oid, ev_scan=ev_scan)
+
+
+def star_query(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_mask: int = 0,
+               a_result: int = an.A_RESULT, a_oid: int = A_OID_ERR, ev_scan: tuple | None = None) -> dict:
+    """The fused star query (ahead_hip_star_query) over device columns.  dims: dicts in the
+    chain's order with fk, key, attr (Column or None), sel (Column of encoded OIDs or None), flags;
+    preds: (Column, op1, th1, op2, th2, comb) with encoded thresholds; meas: Columns.  Returns the
+    entry point's outputs as numpy (status, groups, kept, sums, first_row, keys by dimension, the
+    16 error vectors by kind)."""
+    q = _star_query_obj(n_fact, dims, preds, meas, kind, pre_mask, a_result, a_oid, ev_scan)
     q.launch()
     return q.results()
+
+
+def star_query_batch(calls: list, ev_scan: tuple | None = None) -> list:
+    """Several fused star queries over one fact table with shared fact scans
+    (ahead_hip_star_query_batch): `calls` holds star_query's keyword arguments per query; returns
+    star_query's result per query.  ev_scan brackets all of the batch's scans."""
+    qs = [_star_query_obj(**c) for c in calls]
+    if ev_scan is not None and qs:
+        qs[0].args.ev_scan[0], qs[0].args.ev_scan[1] = ev_scan
+    capi.star_query_batch(qs)
+    return [q.results() for q in qs]
 
 
 def star_query_ws_bytes(n_fact: int, dims: list, meas: list, kind: int) -> int:

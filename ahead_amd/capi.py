@@ -153,6 +153,9 @@ _SIGS = {
     "ahead_hip_select_batch_info": ([_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)], _int),
     "ahead_hip_star_query": ([ctypes.POINTER(SqArgs), ctypes.POINTER(SqOut), _vp, _vp], _int),
     "ahead_hip_star_query_ws_bytes": ([ctypes.POINTER(SqArgs)], ctypes.c_size_t),
+    "ahead_hip_star_query_batch": ([ctypes.POINTER(ctypes.POINTER(SqArgs)), ctypes.POINTER(ctypes.POINTER(SqOut)), _u32,
+                                    _vp, _vp], _int),
+    "ahead_hip_star_query_batch_ws_bytes": ([ctypes.POINTER(ctypes.POINTER(SqArgs)), _u32], ctypes.c_size_t),
     "ahead_hip_star_key_index_create": ([_colp, ctypes.POINTER(_vp), _vp], _int),
     "ahead_hip_star_key_index_destroy": ([_vp], _int),
     "ahead_hip_star_key_index_status": ([_vp], _int),
@@ -677,6 +680,33 @@ class StarQuery:
                 "sums": self.sums.to_numpy(np.uint64, ng), "first_row": self.first.to_numpy(np.uint64, ng),
                 "keys": {d: b.to_numpy(np.uint64, ng) for d, b in self.keys.items()},
                 "err": [self.errs[k].to_numpy(np.uint64, c[4 + k]) for k in range(SQ_KINDS)]}
+
+
+SQ_MAX_BATCH = 8
+
+
+def _batch_ptrs(queries: list):
+    n = len(queries)
+    args = (ctypes.POINTER(SqArgs) * n)(*[ctypes.pointer(q.args) for q in queries])
+    outs = (ctypes.POINTER(SqOut) * n)(*[ctypes.pointer(q.out) for q in queries])
+    return n, args, outs
+
+
+def star_query_batch(queries: list, stream: int | None = None) -> None:
+    """Launches StarQuery objects over one fact table with shared fact scans
+    (ahead_hip_star_query_batch); each query's results() reads its own outputs."""
+    if not queries:
+        return
+    n, args, outs = _batch_ptrs(queries)
+    s = current_stream() if stream is None else stream
+    st = load().ahead_hip_star_query_batch(args, outs, n, workspace().h, s)
+    if st:
+        check(st)
+
+
+def star_query_batch_ws_bytes(queries: list) -> int:
+    n, args, _ = _batch_ptrs(queries)
+    return int(load().ahead_hip_star_query_batch_ws_bytes(args, n))
 
 
 # ------------------------------------------------------------------------------------------------

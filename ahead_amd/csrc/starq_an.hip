@@ -130,8 +130,8 @@ __host__ __device__ __forceinline__ bool pred_pass(const SqPredK &p, uint64_t co
     return cmp(p.op1, code, p.th1) || cmp(p.op2, code, p.th2);
 }
 
-template <typename U>
-__device__ __forceinline__ bool pred_fast(const SqPredK &p, U code) {
+template <typename U, typename P>
+__device__ __forceinline__ bool pred_fast(const P &p, U code) {
     bool in = false;
 #pragma unroll
     for (int i = 0; i < 3; ++i) in = in || static_cast<U>(code - static_cast<U>(p.lo[i])) <= static_cast<U>(p.len[i]);
@@ -553,6 +553,479 @@ __global__ __launch_bounds__(kSqBlock) void k_sq_scan(SqArgs a) {
     }
     const uint32_t wk = wave_sum(kept);
     if (lane == 0 && wk != 0u) atomicAdd(a.kept, static_cast<unsigned long long>(wk));
+}
+
+// ---- shared scan: several star queries over one fact table in one pass ------------------------
+// The batch's fact FK and predicate columns (the union over its queries) are loaded, decoded and
+// checked once per wave step; every query then evaluates its predicates, dimension maps,
+// compaction and measure batch on those registers, with its own group table in LDS.  A query's
+// rows with a corrupt code take the chain-order path with the query's own arguments, exactly as
+// in the single-query scan, so each query's results do not depend on its batch.
+
+constexpr uint32_t kSqbMaxQ = AHEAD_SQ_MAX_BATCH;
+constexpr uint32_t kSqbGroupPool = 3072;   // group table entries in LDS, over the batch
+constexpr uint32_t kSqbMapWords = 26368;   // map words in LDS, over the batch
+
+// a pointer known to address global memory (pointers read from memory are otherwise flat, and a
+// flat access is waited for together with every LDS access)
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gbl(const T *p) {
+    return (const __attribute__((address_space(1))) T *)(p);
+}
+
+__device__ __forceinline__ uint64_t load_code_g(const void *col, uint32_t width, uint64_t i) {
+    switch (width) {
+        case 1: return gbl(static_cast<const uint8_t *>(col))[i];
+        case 2: return gbl(static_cast<const uint16_t *>(col))[i];
+        case 4: return gbl(static_cast<const uint32_t *>(col))[i];
+        default: return gbl(static_cast<const uint64_t *>(col))[i];
+    }
+}
+
+struct SqbDimK {  // a query's dimension over one shared FK column
+    uint32_t kmin, span, lds_off, map_off, stride, errv;
+    uint16_t lb, used;
+};
+
+struct SqbPredK {  // a query's predicate over one shared predicate column (codes of 2 or 4 bytes)
+    uint32_t used, never;
+    uint32_t lo[3], len[3];
+};
+
+struct SqbMeasK {
+    const void *col;
+    uint64_t inv, maxu, wmask;
+    uint32_t width, pre;
+    int32_t pred;  // the shared predicate column over the same column, or -1
+    uint32_t pad;
+};
+
+struct SqbQuery {  // what the scan reads: kernel arguments (scalar loads, global pointers)
+    SqbDimK d[AHEAD_SQ_MAX_DIMS];    // by shared FK column
+    SqbPredK p[AHEAD_SQ_MAX_PREDS];  // by shared predicate column
+    SqbMeasK m[AHEAD_SQ_MAX_MEAS];
+    const uint32_t *maps;
+    uint32_t nm, kind, pre_load, groups, g_lds;
+    // the chain-order path: predicates in the query's order, dimensions in chain order
+    uint8_t np, nd, pslot[AHEAD_SQ_MAX_PREDS], cslot[AHEAD_SQ_MAX_DIMS], dflags[AHEAD_SQ_MAX_DIMS];
+    const uint32_t *row_of[AHEAD_SQ_MAX_DIMS];  // by chain position
+    uint32_t *ev_count;
+    uint64_t *events;
+};
+
+struct SqbCold {  // read once per launch, in device memory
+    const uint32_t *flags;
+    unsigned long long *gsum;
+    uint32_t *gfirst;
+    unsigned long long *kept;
+    uint32_t lds_base, lds_words;
+};
+
+struct SqbColK {
+    const void *col;
+    uint64_t inv, maxu, wmask;
+    uint32_t kmask, width;
+};
+
+struct SqbArgs {  // kernel arguments; the shared columns also sit in SqbDesc::cols for the chain-order path
+    uint64_t n, rows_per_wave;
+    uint32_t nq, pad;
+    SqbColK f[AHEAD_SQ_MAX_DIMS];
+    SqbColK pc[AHEAD_SQ_MAX_PREDS];
+};
+
+// One pass's descriptors in device memory, read through a __restrict__ pointer: nothing in the scan
+// writes them, so their uniform loads stay scalar.
+struct SqbDesc {
+    SqbQuery q[kSqbMaxQ];
+    SqbCold c[kSqbMaxQ];
+    SqbColK cols[AHEAD_SQ_MAX_DIMS + AHEAD_SQ_MAX_PREDS];  // FK columns, then predicate columns
+};
+static_assert(sizeof(SqbDesc) <= 4096 - 64, "descriptor block written through kernel arguments");
+
+template <typename T>
+__global__ void k_sq_put(T *dst, T v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
+}
+
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gblw(T *p) {
+    return (__attribute__((address_space(1))) T *)(p);
+}
+
+__device__ __forceinline__ void push_event_b(const SqbQuery &Q, uint64_t row, uint32_t kind, uint32_t drow) {
+    const uint32_t i = __hip_atomic_fetch_add(gblw(Q.ev_count), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (i < kSqMaxEvents)
+        gblw(Q.events)[i] = (row << 32) | (static_cast<uint64_t>(kind) << 27) | (drow & 0x7FFFFFFu);
+}
+
+// exact_row over the batch's descriptors: the chain's semantics for one row of query Q, with events
+// (predicates in the query's order, dimensions in chain order; the same rules and event kinds).
+__device__ void exact_row_b(const SqbColK *__restrict__ cols, const SqbQuery &Q, uint64_t row,
+                            unsigned long long *s_sum, uint32_t *s_first, uint32_t &kept) {
+    bool alive = true;
+#pragma unroll
+    for (int p = 0; p < AHEAD_SQ_MAX_PREDS; ++p) {
+        if (p >= Q.np) break;
+        const uint32_t sl = Q.pslot[p];
+        const SqbColK &pc = cols[AHEAD_SQ_MAX_DIMS + sl];
+        const uint64_t c = load_code_g(pc.col, pc.width, row);
+        if (((c * pc.inv) & pc.wmask) > pc.maxu) push_event_b(Q, row, AHEAD_SQ_E_PRED + p, 0u);
+        if (!pred_fast<uint32_t>(Q.p[sl], static_cast<uint32_t>(c))) alive = false;
+    }
+    if (!alive) return;
+    bool dropped = false;
+    uint64_t dm[AHEAD_SQ_MAX_MEAS] = {0, 0};
+#pragma unroll
+    for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m) {
+        const SqbMeasK &mk = Q.m[m];
+        if (m >= static_cast<int>(Q.nm) || !mk.pre) continue;
+        dm[m] = (load_code_g(mk.col, mk.width, row) * mk.inv) & mk.wmask;
+        if (dm[m] > mk.maxu) {
+            push_event_b(Q, row, AHEAD_SQ_E_MEAS + m, 0u);
+            dropped = true;
+        }
+    }
+    uint32_t ent[AHEAD_SQ_MAX_DIMS] = {0, 0, 0, 0}, keyv[AHEAD_SQ_MAX_DIMS] = {0, 0, 0, 0};
+    bool fk_bad[AHEAD_SQ_MAX_DIMS] = {false, false, false, false};
+#pragma unroll
+    for (int cp = 0; cp < AHEAD_SQ_MAX_DIMS; ++cp) {
+        if (cp >= Q.nd) break;
+        const uint32_t sl = Q.cslot[cp];
+        const SqbColK &f = cols[sl];
+        const SqbDimK &d = Q.d[sl];
+        const uint64_t dec = gbl(static_cast<const uint64_t *>(f.col))[row] * f.inv;
+        if (dec > f.maxu) {
+            fk_bad[cp] = true;
+            push_event_b(Q, row, AHEAD_SQ_E_FK + cp, 0u);
+            if (!(Q.dflags[cp] & AHEAD_SQ_PROBE)) return;
+        }
+        keyv[cp] = static_cast<uint32_t>(dec) & f.kmask;
+        const uint32_t idx = keyv[cp] - d.kmin;
+        ent[cp] = idx < d.span ? map_entry(gbl(Q.maps)[d.map_off + (idx >> (5u - d.lb))], idx, d.lb) : 0u;
+        if (ent[cp] == 0u) return;
+    }
+#pragma unroll
+    for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m) {
+        const SqbMeasK &mk = Q.m[m];
+        if (m >= static_cast<int>(Q.nm) || mk.pre) continue;
+        dm[m] = (load_code_g(mk.col, mk.width, row) * mk.inv) & mk.wmask;
+        if (dm[m] > mk.maxu) {
+            push_event_b(Q, row, AHEAD_SQ_E_MEAS + m, 0u);
+            dropped = true;
+        }
+    }
+    uint32_t g = 0;
+#pragma unroll
+    for (int cp = 0; cp < AHEAD_SQ_MAX_DIMS; ++cp) {
+        if (cp >= Q.nd) break;
+        if ((Q.dflags[cp] & AHEAD_SQ_REFETCH) && fk_bad[cp]) {
+            push_event_b(Q, row, AHEAD_SQ_E_REFETCH + cp, 0u);
+            dropped = true;
+        }
+        const SqbDimK &d = Q.d[Q.cslot[cp]];
+        if (d.errv == 0u) continue;
+        if (ent[cp] == d.errv) {
+            push_event_b(Q, row, AHEAD_SQ_E_ATTR + cp, gbl(Q.row_of[cp])[keyv[cp] - d.kmin]);
+            dropped = true;
+        } else {
+            g += (ent[cp] - 1u) * d.stride;
+        }
+    }
+    if (dropped || g >= Q.groups) return;
+    const uint64_t v = Q.kind == AHEAD_SQ_SUB ? dm[0] - dm[1] : Q.kind == AHEAD_SQ_MULSUM ? dm[0] * dm[1] : dm[0];
+    atomicAdd(&s_sum[g], static_cast<unsigned long long>(v));
+    atomicMin(&s_first[g], static_cast<uint32_t>(row));
+    ++kept;
+}
+
+// NF shared FK columns, NPC shared predicate columns of PWB bytes (0: widths differ)
+template <int NF, int NPC, int PWB>
+__global__ __launch_bounds__(kSqBlock) void k_sqb_scan(SqbArgs a, const SqbDesc *__restrict__ desc) {
+    constexpr int kSeg = 256;  // rows per wave step: 2 chunks of 128
+    constexpr int R = 4;       // rows per lane per step
+    constexpr uint32_t OFFB = 8u;
+    __shared__ uint32_t s_map[kSqbMapWords];
+    __shared__ unsigned long long s_sum[kSqbGroupPool];
+    __shared__ uint32_t s_first[kSqbGroupPool];
+    __shared__ uint32_t s_stage[kSqWaves * kSeg];
+    __shared__ uint32_t s_kept[kSqbMaxQ];
+    static_assert(sizeof(s_map) + sizeof(s_sum) + sizeof(s_first) + sizeof(s_stage) + sizeof(s_kept) <=
+                      160 * 1024 - 1024,
+                  "LDS budget");
+    static_assert((1u << OFFB) == kSeg && kSqMaxGroups <= (1u << (32 - OFFB)), "staged entry layout");
+    uint32_t qmask = 0;  // queries whose plan holds (no repeated key)
+    for (uint32_t q = 0; q < a.nq; ++q) {
+        const SqbQuery &Q = desc->q[q];
+        const SqbCold &C = desc->c[q];
+        if (C.flags[kFlagDup] != 0u) continue;
+        qmask |= 1u << q;
+        for (uint32_t i = threadIdx.x; i < C.lds_words; i += kSqBlock) s_map[C.lds_base + i] = Q.maps[i];
+        for (uint32_t g = threadIdx.x; g < Q.groups; g += kSqBlock) {
+            s_sum[Q.g_lds + g] = 0ull;
+            s_first[Q.g_lds + g] = 0xFFFFFFFFu;
+        }
+    }
+    if (threadIdx.x < kSqbMaxQ) s_kept[threadIdx.x] = 0u;
+    __syncthreads();
+    if (qmask == 0u) return;
+
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t gw = static_cast<uint64_t>(blockIdx.x) * kSqWaves + wave;
+    const uint64_t begin = gw * a.rows_per_wave;
+    const uint64_t end = begin + a.rows_per_wave < a.n ? begin + a.rows_per_wave : a.n;
+
+    using PT = uint32_t;  // predicate codes of 2 or 4 bytes
+    uint64_t fk[NF][R];
+    PT pc[NPC > 0 ? NPC : 1][R];
+    auto load_step = [&](uint64_t seg) {
+        if (seg + kSeg <= end) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const uint64_t r0 = seg + c * 128u + lane * 2u;
+#pragma unroll
+                for (int d = 0; d < NF; ++d) {
+                    const u32x4 q = load16_nt(static_cast<const uint64_t *>(a.f[d].col) + r0);
+                    fk[d][2 * c] = (static_cast<uint64_t>(q[1]) << 32) | q[0];
+                    fk[d][2 * c + 1] = (static_cast<uint64_t>(q[3]) << 32) | q[2];
+                }
+#pragma unroll
+                for (int p = 0; p < NPC; ++p) {
+                    if constexpr (PWB > 0) load_pair_w<PWB>(a.pc[p].col, r0, pc[p][2 * c], pc[p][2 * c + 1]);
+                    else load_pair(a.pc[p].col, a.pc[p].width, r0, pc[p][2 * c], pc[p][2 * c + 1]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint64_t row = seg + (r >> 1) * 128u + lane * 2u + (r & 1);
+                const bool in = row < end;
+#pragma unroll
+                for (int d = 0; d < NF; ++d) fk[d][r] = in ? static_cast<const uint64_t *>(a.f[d].col)[row] : 0ull;
+#pragma unroll
+                for (int p = 0; p < NPC; ++p) {
+                    if constexpr (PWB > 0) pc[p][r] = in ? static_cast<PT>(load_code_w<PWB>(a.pc[p].col, row)) : PT(0);
+                    else pc[p][r] = in ? static_cast<PT>(load_code(a.pc[p].col, a.pc[p].width, row)) : PT(0);
+                }
+            }
+        }
+    };
+
+    // the pending batch: one joined row per lane of query pend_q, its measure reads in flight
+    bool pend_act = false;
+    uint64_t pend_row = 0;
+    uint32_t pend_g = 0, pend_q = 0;
+    uint64_t pend_code[AHEAD_SQ_MAX_MEAS] = {0, 0};
+    // call in wave-uniform control flow; returns whether this lane's row holds a corrupt measure
+    auto finish_pending = [&]() -> bool {
+        bool bad = false, keep = false;
+        if (pend_act) {
+            const SqbQuery &P = desc->q[pend_q];
+            uint64_t dm[AHEAD_SQ_MAX_MEAS] = {0, 0};
+            bad = pend_g >= P.groups;
+#pragma unroll
+            for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m) {
+                if (m >= static_cast<int>(P.nm)) continue;
+                dm[m] = (pend_code[m] * P.m[m].inv) & P.m[m].wmask;
+                bad |= dm[m] > P.m[m].maxu;
+            }
+            if (!bad) {
+                const uint64_t v = P.kind == AHEAD_SQ_SUB ? dm[0] - dm[1] : P.kind == AHEAD_SQ_MULSUM ? dm[0] * dm[1] : dm[0];
+                atomicAdd(&s_sum[P.g_lds + pend_g], static_cast<unsigned long long>(v));
+                atomicMin(&s_first[P.g_lds + pend_g], static_cast<uint32_t>(pend_row));
+                keep = true;
+            }
+        }
+        pend_act = false;
+        const uint64_t kb = __ballot(keep);
+        if (lane == 0 && kb != 0ull) atomicAdd(&s_kept[pend_q], static_cast<uint32_t>(__popcll(kb)));
+        return bad;
+    };
+    auto exact = [&](uint32_t qq, uint64_t row) {
+        const SqbQuery &QQ = desc->q[qq];
+        uint32_t k = 0;
+        exact_row_b(desc->cols, QQ, row, s_sum + QQ.g_lds, s_first + QQ.g_lds, k);
+        if (k != 0u) atomicAdd(&s_kept[qq], k);
+    };
+
+    if (begin < end) load_step(begin);
+    for (uint64_t seg = begin; seg < end; seg += kSeg) {
+        // 1. shared: decode and check the FK codes, check the predicate codes
+        uint32_t key[NF][R];
+        uint32_t ferr[NF];
+        uint32_t perr[NPC > 0 ? NPC : 1];
+        PT pv[NPC > 0 ? NPC : 1][R];
+        uint32_t live = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (seg + (r >> 1) * 128u + lane * 2u + (r & 1) < end) live |= 1u << r;
+#pragma unroll
+        for (int c = 0; c < NF; ++c) {
+            ferr[c] = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint64_t dec = fk[c][r] * a.f[c].inv;
+                if (dec > a.f[c].maxu) ferr[c] |= 1u << r;
+                key[c][r] = static_cast<uint32_t>(dec) & a.f[c].kmask;
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < NPC; ++p) {
+            perr[p] = 0;
+            const SqbColK &pk = a.pc[p];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                pv[p][r] = pc[p][r];
+                // codes of at most 4 bytes: the low 32 bits of the product decide
+                if (((pc[p][r] * static_cast<uint32_t>(pk.inv)) & static_cast<uint32_t>(pk.wmask)) >
+                    static_cast<uint32_t>(pk.maxu))
+                    perr[p] |= 1u << r;
+            }
+        }
+        // 2. the next step's codes go in flight now
+        if (seg + kSeg < end) load_step(seg + kSeg);
+        const uint64_t dummy = seg;
+        uint32_t *stage = s_stage + wave * kSeg;
+        for (uint32_t q = 0; q < a.nq; ++q) {
+            if (((qmask >> q) & 1u) == 0u) continue;
+            const SqbQuery &Q = desc->q[q];
+            // 3. the query's predicates; its columns' corrupt codes
+            uint32_t err = 0, alive = live;
+#pragma unroll
+            for (int p = 0; p < NPC; ++p) {
+                if (!Q.p[p].used) continue;
+                err |= perr[p];
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (!pred_fast<PT>(Q.p[p], pv[p][r])) alive &= ~(1u << r);
+            }
+#pragma unroll
+            for (int c = 0; c < NF; ++c)
+                if (Q.d[c].used) err |= ferr[c];
+            err &= live;
+            alive &= ~err;
+            // 4. measures fetched right after the predicates (not over a predicate's column)
+            if (Q.pre_load) {
+#pragma unroll
+                for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m) {
+                    const SqbMeasK &mk = Q.m[m];
+                    if (m >= static_cast<int>(Q.nm) || !mk.pre || mk.pred >= 0) continue;
+                    uint64_t code[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const uint64_t row = seg + (r >> 1) * 128u + lane * 2u + (r & 1);
+                        code[r] = load_code_g(mk.col, mk.width, ((alive >> r) & 1u) ? row : dummy);
+                    }
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (((alive >> r) & 1u) && ((code[r] * mk.inv) & mk.wmask) > mk.maxu) err |= 1u << r;
+                }
+                alive &= ~err;
+            }
+            // 5. the query's dimensions, in the batch's column order
+            uint32_t g[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) g[r] = 0;
+#pragma unroll
+            for (int c = 0; c < NF; ++c) {
+                const SqbDimK &dd = Q.d[c];
+                if (!dd.used) continue;
+                const uint32_t sh = 5u - dd.lb;
+                uint32_t w[R], idx[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    idx[r] = key[c][r] - dd.kmin;
+                    const bool use = ((alive >> r) & 1u) && idx[r] < dd.span;
+                    if (!use) idx[r] = 0;
+                    if (!use) alive &= ~(1u << r);
+                }
+                if (dd.lds_off != kNoLds) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) w[r] = s_map[dd.lds_off + (idx[r] >> sh)];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) w[r] = gbl(Q.maps)[dd.map_off + (idx[r] >> sh)];
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint32_t e = map_entry(w[r], idx[r], dd.lb);
+                    if (e == 0u) alive &= ~(1u << r);
+                    if (dd.errv != 0u && ((alive >> r) & 1u)) {
+                        if (e == dd.errv) err |= 1u << r;
+                        g[r] += (e - 1u) * dd.stride;
+                    }
+                }
+                alive &= ~err;
+            }
+            // 6. joined rows: compaction, measure reads (the last batch stays pending)
+            const bool prev_bad = finish_pending();
+            const uint64_t prev_row = pend_row;
+            const uint32_t prev_q = pend_q;
+            uint32_t extra_bad = 0;
+            uint32_t total = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const bool al = (alive >> r) & 1u;
+                const uint64_t b = __ballot(al);
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
+                if (al) stage[total + below] = ((r >> 1) * 128u + lane * 2u + (r & 1)) | (g[r] << OFFB);
+                total += static_cast<uint32_t>(__popcll(b));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (uint32_t i0 = 0; i0 < total; i0 += 64u) {
+                const bool act = i0 + lane < total;
+                const uint32_t ent = act ? stage[i0 + lane] : 0u;
+                pend_act = act;
+                pend_row = seg + (ent & ((1u << OFFB) - 1u));
+                pend_g = ent >> OFFB;
+                pend_q = q;
+#pragma unroll
+                for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m)
+                    if (m < static_cast<int>(Q.nm)) pend_code[m] = load_code_g(Q.m[m].col, Q.m[m].width, act ? pend_row : dummy);
+                if (i0 + 64u < total && finish_pending()) extra_bad |= 1u << (i0 >> 6);
+            }
+            // 7. the chain-order path (rare): rows with a corrupt code, the previous pending row,
+            // rows of extra batches with a corrupt measure
+            uint32_t todo = err | (prev_bad ? 1u << R : 0u) | (extra_bad << (R + 1));
+            while (__ballot(todo != 0u) != 0ull) {
+                if (todo != 0u) {
+                    const int b = __ffs(todo) - 1;
+                    todo &= todo - 1u;
+                    uint64_t row;
+                    uint32_t qq = q;
+                    if (b < R) {
+                        row = seg + (b >> 1) * 128u + lane * 2u + (b & 1);
+                    } else if (b == R) {
+                        row = prev_row;
+                        qq = prev_q;
+                    } else {
+                        row = seg + (stage[(b - R - 1) * 64 + lane] & ((1u << OFFB) - 1u));
+                    }
+                    exact(qq, row);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    if (finish_pending()) exact(pend_q, pend_row);
+    __syncthreads();
+    for (uint32_t q = 0; q < a.nq; ++q) {
+        if (((qmask >> q) & 1u) == 0u) continue;
+        const SqbQuery &Q = desc->q[q];
+        const SqbCold &C = desc->c[q];
+        for (uint32_t g = threadIdx.x; g < Q.groups; g += kSqBlock) {
+            if (s_first[Q.g_lds + g] != 0xFFFFFFFFu) {
+                atomicAdd(&C.gsum[g], s_sum[Q.g_lds + g]);
+                atomicMin(&C.gfirst[g], s_first[Q.g_lds + g]);
+            }
+        }
+        if (threadIdx.x == 0 && s_kept[q] != 0u) atomicAdd(C.kept, static_cast<unsigned long long>(s_kept[q]));
+    }
 }
 
 // ---- dimension side ---------------------------------------------------------------------------
@@ -1019,37 +1492,53 @@ extern "C" int ahead_hip_star_key_index_status(const ahead_sq_key_index *ix) {
 
 namespace {
 
-// Workspace bytes of one star query (the layout of ahead_hip_star_query below, computed alike).
-size_t sq_ws_bytes(const ahead_sq_args *q) {
-    size_t off = 256;
-    off += align_up(static_cast<size_t>(AHEAD_SQ_MAX_DIMS) * kStatWords * 4, 256);
-    off += static_cast<size_t>(kSqMaxGroups) * 8;
-    off += static_cast<size_t>(kSqMaxGroups) * 4;
+// Workspace layout of one star query: flags, statistics, group tables, kept rows and the event
+// count, events, then the dimension maps sized for the largest key span the plan accepts
+// (4 x rows + 1024, 16-bit entries at most) and, for dimensions without a key index, the key
+// bitmap and the row of every key.
+struct SqLayout {
+    size_t off_flags = 0, off_stats = 0, off_gsum = 0, off_gfirst = 0, off_kept = 0, clear_end = 0;
+    size_t off_events = 0, off_maps = 0, total = 0;
+    size_t off_seen[AHEAD_SQ_MAX_DIMS] = {}, off_row[AHEAD_SQ_MAX_DIMS] = {};
+    uint64_t span_cap[AHEAD_SQ_MAX_DIMS] = {};
+};
+
+SqLayout sq_layout(const ahead_sq_args *q) {
+    SqLayout L;
+    size_t off = 0;
+    L.off_flags = off;
     off += 256;
+    L.off_stats = off;
+    off += align_up(static_cast<size_t>(AHEAD_SQ_MAX_DIMS) * kStatWords * 4, 256);
+    L.off_gsum = off;
+    off += static_cast<size_t>(kSqMaxGroups) * 8;
+    L.off_gfirst = off;
+    off += static_cast<size_t>(kSqMaxGroups) * 4;
+    L.off_kept = off;
+    off += 256;  // kept rows (u64) and the event count (u32)
+    L.clear_end = off;
+    L.off_events = off;
     off += static_cast<size_t>(kSqMaxEvents) * 8;
-    const size_t off_maps = align_up(off, 256);
+    L.off_maps = align_up(off, 256);
     size_t maps_words_cap = 0;
-    for (uint32_t d = 0; d < q->n_dims; ++d) maps_words_cap += static_cast<size_t>((4ull * q->dims[d].key.n + 1024) / 2 + 1);
-    off = off_maps + align_up(maps_words_cap * 4, 256);
+    for (uint32_t d = 0; d < q->n_dims; ++d) {
+        L.span_cap[d] = 4ull * q->dims[d].key.n + 1024;
+        maps_words_cap += static_cast<size_t>(L.span_cap[d] / 2 + 1);
+    }
+    off = L.off_maps + align_up(maps_words_cap * 4, 256);
     for (uint32_t d = 0; d < q->n_dims; ++d) {
         if (q->dims[d].key_index != nullptr) continue;
-        const uint64_t span_cap = 4ull * q->dims[d].key.n + 1024;
-        off += align_up((span_cap / 32 + 1) * 4, 256);
-        off += align_up(span_cap * 4, 256);
+        L.off_seen[d] = off;
+        off += align_up((L.span_cap[d] / 32 + 1) * 4, 256);
+        L.off_row[d] = off;
+        off += align_up(L.span_cap[d] * 4, 256);
     }
-    return off;
+    L.total = off;
+    return L;
 }
 
-}  // namespace
-
-extern "C" size_t ahead_hip_star_query_ws_bytes(const ahead_sq_args *q) {
-    if (q == nullptr || q->n_dims < 1 || q->n_dims > AHEAD_SQ_MAX_DIMS) return 0;
-    return kHeaderBytes + sq_ws_bytes(q);
-}
-
-extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *out, ahead_hip_ws *ws,
-                                    void *stream) {
-    if (q == nullptr || out == nullptr || ws == nullptr) return fail(AHEAD_ERR_INVALID, "star_query: NULL argument");
+int sq_validate(const ahead_sq_args *q, const ahead_sq_out *out) {
+    if (q == nullptr || out == nullptr) return fail(AHEAD_ERR_INVALID, "star_query: NULL argument");
     if (out->d_counts == nullptr) return fail(AHEAD_ERR_INVALID, "star_query: d_counts is NULL");
     if (q->n_dims < 1 || q->n_dims > AHEAD_SQ_MAX_DIMS) return fail(AHEAD_ERR_INVALID, "star_query: 1 to 4 dimensions");
     if (q->n_preds > AHEAD_SQ_MAX_PREDS) return fail(AHEAD_ERR_INVALID, "star_query: at most 2 fact predicates");
@@ -1057,7 +1546,6 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
     const uint32_t nm = q->meas_kind == AHEAD_SQ_SUM ? 1u : 2u;
     const uint64_t n = q->n_fact;
     if (n >= (1ull << 32)) return fail(AHEAD_ERR_TOO_LARGE, "star_query: more than 2^32-1 fact rows");
-    hipStream_t s = static_cast<hipStream_t>(stream);
     int st;
     for (uint32_t d = 0; d < q->n_dims; ++d) {
         const ahead_sq_dim &dm = q->dims[d];
@@ -1088,53 +1576,44 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
     }
     for (uint32_t m = 0; m < nm; ++m)
         if ((st = check_fact_col(&q->meas[m], n, "a measure column", false)) != AHEAD_OK) return st;
+    return AHEAD_OK;
+}
 
-    // ---- workspace: flags, statistics, group tables, events, then per-dimension maps sized for
-    // the largest key span the plan accepts (4 x rows + 1024)
-    const uint32_t nd = q->n_dims;
-    std::vector<DimPlan> P(nd);
-    size_t off = 0;
-    const size_t ws_need = sq_ws_bytes(q);
-    const size_t off_flags = off;
-    off += 256;
-    const size_t off_stats = off;
-    off += align_up(static_cast<size_t>(AHEAD_SQ_MAX_DIMS) * kStatWords * 4, 256);
-    const size_t off_gsum = off;
-    off += static_cast<size_t>(kSqMaxGroups) * 8;
-    const size_t off_gfirst = off;
-    off += static_cast<size_t>(kSqMaxGroups) * 4;
-    const size_t off_kept = off;
-    off += 256;  // kept rows (u64) and the event count (u32)
-    const size_t clear_end = off;
-    const size_t off_events = off;
-    off += static_cast<size_t>(kSqMaxEvents) * 8;
-    const size_t off_maps = align_up(off, 256);
-    size_t maps_words_cap = 0;
-    for (uint32_t d = 0; d < nd; ++d) {
-        P[d].span_cap = 4ull * q->dims[d].key.n + 1024;
-        maps_words_cap += static_cast<size_t>(P[d].span_cap / 2 + 1);  // 16-bit entries at most
-    }
-    off = off_maps + align_up(maps_words_cap * 4, 256);
-    for (uint32_t d = 0; d < nd; ++d) {
-        if (q->dims[d].key_index != nullptr) continue;
-        P[d].off_seen = off;
-        off += align_up((P[d].span_cap / 32 + 1) * 4, 256);
-        P[d].off_row = off;
-        off += align_up(P[d].span_cap * 4, 256);
-    }
-    if (off != ws_need) return fail(AHEAD_ERR_DEVICE, "star_query: workspace layout mismatch");
-    if ((st = ws_ensure(ws, off)) != AHEAD_OK) return st;
-    char *base = ws_region(ws, 0);
-    uint32_t *flags = reinterpret_cast<uint32_t *>(base + off_flags);
-    uint32_t *stats = reinterpret_cast<uint32_t *>(base + off_stats);
+// One query between its statistics and its scan.
+struct SqQueryPlan {
+    const ahead_sq_args *q = nullptr;
+    const ahead_sq_out *out = nullptr;
+    char *base = nullptr;  // the query's workspace region
+    SqLayout L;
+    std::vector<DimPlan> P;
+    uint64_t status = AHEAD_SQ_ST_OK;  // a plan-time status: the fused form does not apply
+    uint64_t groups = 1;
+    std::vector<uint32_t> order;  // evaluation order (LDS maps first)
+    uint32_t lds_words = 0;       // leading words of the map region that go to LDS
+    SqArgs a{};
+    SqFinalArgs f{};
+    uint32_t *flags() const { return reinterpret_cast<uint32_t *>(base + L.off_flags); }
+    uint32_t *stats() const { return reinterpret_cast<uint32_t *>(base + L.off_stats); }
+};
+
+// Clears the query's counters and launches its statistics kernels.
+int sq_prep(SqQueryPlan &Q, hipStream_t s) {
+    const ahead_sq_args *q = Q.q;
+    char *base = Q.base;
+    const SqLayout &L = Q.L;
+    int st;
     // clear flags .. event count; the statistics start at all-ones (running minima), count 0
-    if ((st = fill_u64(reinterpret_cast<uint64_t *>(base), clear_end / 8, 0, s)) != AHEAD_OK) return st;
-    if ((st = fill_u32(stats, static_cast<uint64_t>(AHEAD_SQ_MAX_DIMS) * kStatWords, 0xFFFFFFFFu, s)) != AHEAD_OK) return st;
-    if ((st = fill_u32(reinterpret_cast<uint32_t *>(base + off_gfirst), kSqMaxGroups, 0xFFFFFFFFu, s)) != AHEAD_OK) return st;
-
-    for (uint32_t d = 0; d < nd; ++d) {
+    if ((st = fill_u64(reinterpret_cast<uint64_t *>(base), L.clear_end / 8, 0, s)) != AHEAD_OK) return st;
+    if ((st = fill_u32(Q.stats(), static_cast<uint64_t>(AHEAD_SQ_MAX_DIMS) * kStatWords, 0xFFFFFFFFu, s)) != AHEAD_OK) return st;
+    if ((st = fill_u32(reinterpret_cast<uint32_t *>(base + L.off_gfirst), kSqMaxGroups, 0xFFFFFFFFu, s)) != AHEAD_OK) return st;
+    Q.P.assign(q->n_dims, DimPlan{});
+    for (uint32_t d = 0; d < q->n_dims; ++d) {
         const ahead_sq_dim &dm = q->dims[d];
-        DimPrepArgs &a = P[d].prep;
+        DimPlan &pl = Q.P[d];
+        pl.span_cap = L.span_cap[d];
+        pl.off_seen = L.off_seen[d];
+        pl.off_row = L.off_row[d];
+        DimPrepArgs &a = pl.prep;
         a.key = static_cast<const uint64_t *>(dm.key.data);
         a.n = dm.key.n;
         a.key_inv = dm.key.a_inv;
@@ -1149,36 +1628,39 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         a.attr_wmask = width_mask(dm.attr.width);
         a.attr_maxu = dm.attr.max_u & a.attr_wmask;
         a.amask = static_cast<uint32_t>(bits_mask(dm.attr.unenc_bits));
-        a.stats = stats + d * kStatWords;
-        a.flags = flags;
+        a.stats = Q.stats() + d * kStatWords;
+        a.flags = Q.flags();
         if (dm.key_index == nullptr) hipLaunchKernelGGL(k_sq_keystats, dim3(prep_grid(a.n)), dim3(kBlock), 0, s, a);
         hipLaunchKernelGGL(k_sq_selstats, dim3(prep_grid(a.n)), dim3(kBlock), 0, s, a);
     }
     AHEAD_HIP_TRY(hipGetLastError());
-    uint32_t h_stats[AHEAD_SQ_MAX_DIMS * kStatWords];
-    uint32_t h_flags[4];
-    AHEAD_HIP_TRY(hipMemcpyAsync(h_stats, stats, sizeof h_stats, hipMemcpyDeviceToHost, s));
-    AHEAD_HIP_TRY(hipMemcpyAsync(h_flags, flags, sizeof h_flags, hipMemcpyDeviceToHost, s));
-    AHEAD_HIP_TRY(hipStreamSynchronize(s));
+    return AHEAD_OK;
+}
 
-    // ---- plan
-    auto not_fused = [&](uint64_t status) -> int {
-        hipLaunchKernelGGL(k_sq_status, dim3(1), dim3(64), 0, s, out->d_counts, status);
-        AHEAD_HIP_TRY(hipGetLastError());
-        return AHEAD_OK;
-    };
-    // statuses in dimension order, as the oracle reports them
+// From the statistics read back: key spans, attribute ranges, map sizes, groups; or the status
+// that makes the fused form inapplicable (in dimension order, as the oracle reports them).
+void sq_plan(SqQueryPlan &Q, const uint32_t *h_stats, const uint32_t *h_flags) {
+    const ahead_sq_args *q = Q.q;
+    const uint32_t nd = q->n_dims;
     for (uint32_t d = 0; d < nd; ++d) {
         const ahead_sq_key_index *ix = q->dims[d].key_index;
-        if (ix != nullptr && ix->status != AHEAD_SQ_ST_OK) return not_fused(ix->status);
+        if (ix != nullptr && ix->status != AHEAD_SQ_ST_OK) {
+            Q.status = static_cast<uint64_t>(ix->status);
+            return;
+        }
     }
-    if (h_flags[kFlagBadKey]) return not_fused(AHEAD_SQ_ST_BAD_KEY);
-    if (h_flags[kFlagBadOid]) return not_fused(AHEAD_SQ_ST_BAD_OID);
+    if (h_flags[kFlagBadKey]) {
+        Q.status = AHEAD_SQ_ST_BAD_KEY;
+        return;
+    }
+    if (h_flags[kFlagBadOid]) {
+        Q.status = AHEAD_SQ_ST_BAD_OID;
+        return;
+    }
     uint64_t groups = 1;
-    size_t map_off = 0;
     for (uint32_t d = 0; d < nd; ++d) {
         const uint32_t *sd = h_stats + d * kStatWords;
-        DimPlan &pl = P[d];
+        DimPlan &pl = Q.P[d];
         const ahead_sq_key_index *ix = q->dims[d].key_index;
         if (ix != nullptr) {
             pl.kmin = ix->kmin;
@@ -1187,7 +1669,10 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
             pl.kmin = sd[0];
             const uint32_t kmax = ~sd[1];
             const uint64_t span = static_cast<uint64_t>(kmax) - pl.kmin + 1;
-            if (span > pl.span_cap) return not_fused(AHEAD_SQ_ST_KEY_SPAN);
+            if (span > pl.span_cap) {
+                Q.status = AHEAD_SQ_ST_KEY_SPAN;
+                return;
+            }
             pl.span = static_cast<uint32_t>(span);
         }
         const uint32_t nsel = sd[4];
@@ -1196,13 +1681,19 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
             const bool any_valid = sd[2] != 0xFFFFFFFFu || ~sd[3] != 0xFFFFFFFFu;
             pl.amin = any_valid ? sd[2] : 0;
             const uint64_t aspan = any_valid ? static_cast<uint64_t>(~sd[3]) - pl.amin + 1 : 1;
-            if (aspan > 65534) return not_fused(AHEAD_SQ_ST_GROUPS);
+            if (aspan > 65534) {
+                Q.status = AHEAD_SQ_ST_GROUPS;
+                return;
+            }
             pl.aspan = static_cast<uint32_t>(aspan);
             uint32_t lb = 1;  // 2-bit entries at least
             while ((1ull << (1u << lb)) < aspan + 2) ++lb;
             pl.lb = lb;
             groups *= aspan;
-            if (groups > kSqMaxGroups) return not_fused(AHEAD_SQ_ST_GROUPS);
+            if (groups > kSqMaxGroups) {
+                Q.status = AHEAD_SQ_ST_GROUPS;
+                return;
+            }
         } else {
             pl.lb = 0;
             pl.aspan = 0;
@@ -1210,37 +1701,47 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         const uint32_t per_word = 32u >> pl.lb;
         pl.map_words = (pl.span + per_word - 1) / per_word;
     }
-    // LDS placement: most selective dimension first while the maps fit; evaluation order: LDS
-    // maps first, then the L2 maps, each by selected fraction
-    std::vector<uint32_t> by_frac(nd);
-    std::iota(by_frac.begin(), by_frac.end(), 0u);
-    std::stable_sort(by_frac.begin(), by_frac.end(), [&](uint32_t x, uint32_t y) { return P[x].frac < P[y].frac; });
-    uint32_t lds_used = 0;
-    const uint32_t lds_cap = sq_map_words(sq_chunks(static_cast<int>(nd)));
-    for (uint32_t d : by_frac) {
-        if (lds_used + P[d].map_words <= lds_cap) {
-            P[d].in_lds = true;
-            lds_used += P[d].map_words;
-        }
-    }
-    std::vector<uint32_t> order;
+    Q.groups = groups;
+}
+
+// Dimensions by selected fraction, most selective first.
+std::vector<uint32_t> by_selectivity(const SqQueryPlan &Q) {
+    std::vector<uint32_t> v(Q.P.size());
+    std::iota(v.begin(), v.end(), 0u);
+    std::stable_sort(v.begin(), v.end(), [&](uint32_t x, uint32_t y) { return Q.P[x].frac < Q.P[y].frac; });
+    return v;
+}
+
+// Fills the maps (LDS maps first in the map region, in evaluation order), builds the key bitmaps
+// where no key index holds them, and the scan and finish arguments.  P[d].in_lds is decided.
+int sq_build(SqQueryPlan &Q, hipStream_t s) {
+    const ahead_sq_args *q = Q.q;
+    const uint32_t nd = q->n_dims;
+    std::vector<DimPlan> &P = Q.P;
+    const std::vector<uint32_t> by_frac = by_selectivity(Q);
+    Q.order.clear();
     for (uint32_t d : by_frac)
-        if (P[d].in_lds) order.push_back(d);
+        if (P[d].in_lds) Q.order.push_back(d);
     for (uint32_t d : by_frac)
-        if (!P[d].in_lds) order.push_back(d);
-    // maps in the HBM region: LDS maps first (copied as one block), in evaluation order
-    for (uint32_t d : order) {
+        if (!P[d].in_lds) Q.order.push_back(d);
+    size_t map_off = 0;
+    Q.lds_words = 0;
+    for (uint32_t d : Q.order) {
         P[d].off_map = map_off;
         map_off += P[d].map_words;
+        if (P[d].in_lds) Q.lds_words += P[d].map_words;
     }
-    uint32_t *maps = reinterpret_cast<uint32_t *>(base + off_maps);
+    int st;
+    uint32_t *maps = reinterpret_cast<uint32_t *>(Q.base + Q.L.off_maps);
     if ((st = fill_u32(maps, map_off, 0u, s)) != AHEAD_OK) return st;
 
-    SqArgs a{};
-    SqFinalArgs f{};
+    SqArgs &a = Q.a;
+    SqFinalArgs &f = Q.f;
+    a = SqArgs{};
+    f = SqFinalArgs{};
     uint32_t stride = 1;
     for (uint32_t i = 0; i < nd; ++i) {
-        const uint32_t d = order[i];
+        const uint32_t d = Q.order[i];
         DimPlan &pl = P[d];
         DimPrepArgs &pa = pl.prep;
         const ahead_sq_key_index *ix = q->dims[d].key_index;
@@ -1253,10 +1754,10 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         if (ix != nullptr) {
             pa.row_of = ix->row_of;
         } else {
-            uint32_t *seen = reinterpret_cast<uint32_t *>(base + pl.off_seen);
+            uint32_t *seen = reinterpret_cast<uint32_t *>(Q.base + pl.off_seen);
             if ((st = fill_u32(seen, pl.span / 32 + 1, 0u, s)) != AHEAD_OK) return st;
             pa.seen = seen;
-            pa.row_of = reinterpret_cast<uint32_t *>(base + pl.off_row);
+            pa.row_of = reinterpret_cast<uint32_t *>(Q.base + pl.off_row);
             hipLaunchKernelGGL(k_sq_keys, dim3(prep_grid(pa.n)), dim3(kBlock), 0, s, pa);
         }
         hipLaunchKernelGGL(k_sq_sel, dim3(prep_grid(pa.n)), dim3(kBlock), 0, s, pa);
@@ -1288,7 +1789,8 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         if (pa.errv != 0u) stride *= pl.aspan;
     }
     AHEAD_HIP_TRY(hipGetLastError());
-    a.n = n;
+    const uint32_t nm = q->meas_kind == AHEAD_SQ_SUM ? 1u : 2u;
+    a.n = q->n_fact;
     a.nd = nd;
     a.np = q->n_preds;
     a.nm = nm;
@@ -1325,22 +1827,84 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         if (k.pre && k.pred < 0) a.pre_load = 1;
     }
     a.maps = maps;
-    a.lds_words = lds_used;
-    a.groups = static_cast<uint32_t>(groups);
-    a.flags = flags;
-    a.gsum = reinterpret_cast<unsigned long long *>(base + off_gsum);
-    a.gfirst = reinterpret_cast<uint32_t *>(base + off_gfirst);
-    a.kept = reinterpret_cast<unsigned long long *>(base + off_kept);
-    a.ev_count = reinterpret_cast<uint32_t *>(base + off_kept + 8);
-    a.events = reinterpret_cast<uint64_t *>(base + off_events);
-    const uint32_t grid = sq_grid();
+    a.lds_words = Q.lds_words;
+    a.groups = static_cast<uint32_t>(Q.groups);
+    a.flags = Q.flags();
+    a.gsum = reinterpret_cast<unsigned long long *>(Q.base + Q.L.off_gsum);
+    a.gfirst = reinterpret_cast<uint32_t *>(Q.base + Q.L.off_gfirst);
+    a.kept = reinterpret_cast<unsigned long long *>(Q.base + Q.L.off_kept);
+    a.ev_count = reinterpret_cast<uint32_t *>(Q.base + Q.L.off_kept + 8);
+    a.events = reinterpret_cast<uint64_t *>(Q.base + Q.L.off_events);
+    f.groups = a.groups;
+    f.nd = nd;
+    f.gsum = a.gsum;
+    f.gfirst = a.gfirst;
+    f.kept = a.kept;
+    f.flags = a.flags;
+    f.ev_count = a.ev_count;
+    f.events = a.events;
+    f.a_result = q->a_result;
+    f.a_oid_err = q->a_oid_err;
+    f.out = *Q.out;
+    return AHEAD_OK;
+}
+
+int sq_not_fused(const SqQueryPlan &Q, hipStream_t s) {
+    hipLaunchKernelGGL(k_sq_status, dim3(1), dim3(64), 0, s, Q.out->d_counts, Q.status);
+    AHEAD_HIP_TRY(hipGetLastError());
+    return AHEAD_OK;
+}
+
+uint64_t rows_per_wave(uint64_t n, uint32_t grid) {
     const uint64_t waves = static_cast<uint64_t>(grid) * kSqWaves;
     const uint64_t per = (n + waves - 1) / waves;
-    const uint64_t seg_rows = 128ull * sq_chunks(static_cast<int>(nd));
-    a.rows_per_wave = (per + seg_rows - 1) / seg_rows * seg_rows;
+    const uint64_t seg_rows = 128ull * sq_chunks(0);
+    return (per + seg_rows - 1) / seg_rows * seg_rows;
+}
+
+}  // namespace
+
+extern "C" size_t ahead_hip_star_query_ws_bytes(const ahead_sq_args *q) {
+    if (q == nullptr || q->n_dims < 1 || q->n_dims > AHEAD_SQ_MAX_DIMS) return 0;
+    return kHeaderBytes + sq_layout(q).total;
+}
+
+extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *out, ahead_hip_ws *ws,
+                                    void *stream) {
+    if (ws == nullptr) return fail(AHEAD_ERR_INVALID, "star_query: NULL argument");
+    int st = sq_validate(q, out);
+    if (st != AHEAD_OK) return st;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    SqQueryPlan Q;
+    Q.q = q;
+    Q.out = out;
+    Q.L = sq_layout(q);
+    if ((st = ws_ensure(ws, Q.L.total)) != AHEAD_OK) return st;
+    Q.base = ws_region(ws, 0);
+    if ((st = sq_prep(Q, s)) != AHEAD_OK) return st;
+    uint32_t h_stats[AHEAD_SQ_MAX_DIMS * kStatWords];
+    uint32_t h_flags[4];
+    AHEAD_HIP_TRY(hipMemcpyAsync(h_stats, Q.stats(), sizeof h_stats, hipMemcpyDeviceToHost, s));
+    AHEAD_HIP_TRY(hipMemcpyAsync(h_flags, Q.flags(), sizeof h_flags, hipMemcpyDeviceToHost, s));
+    AHEAD_HIP_TRY(hipStreamSynchronize(s));
+    sq_plan(Q, h_stats, h_flags);
+    if (Q.status != AHEAD_SQ_ST_OK) return sq_not_fused(Q, s);
+    // LDS placement: most selective dimension first while the maps fit
+    uint32_t lds_used = 0;
+    const uint32_t lds_cap = sq_map_words(sq_chunks(static_cast<int>(q->n_dims)));
+    for (uint32_t d : by_selectivity(Q)) {
+        if (lds_used + Q.P[d].map_words <= lds_cap) {
+            Q.P[d].in_lds = true;
+            lds_used += Q.P[d].map_words;
+        }
+    }
+    if ((st = sq_build(Q, s)) != AHEAD_OK) return st;
+    SqArgs &a = Q.a;
+    const uint32_t grid = sq_grid();
+    a.rows_per_wave = rows_per_wave(q->n_fact, grid);
     if (q->ev_scan[0] != nullptr) AHEAD_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(q->ev_scan[0]), s));
-    if (n > 0) {
-        switch (nd) {
+    if (q->n_fact > 0) {
+        switch (q->n_dims) {
             case 1: launch_scan_np<1>(a, grid, s); break;
             case 2: launch_scan_np<2>(a, grid, s); break;
             case 3: launch_scan_np<3>(a, grid, s); break;
@@ -1349,18 +1913,308 @@ extern "C" int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *
         AHEAD_HIP_TRY(hipGetLastError());
     }
     if (q->ev_scan[1] != nullptr) AHEAD_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(q->ev_scan[1]), s));
-    f.groups = a.groups;
-    f.nd = nd;
-    f.gsum = a.gsum;
-    f.gfirst = a.gfirst;
-    f.kept = a.kept;
-    f.flags = flags;
-    f.ev_count = a.ev_count;
-    f.events = a.events;
-    f.a_result = q->a_result;
-    f.a_oid_err = q->a_oid_err;
-    f.out = *out;
-    hipLaunchKernelGGL(k_sq_final, dim3(1), dim3(kSqBlock), 0, s, f);
+    hipLaunchKernelGGL(k_sq_final, dim3(1), dim3(kSqBlock), 0, s, Q.f);
+    AHEAD_HIP_TRY(hipGetLastError());
+    return AHEAD_OK;
+}
+
+namespace {
+
+// Batch workspace: every query's own region (as ahead_hip_star_query lays it out), then one
+// descriptor block per pass (at most nq).
+struct SqbLayout {
+    std::vector<size_t> off;
+    size_t off_qdesc = 0, total = 0;
+};
+
+SqbLayout sqb_layout(const ahead_sq_args *const *qs, uint32_t nq) {
+    SqbLayout B;
+    size_t off = 0;
+    for (uint32_t i = 0; i < nq; ++i) {
+        B.off.push_back(off);
+        off += align_up(sq_layout(qs[i]).total, 256);
+    }
+    B.off_qdesc = off;
+    off += static_cast<size_t>(nq) * align_up(sizeof(SqbDesc), 256);
+    B.total = off;
+    return B;
+}
+
+bool same_col(const ahead_col &x, const ahead_col &y) {
+    return x.data == y.data && x.width == y.width && x.a_inv == y.a_inv && x.max_u == y.max_u &&
+           x.unenc_bits == y.unenc_bits;
+}
+
+// One shared scan: its queries, the shared columns, each query's dimension and predicate slots.
+struct SqbPass {
+    std::vector<uint32_t> qi;
+    std::vector<const ahead_col *> fcols, pcols;
+    std::vector<std::vector<int>> dim_slot, pred_slot;
+    uint32_t groups = 0;
+    bool alone = false;  // a query the shared scan cannot take: it runs by itself
+};
+
+// Adds the query to the pass if its columns and group table fit.
+bool sqb_try_add(SqbPass &ps, uint32_t qi, const SqQueryPlan &Q) {
+    const ahead_sq_args *q = Q.q;
+    bool shareable = true;  // the shared scan reads predicate codes of 2 or 4 bytes
+    for (uint32_t k = 0; k < q->n_preds; ++k)
+        if (q->preds[k].col.width != 2 && q->preds[k].col.width != 4) shareable = false;
+    if (ps.alone || (!shareable && !ps.qi.empty())) return false;
+    if (ps.qi.size() >= kSqbMaxQ || ps.groups + Q.groups > kSqbGroupPool) return false;
+    std::vector<const ahead_col *> f = ps.fcols, p = ps.pcols;
+    std::vector<int> ds(q->n_dims, -1), pslot(q->n_preds, -1);
+    auto slot_of = [](std::vector<const ahead_col *> &cols, const ahead_col &c, const std::vector<int> &taken) {
+        for (size_t j = 0; j < cols.size(); ++j)
+            if (same_col(*cols[j], c) && std::find(taken.begin(), taken.end(), static_cast<int>(j)) == taken.end())
+                return static_cast<int>(j);
+        cols.push_back(&c);
+        return static_cast<int>(cols.size() - 1);
+    };
+    for (uint32_t d = 0; d < q->n_dims; ++d) ds[d] = slot_of(f, q->dims[d].fk, ds);
+    for (uint32_t k = 0; k < q->n_preds; ++k) pslot[k] = slot_of(p, q->preds[k].col, pslot);
+    if (f.size() > AHEAD_SQ_MAX_DIMS || p.size() > AHEAD_SQ_MAX_PREDS) return false;
+    ps.fcols = f;
+    ps.pcols = p;
+    ps.alone = !shareable;
+    ps.qi.push_back(qi);
+    ps.dim_slot.push_back(ds);
+    ps.pred_slot.push_back(pslot);
+    ps.groups += static_cast<uint32_t>(Q.groups);
+    return true;
+}
+
+template <int NF, int NPC>
+void launch_sqb_w(const SqbArgs &a, const SqbDesc *d, uint32_t pwb, uint32_t grid, hipStream_t s) {
+    if (pwb == 2) hipLaunchKernelGGL((k_sqb_scan<NF, NPC, 2>), dim3(grid), dim3(kSqBlock), 0, s, a, d);
+    else hipLaunchKernelGGL((k_sqb_scan<NF, NPC, 0>), dim3(grid), dim3(kSqBlock), 0, s, a, d);
+}
+
+template <int NF>
+void launch_sqb_np(const SqbArgs &a, const SqbDesc *d, uint32_t npc, uint32_t pwb, uint32_t grid, hipStream_t s) {
+    switch (npc) {
+        case 0: hipLaunchKernelGGL((k_sqb_scan<NF, 0, 0>), dim3(grid), dim3(kSqBlock), 0, s, a, d); break;
+        case 1: launch_sqb_w<NF, 1>(a, d, pwb, grid, s); break;
+        default: launch_sqb_w<NF, 2>(a, d, pwb, grid, s); break;
+    }
+}
+
+void launch_sqb(const SqbArgs &a, const SqbDesc *d, uint32_t nf, uint32_t npc, uint32_t pwb, uint32_t grid,
+                hipStream_t s) {
+    switch (nf) {
+        case 1: launch_sqb_np<1>(a, d, npc, pwb, grid, s); break;
+        case 2: launch_sqb_np<2>(a, d, npc, pwb, grid, s); break;
+        case 3: launch_sqb_np<3>(a, d, npc, pwb, grid, s); break;
+        default: launch_sqb_np<4>(a, d, npc, pwb, grid, s); break;
+    }
+}
+
+void launch_single_scan(const SqArgs &a, uint32_t nd, uint32_t grid, hipStream_t s) {
+    switch (nd) {
+        case 1: launch_scan_np<1>(a, grid, s); break;
+        case 2: launch_scan_np<2>(a, grid, s); break;
+        case 3: launch_scan_np<3>(a, grid, s); break;
+        default: launch_scan_np<4>(a, grid, s); break;
+    }
+}
+
+// Builds and launches one pass (its queries' maps are filled here).
+int sqb_run_pass(std::vector<SqQueryPlan> &plans, SqbPass &ps, SqbDesc *ddesc, uint32_t grid, hipStream_t s) {
+    int st;
+    if (ps.qi.size() == 1) {  // one query: the single-query scan, with its LDS placement
+        SqQueryPlan &Q = plans[ps.qi[0]];
+        uint32_t lds_used = 0;
+        const uint32_t lds_cap = sq_map_words(sq_chunks(static_cast<int>(Q.q->n_dims)));
+        for (uint32_t d : by_selectivity(Q)) {
+            if (lds_used + Q.P[d].map_words <= lds_cap) {
+                Q.P[d].in_lds = true;
+                lds_used += Q.P[d].map_words;
+            }
+        }
+        if ((st = sq_build(Q, s)) != AHEAD_OK) return st;
+        Q.a.rows_per_wave = rows_per_wave(Q.q->n_fact, grid);
+        if (Q.q->n_fact > 0) launch_single_scan(Q.a, Q.q->n_dims, grid, s);
+        AHEAD_HIP_TRY(hipGetLastError());
+        return AHEAD_OK;
+    }
+    // LDS: over all of the pass's dimensions, most selective first, while the maps fit
+    struct Cand {
+        uint32_t k, d;
+        double frac;
+    };
+    std::vector<Cand> cand;
+    for (uint32_t k = 0; k < ps.qi.size(); ++k) {
+        const SqQueryPlan &Q = plans[ps.qi[k]];
+        for (uint32_t d = 0; d < Q.q->n_dims; ++d) cand.push_back({k, d, Q.P[d].frac});
+    }
+    std::stable_sort(cand.begin(), cand.end(), [](const Cand &x, const Cand &y) { return x.frac < y.frac; });
+    uint32_t lds_used = 0;
+    for (const Cand &c : cand) {
+        DimPlan &pl = plans[ps.qi[c.k]].P[c.d];
+        if (lds_used + pl.map_words <= kSqbMapWords) {
+            pl.in_lds = true;
+            lds_used += pl.map_words;
+        }
+    }
+    // shared FK columns in order of their most selective dimension
+    const uint32_t nf = static_cast<uint32_t>(ps.fcols.size()), npc = static_cast<uint32_t>(ps.pcols.size());
+    std::vector<double> fmin(nf, 2.0);
+    for (uint32_t k = 0; k < ps.qi.size(); ++k) {
+        const SqQueryPlan &Q = plans[ps.qi[k]];
+        for (uint32_t d = 0; d < Q.q->n_dims; ++d) {
+            const int c = ps.dim_slot[k][d];
+            fmin[c] = std::min(fmin[c], Q.P[d].frac + (Q.P[d].in_lds ? 0.0 : 1.0));
+        }
+    }
+    std::vector<uint32_t> fo(nf), rank(nf);
+    std::iota(fo.begin(), fo.end(), 0u);
+    std::stable_sort(fo.begin(), fo.end(), [&](uint32_t x, uint32_t y) { return fmin[x] < fmin[y]; });
+    for (uint32_t i = 0; i < nf; ++i) rank[fo[i]] = i;
+
+    SqbArgs b{};
+    b.n = plans[ps.qi[0]].q->n_fact;
+    b.rows_per_wave = rows_per_wave(b.n, grid);
+    b.nq = static_cast<uint32_t>(ps.qi.size());
+    for (uint32_t i = 0; i < nf; ++i) {
+        const ahead_col &c = *ps.fcols[fo[i]];
+        b.f[i] = SqbColK{c.data, c.a_inv, c.max_u, ~0ull, static_cast<uint32_t>(bits_mask(c.unenc_bits)), 8u};
+    }
+    uint32_t pwb = npc > 0 ? ps.pcols[0]->width : 0u;
+    for (uint32_t p = 0; p < npc; ++p) {
+        const ahead_col &c = *ps.pcols[p];
+        const uint64_t wm = width_mask(c.width);
+        b.pc[p] = SqbColK{c.data, c.a_inv, c.max_u & wm, wm, 0u, c.width};
+        if (c.width != pwb) pwb = 0;
+    }
+    SqbDesc hd{};
+    for (uint32_t i = 0; i < nf; ++i) hd.cols[i] = b.f[i];
+    for (uint32_t p = 0; p < npc; ++p) hd.cols[AHEAD_SQ_MAX_DIMS + p] = b.pc[p];
+    uint32_t lds_base = 0, g_lds = 0;
+    for (uint32_t k = 0; k < ps.qi.size(); ++k) {
+        SqQueryPlan &Q = plans[ps.qi[k]];
+        if ((st = sq_build(Q, s)) != AHEAD_OK) return st;
+        const SqArgs &a = Q.a;
+        SqbQuery &x = hd.q[k];
+        for (uint32_t i = 0; i < a.nd; ++i) {
+            const SqDimK &k2 = a.d[i];
+            SqbDimK &dd = x.d[rank[ps.dim_slot[k][k2.cpos]]];
+            dd.used = 1;
+            dd.kmin = k2.kmin;
+            dd.span = k2.span;
+            dd.lb = static_cast<uint16_t>(k2.lb);
+            dd.lds_off = k2.lds_off != kNoLds ? lds_base + k2.lds_off : kNoLds;
+            dd.map_off = k2.map_off;
+            dd.stride = k2.stride;
+            dd.errv = k2.errv;
+        }
+        for (uint32_t p = 0; p < a.np; ++p) {
+            SqbPredK &pp = x.p[ps.pred_slot[k][p]];
+            pp.used = 1;
+            pp.never = a.p[p].never;
+            for (int j = 0; j < 3; ++j) {  // codes of at most 4 bytes: the intervals fit 32 bits
+                pp.lo[j] = static_cast<uint32_t>(a.p[p].lo[j]);
+                pp.len[j] = static_cast<uint32_t>(a.p[p].len[j]);
+            }
+        }
+        for (uint32_t m = 0; m < a.nm; ++m) {
+            const SqMeasK &mk = a.m[m];
+            x.m[m] = SqbMeasK{mk.col, mk.inv, mk.maxu, mk.wmask, mk.width, mk.pre,
+                              mk.pred >= 0 ? ps.pred_slot[k][mk.pred] : -1, 0u};
+        }
+        x.maps = a.maps;
+        x.nm = a.nm;
+        x.kind = a.kind;
+        x.pre_load = a.pre_load;
+        x.groups = a.groups;
+        x.g_lds = g_lds;
+        x.np = static_cast<uint8_t>(a.np);
+        x.nd = static_cast<uint8_t>(a.nd);
+        for (uint32_t p = 0; p < a.np; ++p) x.pslot[p] = static_cast<uint8_t>(ps.pred_slot[k][p]);
+        for (uint32_t i = 0; i < a.nd; ++i) {
+            const uint32_t cp = a.d[i].cpos;
+            x.cslot[cp] = static_cast<uint8_t>(rank[ps.dim_slot[k][cp]]);
+            x.dflags[cp] = static_cast<uint8_t>(a.d[i].flags);
+            x.row_of[cp] = a.d[i].row_of;
+        }
+        x.ev_count = a.ev_count;
+        x.events = a.events;
+        hd.c[k] = SqbCold{a.flags, a.gsum, a.gfirst, a.kept, lds_base, a.lds_words};
+        lds_base += a.lds_words;
+        g_lds += a.groups;
+    }
+    hipLaunchKernelGGL((k_sq_put<SqbDesc>), dim3(1), dim3(64), 0, s, ddesc, hd);
+    AHEAD_HIP_TRY(hipGetLastError());
+    if (b.n > 0) launch_sqb(b, ddesc, nf, npc, pwb, grid, s);
+    AHEAD_HIP_TRY(hipGetLastError());
+    return AHEAD_OK;
+}
+
+}  // namespace
+
+extern "C" size_t ahead_hip_star_query_batch_ws_bytes(const ahead_sq_args *const *qs, uint32_t nq) {
+    if (qs == nullptr || nq == 0) return 0;
+    for (uint32_t i = 0; i < nq; ++i)
+        if (qs[i] == nullptr || qs[i]->n_dims < 1 || qs[i]->n_dims > AHEAD_SQ_MAX_DIMS) return 0;
+    return kHeaderBytes + sqb_layout(qs, nq).total;
+}
+
+extern "C" int ahead_hip_star_query_batch(const ahead_sq_args *const *qs, const ahead_sq_out *const *outs, uint32_t nq,
+                                          ahead_hip_ws *ws, void *stream) {
+    if (qs == nullptr || outs == nullptr || ws == nullptr) return fail(AHEAD_ERR_INVALID, "star_query_batch: NULL argument");
+    if (nq == 0) return AHEAD_OK;
+    int st;
+    for (uint32_t i = 0; i < nq; ++i) {
+        if ((st = sq_validate(qs[i], outs[i])) != AHEAD_OK) return st;
+        if (qs[i]->n_fact != qs[0]->n_fact)
+            return fail(AHEAD_ERR_SIZE_MISMATCH, "star_query_batch: queries over fact tables of different lengths");
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const SqbLayout B = sqb_layout(qs, nq);
+    if ((st = ws_ensure(ws, B.total)) != AHEAD_OK) return st;
+    char *base = ws_region(ws, 0);
+    std::vector<SqQueryPlan> plans(nq);
+    for (uint32_t i = 0; i < nq; ++i) {
+        SqQueryPlan &Q = plans[i];
+        Q.q = qs[i];
+        Q.out = outs[i];
+        Q.L = sq_layout(qs[i]);
+        Q.base = base + B.off[i];
+        if ((st = sq_prep(Q, s)) != AHEAD_OK) return st;
+    }
+    // every query's statistics, one wait
+    std::vector<uint32_t> h(static_cast<size_t>(nq) * (AHEAD_SQ_MAX_DIMS * kStatWords + 4));
+    for (uint32_t i = 0; i < nq; ++i) {
+        uint32_t *hs = h.data() + static_cast<size_t>(i) * (AHEAD_SQ_MAX_DIMS * kStatWords + 4);
+        AHEAD_HIP_TRY(hipMemcpyAsync(hs, plans[i].stats(), AHEAD_SQ_MAX_DIMS * kStatWords * 4, hipMemcpyDeviceToHost, s));
+        AHEAD_HIP_TRY(hipMemcpyAsync(hs + AHEAD_SQ_MAX_DIMS * kStatWords, plans[i].flags(), 16, hipMemcpyDeviceToHost, s));
+    }
+    AHEAD_HIP_TRY(hipStreamSynchronize(s));
+    std::vector<SqbPass> passes;
+    for (uint32_t i = 0; i < nq; ++i) {
+        SqQueryPlan &Q = plans[i];
+        const uint32_t *hs = h.data() + static_cast<size_t>(i) * (AHEAD_SQ_MAX_DIMS * kStatWords + 4);
+        sq_plan(Q, hs, hs + AHEAD_SQ_MAX_DIMS * kStatWords);
+        if (Q.status != AHEAD_SQ_ST_OK) {
+            if ((st = sq_not_fused(Q, s)) != AHEAD_OK) return st;
+            continue;
+        }
+        if (passes.empty() || !sqb_try_add(passes.back(), i, Q)) {
+            passes.emplace_back();
+            if (!sqb_try_add(passes.back(), i, Q)) return fail(AHEAD_ERR_DEVICE, "star_query_batch: a query fits no pass");
+        }
+    }
+    const uint32_t grid = sq_grid();
+    const ahead_sq_args *q0 = qs[0];
+    if (q0->ev_scan[0] != nullptr) AHEAD_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(q0->ev_scan[0]), s));
+    size_t desc = 0;  // descriptor blocks of the passes, one after another
+    for (SqbPass &ps : passes) {
+        SqbDesc *d = reinterpret_cast<SqbDesc *>(base + B.off_qdesc + desc * align_up(sizeof(SqbDesc), 256));
+        if ((st = sqb_run_pass(plans, ps, d, grid, s)) != AHEAD_OK) return st;
+        ++desc;
+    }
+    if (q0->ev_scan[1] != nullptr) AHEAD_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(q0->ev_scan[1]), s));
+    for (const SqbPass &ps : passes)
+        for (uint32_t i : ps.qi) hipLaunchKernelGGL(k_sq_final, dim3(1), dim3(kSqBlock), 0, s, plans[i].f);
     AHEAD_HIP_TRY(hipGetLastError());
     return AHEAD_OK;
 }

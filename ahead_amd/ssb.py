@@ -616,11 +616,9 @@ FUSED_KINDS = ["pred0", "pred1", "meas0", "meas1", "fk0", "fk1", "fk2", "fk3", "
                "refetch3", "attr0", "attr1", "attr2", "attr3"]
 
 
-def fused(t: dict, q: Query, ops=B, fallback: bool = True, ev_scan: tuple | None = None) -> dict:
-    """The query with one fact scan (ops.star_query) after the chain's dimension selections.  The
-    result has the chain's form; `status` is the fused entry point's (0: it applied).  With a
-    status other than 0 and `fallback`, the chain's result is returned (with that status)."""
-    E = Errors()
+def _fused_args(t: dict, q: Query, ops, E: "Errors") -> dict:
+    """The chain's dimension selections (their error vectors into E) and the fused entry point's
+    arguments."""
     dims = []
     for dq in q.dims:
         fk, key = DIM_KEYS[dq.name]
@@ -642,8 +640,11 @@ def fused(t: dict, q: Query, ops=B, fallback: bool = True, ev_scan: tuple | None
         else:
             preds.append((c.tail, p[0], _enc(c, p[1]), p[2], _enc(c, p[3]), p[4]))
     meas = [t[m].tail for m in q.meas]
-    n_fact = t[q.meas[0]].tail.n
-    r = ops.star_query(n_fact, dims, preds, meas, q.kind, q.pre_mask, ev_scan=ev_scan)
+    return {"n_fact": t[q.meas[0]].tail.n, "dims": dims, "preds": preds, "meas": meas, "kind": q.kind,
+            "pre_mask": q.pre_mask}
+
+
+def _fused_result(t: dict, q: Query, r: dict, E: "Errors", ops, fallback: bool) -> dict:
     if r["status"] != 0:
         if not fallback:
             return {"status": r["status"], "errors": E}
@@ -665,6 +666,24 @@ def fused(t: dict, q: Query, ops=B, fallback: bool = True, ev_scan: tuple | None
         keys.append((r["keys"][d] * np.uint64(m.a_inv)) & np.uint64((1 << (8 * m.width)) - 1))
     out.update({"sums": sums, "keys": keys, "groups": r["groups"]})
     return out
+
+
+def fused(t: dict, q: Query, ops=B, fallback: bool = True, ev_scan: tuple | None = None) -> dict:
+    """The query with one fact scan (ops.star_query) after the chain's dimension selections.  The
+    result has the chain's form; `status` is the fused entry point's (0: it applied).  With a
+    status other than 0 and `fallback`, the chain's result is returned (with that status)."""
+    E = Errors()
+    r = ops.star_query(**_fused_args(t, q, ops, E), ev_scan=ev_scan)
+    return _fused_result(t, q, r, E, ops, fallback)
+
+
+def fused_batch(t: dict, qs: list, ops=B, fallback: bool = True, ev_scan: tuple | None = None) -> list:
+    """Several queries over the same tables with shared fact scans (ops.star_query_batch): each
+    result is the one fused() returns for that query alone."""
+    Es = [Errors() for _ in qs]
+    calls = [_fused_args(t, q, ops, E) for q, E in zip(qs, Es)]
+    rs = ops.star_query_batch(calls, ev_scan=ev_scan)
+    return [_fused_result(t, q, r, E, ops, fallback) for q, r, E in zip(qs, rs, Es)]
 
 
 def fused_ws_bytes(t: dict, q: Query, ops=B) -> int:

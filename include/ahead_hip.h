@@ -412,6 +412,21 @@ int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *out, ahead_
  * largest key span the fused form accepts), for admission by memory; 0 for invalid arguments.  */
 size_t ahead_hip_star_query_ws_bytes(const ahead_sq_args *q);
 
+/* Several star queries over one fact table (same n_fact) with shared fact scans: the queries'
+ * dimension statistics are read back together (one wait), then queries are packed in order into
+ * passes of at most AHEAD_SQ_MAX_BATCH queries whose fact FK columns (at most 4 distinct) and
+ * predicate columns (at most 2 distinct) fit one scan and whose group tables fit its LDS; each
+ * pass reads those columns once for all its queries.  Each query's outputs, error vectors and
+ * status are exactly those of ahead_hip_star_query with the same arguments; a query whose fused
+ * form does not apply gets its status and takes no part in the scans.  No counterpart in the
+ * reference (its drivers run one query at a time): it serves concurrent queries of a
+ * device-resident schema.  ev_scan of qs[0] (if set) brackets all of the batch's scans. */
+#define AHEAD_SQ_MAX_BATCH 8
+int ahead_hip_star_query_batch(const ahead_sq_args *const *qs, const ahead_sq_out *const *outs, uint32_t nq,
+                               ahead_hip_ws *ws, void *stream);
+/* Device bytes of workspace ahead_hip_star_query_batch uses for these queries; 0 for invalid. */
+size_t ahead_hip_star_query_batch_ws_bytes(const ahead_sq_args *const *qs, uint32_t nq);
+
 /* ---- data generation (seeded, counter-based; the same streams as ahead_amd/datagen.py) ------
  * Not part of the reference's operator API: the device-side generator for the synthetic
  * columns of the benchmark workloads (the reference loads SSB tables from dbgen instead).

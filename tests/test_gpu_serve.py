@@ -45,10 +45,10 @@ def test_admission_budget_serializes_when_memory_is_short(mode):
     tables = ssb.device_tables(spec)
     print(f"serve {mode}: admission budget", flush=True)
     with serve.Server(tables, spec.n_fact, workers=4, mode=mode) as srv:
-        needs = [srv.need(s) for s in ("P1", "P2", "P3", "P4")]
-        assert all(n > 0 for n in needs)
-        # a budget that fits one query at a time: queries must not overlap
-        srv.admission.budget = max(needs) + min(needs) - 1
+        needs = {s: srv.need(s) for s in ("P1", "P2", "P3", "P4")}
+        assert all(n > 0 for n in needs.values())
+        # a budget below any two needs together (a need above it is capped at it): no overlap
+        srv.admission.budget = 2 * min(needs.values()) - 1
         qs = srv.run(serve.random_queries(8, 0, 6), realtime=False)
     assert [q.start for q in qs] == sorted(q.start for q in qs)   # admitted in arrival order
     spans = sorted((q.start, q.finish) for q in qs)

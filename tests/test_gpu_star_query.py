@@ -124,3 +124,78 @@ def test_star_query_key_index_statuses():
     assert B.key_index(t3["s_suppkey"].tail).status == 2
     assert ssb.fused(t3, q, fallback=False)["status"] == 2
     assert an.A_OID_ERR == 16089
+
+
+# ---- shared scans (ahead_hip_star_query_batch) ------------------------------------------------
+
+def test_batch_of_all_queries_matches_oracle_with_flips_everywhere():
+    """All thirteen queries in one call: packed into shared scans of up to eight queries (four FK
+    and two predicate columns each); every query's result, group order, first rows and error
+    vectors equal the oracle's for that query alone."""
+    spec = ssb.SsbSpec(rate=1e-3, seed=41, flip_cols=ALL_FLIPS, **SMALL)
+    qs = list(ssb.QUERIES.values())
+    print(f"star_query_batch of {len(qs)} queries vs oracle, {spec.n_fact} rows", flush=True)
+    got = ssb.fused_batch(ssb.device_tables(spec), qs, fallback=False)
+    host = OB.tables(spec)
+    for name, q, g in zip(ssb.QUERIES, qs, got):
+        ref = ssb.fused(host, q, OB, fallback=False)
+        assert g["status"] == 0 and ref["status"] == 0, name
+        same(g, ref)
+        assert np.array_equal(g["first_row"], ref["first_row"]), name
+
+
+@pytest.mark.parametrize("names", [("Q1.1", "Q2.1", "Q3.1", "Q4.1"), ("Q4.1", "Q4.1"), ("Q2.3", "Q1.3", "Q4.3"),
+                                   ("Q3.4",)])
+def test_batch_equals_single_queries(names):
+    """Small batches (the serve shapes, a query twice, mixed measure kinds, one query) against the
+    same queries run alone on the GPU, with flips in every column the chains survive."""
+    spec = ssb.SsbSpec(rate=2e-3, seed=42, flip_cols=ALL_FLIPS, **SMALL)
+    t = ssb.device_tables(spec)
+    qs = [ssb.QUERIES[n] for n in names]
+    print(f"star_query_batch {names}", flush=True)
+    got = ssb.fused_batch(t, qs, fallback=False)
+    for n, q, g in zip(names, qs, got):
+        ref = ssb.fused(t, q, fallback=False)
+        assert g["status"] == 0 and ref["status"] == 0, n
+        same(g, ref)
+        assert np.array_equal(g["first_row"], ref["first_row"]), n
+
+
+def test_batch_where_every_row_joins_and_tiny_tables():
+    every = ssb.Query("q41", [ssb.Dim("supp", [], "s_nation"), ssb.Dim("date", [], "d_year", ssb.REFETCH)],
+                      ["lo_revenue", "lo_supplycost"], ssb.SUBTRACT, groups=[1, 0])
+    spec = ssb.SsbSpec(rate=4e-3, seed=43, flip_cols=("lo_revenue", "lo_supplycost", "lo_orderdate", "s_nation"),
+                       **SMALL)
+    print("star_query_batch every row joins", flush=True)
+    qs = [every, ssb.QUERIES["Q1.2"], every]
+    got = ssb.fused_batch(ssb.device_tables(spec), qs, fallback=False)
+    host = OB.tables(spec)
+    for q, g in zip(qs, got):
+        same(g, ssb.fused(host, q, OB, fallback=False))
+    for n in (0, 1, 511, 513):
+        tiny = ssb.SsbSpec(rate=0.0, seed=44, n_fact=n, n_date=2556, n_cust=300, n_supp=50, n_part=140)
+        print(f"star_query_batch tiny fact {n}", flush=True)
+        qs = [ssb.QUERIES[k] for k in ("Q1.1", "Q3.1", "Q4.1", "Q2.2")]
+        got = ssb.fused_batch(ssb.device_tables(tiny), qs, fallback=False)
+        host = OB.tables(tiny)
+        for q, g in zip(qs, got):
+            same(g, ssb.fused(host, q, OB, fallback=False))
+
+
+def test_batch_reports_each_querys_status():
+    """A query over a dimension whose key index reports a corrupt key gets that status (and the
+    chain's result with fallback); the others in the batch are unaffected."""
+    from ahead_amd import bat as B, capi
+
+    spec = ssb.SsbSpec(rate=0.0, seed=45, **SMALL)
+    t = ssb.device_tables(spec)
+    bad = t["s_suppkey"].tail.to_numpy().copy()
+    bad[7] ^= np.uint64(1 << 45)
+    t3 = dict(t)
+    t3["s_suppkey"] = B.BAT(capi.Column.void(bad.size), capi.Column.from_numpy(bad, t["s_suppkey"].tail.meta))
+    qs = [ssb.QUERIES[k] for k in ("Q1.1", "Q2.1", "Q4.1", "Q1.3")]  # Q2.x-Q4.x join the supplier
+    print("star_query_batch statuses", flush=True)
+    got = ssb.fused_batch(t3, qs)
+    assert [g["status"] for g in got] == [0, 2, 2, 0]
+    for q, g in zip(qs, got):
+        same(g, ssb.chain(t3, q))
