@@ -108,3 +108,29 @@ def an_meta(t: ColType, a: int) -> ColMeta:
 
 def plain_meta(t: ColType, seqbase: int = 0) -> ColMeta:
     return ColMeta(t, 1, 1, seqbase)
+
+
+_UNCHECKED: dict = {}
+
+
+def unchecked_type(t: ColType) -> ColType:
+    """Values stored at the code width of t with no check: every code is valid (maxU = all ones)
+    and the whole width is the value.  With A = 1 the hardened operators over such columns are the
+    plain operators of Operators.hpp (the special case A = 1 of every AN operator)."""
+    if t.name.endswith("!unchecked"):
+        return t
+    u = ColType(f"{t.name}!unchecked", t.width, True, 8 * t.width, (1 << (8 * t.width)) - 1,
+                max(t.larger_bits, 8 * t.width))
+    _UNCHECKED[u.name] = t
+    return u
+
+
+def checked_of(t: ColType) -> ColType:
+    """The AN type an unchecked type was made from."""
+    return _UNCHECKED.get(t.name, t)
+
+
+def unchecked_meta(t: ColType, a: int = 1) -> ColMeta:
+    """An unchecked column of t's code width; a = 1 for values, a = A for raw AN codes (thresholds
+    are still encoded with A, and the operators compare, join and group the codes themselves)."""
+    return ColMeta(unchecked_type(t), a, 1)

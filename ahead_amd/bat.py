@@ -358,6 +358,61 @@ def arithmeticAN(op: int, b1: BAT, b2: BAT, a_result: int = an.A_RESULT, a_oid: 
     return (BAT(Column.void(n), Column(bufs[0], n, meta)),) + tuple(e)
 
 
+def _decoded(col: Column, call, a_oid: int) -> tuple:
+    """Runs a decode entry point over a column into device memory, then widens the values back to
+    the code width (an.unchecked_type, A = 1).  Returns (column, error positions)."""
+    lib, ws = capi.load(), capi.workspace()
+    wd = col.meta.type.unenc_bits // 8
+    dec = DeviceBuffer(wd * max(col.n, 1))
+    cnt = DeviceBuffer(16)
+    d = col.desc()
+    err_cap = 4096
+    while True:
+        err = DeviceBuffer(8 * err_cap)
+        capi.check(call(lib, d, a_oid, dec, err, err_cap, cnt, ws))
+        capi.synchronize()
+        n_dec, n_err = capi._counts(cnt, 2)
+        if n_err <= err_cap:
+            break
+        err_cap = n_err
+    ut = an.unchecked_type(col.meta.type)
+    out = DeviceBuffer(ut.width * max(n_dec, 1))
+    plain = {1: an.TINYINT, 2: an.SHORTINT, 4: an.INT, 8: an.BIGINT}[wd]
+    if n_dec:
+        capi.encode_an_into(Column(dec, n_dec, an.plain_meta(plain)), 1, ut.width, out)
+    return Column(out, n_dec, an.ColMeta(ut, 1, 1)), err.to_numpy(np.uint64, n_err)
+
+
+def checkAndDecodeAN(bat: BAT, a_oid: int = A_OID_ERR):
+    """checkAndDecodeAN of a BAT's tail (OperatorsAN.hpp:157-161; encdecAN_scalar.tcc:185-210):
+    corrupt rows are dropped and reported (i * a_oid).  The decoded values come back at the code
+    width with A = 1 and no check, the form the other operators take.  Returns (BAT<void, values>,
+    errors)."""
+    def call(lib, d, a, dec, err, cap, cnt, ws):
+        return lib.ahead_hip_check_and_decode_an(ctypes.byref(d), a, dec.ptr, d.n, err.ptr, cap, cnt.ptr, ws.h,
+                                                 capi.current_stream())
+
+    c, e = _decoded(bat.tail, call, a_oid)
+    return BAT(Column.void(c.n), c), e
+
+
+def decodeAN(bat: BAT, a_oid: int = A_OID_ERR):
+    """decodeAN of a BAT's tail (encdecAN_scalar.tcc:96-126): every row decoded (corrupt ones
+    too, and reported), values at the code width with A = 1 and no check."""
+    def call(lib, d, a, dec, err, cap, cnt, ws):
+        return lib.ahead_hip_decode_an(ctypes.byref(d), a, dec.ptr, err.ptr, cap, cnt.ptr, ws.h, capi.current_stream())
+
+    c, e = _decoded(bat.tail, call, a_oid)
+    return BAT(Column.void(c.n), c), e
+
+
+def copy(bat: BAT) -> BAT:
+    """A device copy of a <void, T> BAT (the reference's copy(), miscellaneous.tcc): a replica."""
+    t = bat.tail
+    buf = t.buf.prefix_copy(t.buf.nbytes) if t.buf is not None else None
+    return BAT(bat.head, Column(buf, t.n, t.meta))
+
+
 def _star_query_obj(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_mask: int = 0,
                     a_result: int = an.A_RESULT, a_oid: int = A_OID_ERR, ev_scan: tuple | None = None):
     cdims = []

@@ -18,6 +18,8 @@ from . import an
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libahead_hip.so"
 
 AHEAD_OK = 0
+(AHEAD_ERR_INVALID, AHEAD_ERR_SIZE_MISMATCH, AHEAD_ERR_HIP, AHEAD_ERR_NOMEM, AHEAD_ERR_TOO_LARGE,
+ AHEAD_ERR_DEVICE) = range(1, 7)
 STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "SIZE_MISMATCH", 3: "HIP", 4: "NOMEM", 5: "TOO_LARGE", 6: "DEVICE"}
 
 LT, LE, GT, GE, EQ, NE = range(6)

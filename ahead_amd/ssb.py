@@ -119,6 +119,7 @@ COLUMNS = {
 # dimension -> (fact FK column, dimension key column)
 DIM_KEYS = {"date": ("lo_orderdate", "d_datekey"), "cust": ("lo_custkey", "c_custkey"),
             "supp": ("lo_suppkey", "s_suppkey"), "part": ("lo_partkey", "p_partkey")}
+DIM_KEYS_SET = {c for pair in DIM_KEYS.values() for c in pair}
 
 
 def _rows(spec: SsbSpec, table: str) -> int:
@@ -348,6 +349,14 @@ def _enc(col, v: int) -> int:
     return (v * col.tail.meta.a) & an.MASK64
 
 
+def _measure(ops, E: Errors, bat, name: str):
+    """A fetched attribute or measure as grouping, arithmetic and aggregation take it: unchanged,
+    or checked and decoded here when the strategy decodes late (ops.late_decode;
+    ssbm-q11_late.cpp:110-120, ssbm-q21_late.cpp:146-153)."""
+    late = getattr(ops, "late_decode", None)
+    return bat if late is None else late(bat, E, name)
+
+
 def _select(ops, t: dict, E: Errors, attr: str, pred: tuple):
     col = t[attr]
     if len(pred) == 2:
@@ -397,13 +406,16 @@ def chain_q1(t: dict, q: Query, ops=B) -> dict:
     E.add("fetch_discount", *rE[1:])
     rF = ops.matchjoinAN(batC, rE[0])
     E.add("match_discount", *rF[1:])
-    rG = ops.aggregate_mul_sumAN(rD[0], rF[0])
+    rG = ops.aggregate_mul_sumAN(_measure(ops, E, rD[0], "price"), _measure(ops, E, rF[0], "discount"))
     E.add("mul_sum", *rG[1:])
     return {"result": _decode_sum(rG[0]), "rows": rD[0].size(), "errors": E}
 
 
-def _groups(ops, E: Errors, attrs: list, measure) -> dict:
+def _groups(ops, E: Errors, attrs: list, measure, measure_ready: bool = False) -> dict:
     """groupbyAN over the attributes (unary, then binary), grouped sum, and the group keys."""
+    attrs = [_measure(ops, E, a, f"attr_{k}") for k, a in enumerate(attrs)]
+    if not measure_ready:
+        measure = _measure(ops, E, measure, "measure")
     g = ops.groupbyAN(attrs[0])
     E.add("group_0", *g[2:])
     for k, a in enumerate(attrs[1:]):
@@ -503,7 +515,8 @@ def _profit(ops, t: dict, E: Errors, rows):
     E.add("fetch_revenue", *ar[1:])
     asc = ops.fetchjoinAN(rows, t["lo_supplycost"])
     E.add("fetch_supplycost", *asc[1:])
-    ap = ops.arithmeticAN(ops.SUB, ar[0], asc[0], a_result=A_MEAS, result_type=an.RESINT)
+    ap = ops.arithmeticAN(ops.SUB, _measure(ops, E, ar[0], "revenue"), _measure(ops, E, asc[0], "supplycost"),
+                          a_result=A_MEAS, result_type=an.RESINT)
     E.add("profit", *ap[1:])
     return ap[0]
 
@@ -542,7 +555,7 @@ def chain_q41(t: dict, q: Query, ops=B) -> dict:
     d_oid = _join_dim_oids(ops, t, E, rows, "lo_orderdate", t["d_datekey"].reverse(), "orderdate")
     ay = ops.fetchjoinAN(d_oid, t["d_year"])
     E.add("fetch_year", *ay[1:])
-    g = _groups(ops, E, [ay[0], an_[0]], profit)
+    g = _groups(ops, E, [ay[0], an_[0]], profit, measure_ready=True)
     g.update({"rows": profit.size(), "errors": E})
     return g
 
@@ -570,7 +583,7 @@ def chain_q42(t: dict, q: Query, ops=B) -> dict:
     p_oid = _join_dim_oids(ops, t, E, rows, "lo_partkey", t["p_partkey"].reverse(), "partkey_2")
     ac = ops.fetchjoinAN(p_oid, t[dp.attr])
     E.add(f"fetch_{dp.attr}", *ac[1:])
-    g = _groups(ops, E, [ay[0], an_[0], ac[0]], profit)
+    g = _groups(ops, E, [ay[0], an_[0], ac[0]], profit, measure_ready=True)
     g.update({"rows": profit.size(), "errors": E})
     return g
 
@@ -596,7 +609,7 @@ def chain_q43(t: dict, q: Query, ops=B) -> dict:
     p_oid = _join_dim_oids(ops, t, E, rows, "lo_partkey", t["p_partkey"].reverse(), "partkey_2")
     ab = ops.fetchjoinAN(p_oid, t[dp.attr])
     E.add(f"fetch_{dp.attr}", *ab[1:])
-    g = _groups(ops, E, [ay[0], ac[0], ab[0]], profit)
+    g = _groups(ops, E, [ay[0], ac[0], ab[0]], profit, measure_ready=True)
     g.update({"rows": profit.size(), "errors": E})
     return g
 

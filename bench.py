@@ -33,6 +33,9 @@ HBM.  Workloads:
                 (own streams and workspaces, first-come-first-served admission by the bytes each
                 query needs) against the plain operator chains (one launch per operator) on 1 and
                 4 slots; reports queries/s with p50/p99 latency per arrival rate.
+  strategies  : the four query shapes under the reference's other strategies (early / late
+                check-and-decode, unhardened, DMR and TMR replicas) beside the hardened chain and
+                the fused star query.
   hbm_copy    : a plain device-to-device copy of 4 GiB (read 4 GiB, write 4 GiB): the calibration
                 kernel for the FETCH_SIZE / WRITE_SIZE counters behind `roofline.traffic`.
 
@@ -63,7 +66,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="ssb", choices=["star1", "select_sum", "ops", "batch", "ssb", "serve",
-                                                                 "hbm_copy"])
+                                                                 "hbm_copy", "strategies"])
     ap.add_argument("--rows", type=int, default=None, help="override the row count (scaling studies)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the host-side checks of the ssb outputs")
@@ -697,6 +700,72 @@ def run_serve(args):
     }
 
 
+def run_strategies(args):
+    """The four SSB query shapes under each of the reference's strategies, one after another: the
+    hardened chain (continuous), early and late check-and-decode, the unhardened chain (normal),
+    DMR (sequential and concurrent replicas) and TMR, plus the fused star query."""
+    import torch
+
+    from ahead_amd import ssb
+    from ahead_amd import strategies as S
+
+    spec = ssb.SsbSpec()
+    if args.rows:
+        spec.n_fact = args.rows
+    t = ssb.device_tables(spec)
+    plain = S.plain_tables(t)
+    reps3 = S.replicas(plain, 3)
+    torch.cuda.synchronize()
+    qs = [ssb.SHAPES[k]() for k in SSB_SHAPES]
+    forms = {
+        "continuous": lambda q: ssb.chain(t, q),
+        "early": lambda q: S.early(t, q),
+        "late": lambda q: S.late(t, q),
+        "normal": lambda q: S.normal(plain, q),
+        "dmr": lambda q: S.modular(reps3[:2], q),
+        "dmr_concurrent": lambda q: S.modular(reps3[:2], q, concurrent=True),
+        "tmr": lambda q: S.modular(reps3, q),
+        "fused": lambda q: ssb.fused(t, q),
+    }
+    ref = [ssb.chain(t, q) for q in qs]
+    steps = max(1, args.steps)
+    out = {}
+    for name, fn in forms.items():
+        got = [fn(q) for q in qs]  # warm-up, and the results the step must reproduce
+        for q, g, r in zip(SSB_SHAPES, got, ref):
+            if not S.same(g, r):
+                raise RuntimeError(f"{name} differs from the hardened chain on {q}")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for q in qs:
+                fn(q)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / steps
+        out[name] = {"ms_per_step": el * 1e3, "queries_per_s": len(qs) / el}
+        print(f"strategies: {name} {el * 1e3:.1f} ms per step of {len(qs)} queries", file=sys.stderr, flush=True)
+    base = out["normal"]["ms_per_step"]
+    for v in out.values():
+        v["overhead_vs_normal"] = v["ms_per_step"] / base
+    return {
+        "metric": "SSB queries/s per execution strategy (Q1.1, Q2.1, Q3.1, Q4.1 per step)",
+        "value": out["continuous"]["queries_per_s"],
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 1,
+        "ms_per_step": out["continuous"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16/u32/u64",
+        "data": "synthetic (seeded; fact columns generated on device)",
+        "config": {"workload": f"strategies: ssb schema with {spec.n_fact} fact rows; every strategy's results "
+                               f"checked equal to the hardened chain's", "strategies": out},
+        "roofline": None,
+    }
+
+
 SSB_SHAPES = ("P1", "P2", "P3", "P4")  # Q1.1, Q2.1, Q3.1, Q4.1 of the reference's SSB drivers
 SSB_BYTES_PER_ROW = {"P1": 20, "P2": 32, "P3": 32, "P4": 48}  # fact-column bytes each query reads once
 SSB_SCAN_KERNEL = {"P1": "k_sq_scan<1, 2>", "P2": "k_sq_scan<3, 0>", "P3": "k_sq_scan<3, 0>", "P4": "k_sq_scan<4, 0>"}
@@ -904,7 +973,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs an MI355X (cuda:0)"
     torch.cuda.set_device(0)
     run = {"star1": run_star1, "select_sum": run_select_sum, "batch": run_batch, "ssb": run_ssb,
-           "serve": run_serve, "hbm_copy": run_hbm_copy, "ops": run_ops}[args.workload]
+           "serve": run_serve, "hbm_copy": run_hbm_copy, "ops": run_ops, "strategies": run_strategies}[args.workload]
     res = run(args)
     print(json.dumps(res), flush=True)
 

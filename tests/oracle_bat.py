@@ -177,6 +177,27 @@ def arithmeticAN(op, b1, b2, a_result=an.A_RESULT, a_oid=A_OID_ERR, result_type=
 SIMD_THREADS = None  # set to a thread count (0: all) to run star queries on the SIMD restatement
 
 
+def _decoded(bat, fn, a_oid):
+    dec, err = fn(bat.tail.oc(), a_oid)
+    ut = an.unchecked_type(bat.tail.meta.type)
+    c = _col(np.asarray(dec), ut, 1, 1)
+    return BAT(void(c.n), c), err
+
+
+def checkAndDecodeAN(bat, a_oid=A_OID_ERR):
+    """bat.checkAndDecodeAN on the oracle: corrupt rows dropped, values at the code width."""
+    return _decoded(bat, O.check_and_decode_an, a_oid)
+
+
+def decodeAN(bat, a_oid=A_OID_ERR):
+    return _decoded(bat, O.decode_an, a_oid)
+
+
+def copy(bat):
+    t = bat.tail
+    return BAT(bat.head, HCol(None if t.arr is None else t.arr.copy(), t.n, t.meta))
+
+
 def star_query(n_fact, dims, preds, meas, kind, pre_mask=0, a_result=an.A_RESULT, a_oid=A_OID_ERR, ev_scan=None):
     """The oracle's fused star query (or_star_query, or or_star_query_simd when SIMD_THREADS is set)
     with the argument form of bat.star_query."""

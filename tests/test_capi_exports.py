@@ -63,3 +63,16 @@ def test_noalloc_select_rejects_columns_beyond_its_scratch():
         assert b"static scratch" in lib.ahead_hip_last_error()
     finally:
         capi.set_tiling(*old)
+
+
+def test_status_constants_match_the_header():
+    import re
+    from pathlib import Path
+
+    from ahead_amd import capi
+
+    h = (Path(__file__).resolve().parent.parent / "include" / "ahead_hip.h").read_text()
+    for name in ("AHEAD_ERR_INVALID", "AHEAD_ERR_SIZE_MISMATCH", "AHEAD_ERR_HIP", "AHEAD_ERR_NOMEM",
+                 "AHEAD_ERR_TOO_LARGE", "AHEAD_ERR_DEVICE"):
+        m = re.search(rf"{name}\s*=\s*(\d+)", h)
+        assert m and int(m.group(1)) == getattr(capi, name), name
