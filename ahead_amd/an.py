@@ -113,14 +113,17 @@ def plain_meta(t: ColType, seqbase: int = 0) -> ColMeta:
 _UNCHECKED: dict = {}
 
 
-def unchecked_type(t: ColType) -> ColType:
-    """Values stored at the code width of t with no check: every code is valid (maxU = all ones)
-    and the whole width is the value.  With A = 1 the hardened operators over such columns are the
-    plain operators of Operators.hpp (the special case A = 1 of every AN operator)."""
-    if t.name.endswith("!unchecked"):
-        return t
-    u = ColType(f"{t.name}!unchecked", t.width, True, 8 * t.width, (1 << (8 * t.width)) - 1,
-                max(t.larger_bits, 8 * t.width))
+def unchecked_type(t: ColType, raw: bool = False) -> ColType:
+    """Values stored at the code width of t with no check: every code is valid (maxU = all ones).
+    With A = 1 the hardened operators over such columns are the plain operators of Operators.hpp
+    (the special case A = 1 of every AN operator).  Decoded values keep t's unencoded width as
+    their key width; raw AN codes (raw=True) use the whole code width."""
+    for suffix in ("!unchecked", "!raw"):
+        if t.name.endswith(suffix):
+            t = _UNCHECKED[t.name]
+    bits = 8 * t.width if raw else t.unenc_bits
+    u = ColType(f"{t.name}{'!raw' if raw else '!unchecked'}", t.width, True, bits, (1 << (8 * t.width)) - 1,
+                max(t.larger_bits, bits))
     _UNCHECKED[u.name] = t
     return u
 
@@ -133,4 +136,4 @@ def checked_of(t: ColType) -> ColType:
 def unchecked_meta(t: ColType, a: int = 1) -> ColMeta:
     """An unchecked column of t's code width; a = 1 for values, a = A for raw AN codes (thresholds
     are still encoded with A, and the operators compare, join and group the codes themselves)."""
-    return ColMeta(unchecked_type(t), a, 1)
+    return ColMeta(unchecked_type(t, raw=a != 1), a, 1)

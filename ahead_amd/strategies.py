@@ -19,6 +19,7 @@ tests) and returns the chain's result form.
 """
 from __future__ import annotations
 
+import threading
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import replace
 
@@ -73,10 +74,23 @@ def normal(plain: dict, q: ssb.Query, ops=B) -> dict:
     return ssb.chain(plain, q, ops)
 
 
+_POOLS: dict = {}
+_local = threading.local()
+
+
+def _pool(k: int) -> ThreadPoolExecutor:
+    """k long-lived replica threads: each keeps its stream and its workspace across queries."""
+    if k not in _POOLS:
+        _POOLS[k] = ThreadPoolExecutor(max_workers=k)
+    return _POOLS[k]
+
+
 def _on_stream(fn):
     import torch
 
-    s = torch.cuda.Stream()
+    s = getattr(_local, "stream", None)
+    if s is None:
+        s = _local.stream = torch.cuda.Stream()
     with torch.cuda.stream(s):
         r = fn()
     s.synchronize()
@@ -86,8 +100,7 @@ def _on_stream(fn):
 def modular(reps: list, q: ssb.Query, ops=B, concurrent: bool = False) -> dict:
     """DMR (2 replicas) or TMR (3): the plain chain once per replica, then the vote."""
     if concurrent and ops is B:
-        with ThreadPoolExecutor(max_workers=len(reps)) as ex:
-            results = list(ex.map(lambda r: _on_stream(lambda: ssb.chain(r, q, ops)), reps))
+        results = list(_pool(len(reps)).map(lambda r: _on_stream(lambda: ssb.chain(r, q, ops)), reps))
     else:
         results = [ssb.chain(r, q, ops) for r in reps]
     return vote(results)

@@ -709,9 +709,9 @@ def run_strategies(args):
     from ahead_amd import ssb
     from ahead_amd import strategies as S
 
-    spec = ssb.SsbSpec()
-    if args.rows:
-        spec.n_fact = args.rows
+    # fault-free (early drops corrupt rows, which misaligns the columns); 300 M fact rows: the schema,
+    # its unhardened values and two more replicas of them stay resident together
+    spec = ssb.SsbSpec(rate=0.0, n_fact=args.rows or 300_000_000)
     t = ssb.device_tables(spec)
     plain = S.plain_tables(t)
     reps3 = S.replicas(plain, 3)
@@ -743,6 +743,8 @@ def run_strategies(args):
         torch.cuda.synchronize()
         el = (time.perf_counter() - t0) / steps
         out[name] = {"ms_per_step": el * 1e3, "queries_per_s": len(qs) / el}
+        del got
+        torch.cuda.empty_cache()  # the next strategy's intermediates (and stream workspaces) start afresh
         print(f"strategies: {name} {el * 1e3:.1f} ms per step of {len(qs)} queries", file=sys.stderr, flush=True)
     base = out["normal"]["ms_per_step"]
     for v in out.values():
