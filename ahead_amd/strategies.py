@@ -9,6 +9,8 @@
               compared, joined and grouped as they are) and only the fetched group attributes and
               measures are checked and decoded, right before grouping and aggregation
               (ssbm-q11_late.cpp:70-120, ssbm-q21_late.cpp:85-160);
+  continuous_reenc  the hardened chain re-encoding every join's and fetch's outputs with a
+              different A (ssbm-q*_continuous_reenc.cpp);
   dmr / tmr   the plain chain on 2 / 3 replicas of the unhardened values, then a vote
               (ModularRedundant.hpp:111-180, ssbm-q11_dmr_seq.cpp:98-147); `concurrent` runs the
               replicas at the same time, one stream each (the OpenMP replicas of
@@ -170,4 +172,44 @@ def late(t: dict, q: ssb.Query, ops=B) -> dict:
     return ssb.chain(late_tables(t), q, _LateOps(ops))
 
 
-STRATEGIES = ("continuous", "normal", "early", "late", "dmr", "dmr_concurrent", "tmr")
+class _ReencOps:
+    """The operator module with the re-encoding of ssbm-q*_continuous_reenc.cpp: every join and
+    fetch re-encodes its outputs (matchjoinAN.tcc:394-432, hashjoinAN.tcc:561-585) with the next A
+    of the code width's parameter table, counting down from the one below the default (the drivers
+    take std::get<14>, <13>, ... of v2_resoid_t::As / v2_resint_t::As, <6>, <5>, ... of
+    v2_restiny_t::As), so consecutive operators' outputs carry different As.  selectAN's
+    re-encoding arguments are ignored by the reference (selectAN.tcc:51-70) and are not passed."""
+
+    TABLES = {2: an.RESTINY_AS, 4: an.RESSHORT_AS, 8: an.RESINT_AS}
+
+    def __init__(self, ops):
+        self._ops = ops
+        self._next: dict = {}
+
+    def __getattr__(self, name):
+        return getattr(self._ops, name)
+
+    def _a(self, side) -> tuple:
+        w = side.meta.width if side.meta.type.encoded else 8  # void / plain OIDs become 8-byte OID codes
+        table = self.TABLES[w]
+        i = self._next.get(w, len(table) - 2)
+        self._next[w] = i - 1 if i > 1 else len(table) - 2
+        return table[i], an.inverse(table[i], 8 * w)
+
+    def matchjoinAN(self, b1, b2, reenc_h=None, reenc_t=None, **kw):
+        return self._ops.matchjoinAN(b1, b2, self._a(b1.head), self._a(b2.tail), **kw)
+
+    def hashjoinAN(self, b1, b2, reenc_h=None, reenc_t=None, **kw):
+        return self._ops.hashjoinAN(b1, b2, self._a(b1.head), self._a(b2.tail), **kw)
+
+    def fetchjoinAN(self, oids, col, reenc=None, **kw):
+        return self._ops.fetchjoinAN(oids, col, self._a(col.tail), **kw)
+
+
+def continuous_reenc(t: dict, q: ssb.Query, ops=B) -> dict:
+    """The hardened chain with re-encoding at every join and fetch (its results and error vectors
+    are the continuous chain's: decoding and error positions do not depend on the As)."""
+    return ssb.chain(t, q, _ReencOps(ops))
+
+
+STRATEGIES = ("continuous", "continuous_reenc", "normal", "early", "late", "dmr", "dmr_concurrent", "tmr")

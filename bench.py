@@ -719,6 +719,7 @@ def run_strategies(args):
     qs = [ssb.SHAPES[k]() for k in SSB_SHAPES]
     forms = {
         "continuous": lambda q: ssb.chain(t, q),
+        "continuous_reenc": lambda q: S.continuous_reenc(t, q),
         "early": lambda q: S.early(t, q),
         "late": lambda q: S.late(t, q),
         "normal": lambda q: S.normal(plain, q),

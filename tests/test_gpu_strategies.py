@@ -1,6 +1,6 @@
 """The reference's other strategies (ahead_amd.strategies) on the GPU operators against the same
-strategies on the oracle's: unhardened chains, early and late check-and-decode, DMR / TMR with a
-replica whose values were changed, all thirteen queries."""
+strategies on the oracle's: unhardened chains, early and late check-and-decode, the re-encoding
+chains, DMR / TMR with a replica whose values were changed, all thirteen queries."""
 import numpy as np
 import pytest
 
@@ -41,13 +41,13 @@ def test_unhardened_and_modular_strategies(schema, name):
 
 
 @pytest.mark.parametrize("name", list(ssb.QUERIES))
-def test_early_and_late_with_flips(name):
+def test_early_late_and_reencoding_with_flips(name):
     q = ssb.QUERIES[name]
     attrs = {d.attr for d in q.dims}
     flips = tuple(c for c in SEL_FLIPS if c not in attrs)
     spec = ssb.SsbSpec(rate=2e-3, seed=62, flip_cols=flips, **SMALL)
     t, host = ssb.device_tables(spec), OB.tables(spec)
-    print(f"strategies {name}: early, late", flush=True)
+    print(f"strategies {name}: early, late, re-encoding", flush=True)
     _, eg = S.early_decode(t, q)
     _, eo = S.early_decode(host, q, OB)
     same_errors(eg, eo)
@@ -62,6 +62,9 @@ def test_early_and_late_with_flips(name):
     lg, lo = S.late(t, q), S.late(host, q, OB)
     assert S.same(lg, lo)
     same_errors(lg["errors"], lo["errors"])
+    rg, ro = S.continuous_reenc(t, q), S.continuous_reenc(host, q, OB)
+    assert S.same(rg, ro) and S.same(rg, ssb.chain(host, q, OB))
+    same_errors(rg["errors"], ro["errors"])
 
 
 def test_a_changed_replica_is_detected_and_outvoted(schema):

@@ -133,3 +133,33 @@ def test_late_with_a_flipped_measure_fails_its_size_check():
     spec = ssb.SsbSpec(rate=5e-2, seed=54, flip_cols=("lo_revenue",), **SMALL)
     with pytest.raises(RuntimeError):
         S.late(OB.tables(spec), q, OB)
+
+
+@pytest.mark.parametrize("name", list(ssb.QUERIES))
+def test_reencoding_chain_equals_the_continuous_chain(name):
+    """Re-encoding changes the codes of every intermediate, not the decoded results or the error
+    positions: the continuous chain's result and every error vector, with selection flips."""
+    q = ssb.QUERIES[name]
+    attrs = {d.attr for d in q.dims}
+    flips = tuple(c for c, v in ssb.COLUMNS.items() if v[4] and c not in attrs)
+    spec = ssb.SsbSpec(rate=1e-2, seed=55, flip_cols=flips, **SMALL)
+    t = OB.tables(spec)
+    ref = ssb.chain(t, q, OB)
+    r = S.continuous_reenc(t, q, OB)
+    assert S.same(r, ref)
+    er, eref = dict(r["errors"].items), dict(ref["errors"].items)
+    assert sorted(er) == sorted(eref)
+    for k in eref:
+        assert np.array_equal(er[k], eref[k]), k
+
+
+def test_reencoding_uses_a_different_a_per_operator():
+    ops = S._ReencOps(OB)
+    side8 = OB.void(3)
+    side2 = OB.HCol(np.zeros(3, np.uint16), 3, an.an_meta(an.RESTINY, 233))
+    a8 = [ops._a(side8)[0] for _ in range(15)]
+    assert a8[:3] == [an.RESINT_AS[14], an.RESINT_AS[13], an.RESINT_AS[12]]
+    assert 1 not in a8 and a8[14] == an.RESINT_AS[14]  # indexes 14 .. 1, then again
+    assert [ops._a(side2)[0] for _ in range(2)] == [an.RESTINY_AS[6], an.RESTINY_AS[5]]
+    a, ainv = ops._a(side2)
+    assert (a * ainv) % (1 << 16) == 1
