@@ -238,10 +238,8 @@ def hashjoinAN(b1: BAT, b2: BAT, reenc_h: tuple[int, int] | None = None, reenc_t
                                            *[_b(x, c[1]) for x, c in zip(b, caps)], cnt, r.ws.h, current_stream())
 
     bufs, c = r.run(call, caps, [0, 0, 1, 2, 3, 4], shrink=(0, 1))
-    if c[5] == 1:
-        raise capi.AheadError(capi.AHEAD_ERR_DEVICE, "hashjoinAN: the build table overflowed (internal limit)")
     if c[5] != 0:
-        raise capi.AheadError(capi.AHEAD_ERR_DEVICE, "hashjoinAN: a build key has more rows than the device join supports")
+        raise capi.AheadError(capi.AHEAD_ERR_DEVICE, "hashjoinAN: the build table overflowed (internal limit)")
     mh = an.ColMeta(th, reenc_h[0], reenc_h[1]) if reenc_h else an.ColMeta(th, h1.meta.a, h1.meta.a_inv)
     mt = an.ColMeta(tt, reenc_t[0], reenc_t[1]) if reenc_t else an.ColMeta(tt, t2.meta.a, t2.meta.a_inv)
     errs = _errs(bufs[2:], c[1:5], [h1.meta.type.encoded, t1.meta.type.encoded, h2.meta.type.encoded,

@@ -6,7 +6,8 @@
 //             (static_cast<h2unenc_t>(h2), :186) into an HBM hash table (htable.hpp) that counts
 //             the rows per key and keeps the first row.  Keys with several rows additionally get
 //             a row list: counts are scanned into offsets, rows are scattered into the lists and
-//             each list is sorted back into build (insertion) order.
+//             each list is sorted back into build (insertion) order (in LDS, and merged in HBM
+//             for lists of more than 8192 rows).
 //  * dense  : when the build keys span a range of at most 4 x |build| + 1024 values, a direct
 //             map over that range (u32 per key: 0 = no row, first row + 1 for a key with one row,
 //             bit 31 | table slot for a key with several) lets a probe row make one gather into a
@@ -27,7 +28,7 @@ using namespace ahead_dev;
 
 namespace {
 
-constexpr uint32_t kMaxSortedList = 8192;  // longest per-key row list the build can order in LDS
+constexpr uint32_t kMaxSortedList = 8192;  // longest per-key row list ordered in LDS alone (longer: merged in HBM)
 
 constexpr uint32_t kDmapMulti = 0x80000000u;  // direct-map entry: the key has several rows (low bits: slot)
 
@@ -40,7 +41,7 @@ struct HjTable {
     uint32_t *off;    // [size + 1] list offsets (keys with several rows)
     uint32_t *cur;    // [size + 1] fill cursors
     uint32_t *rows;   // [n_build] row lists
-    uint32_t *flags;  // [0] several rows share a key, [1] table overflow, [2] list too long to order,
+    uint32_t *flags;  // [0] several rows share a key, [1] table overflow, [2] unused,
                       // [3] smallest build key, [5] largest build key, [4] direct map in use,
                       // [6] some sampled probe row finds no key
     uint64_t *filt;   // [kHjFiltWords] probe filter: the bit of every build key set (filt_bit)
@@ -195,39 +196,95 @@ __global__ __launch_bounds__(kBlock) void k_hj_fill(BuildArgs a) {
     }
 }
 
-// Sorts every row list back into build order: one workgroup per list (grid-stride over slots),
-// bitonic sort in LDS.
-__global__ __launch_bounds__(kBlock) void k_hj_sort(HjTable t, uint32_t slots) {
+// Sorts every row list back into build order: one workgroup per list (grid-stride over slots).
+// A list of up to kMaxSortedList rows is sorted by a bitonic sort in LDS; a longer one (a left
+// build over a small dimension puts every fact row of a key in one list) has each of its
+// kMaxSortedList-row chunks sorted that way, then the sorted runs are merged pairwise in HBM, run
+// length doubling each pass (merge path: every thread finds where its share of the output starts
+// in both runs by a binary search, then merges sequentially), ping-ponging with `tmp` (one u32
+// per build row).  Row numbers are distinct, so the order is total.
+__device__ void hj_bitonic_lds(uint32_t *s_v, uint32_t p2) {
+    for (uint32_t k = 2; k <= p2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < p2; i += kBlock) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint32_t x = s_v[i], y = s_v[l];
+                    if ((x > y) == ((i & k) == 0)) {
+                        s_v[i] = y;
+                        s_v[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Merges the sorted runs [p0, p0 + w) and [p0 + w, p0 + 2w) (clipped to c) of src into dst, the
+// outputs [d0, d1) relative to p0.
+__device__ void hj_merge_part(const uint32_t *src, uint32_t *dst, uint64_t p0, uint32_t w, uint32_t c,
+                              uint32_t d0, uint32_t d1) {
+    const uint32_t la = w < c - p0 ? w : static_cast<uint32_t>(c - p0);
+    const uint32_t lb = c - p0 - la < w ? static_cast<uint32_t>(c - p0 - la) : w;
+    const uint32_t *A = src + p0, *Bv = src + p0 + la;
+    // i: how many of the first d0 outputs come from A
+    uint32_t lo = d0 > lb ? d0 - lb : 0u, hi = d0 < la ? d0 : la;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (A[mid] < Bv[d0 - mid - 1]) lo = mid + 1;
+        else hi = mid;
+    }
+    uint32_t i = lo, j = d0 - lo;
+    for (uint32_t d = d0; d < d1; ++d) {
+        const bool takeA = i < la && (j >= lb || A[i] < Bv[j]);
+        dst[p0 + d] = takeA ? A[i++] : Bv[j++];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_hj_sort(HjTable t, uint32_t slots, uint32_t *tmp) {
     __shared__ uint32_t s_v[kMaxSortedList];
     if (t.flags[0] == 0u) return;
     for (uint32_t s = blockIdx.x; s < slots; s += gridDim.x) {
         const uint32_t c = t.cnt[s];
         if (c < 2u) continue;
-        if (c > kMaxSortedList) {
-            if (threadIdx.x == 0) atomicOr(&t.flags[2], 1u);
-            continue;
+        uint32_t *list = t.rows + t.off[s];
+        // sorted chunks of at most kMaxSortedList rows
+        for (uint32_t c0 = 0; c0 < c; c0 += kMaxSortedList) {
+            const uint32_t cc = c - c0 < kMaxSortedList ? c - c0 : kMaxSortedList;
+            uint32_t p2 = 1;
+            while (p2 < cc) p2 <<= 1;
+            for (uint32_t i = threadIdx.x; i < p2; i += kBlock) s_v[i] = i < cc ? list[c0 + i] : 0xFFFFFFFFu;
+            __syncthreads();
+            hj_bitonic_lds(s_v, p2);
+            for (uint32_t i = threadIdx.x; i < cc; i += kBlock) list[c0 + i] = s_v[i];
+            __syncthreads();
         }
-        uint32_t p2 = 1;
-        while (p2 < c) p2 <<= 1;
-        const uint32_t o = t.off[s];
-        for (uint32_t i = threadIdx.x; i < p2; i += kBlock) s_v[i] = i < c ? t.rows[o + i] : 0xFFFFFFFFu;
-        __syncthreads();
-        for (uint32_t k = 2; k <= p2; k <<= 1) {
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = threadIdx.x; i < p2; i += kBlock) {
-                    const uint32_t l = i ^ j;
-                    if (l > i) {
-                        const uint32_t x = s_v[i], y = s_v[l];
-                        if ((x > y) == ((i & k) == 0)) {
-                            s_v[i] = y;
-                            s_v[l] = x;
-                        }
-                    }
-                }
-                __syncthreads();
+        if (c <= kMaxSortedList) continue;
+        // merge passes in HBM
+        uint32_t *src = list, *dst = tmp + t.off[s];
+        const uint32_t per = (c + kBlock - 1) / kBlock;
+        for (uint32_t w = kMaxSortedList; w < c; w <<= 1) {
+            __threadfence();
+            __syncthreads();
+            uint32_t pos = threadIdx.x * per;
+            const uint32_t end = pos + per < c ? pos + per : c;
+            while (pos < end) {
+                const uint64_t p0 = static_cast<uint64_t>(pos) / (2ull * w) * (2ull * w);
+                const uint64_t pair_end = p0 + 2ull * w < c ? p0 + 2ull * w : c;
+                const uint32_t seg_end = end < pair_end ? end : static_cast<uint32_t>(pair_end);
+                hj_merge_part(src, dst, p0, w, c, static_cast<uint32_t>(pos - p0), static_cast<uint32_t>(seg_end - p0));
+                pos = seg_end;
             }
+            uint32_t *x = src;
+            src = dst;
+            dst = x;
         }
-        for (uint32_t i = threadIdx.x; i < c; i += kBlock) t.rows[o + i] = s_v[i];
+        __threadfence();
+        __syncthreads();
+        if (src != list)
+            for (uint32_t i = threadIdx.x; i < c; i += kBlock) list[i] = src[i];
+        __threadfence();
         __syncthreads();
     }
 }
@@ -638,7 +695,7 @@ __global__ __launch_bounds__(kBlock) void k_hj_sample(ProbeOp p, uint64_t np) {
 }
 
 __global__ void k_hj_status(const uint32_t *flags, uint64_t *d_counts) {
-    d_counts[5] = flags[1] ? 1u : (flags[2] ? 2u : 0u);
+    d_counts[5] = flags[1] ? 1u : 0u;
 }
 
 }  // namespace
@@ -692,6 +749,7 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
                       (out_h.cap < out_t.cap ? out_h.cap : out_t.cap) >= np;
     // the streaming probe's tables follow the build's
     size_t need = o_end + map_scratch_bytes<ProbeMapOp>(np);
+    if (need < o_end + align_up(nb * 4, 256)) need = o_end + align_up(nb * 4, 256);  // long-list merge scratch
     if (lean) {
         lean_shape(lean_kw, !ppay.data, !bpay.data, [&](auto proto) {
             const size_t b = o_end + map_scratch_bytes<decltype(proto)>(np);
@@ -739,7 +797,9 @@ extern "C" int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, c
     off.t = t;
     if ((st = launch_rowtile(off, slots, 1, ws, s)) != AHEAD_OK) return st;
     hipLaunchKernelGGL(k_hj_fill, grid, dim3(kBlock), 0, s, b);
-    hipLaunchKernelGGL(k_hj_sort, dim3(1024), dim3(kBlock), 0, s, t, static_cast<uint32_t>(slots));
+    // the merge scratch of long row lists shares the probe's scratch region (the build is done first)
+    hipLaunchKernelGGL(k_hj_sort, dim3(1024), dim3(kBlock), 0, s, t, static_cast<uint32_t>(slots),
+                       reinterpret_cast<uint32_t *>(ws_region(ws, o_end)));
     AHEAD_HIP_TRY(hipGetLastError());
 
     // build-side checks, in build-scan order: e3/e4 for a right build, e1/e2 for a left build

@@ -520,10 +520,8 @@ namespace ahead {
                                 reencT ? 1 : 0, f_t2, sizeof(typename RT::type_t), AOID, ahead_buf {oh, cap}, ahead_buf {ot, cap}, cx.errbuf(0), cx.errbuf(1),
                                 cx.errbuf(2), cx.errbuf(3), cx.d_counts(), cx.ws, cx.stream));
                         k = cx.read_counts(6);
-                        if (k[5] == 1)
-                            throw std::runtime_error("hashjoinAN: the build table overflowed (internal limit)");
                         if (k[5] != 0)
-                            throw std::runtime_error("hashjoinAN: a build key has more rows than the device join supports");
+                            throw std::runtime_error("hashjoinAN: the build table overflowed (internal limit)");
                         const bool err_ok = cx.errors_fit( {k[1], k[2], k[3], k[4]});
                         if (k[0] > cap) {
                             cap = k[0];

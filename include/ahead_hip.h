@@ -217,8 +217,10 @@ int ahead_hip_matchjoin_seq_an(const ahead_col *h1, const ahead_col *t1, const a
  * hash map.  build_right != 0 is hash_side_t::right (the default).  Pairs (h1, t2) in probe
  * order, build rows in insertion order per key; f_h1 / f_t2 are the re-encoding factors
  * mulIfEncoded(A_R, A^-1) of the re-encoding overloads.  e1..e4 check h1, t1, h2, t2 by scan
- * position.  Build keys up to 64 bits; at most 8192 build rows per key.
- * d_counts[0..5] = #pairs, #e1, #e2, #e3, #e4, status (0 ok, 2 = a key has > 8192 rows).   */
+ * position.  Build keys up to 64 bits, any number of build rows per key (lists of more than
+ * 8192 rows are merged in the workspace, 4 bytes per build row).
+ * d_counts[0..5] = #pairs, #e1, #e2, #e3, #e4, status (0 ok, 1 = the build table overflowed,
+ * an internal limit).                                                                          */
 int ahead_hip_hashjoin_an(const ahead_col *h1, const ahead_col *t1, const ahead_col *h2,
                           const ahead_col *t2, int build_right, int reenc_h, uint64_t f_h1,
                           uint32_t wout_h, int reenc_t, uint64_t f_t2, uint32_t wout_t,

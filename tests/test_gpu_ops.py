@@ -101,6 +101,24 @@ def test_hashjoin_void_sides_and_dense_keys():
     same(got, ref)
 
 
+@pytest.mark.parametrize("nf,nd", [(120_000, 5), (300_000, 13), (40_000, 2), (470_000, 2)])
+def test_hashjoin_long_row_lists(nf, nd):
+    """A left build over the fact FK against a few dimension keys: every key's row list is far
+    longer than the 8192 rows ordered in LDS alone (about 235 000 with 470 000 rows over 2 keys,
+    the rows per date key of a 600 M-row fact), so the lists are merged in HBM; pairs in probe order, build rows in insertion order (hashjoinAN.tcc:130-172), as the
+    oracle gives them.  Flips in the build keys drop rows from the lists and report them."""
+    fk = datagen.encode(datagen.uniform(nf, nd, 1, 0, nd - 1), 63877, 8)
+    datagen.inject_flips(fk, nd, 2, datagen.flip_threshold(1e-4))
+    key = datagen.encode(np.arange(nd, dtype=np.uint64), 63877, 8)
+    meta_k = an.an_meta(an.RESINT, 63877)
+    for reenc in (0, 1):
+        args = (0, reenc, an.A_OID, 8, reenc, an.A_OID, 8)  # build_right = 0: the fact side is built
+        ref = O.hashjoin_an(O.void(nf), O.Col(fk, meta_k), O.Col(key, meta_k), O.void(nd), *args)
+        got = capi.hashjoin_an(capi.Column.void(nf), gcol(fk, meta_k), gcol(key, meta_k), capi.Column.void(nd), *args)
+        same(got, ref)
+        assert got[0].size >= nf - 50
+
+
 # The probe for the common shapes (8- or 4-byte key codes, void or 8-byte payloads, 4- or 8-byte
 # outputs, room for a pair per probe row): all rows joining (one pass), some missing (the variant
 # with the key filter in LDS and the second, writing pass), both build sides, re-encoding.
