@@ -462,7 +462,10 @@ namespace ahead {
                         r->head.container->resize(np);
                         r->tail.container->resize(np);
                         return std::make_tuple(r, cx.take(0, e1, is_enc<H1>), cx.take(1, e2, true), nullptr, cx.take(3, e4, is_enc<T2>));
-                    } else if constexpr (!is_void<T1> && !is_void<H2>) {
+                    } else if constexpr (!(is_void<T1> && is_void<H2>)) {
+                        // the merge form; a void side is its virtual ascending OID sequence (the
+                        // drivers' matchjoinAN(reverse(key), mirror_head(selection)),
+                        // ssbm-q21_continuous.cpp:92)
                         const uint64_t cap = std::min(n1, n2);
                         void *oh = r->head.container->overwrite(cap);
                         void *ot = r->tail.container->overwrite(cap);
@@ -504,8 +507,17 @@ namespace ahead {
                     const ahead_col h2 = col(arg2->head, n2), t2 = col(arg2->tail, n2);
                     const auto & hmd = arg1->head.metaData;
                     const auto & tmd = arg2->tail.metaData;
-                    ColumnMetaData mh = reencH ? ColumnMetaData(hmd.width, static_cast<A_t>(AH1R), AH1InvR, hmd.AN_unencMaxU, hmd.AN_unencMinS) : hmd;
-                    ColumnMetaData mt = reencT ? ColumnMetaData(tmd.width, static_cast<A_t>(AT2R), AT2InvR, tmd.AN_unencMaxU, tmd.AN_unencMinS) : tmd;
+                    // a re-encoded side takes its result type's metadata: a void side becomes OIDs
+                    // encoded with A_R (resoid width and maxU), not the void column's
+                    const auto reenc_md = [](const ColumnMetaData &md, bool enc, uint64_t a, uint64_t ainv, const bytes_t &width, uint64_t maxu) {
+                        return enc ? ColumnMetaData(md.width, static_cast<A_t>(a), ainv, md.AN_unencMaxU, md.AN_unencMinS)
+                                   : ColumnMetaData(width, static_cast<A_t>(a), ainv, maxu, 0);
+                    };
+                    ColumnMetaData mh = hmd, mt = tmd;
+                    if constexpr (reencH)
+                        mh = reenc_md(hmd, is_enc<H1>, AH1R, AH1InvR, size_bytes<typename RH::type_t>, RH::UNENC_MAX_U);
+                    if constexpr (reencT)
+                        mt = reenc_md(tmd, is_enc<T2>, AT2R, AT2InvR, size_bytes<typename RT::type_t>, RT::UNENC_MAX_U);
                     // hashjoinAN.tcc:94-95: reencFactor = mulIfEncoded(A_R, A^-1), applied to every output value
                     const uint64_t f_h1 = is_enc<H1> ? static_cast<h1enc_t>(AH1R * static_cast<h1enc_t>(hmd.AN_Ainv)) : static_cast<h1enc_t>(AH1R);
                     const uint64_t f_t2 = is_enc<T2> ? static_cast<t2enc_t>(AT2R * static_cast<t2enc_t>(tmd.AN_Ainv)) : static_cast<t2enc_t>(AT2R);
