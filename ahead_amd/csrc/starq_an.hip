@@ -1158,6 +1158,542 @@ __global__ __launch_bounds__(kBlock) void k_sq_sel(DimPrepArgs a) {
     }
 }
 
+// ---- same-shape shared scan --------------------------------------------------------------------
+// Queries of one shape (the same dimensions, FK, key and group attribute columns, join forms, fact
+// predicate columns and measures; only their selections and constants differ) share the whole
+// per-row work.  Per dimension one combined map holds, per key, a presence bit per query (the key
+// belongs to that query's selection) and the attribute's index over the union of the queries'
+// attribute ranges; a row costs one lookup per dimension for all of them.  A fact predicate over
+// values of at most 8 bits is a 256-entry table of per-query pass bits, indexed by the decoded
+// value.  What remains per query is the work of its joined rows: the group index in its own
+// numbering and its LDS group sums.  A row with a corrupt code takes each query's chain-order path
+// over that query's own maps (exact_row_s), as in the single-query scan.
+
+constexpr uint32_t kSqsMaxQ = AHEAD_SQ_MAX_BATCH;
+constexpr uint32_t kSqsMapWords = 24320;  // LDS words for the combined maps (beside groups, staging, tables)
+constexpr uint32_t kSqsQWords = 10;       // per query in LDS: attribute offsets[4], strides[4], groups, group base
+
+struct SqsDimK {  // a dimension of the pass, in evaluation order, over its combined map
+    const uint64_t *fk;
+    uint64_t fk_inv, fk_maxu;
+    uint32_t kmask, kmin, span, lb, lds_off, map_off;
+    uint32_t ab;      // bits of the union attribute field (0: no group attribute)
+    uint32_t ashift;  // position of the field in a row's packed attribute word
+    uint32_t flags;   // AHEAD_SQ_PROBE | AHEAD_SQ_REFETCH
+    uint32_t cpos;    // position in the chain
+    const uint32_t *row_of;  // dimension row per key index
+};
+
+struct SqsPredK {  // a fact predicate column (codes of at most 4 bytes)
+    const void *col;
+    uint32_t width, table;  // table: the predicate's 256-entry pass-bit table is in LDS
+    uint64_t inv, maxu, wmask;
+};
+
+struct SqsArgs {
+    uint64_t n, rows_per_wave;
+    uint32_t nq, nd, np, nm, kind, pre_load;
+    SqsDimK d[AHEAD_SQ_MAX_DIMS];
+    SqsPredK p[AHEAD_SQ_MAX_PREDS];
+    SqMeasK m[AHEAD_SQ_MAX_MEAS];
+    const uint32_t *maps;  // combined maps; the first lds_words words go to LDS
+    uint32_t lds_words;
+    const uint32_t *ptab;  // [np][256] pass bits
+};
+
+struct SqsQ {  // one query of the pass, by the pass's dimension index (device memory)
+    const uint32_t *maps;  // the query's own maps (chain-order path)
+    uint32_t map_off[AHEAD_SQ_MAX_DIMS], lb[AHEAD_SQ_MAX_DIMS], errv[AHEAD_SQ_MAX_DIMS];
+    uint32_t stride[AHEAD_SQ_MAX_DIMS];  // group strides in the query's own numbering (0: no attribute)
+    uint32_t off[AHEAD_SQ_MAX_DIMS];     // the query's attribute minimum minus the union's
+    uint32_t groups, goff;               // its groups, their base in the LDS group tables
+    SqbPredK p[AHEAD_SQ_MAX_PREDS];      // its predicates as intervals of codes
+    const uint32_t *flags;
+    uint32_t *ev_count;
+    uint64_t *events;
+    unsigned long long *gsum;
+    uint32_t *gfirst;
+    unsigned long long *kept;
+};
+
+struct SqsDesc {
+    SqsQ q[kSqsMaxQ];
+};
+static_assert(sizeof(SqsDesc) <= 4096 - 64, "descriptor block written through kernel arguments");
+
+struct SqsTab {
+    uint32_t t[AHEAD_SQ_MAX_PREDS * 256];
+};
+
+__device__ __forceinline__ void push_event_s(const SqsQ &Q, uint64_t row, uint32_t kind, uint32_t drow) {
+    const uint32_t i = __hip_atomic_fetch_add(gblw(Q.ev_count), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (i < kSqMaxEvents)
+        gblw(Q.events)[i] = (row << 32) | (static_cast<uint64_t>(kind) << 27) | (drow & 0x7FFFFFFu);
+}
+
+// The chain's semantics for one row of query Q of a same-shape pass: its own maps and constants,
+// the pass's columns; dimensions in chain order (the same rules and event kinds as exact_row).
+template <int ND, int NP>
+__device__ void exact_row_s(const SqsArgs &a, const SqsQ &Q, uint64_t row, unsigned long long *s_sum,
+                            uint32_t *s_first, uint32_t &kept) {
+    bool alive = true;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const SqsPredK &pc = a.p[p];
+        const uint64_t c = load_code_g(pc.col, pc.width, row);
+        if (((c * pc.inv) & pc.wmask) > pc.maxu) push_event_s(Q, row, AHEAD_SQ_E_PRED + p, 0u);
+        if (!pred_fast<uint32_t>(Q.p[p], static_cast<uint32_t>(c))) alive = false;
+    }
+    if (!alive) return;
+    bool dropped = false;
+    uint64_t dm[AHEAD_SQ_MAX_MEAS] = {0, 0};
+#pragma unroll
+    for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m) {
+        const SqMeasK &mk = a.m[m];
+        if (m >= static_cast<int>(a.nm) || !mk.pre) continue;
+        dm[m] = (load_code_g(mk.col, mk.width, row) * mk.inv) & mk.wmask;
+        if (dm[m] > mk.maxu) {
+            push_event_s(Q, row, AHEAD_SQ_E_MEAS + m, 0u);
+            dropped = true;
+        }
+    }
+    uint32_t ent[ND], keyv[ND];
+    bool fk_bad[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        ent[d] = 0u;
+        keyv[d] = 0u;
+        fk_bad[d] = false;
+    }
+    // chain order: position cp is the pass dimension d with cpos == cp (static indices only)
+#pragma unroll
+    for (int cp = 0; cp < ND; ++cp) {
+        bool stop = false;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const SqsDimK &dk = a.d[d];
+            if (dk.cpos != static_cast<uint32_t>(cp)) continue;
+            const uint64_t dec = gbl(dk.fk)[row] * dk.fk_inv;
+            if (dec > dk.fk_maxu) {
+                fk_bad[d] = true;
+                push_event_s(Q, row, AHEAD_SQ_E_FK + cp, 0u);
+                if (!(dk.flags & AHEAD_SQ_PROBE)) stop = true;
+            }
+            if (!stop) {
+                keyv[d] = static_cast<uint32_t>(dec) & dk.kmask;
+                const uint32_t idx = keyv[d] - dk.kmin;
+                ent[d] = idx < dk.span ? map_entry(gbl(Q.maps)[Q.map_off[d] + (idx >> (5u - Q.lb[d]))], idx, Q.lb[d])
+                                       : 0u;
+                if (ent[d] == 0u) stop = true;
+            }
+        }
+        if (stop) return;
+    }
+#pragma unroll
+    for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m) {
+        const SqMeasK &mk = a.m[m];
+        if (m >= static_cast<int>(a.nm) || mk.pre) continue;
+        dm[m] = (load_code_g(mk.col, mk.width, row) * mk.inv) & mk.wmask;
+        if (dm[m] > mk.maxu) {
+            push_event_s(Q, row, AHEAD_SQ_E_MEAS + m, 0u);
+            dropped = true;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        if ((a.d[d].flags & AHEAD_SQ_REFETCH) && fk_bad[d]) {
+            push_event_s(Q, row, AHEAD_SQ_E_REFETCH + a.d[d].cpos, 0u);
+            dropped = true;
+        }
+    }
+    uint32_t g = 0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        if (Q.errv[d] == 0u) continue;
+        if (ent[d] == Q.errv[d]) {
+            push_event_s(Q, row, AHEAD_SQ_E_ATTR + a.d[d].cpos, gbl(a.d[d].row_of)[keyv[d] - a.d[d].kmin]);
+            dropped = true;
+        } else {
+            g += (ent[d] - 1u) * Q.stride[d];
+        }
+    }
+    if (dropped || g >= Q.groups) return;
+    const uint64_t v = a.kind == AHEAD_SQ_SUB ? dm[0] - dm[1] : a.kind == AHEAD_SQ_MULSUM ? dm[0] * dm[1] : dm[0];
+    atomicAdd(&s_sum[g], static_cast<unsigned long long>(v));
+    atomicMin(&s_first[g], static_cast<uint32_t>(row));
+    ++kept;
+}
+
+// ND dimensions, NP fact predicates over codes of PWB bytes (0: widths differ; at most 4 bytes)
+template <int ND, int NP, int PWB>
+__global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc *__restrict__ desc) {
+    constexpr int kSeg = 256, R = 4;  // rows per wave step (2 chunks of 128), rows per lane
+    constexpr uint32_t OFFB = 8u;     // a staged row: offset | pass bits << OFFB, then its attribute word
+    __shared__ uint32_t s_map[kSqsMapWords];
+    __shared__ unsigned long long s_sum[kSqMaxGroups];
+    __shared__ uint32_t s_first[kSqMaxGroups];
+    __shared__ uint32_t s_stage[kSqWaves * kSeg * 2];
+    __shared__ uint32_t s_ptab[AHEAD_SQ_MAX_PREDS * 256];
+    __shared__ uint32_t s_qd[kSqsMaxQ * kSqsQWords];
+    __shared__ uint32_t s_kept[kSqsMaxQ];
+    static_assert(sizeof(s_map) + sizeof(s_sum) + sizeof(s_first) + sizeof(s_stage) + sizeof(s_ptab) + sizeof(s_qd) +
+                      sizeof(s_kept) <= 160 * 1024 - 1024,
+                  "LDS budget");
+    static_assert((1u << OFFB) == kSeg && kSqsMaxQ <= 32u - OFFB, "staged entry layout");
+    uint32_t qmask = 0;  // queries whose plan holds (no repeated key)
+    for (uint32_t q = 0; q < a.nq; ++q)
+        if (gbl(desc->q[q].flags)[kFlagDup] == 0u) qmask |= 1u << q;
+    for (uint32_t i = threadIdx.x; i < a.lds_words; i += kSqBlock) s_map[i] = a.maps[i];
+    for (uint32_t i = threadIdx.x; i < a.np * 256u; i += kSqBlock) s_ptab[i] = a.ptab[i];
+    for (uint32_t i = threadIdx.x; i < kSqMaxGroups; i += kSqBlock) {
+        s_sum[i] = 0ull;
+        s_first[i] = 0xFFFFFFFFu;
+    }
+    if (threadIdx.x < a.nq * kSqsQWords) {
+        const SqsQ &Q = desc->q[threadIdx.x / kSqsQWords];
+        const uint32_t k = threadIdx.x % kSqsQWords;
+        s_qd[threadIdx.x] = k < 4 ? Q.off[k] : k < 8 ? Q.stride[k - 4] : k == 8 ? Q.groups : Q.goff;
+    }
+    if (threadIdx.x < kSqsMaxQ) s_kept[threadIdx.x] = 0u;
+    __syncthreads();
+    if (qmask == 0u) return;
+
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t gw = static_cast<uint64_t>(blockIdx.x) * kSqWaves + wave;
+    const uint64_t begin = gw * a.rows_per_wave;
+    const uint64_t end = begin + a.rows_per_wave < a.n ? begin + a.rows_per_wave : a.n;
+
+    using PT = uint32_t;
+    uint64_t fk[ND][R];
+    PT pc[NP > 0 ? NP : 1][R];
+    auto load_step = [&](uint64_t seg) {
+        if (seg + kSeg <= end) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const uint64_t r0 = seg + c * 128u + lane * 2u;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const u32x4 q = load16_nt(a.d[d].fk + r0);
+                    fk[d][2 * c] = (static_cast<uint64_t>(q[1]) << 32) | q[0];
+                    fk[d][2 * c + 1] = (static_cast<uint64_t>(q[3]) << 32) | q[2];
+                }
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    if constexpr (PWB > 0) load_pair_w<PWB>(a.p[p].col, r0, pc[p][2 * c], pc[p][2 * c + 1]);
+                    else load_pair(a.p[p].col, a.p[p].width, r0, pc[p][2 * c], pc[p][2 * c + 1]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint64_t row = seg + (r >> 1) * 128u + lane * 2u + (r & 1);
+                const bool in = row < end;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) fk[d][r] = in ? a.d[d].fk[row] : 0ull;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    if constexpr (PWB > 0) pc[p][r] = in ? static_cast<PT>(load_code_w<PWB>(a.p[p].col, row)) : PT(0);
+                    else pc[p][r] = in ? static_cast<PT>(load_code(a.p[p].col, a.p[p].width, row)) : PT(0);
+                }
+            }
+        }
+    };
+
+    // the pending batch: one staged row per lane, its measure reads in flight
+    bool pend_act = false;
+    uint64_t pend_row = 0;
+    uint32_t pend_pass = 0, pend_attr = 0;
+    uint64_t pend_code[AHEAD_SQ_MAX_MEAS] = {0, 0};
+    // finishes the pending batch: the row into each of its queries' group tables; returns the
+    // queries whose chain-order path the row needs (a corrupt measure: all of them)
+    auto finish_pending = [&]() -> uint32_t {
+        if (!pend_act) return 0u;
+        pend_act = false;
+        uint64_t dm[AHEAD_SQ_MAX_MEAS] = {0, 0};
+        bool bad = false;
+#pragma unroll
+        for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m) {
+            if (m >= static_cast<int>(a.nm)) continue;
+            dm[m] = (pend_code[m] * a.m[m].inv) & a.m[m].wmask;
+            bad |= dm[m] > a.m[m].maxu;
+        }
+        if (bad) return pend_pass;
+        const uint64_t v = a.kind == AHEAD_SQ_SUB ? dm[0] - dm[1] : a.kind == AHEAD_SQ_MULSUM ? dm[0] * dm[1] : dm[0];
+        uint32_t rest = pend_pass, redo = 0;
+        while (rest != 0u) {
+            const uint32_t q = __ffs(rest) - 1;
+            rest &= rest - 1u;
+            const uint32_t *qs = s_qd + q * kSqsQWords;
+            uint32_t g = 0;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                if (a.d[d].ab == 0u) continue;
+                const uint32_t f = (pend_attr >> a.d[d].ashift) & ((1u << a.d[d].ab) - 1u);
+                g += (f - 1u - qs[d]) * qs[4 + d];
+            }
+            if (g >= qs[8]) {
+                redo |= 1u << q;
+                continue;
+            }
+            atomicAdd(&s_sum[qs[9] + g], static_cast<unsigned long long>(v));
+            atomicMin(&s_first[qs[9] + g], static_cast<uint32_t>(pend_row));
+            atomicAdd(&s_kept[q], 1u);
+        }
+        return redo;
+    };
+
+    if (begin < end) load_step(begin);
+    for (uint64_t seg = begin; seg < end; seg += kSeg) {
+        // 1. decode and check the codes; each query's predicate bits
+        uint32_t key[ND][R];
+        uint32_t pass[R], err = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint64_t row = seg + (r >> 1) * 128u + lane * 2u + (r & 1);
+            bool e = false;
+            uint32_t ps = row < end ? qmask : 0u;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                const SqsPredK &pk = a.p[p];
+                // codes of at most 4 bytes: the low 32 bits of the product decide
+                const uint32_t dec = (pc[p][r] * static_cast<uint32_t>(pk.inv)) & static_cast<uint32_t>(pk.wmask);
+                e |= dec > static_cast<uint32_t>(pk.maxu);
+                if (pk.table) {
+                    ps &= s_ptab[p * 256 + (dec & 255u)];
+                } else {
+                    for (uint32_t q = 0; q < a.nq; ++q)
+                        if (!pred_fast<uint32_t>(desc->q[q].p[p], pc[p][r])) ps &= ~(1u << q);
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                const uint64_t dec = fk[d][r] * a.d[d].fk_inv;
+                e |= dec > a.d[d].fk_maxu;
+                key[d][r] = static_cast<uint32_t>(dec) & a.d[d].kmask;
+            }
+            if (row < end && e) err |= 1u << r;
+            pass[r] = e ? 0u : ps;
+        }
+        // 2. the next step's codes go in flight now
+        if (seg + kSeg < end) load_step(seg + kSeg);
+        const uint64_t dummy = seg;
+        // 3. measures fetched right after the predicates (not over a predicate's column), checked
+        // at every row some query's predicates pass; a corrupt one sends the row to every query's
+        // chain-order path
+        if (a.pre_load) {
+#pragma unroll
+            for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m) {
+                const SqMeasK &mk = a.m[m];
+                if (m >= static_cast<int>(a.nm) || !mk.pre || mk.pred >= 0) continue;
+                uint64_t code[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint64_t row = seg + (r >> 1) * 128u + lane * 2u + (r & 1);
+                    code[r] = load_code(mk.col, mk.width, pass[r] != 0u ? row : dummy);
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (pass[r] != 0u && ((code[r] * mk.inv) & mk.wmask) > mk.maxu) {
+                        err |= 1u << r;
+                        pass[r] = 0u;
+                    }
+                }
+            }
+        }
+        // 4. dimensions in evaluation order (LDS maps first): the queries whose selections hold
+        // the key, the union attribute fields; a corrupt attribute sends the row to the
+        // chain-order path of the queries that select its key
+        uint32_t attr[R], redo[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            attr[r] = 0u;
+            redo[r] = 0u;
+        }
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const SqsDimK &dd = a.d[d];
+            const uint32_t sh = 5u - dd.lb;
+            uint32_t w[R], idx[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                idx[r] = key[d][r] - dd.kmin;
+                const bool use = pass[r] != 0u && idx[r] < dd.span;
+                if (!use) idx[r] = 0;
+                if (!use) pass[r] = 0u;
+            }
+            if (dd.lds_off != kNoLds) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) w[r] = s_map[dd.lds_off + (idx[r] >> sh)];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) w[r] = a.maps[dd.map_off + (idx[r] >> sh)];
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t e = map_entry(w[r], idx[r], dd.lb);
+                pass[r] &= e;
+                if (dd.ab != 0u && pass[r] != 0u) {
+                    const uint32_t fmask = (1u << dd.ab) - 1u;
+                    const uint32_t f = (e >> a.nq) & fmask;
+                    if (f == fmask) {
+                        redo[r] |= pass[r];
+                        pass[r] = 0u;
+                    }
+                    attr[r] |= f << dd.ashift;
+                }
+            }
+        }
+        // 5. rows some query joins: compaction into the wave's staging area, one lane per row
+        // reads the measures; the last batch of 64 stays pending until the next step's codes are
+        // decoded, earlier ones (rare) are finished at once
+        const uint32_t prev_redo = finish_pending();
+        const uint64_t prev_row = pend_row;
+        uint32_t extra_redo[R - 1];
+#pragma unroll
+        for (int b = 0; b < R - 1; ++b) extra_redo[b] = 0u;
+        uint32_t *stage = s_stage + wave * kSeg * 2;
+        uint32_t total = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const bool al = pass[r] != 0u;
+            const uint64_t b = __ballot(al);
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
+            if (al) {
+                stage[2 * (total + below)] = ((r >> 1) * 128u + lane * 2u + (r & 1)) | (pass[r] << OFFB);
+                stage[2 * (total + below) + 1] = attr[r];
+            }
+            total += static_cast<uint32_t>(__popcll(b));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int b = 0; b < R; ++b) {
+            const uint32_t i0 = b * 64u;
+            if (i0 < total) {
+                const bool act = i0 + lane < total;
+                const uint32_t ent = act ? stage[2 * (i0 + lane)] : 0u;
+                pend_act = act;
+                pend_row = seg + (ent & ((1u << OFFB) - 1u));
+                pend_pass = ent >> OFFB;
+                pend_attr = act ? stage[2 * (i0 + lane) + 1] : 0u;
+#pragma unroll
+                for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m)
+                    if (m < static_cast<int>(a.nm)) pend_code[m] = load_code(a.m[m].col, a.m[m].width, act ? pend_row : dummy);
+                if (b < R - 1 && i0 + 64u < total) extra_redo[b < R - 1 ? b : 0] = finish_pending();
+            }
+        }
+        // 6. the chain-order path (rare), one call site.  Slots: 0..R-1 this step's rows (a
+        // corrupt code: every query; a corrupt attribute: the queries selecting its key), R the
+        // previous pending row, R+1.. the rows of this step's earlier batches
+        uint32_t slots = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if ((err >> r) & 1u) redo[r] = qmask;
+            if (redo[r] != 0u) slots |= 1u << r;
+        }
+        if (prev_redo != 0u) slots |= 1u << R;
+#pragma unroll
+        for (int b = 0; b < R - 1; ++b)
+            if (extra_redo[b] != 0u) slots |= 1u << (R + 1 + b);
+        uint32_t cur = 0;
+        uint64_t cur_row = 0;
+        while (__ballot(slots != 0u || cur != 0u) != 0ull) {
+            if (cur == 0u && slots != 0u) {
+                const int b = __ffs(slots) - 1;
+                slots &= slots - 1u;
+                uint32_t m = prev_redo;
+#pragma unroll
+                for (int j = 0; j < R; ++j)
+                    if (b == j) m = redo[j];
+#pragma unroll
+                for (int j = 0; j < R - 1; ++j)
+                    if (b == R + 1 + j) m = extra_redo[j];
+                cur = m;
+                if (b < R) cur_row = seg + (b >> 1) * 128u + lane * 2u + (b & 1);
+                else if (b == R) cur_row = prev_row;
+                else cur_row = seg + (stage[2 * ((b - R - 1) * 64 + lane)] & ((1u << OFFB) - 1u));
+            }
+            if (cur != 0u) {
+                const uint32_t q = __ffs(cur) - 1;
+                cur &= cur - 1u;
+                uint32_t k = 0;
+                const uint32_t goff = s_qd[q * kSqsQWords + 9];
+                exact_row_s<ND, NP>(a, desc->q[q], cur_row, s_sum + goff, s_first + goff, k);
+                if (k != 0u) atomicAdd(&s_kept[q], k);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    {
+        uint32_t cur = finish_pending();
+        while (__ballot(cur != 0u) != 0ull) {
+            if (cur != 0u) {
+                const uint32_t q = __ffs(cur) - 1;
+                cur &= cur - 1u;
+                uint32_t k = 0;
+                const uint32_t goff = s_qd[q * kSqsQWords + 9];
+                exact_row_s<ND, NP>(a, desc->q[q], pend_row, s_sum + goff, s_first + goff, k);
+                if (k != 0u) atomicAdd(&s_kept[q], k);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t q = 0; q < a.nq; ++q) {
+        if (((qmask >> q) & 1u) == 0u) continue;
+        const SqsQ &Q = desc->q[q];
+        const uint32_t goff = s_qd[q * kSqsQWords + 9], ng = s_qd[q * kSqsQWords + 8];
+        for (uint32_t g = threadIdx.x; g < ng; g += kSqBlock) {
+            if (s_first[goff + g] != 0xFFFFFFFFu) {
+                atomicAdd(&Q.gsum[g], s_sum[goff + g]);
+                atomicMin(&Q.gfirst[g], s_first[goff + g]);
+            }
+        }
+        if (threadIdx.x == 0 && s_kept[q] != 0u) atomicAdd(Q.kept, static_cast<unsigned long long>(s_kept[q]));
+    }
+}
+
+// A combined map: per key, the presence bits of the pass's queries (low nq bits) and the union
+// attribute field above them (the key's attribute index over the union range + 1, all ones for a
+// corrupt attribute), from the queries' own maps.  Queries with a repeated key are left out.
+struct SqsCombineArgs {
+    uint32_t span, nq, lb, ab, words;
+    const uint32_t *qmap[kSqsMaxQ];  // each query's map of this dimension
+    const uint32_t *qflags[kSqsMaxQ];
+    uint32_t qlb[kSqsMaxQ], qadd[kSqsMaxQ], qerrv[kSqsMaxQ];  // qadd: query's attribute minimum - union's
+    uint32_t *out;
+};
+
+__global__ __launch_bounds__(kBlock) void k_sqs_combine(SqsCombineArgs a) {
+    uint32_t live = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kSqsMaxQ; ++q)
+        if (q < a.nq && a.qflags[q][kFlagDup] == 0u) live |= 1u << q;
+    const uint32_t per = 32u >> a.lb;
+    const uint32_t fall = (1u << a.ab) - 1u;
+    for (uint32_t w = blockIdx.x * kBlock + threadIdx.x; w < a.words; w += gridDim.x * kBlock) {
+        uint32_t word = 0;
+        for (uint32_t j = 0; j < per; ++j) {
+            const uint32_t idx = w * per + j;
+            if (idx >= a.span) break;
+            uint32_t pres = 0, f = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < kSqsMaxQ; ++q) {
+                if (((live >> q) & 1u) == 0u) continue;
+                const uint32_t v = map_entry(a.qmap[q][idx >> (5u - a.qlb[q])], idx, a.qlb[q]);
+                if (v == 0u) continue;
+                pres |= 1u << q;
+                if (a.ab != 0u) f = v == a.qerrv[q] ? fall : v + a.qadd[q];
+            }
+            word |= (pres | (f << a.nq)) << (j << a.lb);
+        }
+        a.out[w] = word;
+    }
+}
+
 // ---- finalize ---------------------------------------------------------------------------------
 
 struct SqAttrK {
@@ -1924,8 +2460,10 @@ namespace {
 // descriptor block per pass (at most nq).
 struct SqbLayout {
     std::vector<size_t> off;
-    size_t off_qdesc = 0, total = 0;
+    size_t off_qdesc = 0, off_sqs = 0, total = 0;
 };
+
+size_t sqs_region_bytes(const ahead_sq_args *const *qs, uint32_t nq);
 
 SqbLayout sqb_layout(const ahead_sq_args *const *qs, uint32_t nq) {
     SqbLayout B;
@@ -1936,6 +2474,8 @@ SqbLayout sqb_layout(const ahead_sq_args *const *qs, uint32_t nq) {
     }
     B.off_qdesc = off;
     off += static_cast<size_t>(nq) * align_up(sizeof(SqbDesc), 256);
+    B.off_sqs = off;
+    off += nq > 1 ? sqs_region_bytes(qs, nq) : 0;
     B.total = off;
     return B;
 }
@@ -2149,6 +2689,275 @@ int sqb_run_pass(std::vector<SqQueryPlan> &plans, SqbPass &ps, SqbDesc *ddesc, u
     return AHEAD_OK;
 }
 
+// ---- same-shape passes ----
+
+bool same_opt_col(const ahead_col &x, const ahead_col &y) {
+    if (x.data == nullptr || y.data == nullptr) return x.data == y.data;
+    return same_col(x, y);
+}
+
+// Whether two planned queries have one shape: the same fact length, dimensions (FK, key and group
+// attribute columns, join forms, key ranges), fact predicate columns and measures.
+bool same_shape(const SqQueryPlan &X, const SqQueryPlan &Y) {
+    const ahead_sq_args *x = X.q, *y = Y.q;
+    if (x->n_fact != y->n_fact || x->n_dims != y->n_dims || x->n_preds != y->n_preds ||
+        x->meas_kind != y->meas_kind || x->meas_pre_mask != y->meas_pre_mask)
+        return false;
+    for (uint32_t d = 0; d < x->n_dims; ++d) {
+        const ahead_sq_dim &a = x->dims[d], &b = y->dims[d];
+        if (!same_col(a.fk, b.fk) || !same_col(a.key, b.key) || !same_opt_col(a.attr, b.attr) || a.flags != b.flags)
+            return false;
+        if (X.P[d].kmin != Y.P[d].kmin || X.P[d].span != Y.P[d].span) return false;
+    }
+    for (uint32_t p = 0; p < x->n_preds; ++p)
+        if (!same_col(x->preds[p].col, y->preds[p].col)) return false;
+    const uint32_t nm = x->meas_kind == AHEAD_SQ_SUM ? 1u : 2u;
+    for (uint32_t m = 0; m < nm; ++m)
+        if (!same_col(x->meas[m], y->meas[m])) return false;
+    return true;
+}
+
+// One same-shape pass: its queries and, per chain dimension, the union of their attribute ranges.
+struct SqsPass {
+    std::vector<uint32_t> qi;
+    uint32_t groups = 0;
+    uint32_t umin[AHEAD_SQ_MAX_DIMS] = {}, umax[AHEAD_SQ_MAX_DIMS] = {};
+    bool closed = false;  // a query that no same-shape pass takes: it runs in a union pass
+};
+
+uint32_t field_bits(uint64_t uspan) {  // bits of a field holding 1 .. uspan and an all-ones error value
+    uint32_t b = 1;
+    while ((1ull << b) < uspan + 2) ++b;
+    return b;
+}
+
+// Adds the query to the pass if its group table, presence bits and union attribute fields fit.
+bool sqs_try_add(SqsPass &ps, uint32_t qi, const std::vector<SqQueryPlan> &plans) {
+    const SqQueryPlan &Q = plans[qi];
+    const ahead_sq_args *q = Q.q;
+    for (uint32_t p = 0; p < q->n_preds; ++p)
+        if (q->preds[p].col.width > 4) return false;  // the shared scan reads codes of at most 4 bytes
+    if (ps.closed || (!ps.qi.empty() && !same_shape(plans[ps.qi[0]], Q))) return false;
+    if (ps.qi.size() >= kSqsMaxQ || ps.groups + Q.groups > kSqMaxGroups) return false;
+    const uint32_t nq = static_cast<uint32_t>(ps.qi.size()) + 1;
+    uint32_t umin[AHEAD_SQ_MAX_DIMS], umax[AHEAD_SQ_MAX_DIMS], fields = 0;
+    for (uint32_t d = 0; d < q->n_dims; ++d) {
+        const DimPlan &pl = Q.P[d];
+        const uint32_t hi = pl.amin + (pl.aspan ? pl.aspan - 1 : 0);
+        umin[d] = ps.qi.empty() ? pl.amin : std::min(ps.umin[d], pl.amin);
+        umax[d] = ps.qi.empty() ? hi : std::max(ps.umax[d], hi);
+        if (q->dims[d].attr.data == nullptr) continue;
+        const uint32_t ab = field_bits(static_cast<uint64_t>(umax[d]) - umin[d] + 1);
+        if (nq + ab > 32) return false;
+        fields += ab;
+    }
+    if (fields > 32) return false;
+    ps.qi.push_back(qi);
+    ps.groups += static_cast<uint32_t>(Q.groups);
+    std::copy(umin, umin + q->n_dims, ps.umin);
+    std::copy(umax, umax + q->n_dims, ps.umax);
+    return true;
+}
+
+// Workspace of the same-shape passes (one at a time, in stream order): descriptors, predicate
+// tables, combined maps sized for the largest query's key spans at 32-bit entries.
+size_t sqs_region_bytes(const ahead_sq_args *const *qs, uint32_t nq) {
+    size_t maps = 0;
+    for (uint32_t i = 0; i < nq; ++i) {
+        const SqLayout L = sq_layout(qs[i]);
+        size_t m = 0;
+        for (uint32_t d = 0; d < qs[i]->n_dims; ++d) m += align_up(static_cast<size_t>(L.span_cap[d]) * 4, 256);
+        maps = std::max(maps, m);
+    }
+    return align_up(sizeof(SqsDesc), 256) + align_up(sizeof(SqsTab), 256) + maps;
+}
+
+template <int ND, int NP>
+void launch_sqs_w(const SqsArgs &a, const SqsDesc *d, uint32_t pwb, uint32_t grid, hipStream_t s) {
+    if (pwb == 2) hipLaunchKernelGGL((k_sqs_scan<ND, NP, 2>), dim3(grid), dim3(kSqBlock), 0, s, a, d);
+    else hipLaunchKernelGGL((k_sqs_scan<ND, NP, 0>), dim3(grid), dim3(kSqBlock), 0, s, a, d);
+}
+
+template <int ND>
+void launch_sqs_np(const SqsArgs &a, const SqsDesc *d, uint32_t pwb, uint32_t grid, hipStream_t s) {
+    switch (a.np) {
+        case 0: hipLaunchKernelGGL((k_sqs_scan<ND, 0, 0>), dim3(grid), dim3(kSqBlock), 0, s, a, d); break;
+        case 1: launch_sqs_w<ND, 1>(a, d, pwb, grid, s); break;
+        default: launch_sqs_w<ND, 2>(a, d, pwb, grid, s); break;
+    }
+}
+
+// Builds the pass's queries' own maps, the combined maps, descriptors and predicate tables, and
+// launches the shared scan.
+int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *region, uint32_t grid, hipStream_t s) {
+    int st;
+    const uint32_t nq = static_cast<uint32_t>(ps.qi.size());
+    for (uint32_t i : ps.qi) {  // own maps in HBM: the chain-order path and the combine kernels read them
+        SqQueryPlan &Q = plans[i];
+        for (DimPlan &pl : Q.P) pl.in_lds = false;
+        if ((st = sq_build(Q, s)) != AHEAD_OK) return st;
+    }
+    const SqQueryPlan &Q0 = plans[ps.qi[0]];
+    const ahead_sq_args *q0 = Q0.q;
+    const uint32_t nd = q0->n_dims;
+    SqsDesc *ddesc = reinterpret_cast<SqsDesc *>(region);
+    SqsTab *dtab = reinterpret_cast<SqsTab *>(region + align_up(sizeof(SqsDesc), 256));
+    uint32_t *cmaps = reinterpret_cast<uint32_t *>(region + align_up(sizeof(SqsDesc), 256) + align_up(sizeof(SqsTab), 256));
+
+    // combined map geometry per chain dimension
+    uint32_t ab[AHEAD_SQ_MAX_DIMS] = {}, lb[AHEAD_SQ_MAX_DIMS] = {}, words[AHEAD_SQ_MAX_DIMS] = {};
+    double frac[AHEAD_SQ_MAX_DIMS] = {};
+    for (uint32_t d = 0; d < nd; ++d) {
+        ab[d] = q0->dims[d].attr.data != nullptr ? field_bits(static_cast<uint64_t>(ps.umax[d]) - ps.umin[d] + 1) : 0u;
+        while ((1u << lb[d]) < nq + ab[d]) ++lb[d];
+        const uint32_t per = 32u >> lb[d];
+        words[d] = (Q0.P[d].span + per - 1) / per;
+        for (uint32_t i : ps.qi) frac[d] += plans[i].P[d].frac;
+        frac[d] = std::min(frac[d], 1.0);
+    }
+    // LDS: most selective (over the union) first while the maps fit; evaluation order: LDS maps
+    // first, each group by selectivity
+    std::vector<uint32_t> by(nd);
+    std::iota(by.begin(), by.end(), 0u);
+    std::stable_sort(by.begin(), by.end(), [&](uint32_t x, uint32_t y) { return frac[x] < frac[y]; });
+    bool lds[AHEAD_SQ_MAX_DIMS] = {};
+    uint32_t lds_words = 0;
+    for (uint32_t d : by) {
+        if (lds_words + words[d] <= kSqsMapWords) {
+            lds[d] = true;
+            lds_words += words[d];
+        }
+    }
+    std::vector<uint32_t> order;
+    for (uint32_t d : by)
+        if (lds[d]) order.push_back(d);
+    for (uint32_t d : by)
+        if (!lds[d]) order.push_back(d);
+
+    SqsArgs a{};
+    a.n = q0->n_fact;
+    a.rows_per_wave = rows_per_wave(a.n, grid);
+    a.nq = nq;
+    a.nd = nd;
+    a.np = q0->n_preds;
+    a.nm = Q0.a.nm;
+    a.kind = Q0.a.kind;
+    a.pre_load = Q0.a.pre_load;
+    uint32_t map_off = 0, ashift = 0;
+    for (uint32_t i = 0; i < nd; ++i) {
+        const uint32_t d = order[i];
+        const ahead_sq_dim &dm = q0->dims[d];
+        SqsDimK &k = a.d[i];
+        k.fk = static_cast<const uint64_t *>(dm.fk.data);
+        k.fk_inv = dm.fk.a_inv;
+        k.fk_maxu = dm.fk.max_u;
+        k.kmask = static_cast<uint32_t>(bits_mask(dm.fk.unenc_bits));
+        k.kmin = Q0.P[d].kmin;
+        k.span = Q0.P[d].span;
+        k.lb = lb[d];
+        k.lds_off = lds[d] ? map_off : kNoLds;
+        k.map_off = map_off;
+        k.ab = ab[d];
+        k.ashift = ashift;
+        k.flags = dm.flags;
+        k.cpos = d;
+        k.row_of = Q0.P[d].prep.row_of;
+        SqsCombineArgs c{};
+        c.span = k.span;
+        c.nq = nq;
+        c.lb = lb[d];
+        c.ab = ab[d];
+        c.words = words[d];
+        for (uint32_t j = 0; j < nq; ++j) {
+            const SqQueryPlan &Q = plans[ps.qi[j]];
+            c.qmap[j] = Q.a.maps + Q.P[d].off_map;
+            c.qflags[j] = Q.flags();
+            c.qlb[j] = Q.P[d].lb;
+            c.qadd[j] = Q.P[d].amin - ps.umin[d];
+            c.qerrv[j] = Q.P[d].prep.errv;
+        }
+        c.out = cmaps + map_off;
+        hipLaunchKernelGGL(k_sqs_combine, dim3(prep_grid(words[d])), dim3(kBlock), 0, s, c);
+        map_off += words[d];
+        ashift += ab[d];
+    }
+    AHEAD_HIP_TRY(hipGetLastError());
+    SqsTab ht{};
+    uint32_t pwb = a.np > 0 ? q0->preds[0].col.width : 0u;
+    for (uint32_t p = 0; p < a.np; ++p) {
+        const SqPredK &k0 = Q0.a.p[p];
+        SqsPredK &k = a.p[p];
+        k.col = k0.col;
+        k.width = k0.width;
+        k.inv = k0.inv;
+        k.maxu = k0.maxu;
+        k.wmask = k0.wmask;
+        k.table = k0.maxu <= 255 ? 1u : 0u;
+        if (k0.width != pwb) pwb = 0;
+        if (!k.table) continue;
+        const uint64_t A = q0->preds[p].col.a;
+        for (uint32_t v = 0; v <= k0.maxu; ++v) {
+            const uint64_t code = (static_cast<uint64_t>(v) * A) & k0.wmask;
+            uint32_t bits = 0;
+            for (uint32_t j = 0; j < nq; ++j)
+                if (pred_pass(plans[ps.qi[j]].a.p[p], code)) bits |= 1u << j;
+            ht.t[p * 256 + v] = bits;
+        }
+    }
+    for (uint32_t m = 0; m < a.nm; ++m) a.m[m] = Q0.a.m[m];
+    a.maps = cmaps;
+    a.lds_words = lds_words;
+    a.ptab = dtab->t;
+
+    SqsDesc hd{};
+    uint32_t goff = 0;
+    for (uint32_t j = 0; j < nq; ++j) {
+        const SqQueryPlan &Q = plans[ps.qi[j]];
+        SqsQ &x = hd.q[j];
+        x.maps = Q.a.maps;
+        for (uint32_t i = 0; i < nd; ++i) {
+            const uint32_t d = order[i];
+            x.map_off[i] = static_cast<uint32_t>(Q.P[d].off_map);
+            x.lb[i] = Q.P[d].lb;
+            x.errv[i] = Q.P[d].prep.errv;
+            for (uint32_t e = 0; e < nd; ++e)
+                if (Q.a.d[e].cpos == d) x.stride[i] = Q.a.d[e].stride;
+            x.off[i] = ab[d] != 0u ? Q.P[d].amin - ps.umin[d] : 0u;
+        }
+        x.groups = static_cast<uint32_t>(Q.groups);
+        x.goff = goff;
+        goff += x.groups;
+        for (uint32_t p = 0; p < a.np; ++p) {
+            SqbPredK &pp = x.p[p];
+            pp.used = 1;
+            pp.never = Q.a.p[p].never;
+            for (int i = 0; i < 3; ++i) {  // codes of at most 4 bytes: the intervals fit 32 bits
+                pp.lo[i] = static_cast<uint32_t>(Q.a.p[p].lo[i]);
+                pp.len[i] = static_cast<uint32_t>(Q.a.p[p].len[i]);
+            }
+        }
+        x.flags = Q.a.flags;
+        x.ev_count = Q.a.ev_count;
+        x.events = Q.a.events;
+        x.gsum = Q.a.gsum;
+        x.gfirst = Q.a.gfirst;
+        x.kept = Q.a.kept;
+    }
+    hipLaunchKernelGGL((k_sq_put<SqsDesc>), dim3(1), dim3(64), 0, s, ddesc, hd);
+    hipLaunchKernelGGL((k_sq_put<SqsTab>), dim3(1), dim3(64), 0, s, dtab, ht);
+    AHEAD_HIP_TRY(hipGetLastError());
+    if (a.n > 0) {
+        switch (nd) {
+            case 1: launch_sqs_np<1>(a, ddesc, pwb, grid, s); break;
+            case 2: launch_sqs_np<2>(a, ddesc, pwb, grid, s); break;
+            case 3: launch_sqs_np<3>(a, ddesc, pwb, grid, s); break;
+            default: launch_sqs_np<4>(a, ddesc, pwb, grid, s); break;
+        }
+        AHEAD_HIP_TRY(hipGetLastError());
+    }
+    return AHEAD_OK;
+}
+
 }  // namespace
 
 extern "C" size_t ahead_hip_star_query_batch_ws_bytes(const ahead_sq_args *const *qs, uint32_t nq) {
@@ -2189,7 +2998,9 @@ extern "C" int ahead_hip_star_query_batch(const ahead_sq_args *const *qs, const 
         AHEAD_HIP_TRY(hipMemcpyAsync(hs + AHEAD_SQ_MAX_DIMS * kStatWords, plans[i].flags(), 16, hipMemcpyDeviceToHost, s));
     }
     AHEAD_HIP_TRY(hipStreamSynchronize(s));
-    std::vector<SqbPass> passes;
+    // queries of one shape share a scan (same-shape passes of two or more); the others are packed
+    // in order into passes over the union of their columns
+    std::vector<SqsPass> spasses;
     for (uint32_t i = 0; i < nq; ++i) {
         SqQueryPlan &Q = plans[i];
         const uint32_t *hs = h.data() + static_cast<size_t>(i) * (AHEAD_SQ_MAX_DIMS * kStatWords + 4);
@@ -2198,6 +3009,26 @@ extern "C" int ahead_hip_star_query_batch(const ahead_sq_args *const *qs, const 
             if ((st = sq_not_fused(Q, s)) != AHEAD_OK) return st;
             continue;
         }
+        bool added = false;
+        for (SqsPass &sp : spasses)
+            if ((added = sqs_try_add(sp, i, plans))) break;
+        if (!added) {
+            spasses.emplace_back();
+            if (!sqs_try_add(spasses.back(), i, plans)) {
+                spasses.back().qi.push_back(i);
+                spasses.back().closed = true;
+            }
+        }
+    }
+    std::vector<uint32_t> rest;
+    for (const SqsPass &sp : spasses)
+        if (sp.qi.size() == 1) rest.push_back(sp.qi[0]);
+    std::sort(rest.begin(), rest.end());
+    spasses.erase(std::remove_if(spasses.begin(), spasses.end(), [](const SqsPass &sp) { return sp.qi.size() < 2; }),
+                  spasses.end());
+    std::vector<SqbPass> passes;
+    for (uint32_t i : rest) {
+        SqQueryPlan &Q = plans[i];
         if (passes.empty() || !sqb_try_add(passes.back(), i, Q)) {
             passes.emplace_back();
             if (!sqb_try_add(passes.back(), i, Q)) return fail(AHEAD_ERR_DEVICE, "star_query_batch: a query fits no pass");
@@ -2206,6 +3037,8 @@ extern "C" int ahead_hip_star_query_batch(const ahead_sq_args *const *qs, const 
     const uint32_t grid = sq_grid();
     const ahead_sq_args *q0 = qs[0];
     if (q0->ev_scan[0] != nullptr) AHEAD_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(q0->ev_scan[0]), s));
+    for (const SqsPass &sp : spasses)
+        if ((st = sqs_run_pass(plans, sp, base + B.off_sqs, grid, s)) != AHEAD_OK) return st;
     size_t desc = 0;  // descriptor blocks of the passes, one after another
     for (SqbPass &ps : passes) {
         SqbDesc *d = reinterpret_cast<SqbDesc *>(base + B.off_qdesc + desc * align_up(sizeof(SqbDesc), 256));
@@ -2213,6 +3046,8 @@ extern "C" int ahead_hip_star_query_batch(const ahead_sq_args *const *qs, const 
         ++desc;
     }
     if (q0->ev_scan[1] != nullptr) AHEAD_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(q0->ev_scan[1]), s));
+    for (const SqsPass &sp : spasses)
+        for (uint32_t i : sp.qi) hipLaunchKernelGGL(k_sq_final, dim3(1), dim3(kSqBlock), 0, s, plans[i].f);
     for (const SqbPass &ps : passes)
         for (uint32_t i : ps.qi) hipLaunchKernelGGL(k_sq_final, dim3(1), dim3(kSqBlock), 0, s, plans[i].f);
     AHEAD_HIP_TRY(hipGetLastError());

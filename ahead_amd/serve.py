@@ -9,18 +9,24 @@ head of the arrival queue starts as soon as a slot is free and the device memory
 budget beside the queries running; later arrivals wait behind it even if they would fit.
 Latency runs from arrival to the query's results on the host.
 
-Two execution forms:
-  fused  each query is a fused star query (ssb.fused): the dimension selections, then one pass over
-         the fact rows; it needs its workspace (dimension maps, group tables, as
-         ahead_hip_star_query_ws_bytes reports) and its output buffers;
-  chain  each query is its operator chain (ssb.chain), one launch per operator, every operator
-         waiting for its counts: the plain path; it needs its materialised intermediates (a bound
-         per fact row measured at 600 M rows).
+Three execution forms:
+  fused    each query is a fused star query (ssb.fused): the dimension selections, then one pass
+           over the fact rows; it needs its workspace (dimension maps, group tables, as
+           ahead_hip_star_query_ws_bytes reports) and its output buffers;
+  batched  a free slot takes the query at the head of the queue together with the queued queries
+           of the same shape behind it (at most MAX_BATCH), and runs them as one batch
+           (ssb.fused_batch): queries of one shape share one pass over the fact rows; each result
+           is the one the query gets alone; a batch needs its queries' workspaces and one more for
+           the shared scan's combined maps;
+  chain    each query is its operator chain (ssb.chain), one launch per operator, every operator
+           waiting for its counts: the plain path; it needs its materialised intermediates (a bound
+           per fact row measured at 600 M rows).
 """
 from __future__ import annotations
 
 import threading
 import time
+from collections import deque
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
@@ -33,6 +39,8 @@ from . import ssb
 HBM_PER_ROW = {"P1": 16, "P2": 20, "P3": 20, "P4": 23}
 # the fused form's output buffers (4096 groups x 6 arrays, 16 error vectors of 1024 positions)
 FUSED_OUT_BYTES = 4096 * 8 * 6 + 16 * 1024 * 8 + 4096
+# queries per batch in the batched form (ahead_hip_star_query_batch's same-shape pass)
+MAX_BATCH = 8
 
 
 @dataclass
@@ -118,21 +126,39 @@ class Admission:
             self._cv.notify_all()
 
 
+def form_batch(queue: deque, max_batch: int) -> list:
+    """Takes the query at the head of the queue and the queued queries of its shape behind it, at
+    most max_batch in all, in arrival order; the other queries stay queued in their order."""
+    head = queue.popleft()
+    batch, rest = [head], deque()
+    while queue and len(batch) < max_batch:
+        q = queue.popleft()
+        (batch if q.shape == head.shape else rest).append(q)
+    rest.extend(queue)
+    queue.clear()
+    queue.extend(rest)
+    return batch
+
+
 class Server:
     """A long-lived query server: its worker threads, their streams and their workspaces persist
     across `run` calls (close() ends them), so device memory cached for one stream's queries is
     reused by that stream's later queries."""
 
     def __init__(self, tables: dict, n_fact: int, workers: int = 4, hbm_budget: float = 64e9, mode: str = "fused"):
-        assert mode in ("fused", "chain")
+        assert mode in ("fused", "batched", "chain")
         self.tables = tables
         self.n_fact = n_fact
         self.workers = workers
         self.mode = mode
+        self.max_batch = MAX_BATCH if mode == "batched" else 1
         self.admission = Admission(hbm_budget)
         self._need_cache: dict = {}
         self._local = threading.local()
         self._ex = ThreadPoolExecutor(max_workers=workers)
+        self._queue: deque = deque()
+        self._qcv = threading.Condition()
+        self._feeding = False
 
     def close(self) -> None:
         self._ex.shutdown(wait=True)
@@ -151,48 +177,81 @@ class Server:
             s = self._local.stream = torch.cuda.Stream()
         return s
 
-    def need(self, shape: str) -> float:
-        """Device bytes a query of this shape holds while it runs: for the fused form its workspace
-        (as the library computes it from the dimension sizes) and outputs, for the chain the bound
-        of its intermediates."""
+    def need(self, shape: str, n: int = 1) -> float:
+        """Device bytes a query of this shape (a batch of n of them) holds while it runs: for the
+        fused forms the workspace (as the library computes it from the dimension sizes; a batch
+        one more for the shared scan's combined maps) and outputs, for the chain the bound of its
+        intermediates."""
         if self.mode == "chain":
             return HBM_PER_ROW[shape] * self.n_fact
         if shape not in self._need_cache:
             spec = SHAPE_DEFAULTS[shape]()
             self._need_cache[shape] = ssb.fused_ws_bytes(self.tables, spec) + FUSED_OUT_BYTES
-        return self._need_cache[shape]
+        return self._need_cache[shape] * (n + 1 if n > 1 else 1)
 
-    def _run_one(self, q: Query, ticket: int, t0: float) -> Query:
+    def _execute(self, batch: list) -> list:
+        specs = [q.spec() for q in batch]
+        if self.mode == "chain":
+            return [ssb.chain(self.tables, specs[0])]
+        if len(batch) == 1:
+            return [ssb.fused(self.tables, specs[0])]
+        return ssb.fused_batch(self.tables, specs)
+
+    def _run_batch(self, batch: list) -> list:
+        """Runs the batch on this slot's stream; returns once its results are on the host."""
         import torch
 
-        need = self.need(q.shape)
-        self.admission.acquire(ticket, need)
-        try:
-            q.start = time.perf_counter() - t0
-            s = self._stream()
-            with torch.cuda.stream(s):
-                spec = q.spec()
-                q.result = ssb.fused(self.tables, spec) if self.mode == "fused" else ssb.chain(self.tables, spec)
-            s.synchronize()
-            q.finish = time.perf_counter() - t0
-        finally:
-            self.admission.release(need)
-        return q
+        s = self._stream()
+        with torch.cuda.stream(s):
+            results = self._execute(batch)
+        s.synchronize()
+        return results
+
+    def _worker(self, t0: float) -> None:
+        """A slot: takes batches from the queue in arrival order until the feed ends and the queue
+        is empty; each batch is admitted first come, first served, and runs on the slot's stream."""
+        while True:
+            with self._qcv:
+                while not self._queue and self._feeding:
+                    self._qcv.wait()
+                if not self._queue:
+                    return
+                batch = form_batch(self._queue, self.max_batch)
+                ticket = self.admission.ticket()
+            need = self.need(batch[0].shape, len(batch))
+            self.admission.acquire(ticket, need)
+            try:
+                start = time.perf_counter() - t0
+                results = self._run_batch(batch)
+                finish = time.perf_counter() - t0
+                for q, r in zip(batch, results):
+                    q.start, q.finish, q.result = start, finish, r
+            finally:
+                self.admission.release(need)
 
     def run(self, queries: list[Query], realtime: bool = True) -> list[Query]:
-        """Replays the arrivals (in real time, or all at once) and returns the queries, completed."""
+        """Replays the arrivals (in real time, or all at once) into the queue the slots serve, and
+        returns the queries, completed."""
         for shape in {q.shape for q in queries}:
             self.need(shape)  # computed once per shape, before the clock starts
         t0 = time.perf_counter()
-        futs = []
-        for q in queries:
-            if realtime:
-                delay = q.arrival - (time.perf_counter() - t0)
-                if delay > 0:
-                    time.sleep(delay)
-            else:
-                q.arrival = time.perf_counter() - t0
-            futs.append(self._ex.submit(self._run_one, q, self.admission.ticket(), t0))
+        self._feeding = True
+        futs = [self._ex.submit(self._worker, t0) for _ in range(self.workers)]
+        try:
+            for q in queries:
+                if realtime:
+                    delay = q.arrival - (time.perf_counter() - t0)
+                    if delay > 0:
+                        time.sleep(delay)
+                else:
+                    q.arrival = time.perf_counter() - t0
+                with self._qcv:
+                    self._queue.append(q)
+                    self._qcv.notify()
+        finally:
+            with self._qcv:
+                self._feeding = False
+                self._qcv.notify_all()
         for f in futs:
             f.result()
         return queries

@@ -1,7 +1,8 @@
-"""Served queries (ahead_amd.serve), in the fused form and as operator chains: every query's result,
-error vectors included, is the same when it runs alone (one query on the GPU at a time) and in the
-mix (four queries overlapping on four streams, all arriving at once), and both equal the oracle's
-operator chain for the query; the admission budget serialises queries that do not fit together."""
+"""Served queries (ahead_amd.serve), in the fused form, in same-shape batches and as operator chains:
+every query's result, error vectors included, is the same when it runs alone (one query or batch on
+the GPU at a time) and in the mix (four slots overlapping on four streams, all arriving at once),
+and both equal the oracle's operator chain for the query; the admission budget serialises queries
+(batches) that do not fit together."""
 import pytest
 
 import oracle_bat as OB
@@ -12,7 +13,7 @@ pytestmark = pytest.mark.gpu
 SMALL = dict(n_fact=250_000, n_date=2556, n_cust=3_000, n_supp=200, n_part=1_400)
 
 
-@pytest.mark.parametrize("mode", ["fused", "chain"])
+@pytest.mark.parametrize("mode", ["fused", "batched", "chain"])
 def test_results_alone_equal_results_in_the_mix(mode):
     spec = ssb.SsbSpec(rate=2e-3, seed=21, **SMALL)
     tables = ssb.device_tables(spec)
@@ -31,7 +32,7 @@ def test_results_alone_equal_results_in_the_mix(mode):
         assert serve.same_result(a.result, ref), ("alone", a.qid, a.shape, a.params)
         assert serve.same_result(m.result, ref), ("mixed", a.qid, a.shape, a.params)
         assert serve.same_result(again[a.qid].result, ref), ("mixed again", a.qid, a.shape, a.params)
-        if mode == "fused":
+        if mode != "chain":
             assert a.result["status"] == 0 and m.result["status"] == 0
         shapes.add(a.shape)
     assert shapes == {"P1", "P2", "P3", "P4"}
@@ -39,7 +40,7 @@ def test_results_alone_equal_results_in_the_mix(mode):
     assert s["queries"] == 24 and s["p99_ms"] >= s["p50_ms"] > 0
 
 
-@pytest.mark.parametrize("mode", ["fused", "chain"])
+@pytest.mark.parametrize("mode", ["fused", "batched", "chain"])
 def test_admission_budget_serializes_when_memory_is_short(mode):
     spec = ssb.SsbSpec(rate=0.0, seed=22, **SMALL)
     tables = ssb.device_tables(spec)
@@ -50,7 +51,10 @@ def test_admission_budget_serializes_when_memory_is_short(mode):
         # a budget below any two needs together (a need above it is capped at it): no overlap
         srv.admission.budget = 2 * min(needs.values()) - 1
         qs = srv.run(serve.random_queries(8, 0, 6), realtime=False)
-    assert [q.start for q in qs] == sorted(q.start for q in qs)   # admitted in arrival order
-    spans = sorted((q.start, q.finish) for q in qs)
+    heads = {}  # start of each batch (one query each, but in the batched form) -> its first query
+    for q in qs:
+        heads.setdefault(q.start, q.qid)
+    assert list(heads) == sorted(heads)   # batches admitted in the arrival order of their first queries
+    spans = sorted({(q.start, q.finish) for q in qs})
     for (s0, f0), (s1, f1) in zip(spans, spans[1:]):
         assert s1 >= f0 - 1e-4

@@ -199,3 +199,76 @@ def test_batch_reports_each_querys_status():
     assert [g["status"] for g in got] == [0, 2, 2, 0]
     for q, g in zip(qs, got):
         same(g, ssb.chain(t3, q))
+
+
+# ---- same-shape shared scans ------------------------------------------------------------------
+
+def _instances(shape: str, n: int, seed: int) -> list:
+    """n queries of one serve shape with seeded random constants."""
+    from ahead_amd import serve
+
+    qs = [q for q in serve.random_queries(12 * n, 0.0, seed) if q.shape == shape][:n]
+    assert len(qs) == n
+    return [q.spec() for q in qs]
+
+
+@pytest.mark.parametrize("shape", ["P1", "P2", "P3", "P4"])
+def test_same_shape_batch_matches_oracle_with_flips_everywhere(shape):
+    """Eight queries of one shape with different constants share one scan (combined dimension maps,
+    predicate pass-bit tables); each query's result, group order, first rows and error vectors
+    equal the oracle's for that query alone, with flips in every fact column and attribute."""
+    spec = ssb.SsbSpec(rate=1e-3, seed=51, flip_cols=ALL_FLIPS, **SMALL)
+    qs = _instances(shape, 8, 52)
+    print(f"star_query_batch 8 x {shape} vs oracle, {spec.n_fact} rows", flush=True)
+    got = ssb.fused_batch(ssb.device_tables(spec), qs, fallback=False)
+    host = OB.tables(spec)
+    for i, (q, g) in enumerate(zip(qs, got)):
+        ref = ssb.fused(host, q, OB, fallback=False)
+        assert g["status"] == 0 and ref["status"] == 0, i
+        same(g, ref)
+        assert np.array_equal(g["first_row"], ref["first_row"]), i
+
+
+def test_same_shape_passes_mixed_with_other_queries():
+    """Eleven P2 queries (passes of eight and three), five P3 (two of them the same query), one P4
+    (a union pass with the other thirteen queries' leftovers) and Q1.2: each equals the same query
+    run alone on the GPU, with flips in every fact column and attribute."""
+    spec = ssb.SsbSpec(rate=2e-3, seed=53, flip_cols=ALL_FLIPS, **SMALL)
+    t = ssb.device_tables(spec)
+    p3 = _instances("P3", 3, 55)
+    qs = _instances("P2", 11, 54) + p3 + [p3[1], p3[1]] + _instances("P4", 1, 56) + [ssb.QUERIES["Q1.2"]]
+    print(f"star_query_batch of {len(qs)} queries (same-shape passes) vs single queries", flush=True)
+    got = ssb.fused_batch(t, qs, fallback=False)
+    for i, (q, g) in enumerate(zip(qs, got)):
+        ref = ssb.fused(t, q, fallback=False)
+        assert g["status"] == 0 and ref["status"] == 0, i
+        same(g, ref)
+        assert np.array_equal(g["first_row"], ref["first_row"]), i
+
+
+def test_same_shape_batch_tiny_fact_tables_and_repeated_keys():
+    """Same-shape passes over fact tables of 0, 1, 511 and 513 rows against the oracle; over a
+    supplier dimension with a repeated key every query of the pass reports it (and gets the
+    chain's result with fallback)."""
+    from ahead_amd import bat as B, capi
+
+    for n in (0, 1, 511, 513):
+        tiny = ssb.SsbSpec(rate=0.0, seed=57, n_fact=n, n_date=2556, n_cust=300, n_supp=50, n_part=140)
+        print(f"star_query_batch same shape, tiny fact {n}", flush=True)
+        qs = _instances("P3", 3, 58) + _instances("P1", 2, 59)
+        got = ssb.fused_batch(ssb.device_tables(tiny), qs, fallback=False)
+        host = OB.tables(tiny)
+        for q, g in zip(qs, got):
+            same(g, ssb.fused(host, q, OB, fallback=False))
+    spec = ssb.SsbSpec(rate=0.0, seed=60, **SMALL)
+    t = ssb.device_tables(spec)
+    dup = t["s_suppkey"].tail.to_numpy().copy()
+    dup[11] = dup[5]
+    t2 = dict(t)
+    t2["s_suppkey"] = B.BAT(capi.Column.void(dup.size), capi.Column.from_numpy(dup, t["s_suppkey"].tail.meta))
+    qs = _instances("P3", 3, 61)
+    print("star_query_batch same shape, repeated supplier key", flush=True)
+    got = ssb.fused_batch(t2, qs)
+    assert [g["status"] for g in got] == [3, 3, 3]
+    for q, g in zip(qs, got):
+        same(g, ssb.chain(t2, q))

@@ -62,3 +62,45 @@ def test_admission_caps_a_need_at_the_budget():
     assert adm.in_use == 4.0
     adm.release(100.0)
     assert adm.in_use == 0.0
+
+
+def test_form_batch_takes_the_heads_shape_in_arrival_order():
+    from collections import deque
+
+    qs = [serve.Query(i, s, {}) for i, s in enumerate(["P2", "P1", "P2", "P3", "P2", "P1", "P2"])]
+    q = deque(qs)
+    b = serve.form_batch(q, 3)
+    assert [x.qid for x in b] == [0, 2, 4]            # the head and the next two of its shape
+    assert [x.qid for x in q] == [1, 3, 5, 6]         # the rest, in arrival order
+    b = serve.form_batch(q, 8)
+    assert [x.qid for x in b] == [1, 5] and [x.qid for x in q] == [3, 6]
+    assert [x.qid for x in serve.form_batch(q, 1)] == [3] and [x.qid for x in q] == [6]
+    assert [x.qid for x in serve.form_batch(q, 8)] == [6] and not q
+
+
+def test_batched_server_serves_every_query_once_in_batches_of_one_shape():
+    """The slots' loop on the host (a stand-in for the device work): every query is served exactly
+    once, and each batch holds one shape, at most MAX_BATCH queries, in arrival order."""
+    import threading
+
+    srv = serve.Server({}, 1000, workers=3, mode="batched")
+    srv.need = lambda shape, n=1: 1.0
+    batches, lock = [], threading.Lock()
+
+    def execute(batch):
+        with lock:
+            batches.append([q.qid for q in batch])
+        return [{"rows": q.qid} for q in batch]
+
+    srv._run_batch = execute
+    try:
+        qs = srv.run(serve.random_queries(200, 0, 8), realtime=False)
+    finally:
+        srv.close()
+    seen = sorted(i for b in batches for i in b)
+    assert seen == list(range(200))
+    shape = {q.qid: q.shape for q in qs}
+    for b in batches:
+        assert len(b) <= serve.MAX_BATCH and len({shape[i] for i in b}) == 1 and b == sorted(b)
+    assert all(q.result["rows"] == q.qid and q.finish >= q.start >= q.arrival for q in qs)
+
