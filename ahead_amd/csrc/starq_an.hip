@@ -1161,22 +1161,31 @@ __global__ __launch_bounds__(kBlock) void k_sq_sel(DimPrepArgs a) {
 // ---- same-shape shared scan --------------------------------------------------------------------
 // Queries of one shape (the same dimensions, FK, key and group attribute columns, join forms, fact
 // predicate columns and measures; only their selections and constants differ) share the whole
-// per-row work.  Per dimension one combined map holds, per key, a presence bit per query (the key
-// belongs to that query's selection) and the attribute's index over the union of the queries'
-// attribute ranges; a row costs one lookup per dimension for all of them.  A fact predicate over
-// values of at most 8 bits is a 256-entry table of per-query pass bits, indexed by the decoded
-// value.  What remains per query is the work of its joined rows: the group index in its own
-// numbering and its LDS group sums.  A row with a corrupt code takes each query's chain-order path
-// over that query's own maps (exact_row_s), as in the single-query scan.
+// per-row work.  Per dimension a presence map holds, per key, a bit per query (the key belongs to
+// that query's selection; one bit for all of them when no query selects on the dimension), and
+// for a group attribute the key's attribute index over the union of the queries' attribute ranges
+// (+ 1; all ones: a corrupt attribute) is either a field of the same entry above the presence bits
+// (combined: one lookup) or the entry of a separate attribute map (split: smaller maps, the second
+// lookup only where some query still holds the row).  A row costs one or two lookups per dimension
+// for all queries.  A fact
+// predicate over values of at most 8 bits is a 256-entry table of per-query pass bits, indexed by
+// the decoded value.  What remains per query is the work of its joined rows: the group index in
+// its own numbering and its LDS group sums.  A row with a corrupt code takes each query's
+// chain-order path over that query's own maps (exact_row_s), as in the single-query scan.
 
 constexpr uint32_t kSqsMaxQ = AHEAD_SQ_MAX_BATCH;
-constexpr uint32_t kSqsMapWords = 24320;  // LDS words for the combined maps (beside groups, staging, tables)
+constexpr uint32_t kSqsMapWords = 24320;  // LDS words for the maps (beside groups, staging, tables)
 constexpr uint32_t kSqsQWords = 10;       // per query in LDS: attribute offsets[4], strides[4], groups, group base
 
-struct SqsDimK {  // a dimension of the pass, in evaluation order, over its combined map
+struct SqsDimK {  // a dimension of the pass, in evaluation order
     const uint64_t *fk;
     uint64_t fk_inv, fk_maxu;
-    uint32_t kmask, kmin, span, lb, lds_off, map_off;
+    uint32_t kmask, kmin, span;
+    uint32_t lb, lds_off, map_off;        // presence map: log2 entry bits, LDS offset (or kNoLds), offset
+    uint32_t all;                         // one presence bit for every query (no query selects here)
+    uint32_t pbits;                       // presence bits of an entry (1 or the pass's queries)
+    uint32_t comb;                        // the attribute field sits above the presence bits
+    uint32_t alb, a_lds_off, a_map_off;   // separate attribute map (comb 0), likewise
     uint32_t ab;      // bits of the union attribute field (0: no group attribute)
     uint32_t ashift;  // position of the field in a row's packed attribute word
     uint32_t flags;   // AHEAD_SQ_PROBE | AHEAD_SQ_REFETCH
@@ -1196,7 +1205,7 @@ struct SqsArgs {
     SqsDimK d[AHEAD_SQ_MAX_DIMS];
     SqsPredK p[AHEAD_SQ_MAX_PREDS];
     SqMeasK m[AHEAD_SQ_MAX_MEAS];
-    const uint32_t *maps;  // combined maps; the first lds_words words go to LDS
+    const uint32_t *maps;  // presence and attribute maps; the first lds_words words go to LDS
     uint32_t lds_words;
     const uint32_t *ptab;  // [np][256] pass bits
 };
@@ -1501,8 +1510,8 @@ __global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc 
             }
         }
         // 4. dimensions in evaluation order (LDS maps first): the queries whose selections hold
-        // the key, the union attribute fields; a corrupt attribute sends the row to the
-        // chain-order path of the queries that select its key
+        // the key, then the union attribute field of rows some query still holds; a corrupt
+        // attribute sends the row to the chain-order path of the queries that select its key
         uint32_t attr[R], redo[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -1528,13 +1537,36 @@ __global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc 
 #pragma unroll
                 for (int r = 0; r < R; ++r) w[r] = a.maps[dd.map_off + (idx[r] >> sh)];
             }
+            const uint32_t fmask = (1u << dd.ab) - 1u;
+            const uint32_t pmask = (1u << dd.pbits) - 1u;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const uint32_t e = map_entry(w[r], idx[r], dd.lb);
-                pass[r] &= e;
-                if (dd.ab != 0u && pass[r] != 0u) {
-                    const uint32_t fmask = (1u << dd.ab) - 1u;
-                    const uint32_t f = (e >> a.nq) & fmask;
+                const uint32_t pres = e & pmask;
+                pass[r] &= dd.all ? (pres != 0u ? ~0u : 0u) : pres;
+                if (pass[r] == 0u) idx[r] = 0;
+                if (dd.comb && dd.ab != 0u && pass[r] != 0u) {
+                    const uint32_t f = (e >> dd.pbits) & fmask;
+                    if (f == fmask) {
+                        redo[r] |= pass[r];
+                        pass[r] = 0u;
+                    }
+                    attr[r] |= f << dd.ashift;
+                }
+            }
+            if (!dd.comb && dd.ab != 0u) {
+                const uint32_t ash = 5u - dd.alb;
+                if (dd.a_lds_off != kNoLds) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) w[r] = s_map[dd.a_lds_off + (idx[r] >> ash)];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) w[r] = a.maps[dd.a_map_off + (idx[r] >> ash)];
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (pass[r] == 0u) continue;
+                    const uint32_t f = map_entry(w[r], idx[r], dd.alb);
                     if (f == fmask) {
                         redo[r] |= pass[r];
                         pass[r] = 0u;
@@ -1656,14 +1688,17 @@ __global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc 
     }
 }
 
-// A combined map: per key, the presence bits of the pass's queries (low nq bits) and the union
-// attribute field above them (the key's attribute index over the union range + 1, all ones for a
-// corrupt attribute), from the queries' own maps.  Queries with a repeated key are left out.
+// A presence map (mode 0: per key the bits of the pass's queries whose selections hold it, or one
+// bit for all of them), an attribute map (mode 1: per key its attribute index over the union range
+// + 1, all ones for a corrupt attribute, 0 where no query selects it) or both in one entry (mode 2:
+// the attribute field above pbits presence bits), from the queries' own maps.  Queries with a
+// repeated key are left out.
 struct SqsCombineArgs {
-    uint32_t span, nq, lb, ab, words;
+    uint32_t span, nq, lb, mode, all, pbits, words;
     const uint32_t *qmap[kSqsMaxQ];  // each query's map of this dimension
     const uint32_t *qflags[kSqsMaxQ];
     uint32_t qlb[kSqsMaxQ], qadd[kSqsMaxQ], qerrv[kSqsMaxQ];  // qadd: query's attribute minimum - union's
+    uint32_t fall;                   // the corrupt-attribute value (all ones of the field)
     uint32_t *out;
 };
 
@@ -1673,7 +1708,6 @@ __global__ __launch_bounds__(kBlock) void k_sqs_combine(SqsCombineArgs a) {
     for (uint32_t q = 0; q < kSqsMaxQ; ++q)
         if (q < a.nq && a.qflags[q][kFlagDup] == 0u) live |= 1u << q;
     const uint32_t per = 32u >> a.lb;
-    const uint32_t fall = (1u << a.ab) - 1u;
     for (uint32_t w = blockIdx.x * kBlock + threadIdx.x; w < a.words; w += gridDim.x * kBlock) {
         uint32_t word = 0;
         for (uint32_t j = 0; j < per; ++j) {
@@ -1686,9 +1720,11 @@ __global__ __launch_bounds__(kBlock) void k_sqs_combine(SqsCombineArgs a) {
                 const uint32_t v = map_entry(a.qmap[q][idx >> (5u - a.qlb[q])], idx, a.qlb[q]);
                 if (v == 0u) continue;
                 pres |= 1u << q;
-                if (a.ab != 0u) f = v == a.qerrv[q] ? fall : v + a.qadd[q];
+                f = v == a.qerrv[q] ? a.fall : v + a.qadd[q];
             }
-            word |= (pres | (f << a.nq)) << (j << a.lb);
+            const uint32_t pv = a.all ? (pres != 0u ? 1u : 0u) : pres;
+            const uint32_t e = a.mode == 1 ? f : a.mode == 2 ? pv | (f << a.pbits) : pv;
+            word |= e << (j << a.lb);
         }
         a.out[w] = word;
     }
@@ -2739,7 +2775,6 @@ bool sqs_try_add(SqsPass &ps, uint32_t qi, const std::vector<SqQueryPlan> &plans
         if (q->preds[p].col.width > 4) return false;  // the shared scan reads codes of at most 4 bytes
     if (ps.closed || (!ps.qi.empty() && !same_shape(plans[ps.qi[0]], Q))) return false;
     if (ps.qi.size() >= kSqsMaxQ || ps.groups + Q.groups > kSqMaxGroups) return false;
-    const uint32_t nq = static_cast<uint32_t>(ps.qi.size()) + 1;
     uint32_t umin[AHEAD_SQ_MAX_DIMS], umax[AHEAD_SQ_MAX_DIMS], fields = 0;
     for (uint32_t d = 0; d < q->n_dims; ++d) {
         const DimPlan &pl = Q.P[d];
@@ -2748,10 +2783,10 @@ bool sqs_try_add(SqsPass &ps, uint32_t qi, const std::vector<SqQueryPlan> &plans
         umax[d] = ps.qi.empty() ? hi : std::max(ps.umax[d], hi);
         if (q->dims[d].attr.data == nullptr) continue;
         const uint32_t ab = field_bits(static_cast<uint64_t>(umax[d]) - umin[d] + 1);
-        if (nq + ab > 32) return false;
+        if (ab > 24) return false;
         fields += ab;
     }
-    if (fields > 32) return false;
+    if (fields > 32) return false;  // a row's union attribute fields share one word
     ps.qi.push_back(qi);
     ps.groups += static_cast<uint32_t>(Q.groups);
     std::copy(umin, umin + q->n_dims, ps.umin);
@@ -2760,13 +2795,15 @@ bool sqs_try_add(SqsPass &ps, uint32_t qi, const std::vector<SqQueryPlan> &plans
 }
 
 // Workspace of the same-shape passes (one at a time, in stream order): descriptors, predicate
-// tables, combined maps sized for the largest query's key spans at 32-bit entries.
+// tables, presence maps (at most 8 bits per key) and attribute maps (at most 32) sized for the
+// largest query's key spans.
 size_t sqs_region_bytes(const ahead_sq_args *const *qs, uint32_t nq) {
     size_t maps = 0;
     for (uint32_t i = 0; i < nq; ++i) {
         const SqLayout L = sq_layout(qs[i]);
         size_t m = 0;
-        for (uint32_t d = 0; d < qs[i]->n_dims; ++d) m += align_up(static_cast<size_t>(L.span_cap[d]) * 4, 256);
+        for (uint32_t d = 0; d < qs[i]->n_dims; ++d)
+            m += align_up(static_cast<size_t>(L.span_cap[d]) + 4, 256) + align_up(static_cast<size_t>(L.span_cap[d]) * 4 + 4, 256);
         maps = std::max(maps, m);
     }
     return align_up(sizeof(SqsDesc), 256) + align_up(sizeof(SqsTab), 256) + maps;
@@ -2804,35 +2841,81 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
     SqsTab *dtab = reinterpret_cast<SqsTab *>(region + align_up(sizeof(SqsDesc), 256));
     uint32_t *cmaps = reinterpret_cast<uint32_t *>(region + align_up(sizeof(SqsDesc), 256) + align_up(sizeof(SqsTab), 256));
 
-    // combined map geometry per chain dimension
-    uint32_t ab[AHEAD_SQ_MAX_DIMS] = {}, lb[AHEAD_SQ_MAX_DIMS] = {}, words[AHEAD_SQ_MAX_DIMS] = {};
+    // map geometry per chain dimension: presence (one bit for all queries where none selects),
+    // union attribute field; combined entries (presence | field) or split maps
+    uint32_t ab[AHEAD_SQ_MAX_DIMS] = {}, pbits[AHEAD_SQ_MAX_DIMS] = {};
+    uint32_t lb[AHEAD_SQ_MAX_DIMS] = {}, alb[AHEAD_SQ_MAX_DIMS] = {}, clb[AHEAD_SQ_MAX_DIMS] = {};
+    uint32_t pwords[AHEAD_SQ_MAX_DIMS] = {}, awords[AHEAD_SQ_MAX_DIMS] = {}, cwords[AHEAD_SQ_MAX_DIMS] = {};
+    bool all[AHEAD_SQ_MAX_DIMS] = {}, comb[AHEAD_SQ_MAX_DIMS] = {};
     double frac[AHEAD_SQ_MAX_DIMS] = {};
+    auto words_of = [](uint32_t span, uint32_t l) { return (span + (32u >> l) - 1) / (32u >> l); };
     for (uint32_t d = 0; d < nd; ++d) {
-        ab[d] = q0->dims[d].attr.data != nullptr ? field_bits(static_cast<uint64_t>(ps.umax[d]) - ps.umin[d] + 1) : 0u;
-        while ((1u << lb[d]) < nq + ab[d]) ++lb[d];
-        const uint32_t per = 32u >> lb[d];
-        words[d] = (Q0.P[d].span + per - 1) / per;
+        all[d] = true;
+        for (uint32_t i : ps.qi) all[d] = all[d] && plans[i].q->dims[d].sel == nullptr;
+        pbits[d] = all[d] ? 1u : nq;
+        while ((1u << lb[d]) < pbits[d]) ++lb[d];
+        pwords[d] = words_of(Q0.P[d].span, lb[d]);
+        if (q0->dims[d].attr.data != nullptr) {
+            ab[d] = field_bits(static_cast<uint64_t>(ps.umax[d]) - ps.umin[d] + 1);
+            while ((1u << alb[d]) < ab[d]) ++alb[d];
+            while ((1u << clb[d]) < pbits[d] + ab[d]) ++clb[d];
+            awords[d] = words_of(Q0.P[d].span, alb[d]);
+            cwords[d] = clb[d] <= 5 ? words_of(Q0.P[d].span, clb[d]) : 0u;
+        }
         for (uint32_t i : ps.qi) frac[d] += plans[i].P[d].frac;
         frac[d] = std::min(frac[d], 1.0);
     }
-    // LDS: most selective (over the union) first while the maps fit; evaluation order: LDS maps
-    // first, each group by selectivity
+    // LDS, most selective dimension (over the union) first while maps fit: a dimension with an
+    // attribute takes the combined map if it fits (one LDS lookup), else the split maps if they
+    // fit; in L2 it takes combined entries of at most 16 bits (one lookup, the map at most twice
+    // the presence map), else split maps (the attribute lookup only where some query holds the
+    // row).  Evaluation order: dimensions with an LDS map first, each group by selectivity.
     std::vector<uint32_t> by(nd);
     std::iota(by.begin(), by.end(), 0u);
     std::stable_sort(by.begin(), by.end(), [&](uint32_t x, uint32_t y) { return frac[x] < frac[y]; });
-    bool lds[AHEAD_SQ_MAX_DIMS] = {};
+    bool plds[AHEAD_SQ_MAX_DIMS] = {}, alds[AHEAD_SQ_MAX_DIMS] = {};
     uint32_t lds_words = 0;
     for (uint32_t d : by) {
-        if (lds_words + words[d] <= kSqsMapWords) {
-            lds[d] = true;
-            lds_words += words[d];
+        if (ab[d] == 0u) {
+            if (lds_words + pwords[d] <= kSqsMapWords) plds[d] = true, lds_words += pwords[d];
+        } else if (cwords[d] != 0u && lds_words + cwords[d] <= kSqsMapWords) {
+            comb[d] = plds[d] = true;
+            lds_words += cwords[d];
+        } else if (lds_words + pwords[d] + awords[d] <= kSqsMapWords) {
+            plds[d] = alds[d] = true;
+            lds_words += pwords[d] + awords[d];
+        } else {
+            comb[d] = cwords[d] != 0u && clb[d] <= 4;
+        }
+    }
+    for (uint32_t d : by) {  // a presence map that did not fit with its attribute map may fit alone
+        if (ab[d] != 0u && !comb[d] && !plds[d] && lds_words + pwords[d] <= kSqsMapWords) {
+            plds[d] = true;
+            lds_words += pwords[d];
+        }
+    }
+    for (uint32_t d = 0; d < nd; ++d) {
+        if (comb[d]) {
+            lb[d] = clb[d];
+            pwords[d] = cwords[d];
+            awords[d] = 0;
         }
     }
     std::vector<uint32_t> order;
     for (uint32_t d : by)
-        if (lds[d]) order.push_back(d);
+        if (plds[d]) order.push_back(d);
     for (uint32_t d : by)
-        if (!lds[d]) order.push_back(d);
+        if (!plds[d]) order.push_back(d);
+    // map offsets: LDS maps first, then the others
+    uint32_t poff[AHEAD_SQ_MAX_DIMS] = {}, aoff[AHEAD_SQ_MAX_DIMS] = {}, words = 0;
+    for (uint32_t d : by) {
+        if (plds[d]) poff[d] = words, words += pwords[d];
+        if (alds[d]) aoff[d] = words, words += awords[d];
+    }
+    for (uint32_t d : by) {
+        if (!plds[d]) poff[d] = words, words += pwords[d];
+        if (awords[d] != 0u && !alds[d]) aoff[d] = words, words += awords[d];
+    }
 
     SqsArgs a{};
     a.n = q0->n_fact;
@@ -2843,7 +2926,7 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
     a.nm = Q0.a.nm;
     a.kind = Q0.a.kind;
     a.pre_load = Q0.a.pre_load;
-    uint32_t map_off = 0, ashift = 0;
+    uint32_t ashift = 0;
     for (uint32_t i = 0; i < nd; ++i) {
         const uint32_t d = order[i];
         const ahead_sq_dim &dm = q0->dims[d];
@@ -2855,8 +2938,14 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
         k.kmin = Q0.P[d].kmin;
         k.span = Q0.P[d].span;
         k.lb = lb[d];
-        k.lds_off = lds[d] ? map_off : kNoLds;
-        k.map_off = map_off;
+        k.lds_off = plds[d] ? poff[d] : kNoLds;
+        k.map_off = poff[d];
+        k.all = all[d] ? 1u : 0u;
+        k.pbits = pbits[d];
+        k.comb = comb[d] ? 1u : 0u;
+        k.alb = alb[d];
+        k.a_lds_off = alds[d] ? aoff[d] : kNoLds;
+        k.a_map_off = aoff[d];
         k.ab = ab[d];
         k.ashift = ashift;
         k.flags = dm.flags;
@@ -2865,9 +2954,9 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
         SqsCombineArgs c{};
         c.span = k.span;
         c.nq = nq;
-        c.lb = lb[d];
-        c.ab = ab[d];
-        c.words = words[d];
+        c.all = k.all;
+        c.pbits = pbits[d];
+        c.fall = ab[d] != 0u ? (1u << ab[d]) - 1u : 0u;
         for (uint32_t j = 0; j < nq; ++j) {
             const SqQueryPlan &Q = plans[ps.qi[j]];
             c.qmap[j] = Q.a.maps + Q.P[d].off_map;
@@ -2876,9 +2965,18 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
             c.qadd[j] = Q.P[d].amin - ps.umin[d];
             c.qerrv[j] = Q.P[d].prep.errv;
         }
-        c.out = cmaps + map_off;
-        hipLaunchKernelGGL(k_sqs_combine, dim3(prep_grid(words[d])), dim3(kBlock), 0, s, c);
-        map_off += words[d];
+        c.mode = comb[d] ? 2u : 0u;
+        c.lb = lb[d];
+        c.words = pwords[d];
+        c.out = cmaps + poff[d];
+        hipLaunchKernelGGL(k_sqs_combine, dim3(prep_grid(c.words)), dim3(kBlock), 0, s, c);
+        if (ab[d] != 0u && !comb[d]) {
+            c.mode = 1;
+            c.lb = alb[d];
+            c.words = awords[d];
+            c.out = cmaps + aoff[d];
+            hipLaunchKernelGGL(k_sqs_combine, dim3(prep_grid(c.words)), dim3(kBlock), 0, s, c);
+        }
         ashift += ab[d];
     }
     AHEAD_HIP_TRY(hipGetLastError());
