@@ -66,7 +66,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="ssb", choices=["star1", "select_sum", "ops", "batch", "ssb", "serve",
-                                                                 "hbm_copy", "strategies"])
+                                                                 "hbm_copy", "strategies", "shape_batches"])
     ap.add_argument("--rows", type=int, default=None, help="override the row count (scaling studies)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the host-side checks of the ssb outputs")
@@ -704,6 +704,67 @@ def run_serve(args):
     }
 
 
+def run_shape_batches(args):
+    """Eight queries of each shape (seeded random constants) as one ahead_hip_star_query_batch call
+    (same-shape shared scans) against the same eight one at a time; results asserted equal."""
+    import torch
+
+    from ahead_amd import capi, serve, ssb
+
+    spec = ssb.SsbSpec()
+    if args.rows:
+        spec.n_fact = args.rows
+    t = ssb.device_tables(spec)
+    torch.cuda.synchronize()
+    per_shape, tot_single, tot_batch = {}, 0.0, 0.0
+    ev = capi.HipEvents(9)
+    for shape in SSB_SHAPES:
+        qs = [q.spec() for q in serve.random_queries(200, 0, 5) if q.shape == shape][:8]
+        singles = [ssb.fused(t, q) for q in qs]
+        if not all(serve.same_result(a, b) for a, b in zip(singles, ssb.fused_batch(t, qs))):
+            raise RuntimeError(f"{shape}: a batched query differs from the query alone")
+        best_s = best_b = float("inf")
+        scan_s = scan_b = float("inf")
+        for _ in range(max(1, args.steps)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for i, q in enumerate(qs):
+                ssb.fused(t, q, ev_scan=ev.pairs[i])
+            torch.cuda.synchronize()
+            best_s = min(best_s, time.perf_counter() - t0)
+            scan_s = min(scan_s, sum(ev.ms(i) for i in range(len(qs))))
+            t0 = time.perf_counter()
+            ssb.fused_batch(t, qs, ev_scan=ev.pairs[8])
+            torch.cuda.synchronize()
+            best_b = min(best_b, time.perf_counter() - t0)
+            scan_b = min(scan_b, ev.ms(8))
+        per_shape[shape] = {"queries": len(qs), "single_wall_ms": best_s * 1e3, "batch_wall_ms": best_b * 1e3,
+                            "single_scans_ms": scan_s, "batch_maps_and_scan_ms": scan_b}
+        tot_single += best_s
+        tot_batch += best_b
+        print(f"shape_batches: {shape}: 8 single {best_s * 1e3:.1f} ms, one batch {best_b * 1e3:.1f} ms",
+              file=sys.stderr, flush=True)
+    ev.close()
+    nq = 8 * len(SSB_SHAPES)
+    return {
+        "metric": "star-schema queries/s, eight queries of one shape per ahead_hip_star_query_batch call",
+        "value": nq / tot_batch,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": tot_batch * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16/u32/u64",
+        "data": "synthetic (seeded; fact columns generated on device)",
+        "config": {"workload": f"shape_batches: ssb schema with {spec.n_fact} fact rows, 8 queries per shape P1-P4",
+                   "single_queries_per_s": nq / tot_single, "per_shape": per_shape},
+        "roofline": None,
+    }
+
+
 def run_strategies(args):
     """The four SSB query shapes under each of the reference's strategies, one after another: the
     hardened chain (continuous), early and late check-and-decode, the unhardened chain (normal),
@@ -980,7 +1041,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs an MI355X (cuda:0)"
     torch.cuda.set_device(0)
     run = {"star1": run_star1, "select_sum": run_select_sum, "batch": run_batch, "ssb": run_ssb,
-           "serve": run_serve, "hbm_copy": run_hbm_copy, "ops": run_ops, "strategies": run_strategies}[args.workload]
+           "serve": run_serve, "hbm_copy": run_hbm_copy, "ops": run_ops, "strategies": run_strategies,
+           "shape_batches": run_shape_batches}[args.workload]
     res = run(args)
     print(json.dumps(res), flush=True)
 

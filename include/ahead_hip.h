@@ -415,10 +415,13 @@ int ahead_hip_star_query(const ahead_sq_args *q, const ahead_sq_out *out, ahead_
 size_t ahead_hip_star_query_ws_bytes(const ahead_sq_args *q);
 
 /* Several star queries over one fact table (same n_fact) with shared fact scans: the queries'
- * dimension statistics are read back together (one wait), then queries are packed in order into
- * passes of at most AHEAD_SQ_MAX_BATCH queries whose fact FK columns (at most 4 distinct) and
- * predicate columns (at most 2 distinct) fit one scan and whose group tables fit its LDS; each
- * pass reads those columns once for all its queries.  Each query's outputs, error vectors and
+ * dimension statistics are read back together (one wait).  Queries of one shape (the same
+ * dimensions, FK / key / attribute columns, join flags, predicate columns of at most 4 bytes and
+ * measures) are grouped into passes of at most AHEAD_SQ_MAX_BATCH whose group tables fit one LDS
+ * table: a row takes one lookup per dimension for all of a pass's queries (per-key presence bits
+ * and a union attribute index merged from the queries' own maps).  The other queries are packed
+ * in order into passes whose fact FK columns (at most 4 distinct) and predicate columns (at most 2
+ * distinct) fit one scan; each such pass reads those columns once for all its queries.  Each query's outputs, error vectors and
  * status are exactly those of ahead_hip_star_query with the same arguments; a query whose fused
  * form does not apply gets its status and takes no part in the scans.  No counterpart in the
  * reference (its drivers run one query at a time): it serves concurrent queries of a
