@@ -213,6 +213,13 @@ class DeviceBuffer:
     def ptr(self) -> int:
         return self.t.data_ptr()
 
+    def view(self, offset: int, nbytes: int) -> "DeviceBuffer":
+        """nbytes of this buffer from offset on, sharing its memory."""
+        b = DeviceBuffer.__new__(DeviceBuffer)
+        b.t = self.t[int(offset): int(offset) + max(int(nbytes), 1)]
+        b.nbytes = int(nbytes)
+        return b
+
     def prefix_copy(self, nbytes: int) -> "DeviceBuffer":
         """A new buffer holding the first nbytes (a device copy on the current stream)."""
         b = DeviceBuffer.__new__(DeviceBuffer)
@@ -603,7 +610,7 @@ class StarQuery:
 
     def __init__(self, n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_mask: int = 0,
                  a_result: int = an.A_RESULT, a_oid_err: int = an.A_OID_ERR, groups_cap: int = 4096,
-                 err_cap: int = 1024, ev_scan: tuple | None = None):
+                 err_cap: int = 1024, ev_scan: tuple | None = None, out_arena: tuple | None = None):
         self.keep = (dims, preds, meas)  # the columns and selections stay alive with the query
         a = SqArgs()
         a.n_fact = n_fact
@@ -634,15 +641,34 @@ class StarQuery:
         self.attr_dims = [i for i, d in enumerate(dims) if d.get("attr") is not None]
         self.groups_cap = groups_cap
         self.err_cap = err_cap
-        self._alloc()
+        self._alloc(out_arena)
 
-    def _alloc(self):
+    @staticmethod
+    def out_bytes(n_attr: int, groups_cap: int, err_cap: int) -> int:
+        """Bytes of one query's outputs carved from an arena (counts, sums, first rows, attribute
+        codes, error vectors), 256-aligned."""
+        b = 8 * (4 + SQ_KINDS) + 8 * groups_cap * (2 + n_attr) + 8 * err_cap * SQ_KINDS
+        return (b + 255) // 256 * 256
+
+    def _alloc(self, arena: tuple | None = None):
+        """The output buffers: own allocations, or consecutive views of arena = (DeviceBuffer,
+        offset) in the order out_bytes counts them."""
         g, e = self.groups_cap, self.err_cap
-        self.sums = DeviceBuffer(8 * g)
-        self.first = DeviceBuffer(8 * g)
-        self.keys = {d: DeviceBuffer(8 * g) for d in self.attr_dims}
-        self.errs = [DeviceBuffer(8 * e) for _ in range(SQ_KINDS)]
-        self.counts = DeviceBuffer(8 * (4 + SQ_KINDS))
+        if arena is None:
+            new = DeviceBuffer
+        else:
+            buf, off = arena
+            pos = [off]
+
+            def new(nbytes):
+                v = buf.view(pos[0], nbytes)
+                pos[0] += nbytes
+                return v
+        self.counts = new(8 * (4 + SQ_KINDS))
+        self.sums = new(8 * g)
+        self.first = new(8 * g)
+        self.keys = {d: new(8 * g) for d in self.attr_dims}
+        self.errs = [new(8 * e) for _ in range(SQ_KINDS)]
         o = SqOut()
         o.sums = self.sums.ptr
         o.first_row = self.first.ptr
@@ -665,6 +691,32 @@ class StarQuery:
         if st:
             check(st)
 
+    def results_from(self, h: np.ndarray, off: int) -> dict | None:
+        """The results from a host copy h (uint8) of the arena this query's outputs were carved
+        from at off; None if an output was truncated (results() then re-runs the query alone)."""
+        g, e = self.groups_cap, self.err_cap
+        c = [int(x) for x in h[off: off + 8 * (4 + SQ_KINDS)].view(np.uint64)]
+        if c[0] > g or max(c[4:]) > e:
+            return None
+
+        def arr(pos, count):
+            return h[pos: pos + 8 * count].view(np.uint64).copy()
+
+        pos = off + 8 * (4 + SQ_KINDS)
+        ng = c[0]
+        sums, first = arr(pos, ng), arr(pos + 8 * g, ng)
+        pos += 16 * g
+        keys = {}
+        for d in self.attr_dims:
+            keys[d] = arr(pos, ng)
+            pos += 8 * g
+        err = []
+        for k in range(SQ_KINDS):
+            err.append(arr(pos, c[4 + k]))
+            pos += 8 * e
+        return {"status": c[2], "groups": ng, "kept": c[1], "events": c[3], "sums": sums, "first_row": first,
+                "keys": keys, "err": err}
+
     def results(self) -> dict:
         """Synchronises, re-runs with larger buffers if an output was truncated, returns numpy."""
         while True:
@@ -685,6 +737,7 @@ class StarQuery:
 
 
 SQ_MAX_BATCH = 8
+SQ_MAX_SHAPE_PASS = 16
 
 
 def _batch_ptrs(queries: list):

@@ -1166,14 +1166,16 @@ __global__ __launch_bounds__(kBlock) void k_sq_sel(DimPrepArgs a) {
 // for a group attribute the key's attribute index over the union of the queries' attribute ranges
 // (+ 1; all ones: a corrupt attribute) is either a field of the same entry above the presence bits
 // (combined: one lookup) or the entry of a separate attribute map (split: smaller maps, the second
-// lookup only where some query still holds the row).  A row costs one or two lookups per dimension
-// for all queries.  A fact
-// predicate over values of at most 8 bits is a 256-entry table of per-query pass bits, indexed by
-// the decoded value.  What remains per query is the work of its joined rows: the group index in
-// its own numbering and its LDS group sums.  A row with a corrupt code takes each query's
-// chain-order path over that query's own maps (exact_row_s), as in the single-query scan.
+// lookup only where some query still holds the row), or, when the distinct (presence bits, field)
+// values of the dimension are few (selections that follow a low-cardinality attribute), a small
+// dictionary code per key whose value sits in an LDS class table.  A row costs one or two lookups
+// per dimension for all queries.  A fact predicate over values of at most 8 bits is a 256-entry
+// table of per-query pass bits, indexed by the decoded value.  What remains per query is the work
+// of its joined rows: the group index in its own numbering and its LDS group sums.  A row with a
+// corrupt code takes each query's chain-order path over that query's own maps (exact_row_s), as
+// in the single-query scan.
 
-constexpr uint32_t kSqsMaxQ = AHEAD_SQ_MAX_BATCH;
+constexpr uint32_t kSqsMaxQ = AHEAD_SQ_MAX_SHAPE_PASS;
 constexpr uint32_t kSqsMapWords = 24320;  // LDS words for the maps (beside groups, staging, tables)
 constexpr uint32_t kSqsQWords = 10;       // per query in LDS: attribute offsets[4], strides[4], groups, group base
 
@@ -1185,6 +1187,8 @@ struct SqsDimK {  // a dimension of the pass, in evaluation order
     uint32_t all;                         // one presence bit for every query (no query selects here)
     uint32_t pbits;                       // presence bits of an entry (1 or the pass's queries)
     uint32_t comb;                        // the attribute field sits above the presence bits
+    uint32_t dict;                        // entries are class codes; their values at s_map[cls_off + code]
+    uint32_t cls_off;
     uint32_t alb, a_lds_off, a_map_off;   // separate attribute map (comb 0), likewise
     uint32_t ab;      // bits of the union attribute field (0: no group attribute)
     uint32_t ashift;  // position of the field in a row's packed attribute word
@@ -1233,6 +1237,11 @@ static_assert(sizeof(SqsDesc) <= 4096 - 64, "descriptor block written through ke
 struct SqsTab {
     uint32_t t[AHEAD_SQ_MAX_PREDS * 256];
 };
+
+struct SqsCls {  // a dictionary-coded dimension's class values by code (code 0: no query holds the key)
+    uint32_t v[256];
+};
+constexpr uint32_t kSqsDictWords = 257;  // per dimension: 256 slots of distinct values, an overflow flag
 
 __device__ __forceinline__ void push_event_s(const SqsQ &Q, uint64_t row, uint32_t kind, uint32_t drow) {
     const uint32_t i = __hip_atomic_fetch_add(gblw(Q.ev_count), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1541,7 +1550,8 @@ __global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc 
             const uint32_t pmask = (1u << dd.pbits) - 1u;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const uint32_t e = map_entry(w[r], idx[r], dd.lb);
+                uint32_t e = map_entry(w[r], idx[r], dd.lb);
+                if (dd.dict) e = s_map[dd.cls_off + e];
                 const uint32_t pres = e & pmask;
                 pass[r] &= dd.all ? (pres != 0u ? ~0u : 0u) : pres;
                 if (pass[r] == 0u) idx[r] = 0;
@@ -1691,10 +1701,14 @@ __global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc 
 // A presence map (mode 0: per key the bits of the pass's queries whose selections hold it, or one
 // bit for all of them), an attribute map (mode 1: per key its attribute index over the union range
 // + 1, all ones for a corrupt attribute, 0 where no query selects it) or both in one entry (mode 2:
-// the attribute field above pbits presence bits), from the queries' own maps.  Queries with a
+// the attribute field above pbits presence bits), from the queries' own maps.  Mode 3 inserts the
+// distinct non-zero mode-2 values into a 256-slot table (an overflow sets *flag; lb must be 5);
+// mode 4 writes each key's class code (code_of[slot of its value], 0 for none).  Queries with a
 // repeated key are left out.
 struct SqsCombineArgs {
     uint32_t span, nq, lb, mode, all, pbits, words;
+    uint32_t *table, *flag;
+    uint8_t code_of[256];
     const uint32_t *qmap[kSqsMaxQ];  // each query's map of this dimension
     const uint32_t *qflags[kSqsMaxQ];
     uint32_t qlb[kSqsMaxQ], qadd[kSqsMaxQ], qerrv[kSqsMaxQ];  // qadd: query's attribute minimum - union's
@@ -1723,10 +1737,25 @@ __global__ __launch_bounds__(kBlock) void k_sqs_combine(SqsCombineArgs a) {
                 f = v == a.qerrv[q] ? a.fall : v + a.qadd[q];
             }
             const uint32_t pv = a.all ? (pres != 0u ? 1u : 0u) : pres;
-            const uint32_t e = a.mode == 1 ? f : a.mode == 2 ? pv | (f << a.pbits) : pv;
+            uint32_t e = a.mode == 1 ? f : a.mode == 0 ? pv : pv | (f << a.pbits);
+            if (a.mode >= 3 && e != 0u) {
+                uint32_t h = (e * 0x9E3779B1u) >> 24;
+                bool done = false;
+                for (uint32_t k = 0; k < 256u && !done; ++k, h = (h + 1u) & 255u) {
+                    const uint32_t cur = a.table[h];
+                    if (cur == e) {
+                        done = true;
+                    } else if (cur == 0u && a.mode == 3) {
+                        const uint32_t old = atomicCAS(&a.table[h], 0u, e);
+                        done = old == 0u || old == e;
+                    }
+                }
+                if (a.mode == 3 && !done) atomicOr(a.flag, 1u);
+                e = done ? a.code_of[(h - 1u) & 255u] : 0u;
+            }
             word |= e << (j << a.lb);
         }
-        a.out[w] = word;
+        if (a.mode != 3) a.out[w] = word;
     }
 }
 
@@ -2806,7 +2835,8 @@ size_t sqs_region_bytes(const ahead_sq_args *const *qs, uint32_t nq) {
             m += align_up(static_cast<size_t>(L.span_cap[d]) + 4, 256) + align_up(static_cast<size_t>(L.span_cap[d]) * 4 + 4, 256);
         maps = std::max(maps, m);
     }
-    return align_up(sizeof(SqsDesc), 256) + align_up(sizeof(SqsTab), 256) + maps;
+    return align_up(sizeof(SqsDesc), 256) + align_up(sizeof(SqsTab), 256) +
+           align_up(AHEAD_SQ_MAX_DIMS * kSqsDictWords * 4, 256) + AHEAD_SQ_MAX_DIMS * sizeof(SqsCls) + maps;
 }
 
 template <int ND, int NP>
@@ -2839,7 +2869,8 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
     const uint32_t nd = q0->n_dims;
     SqsDesc *ddesc = reinterpret_cast<SqsDesc *>(region);
     SqsTab *dtab = reinterpret_cast<SqsTab *>(region + align_up(sizeof(SqsDesc), 256));
-    uint32_t *cmaps = reinterpret_cast<uint32_t *>(region + align_up(sizeof(SqsDesc), 256) + align_up(sizeof(SqsTab), 256));
+    uint32_t *ddict = reinterpret_cast<uint32_t *>(region + align_up(sizeof(SqsDesc), 256) + align_up(sizeof(SqsTab), 256));
+    uint32_t *cmaps = ddict + align_up(AHEAD_SQ_MAX_DIMS * kSqsDictWords * 4, 256) / 4;
 
     // map geometry per chain dimension: presence (one bit for all queries where none selects),
     // union attribute field; combined entries (presence | field) or split maps
@@ -2865,6 +2896,82 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
         for (uint32_t i : ps.qi) frac[d] += plans[i].P[d].frac;
         frac[d] = std::min(frac[d], 1.0);
     }
+    // the queries' maps merged per key into (presence bits | field << pbits) values, as the
+    // combine kernel writes them
+    auto combine_args = [&](uint32_t d) {
+        SqsCombineArgs c{};
+        c.span = Q0.P[d].span;
+        c.nq = nq;
+        c.all = all[d] ? 1u : 0u;
+        c.pbits = pbits[d];
+        for (uint32_t j = 0; j < nq; ++j) {
+            const SqQueryPlan &Q = plans[ps.qi[j]];
+            c.qmap[j] = Q.a.maps + Q.P[d].off_map;
+            c.qflags[j] = Q.flags();
+            c.qlb[j] = Q.P[d].lb;
+            c.qadd[j] = Q.P[d].amin - ps.umin[d];
+            c.qerrv[j] = Q.P[d].prep.errv;
+        }
+        c.fall = ab[d] != 0u ? (1u << ab[d]) - 1u : 0u;
+        return c;
+    };
+    // dictionary codes where the distinct values are few: each candidate dimension's values go
+    // into a 256-slot table on the device, read back with one wait for the pass
+    bool dict[AHEAD_SQ_MAX_DIMS] = {};
+    uint32_t dlb[AHEAD_SQ_MAX_DIMS] = {}, ndict = 0;
+    uint8_t code_of[AHEAD_SQ_MAX_DIMS][256] = {};
+    SqsCls cls[AHEAD_SQ_MAX_DIMS] = {};
+    bool cand[AHEAD_SQ_MAX_DIMS] = {}, any_cand = false;
+    for (uint32_t d = 0; d < nd; ++d) {
+        const uint32_t vb = pbits[d] + ab[d];
+        cand[d] = vb > 4 && vb <= 32;
+        any_cand = any_cand || cand[d];
+    }
+    if (any_cand) {
+        if ((st = fill_u32(ddict, AHEAD_SQ_MAX_DIMS * kSqsDictWords, 0u, s)) != AHEAD_OK) return st;
+        for (uint32_t d = 0; d < nd; ++d) {
+            if (!cand[d]) continue;
+            SqsCombineArgs c = combine_args(d);
+            c.mode = 3;
+            c.lb = 5;
+            c.words = c.span;
+            c.table = ddict + d * kSqsDictWords;
+            c.flag = c.table + 256;
+            hipLaunchKernelGGL(k_sqs_combine, dim3(prep_grid(c.words)), dim3(kBlock), 0, s, c);
+        }
+        AHEAD_HIP_TRY(hipGetLastError());
+        uint32_t h[AHEAD_SQ_MAX_DIMS * kSqsDictWords];
+        AHEAD_HIP_TRY(hipMemcpyAsync(h, ddict, sizeof h, hipMemcpyDeviceToHost, s));
+        AHEAD_HIP_TRY(hipStreamSynchronize(s));
+        for (uint32_t d = 0; d < nd; ++d) {
+            const uint32_t *t = h + d * kSqsDictWords;
+            if (!cand[d] || t[256] != 0u) continue;
+            uint32_t n = 0;
+            for (uint32_t k = 0; k < 256; ++k)
+                if (t[k] != 0u) n++;
+            if (n > 255) continue;
+            uint32_t l = 0;
+            while ((1u << (1u << l)) <= n) ++l;  // codes 0 .. n
+            // codes narrower than the presence entry (no attribute) or than the combined entry
+            // (an attribute: one lookup where the split maps take two)
+            if (ab[d] == 0u ? l >= lb[d] : (cwords[d] != 0u && l >= clb[d])) continue;
+            dict[d] = true;
+            dlb[d] = l;
+            uint32_t code = 0;
+            for (uint32_t k = 0; k < 256; ++k) {
+                if (t[k] == 0u) continue;
+                code_of[d][k] = static_cast<uint8_t>(++code);
+                cls[d].v[code] = t[k];
+            }
+            ++ndict;
+        }
+    }
+    for (uint32_t d = 0; d < nd; ++d) {
+        if (!dict[d]) continue;
+        lb[d] = dlb[d];
+        pwords[d] = words_of(Q0.P[d].span, dlb[d]);
+        ab[d] != 0u ? (void)0 : (void)0;
+    }
     // LDS, most selective dimension (over the union) first while maps fit: a dimension with an
     // attribute takes the combined map if it fits (one LDS lookup), else the split maps if they
     // fit; in L2 it takes combined entries of at most 16 bits (one lookup, the map at most twice
@@ -2874,9 +2981,12 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
     std::iota(by.begin(), by.end(), 0u);
     std::stable_sort(by.begin(), by.end(), [&](uint32_t x, uint32_t y) { return frac[x] < frac[y]; });
     bool plds[AHEAD_SQ_MAX_DIMS] = {}, alds[AHEAD_SQ_MAX_DIMS] = {};
-    uint32_t lds_words = 0;
+    uint32_t lds_words = ndict * 256u;  // the class tables of dictionary-coded dimensions come first
     for (uint32_t d : by) {
-        if (ab[d] == 0u) {
+        if (dict[d]) {
+            comb[d] = true;
+            if (lds_words + pwords[d] <= kSqsMapWords) plds[d] = true, lds_words += pwords[d];
+        } else if (ab[d] == 0u) {
             if (lds_words + pwords[d] <= kSqsMapWords) plds[d] = true, lds_words += pwords[d];
         } else if (cwords[d] != 0u && lds_words + cwords[d] <= kSqsMapWords) {
             comb[d] = plds[d] = true;
@@ -2895,19 +3005,21 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
         }
     }
     for (uint32_t d = 0; d < nd; ++d) {
-        if (comb[d]) {
+        if (comb[d] && !dict[d]) {
             lb[d] = clb[d];
             pwords[d] = cwords[d];
-            awords[d] = 0;
         }
+        if (comb[d]) awords[d] = 0;
     }
     std::vector<uint32_t> order;
     for (uint32_t d : by)
         if (plds[d]) order.push_back(d);
     for (uint32_t d : by)
         if (!plds[d]) order.push_back(d);
-    // map offsets: LDS maps first, then the others
-    uint32_t poff[AHEAD_SQ_MAX_DIMS] = {}, aoff[AHEAD_SQ_MAX_DIMS] = {}, words = 0;
+    // map offsets: class tables, LDS maps, then the others
+    uint32_t poff[AHEAD_SQ_MAX_DIMS] = {}, aoff[AHEAD_SQ_MAX_DIMS] = {}, coff[AHEAD_SQ_MAX_DIMS] = {}, words = 0;
+    for (uint32_t d = 0; d < nd; ++d)
+        if (dict[d]) coff[d] = words, words += 256u;
     for (uint32_t d : by) {
         if (plds[d]) poff[d] = words, words += pwords[d];
         if (alds[d]) aoff[d] = words, words += awords[d];
@@ -2943,6 +3055,8 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
         k.all = all[d] ? 1u : 0u;
         k.pbits = pbits[d];
         k.comb = comb[d] ? 1u : 0u;
+        k.dict = dict[d] ? 1u : 0u;
+        k.cls_off = coff[d];
         k.alb = alb[d];
         k.a_lds_off = alds[d] ? aoff[d] : kNoLds;
         k.a_map_off = aoff[d];
@@ -2951,21 +3065,14 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
         k.flags = dm.flags;
         k.cpos = d;
         k.row_of = Q0.P[d].prep.row_of;
-        SqsCombineArgs c{};
-        c.span = k.span;
-        c.nq = nq;
-        c.all = k.all;
-        c.pbits = pbits[d];
-        c.fall = ab[d] != 0u ? (1u << ab[d]) - 1u : 0u;
-        for (uint32_t j = 0; j < nq; ++j) {
-            const SqQueryPlan &Q = plans[ps.qi[j]];
-            c.qmap[j] = Q.a.maps + Q.P[d].off_map;
-            c.qflags[j] = Q.flags();
-            c.qlb[j] = Q.P[d].lb;
-            c.qadd[j] = Q.P[d].amin - ps.umin[d];
-            c.qerrv[j] = Q.P[d].prep.errv;
+        SqsCombineArgs c = combine_args(d);
+        if (dict[d]) {
+            c.table = ddict + d * kSqsDictWords;
+            std::copy(code_of[d], code_of[d] + 256, c.code_of);
+            hipLaunchKernelGGL((k_sq_put<SqsCls>), dim3(1), dim3(64), 0, s, reinterpret_cast<SqsCls *>(cmaps + coff[d]),
+                               cls[d]);
         }
-        c.mode = comb[d] ? 2u : 0u;
+        c.mode = dict[d] ? 4u : comb[d] ? 2u : 0u;
         c.lb = lb[d];
         c.words = pwords[d];
         c.out = cmaps + poff[d];

@@ -39,8 +39,9 @@ from . import ssb
 HBM_PER_ROW = {"P1": 16, "P2": 20, "P3": 20, "P4": 23}
 # the fused form's output buffers (4096 groups x 6 arrays, 16 error vectors of 1024 positions)
 FUSED_OUT_BYTES = 4096 * 8 * 6 + 16 * 1024 * 8 + 4096
-# queries per batch in the batched form (ahead_hip_star_query_batch's same-shape pass)
-MAX_BATCH = 8
+# queries per batch in the batched form (ahead_hip_star_query_batch's same-shape pass,
+# AHEAD_SQ_MAX_SHAPE_PASS)
+MAX_BATCH = 16
 
 
 @dataclass

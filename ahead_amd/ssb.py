@@ -693,10 +693,100 @@ def fused(t: dict, q: Query, ops=B, fallback: bool = True, ev_scan: tuple | None
 def fused_batch(t: dict, qs: list, ops=B, fallback: bool = True, ev_scan: tuple | None = None) -> list:
     """Several queries over the same tables with shared fact scans (ops.star_query_batch): each
     result is the one fused() returns for that query alone."""
+    if ops is B and qs and all(len(d.sels) <= 1 for q in qs for d in q.dims):
+        return _fused_batch_device(t, qs, fallback, ev_scan)
     Es = [Errors() for _ in qs]
     calls = [_fused_args(t, q, ops, E) for q, E in zip(qs, Es)]
     rs = ops.star_query_batch(calls, ev_scan=ev_scan)
     return [_fused_result(t, q, r, E, ops, fallback) for q, r, E in zip(qs, rs, Es)]
+
+
+# a device batch's read-back arena: the selections' error vectors and every query's outputs
+_SEL_ERR_CAP = 1024
+_BATCH_GROUPS_CAP = 2048   # the fused form's group limit (more groups: its status, then the chain)
+_BATCH_ERR_CAP = 256
+
+
+def _fused_batch_device(t: dict, qs: list, fallback: bool, ev_scan: tuple | None) -> list:
+    """fused_batch on the device operators with one wait: every dimension selection (one per
+    dimension at most) is launched without reading its counts back (the star queries read the OID
+    count on the device), then the batch; the selections' error vectors and all the queries'
+    outputs come back in one copy.  A query whose selection errors or outputs overflow the arena's
+    capacities runs again alone (fused() / StarQuery.results())."""
+    import ctypes
+
+    import torch
+
+    from . import capi
+
+    lib, ws, stream = capi.load(), capi.workspace(), capi.current_stream()
+    sels, q_off, off = [], [], 0
+    for qi, q in enumerate(qs):
+        for dq in q.dims:
+            if dq.sels:
+                sels.append((qi, dq.sels[0], off))
+                off += (16 + 8 * _SEL_ERR_CAP + 255) // 256 * 256
+    for q in qs:
+        q_off.append(off)
+        off += capi.StarQuery.out_bytes(sum(1 for d in q.dims if d.attr), _BATCH_GROUPS_CAP, _BATCH_ERR_CAP)
+    arena = capi.DeviceBuffer(off)
+    oids = {}
+    for qi, (attr, pred), so in sels:
+        col = t[attr]
+        n = col.tail.n
+        buf = capi.DeviceBuffer(8 * max(n, 1))
+        op2, th2, comb = (pred[2], _enc(col, pred[3]), pred[4]) if len(pred) > 2 else (B.LT, 0, B.NONE_)
+        d = col.tail.desc()
+        capi.check(lib.ahead_hip_select_an(ctypes.byref(d), pred[0], _enc(col, pred[1]), op2, th2 & an.MASK64, comb,
+                                           B.A_OID, B.A_OID_ERR, buf.ptr, n, arena.ptr + so + 16, _SEL_ERR_CAP,
+                                           arena.ptr + so, None, ws.h, stream))
+        oids[(qi, attr)] = (buf, so)
+    sqs = []
+    for qi, q in enumerate(qs):
+        dims = []
+        for dq in q.dims:
+            fk, key = DIM_KEYS[dq.name]
+            x = {"fk": t[fk].tail, "key": t[key].tail, "attr": t[dq.attr].tail if dq.attr else None,
+                 "flags": dq.flags, "key_index": B.key_index(t[key].tail)}
+            if dq.sels:
+                buf, so = oids[(qi, dq.sels[0][0])]
+                x.update(sel=buf, sel_count=arena.view(so, 16), sel_a_inv=B.RESOID_META.a_inv)
+            dims.append(x)
+        preds = []
+        for col, p in q.preds:
+            c = t[col]
+            if len(p) == 2:
+                preds.append((c.tail, p[0], _enc(c, p[1]), B.LT, 0, B.NONE_))
+            else:
+                preds.append((c.tail, p[0], _enc(c, p[1]), p[2], _enc(c, p[3]), p[4]))
+        sqs.append(capi.StarQuery(t[q.meas[0]].tail.n, dims, preds, [t[m].tail for m in q.meas], q.kind, q.pre_mask,
+                                  groups_cap=_BATCH_GROUPS_CAP, err_cap=_BATCH_ERR_CAP, out_arena=(arena, q_off[qi])))
+    if ev_scan is not None:
+        sqs[0].args.ev_scan[0], sqs[0].args.ev_scan[1] = ev_scan
+    capi.star_query_batch(sqs)
+    host = torch.empty(off, dtype=torch.uint8, pin_memory=True)
+    host.copy_(arena.t[:off], non_blocking=True)
+    torch.cuda.current_stream().synchronize()
+    h = host.numpy()
+    out = []
+    for qi, q in enumerate(qs):
+        E = Errors()
+        rerun = False
+        for qj, (attr, _), so in sels:
+            if qj != qi:
+                continue
+            ne = int(h[so + 8: so + 16].view(np.uint64)[0])
+            if ne > _SEL_ERR_CAP:
+                rerun = True
+            E.add(f"select_{attr}", h[so + 16: so + 16 + 8 * min(ne, _SEL_ERR_CAP)].view(np.uint64).copy())
+        if rerun:
+            out.append(fused(t, q, fallback=fallback))
+            continue
+        r = sqs[qi].results_from(h, q_off[qi])
+        if r is None:
+            r = sqs[qi].results()
+        out.append(_fused_result(t, q, r, E, B, fallback))
+    return out
 
 
 def fused_ws_bytes(t: dict, q: Query, ops=B) -> int:

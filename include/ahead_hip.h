@@ -417,7 +417,7 @@ size_t ahead_hip_star_query_ws_bytes(const ahead_sq_args *q);
 /* Several star queries over one fact table (same n_fact) with shared fact scans: the queries'
  * dimension statistics are read back together (one wait).  Queries of one shape (the same
  * dimensions, FK / key / attribute columns, join flags, predicate columns of at most 4 bytes and
- * measures) are grouped into passes of at most AHEAD_SQ_MAX_BATCH whose group tables fit one LDS
+ * measures) are grouped into passes of at most AHEAD_SQ_MAX_SHAPE_PASS whose group tables fit one LDS
  * table: a row takes one lookup per dimension for all of a pass's queries (per-key presence bits
  * and a union attribute index merged from the queries' own maps).  The other queries are packed
  * in order into passes whose fact FK columns (at most 4 distinct) and predicate columns (at most 2
@@ -427,6 +427,7 @@ size_t ahead_hip_star_query_ws_bytes(const ahead_sq_args *q);
  * reference (its drivers run one query at a time): it serves concurrent queries of a
  * device-resident schema.  ev_scan of qs[0] (if set) brackets all of the batch's scans. */
 #define AHEAD_SQ_MAX_BATCH 8
+#define AHEAD_SQ_MAX_SHAPE_PASS 16
 int ahead_hip_star_query_batch(const ahead_sq_args *const *qs, const ahead_sq_out *const *outs, uint32_t nq,
                                ahead_hip_ws *ws, void *stream);
 /* Device bytes of workspace ahead_hip_star_query_batch uses for these queries; 0 for invalid. */
