@@ -1176,7 +1176,8 @@ __global__ __launch_bounds__(kBlock) void k_sq_sel(DimPrepArgs a) {
 // in the single-query scan.
 
 constexpr uint32_t kSqsMaxQ = AHEAD_SQ_MAX_SHAPE_PASS;
-constexpr uint32_t kSqsMapWords = 24320;  // LDS words for the maps (beside groups, staging, tables)
+constexpr uint32_t kSqsMapWords = 18176;  // LDS words for the maps (beside groups, staging, tables)
+constexpr uint32_t kSqsGroups = 4096;     // LDS group table entries of a pass, over its queries
 constexpr uint32_t kSqsQWords = 10;       // per query in LDS: attribute offsets[4], strides[4], groups, group base
 
 struct SqsDimK {  // a dimension of the pass, in evaluation order
@@ -1348,8 +1349,8 @@ __global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc 
     constexpr int kSeg = 256, R = 4;  // rows per wave step (2 chunks of 128), rows per lane
     constexpr uint32_t OFFB = 8u;     // a staged row: offset | pass bits << OFFB, then its attribute word
     __shared__ uint32_t s_map[kSqsMapWords];
-    __shared__ unsigned long long s_sum[kSqMaxGroups];
-    __shared__ uint32_t s_first[kSqMaxGroups];
+    __shared__ unsigned long long s_sum[kSqsGroups];
+    __shared__ uint32_t s_first[kSqsGroups];
     __shared__ uint32_t s_stage[kSqWaves * kSeg * 2];
     __shared__ uint32_t s_ptab[AHEAD_SQ_MAX_PREDS * 256];
     __shared__ uint32_t s_qd[kSqsMaxQ * kSqsQWords];
@@ -1363,7 +1364,7 @@ __global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc 
         if (gbl(desc->q[q].flags)[kFlagDup] == 0u) qmask |= 1u << q;
     for (uint32_t i = threadIdx.x; i < a.lds_words; i += kSqBlock) s_map[i] = a.maps[i];
     for (uint32_t i = threadIdx.x; i < a.np * 256u; i += kSqBlock) s_ptab[i] = a.ptab[i];
-    for (uint32_t i = threadIdx.x; i < kSqMaxGroups; i += kSqBlock) {
+    for (uint32_t i = threadIdx.x; i < kSqsGroups; i += kSqBlock) {
         s_sum[i] = 0ull;
         s_first[i] = 0xFFFFFFFFu;
     }
@@ -2803,7 +2804,7 @@ bool sqs_try_add(SqsPass &ps, uint32_t qi, const std::vector<SqQueryPlan> &plans
     for (uint32_t p = 0; p < q->n_preds; ++p)
         if (q->preds[p].col.width > 4) return false;  // the shared scan reads codes of at most 4 bytes
     if (ps.closed || (!ps.qi.empty() && !same_shape(plans[ps.qi[0]], Q))) return false;
-    if (ps.qi.size() >= kSqsMaxQ || ps.groups + Q.groups > kSqMaxGroups) return false;
+    if (ps.qi.size() >= kSqsMaxQ || ps.groups + Q.groups > kSqsGroups) return false;
     uint32_t umin[AHEAD_SQ_MAX_DIMS], umax[AHEAD_SQ_MAX_DIMS], fields = 0;
     for (uint32_t d = 0; d < q->n_dims; ++d) {
         const DimPlan &pl = Q.P[d];

@@ -658,7 +658,7 @@ def run_serve(args):
     # Poisson arrivals at multiples of the plain path's capacity; every form sees the same arrivals
     sweep = []
     base = sat["chain1"]["qps"]
-    for frac in (0.5, 0.9, 2.0, 4.0, 8.0):
+    for frac in (0.5, 0.9, 2.0, 4.0, 8.0, 16.0):
         rate = base * frac
         row = {"rate_qps": rate}
         for k, srv in servers.items():
@@ -705,8 +705,8 @@ def run_serve(args):
 
 
 def run_shape_batches(args):
-    """Eight queries of each shape (seeded random constants) as one ahead_hip_star_query_batch call
-    (same-shape shared scans) against the same eight one at a time; results asserted equal."""
+    """Sixteen queries of each shape (seeded random constants) as one ahead_hip_star_query_batch
+    call (same-shape shared scans) against the same sixteen one at a time; results asserted equal."""
     import torch
 
     from ahead_amd import capi, serve, ssb
@@ -717,9 +717,10 @@ def run_shape_batches(args):
     t = ssb.device_tables(spec)
     torch.cuda.synchronize()
     per_shape, tot_single, tot_batch = {}, 0.0, 0.0
-    ev = capi.HipEvents(9)
+    nb = capi.SQ_MAX_SHAPE_PASS
+    ev = capi.HipEvents(nb + 1)
     for shape in SSB_SHAPES:
-        qs = [q.spec() for q in serve.random_queries(200, 0, 5) if q.shape == shape][:8]
+        qs = [q.spec() for q in serve.random_queries(400, 0, 5) if q.shape == shape][:nb]
         singles = [ssb.fused(t, q) for q in qs]
         if not all(serve.same_result(a, b) for a, b in zip(singles, ssb.fused_batch(t, qs))):
             raise RuntimeError(f"{shape}: a batched query differs from the query alone")
@@ -734,20 +735,20 @@ def run_shape_batches(args):
             best_s = min(best_s, time.perf_counter() - t0)
             scan_s = min(scan_s, sum(ev.ms(i) for i in range(len(qs))))
             t0 = time.perf_counter()
-            ssb.fused_batch(t, qs, ev_scan=ev.pairs[8])
+            ssb.fused_batch(t, qs, ev_scan=ev.pairs[nb])
             torch.cuda.synchronize()
             best_b = min(best_b, time.perf_counter() - t0)
-            scan_b = min(scan_b, ev.ms(8))
+            scan_b = min(scan_b, ev.ms(nb))
         per_shape[shape] = {"queries": len(qs), "single_wall_ms": best_s * 1e3, "batch_wall_ms": best_b * 1e3,
                             "single_scans_ms": scan_s, "batch_maps_and_scan_ms": scan_b}
         tot_single += best_s
         tot_batch += best_b
-        print(f"shape_batches: {shape}: 8 single {best_s * 1e3:.1f} ms, one batch {best_b * 1e3:.1f} ms",
+        print(f"shape_batches: {shape}: {nb} single {best_s * 1e3:.1f} ms, one batch {best_b * 1e3:.1f} ms",
               file=sys.stderr, flush=True)
     ev.close()
-    nq = 8 * len(SSB_SHAPES)
+    nq = nb * len(SSB_SHAPES)
     return {
-        "metric": "star-schema queries/s, eight queries of one shape per ahead_hip_star_query_batch call",
+        "metric": f"star-schema queries/s, {nb} queries of one shape per ahead_hip_star_query_batch call",
         "value": nq / tot_batch,
         "unit": "queries/s",
         "n_gpus": 1,
@@ -759,7 +760,7 @@ def run_shape_batches(args):
         "vs_baseline": None,
         "dtype": "u16/u32/u64",
         "data": "synthetic (seeded; fact columns generated on device)",
-        "config": {"workload": f"shape_batches: ssb schema with {spec.n_fact} fact rows, 8 queries per shape P1-P4",
+        "config": {"workload": f"shape_batches: ssb schema with {spec.n_fact} fact rows, {nb} queries per shape P1-P4",
                    "single_queries_per_s": nq / tot_single, "per_shape": per_shape},
         "roofline": None,
     }
