@@ -105,6 +105,25 @@ class SqOut(ctypes.Structure):
                 ("err_cap", ctypes.c_uint64 * SQ_KINDS), ("d_counts", ctypes.c_void_p)]
 
 
+class ServeSel(ctypes.Structure):
+    _fields_ = [("col", AheadCol), ("op1", ctypes.c_int32), ("op2", ctypes.c_int32), ("comb", ctypes.c_int32),
+                ("reserved", ctypes.c_uint32), ("th1", ctypes.c_uint64), ("th2", ctypes.c_uint64),
+                ("a_oid", ctypes.c_uint64), ("a_oid_inv", ctypes.c_uint64)]
+
+
+class ServeQuery(ctypes.Structure):
+    _fields_ = [("q", SqArgs), ("has_sel", ctypes.c_uint32 * SQ_MAX_DIMS), ("sel", ServeSel * SQ_MAX_DIMS)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_uint64), ("groups", ctypes.c_uint64), ("kept", ctypes.c_uint64),
+                ("events", ctypes.c_uint64), ("sums", ctypes.c_void_p), ("first_row", ctypes.c_void_p),
+                ("keys", ctypes.c_void_p * SQ_MAX_DIMS), ("err", ctypes.c_void_p * SQ_KINDS),
+                ("n_err", ctypes.c_uint64 * SQ_KINDS), ("sel_err", ctypes.c_void_p * SQ_MAX_DIMS),
+                ("n_sel_err", ctypes.c_uint64 * SQ_MAX_DIMS), ("t_arrival", ctypes.c_double),
+                ("t_start", ctypes.c_double), ("t_finish", ctypes.c_double)]
+
+
 _u64 = ctypes.c_uint64
 _u32 = ctypes.c_uint32
 _int = ctypes.c_int
@@ -158,6 +177,12 @@ _SIGS = {
     "ahead_hip_star_query_batch": ([ctypes.POINTER(ctypes.POINTER(SqArgs)), ctypes.POINTER(ctypes.POINTER(SqOut)), _u32,
                                     _vp, _vp], _int),
     "ahead_hip_star_query_batch_ws_bytes": ([ctypes.POINTER(ctypes.POINTER(SqArgs)), _u32], ctypes.c_size_t),
+    "ahead_serve_create": ([_int, _u32, _u32, _u64, ctypes.POINTER(_vp)], _int),
+    "ahead_serve_submit": ([_vp, ctypes.POINTER(ServeQuery), ctypes.POINTER(_u64)], _int),
+    "ahead_serve_wait": ([_vp, _u64, ctypes.POINTER(ServeResult)], _int),
+    "ahead_serve_release": ([_vp, _u64], _int),
+    "ahead_serve_clock": ([_vp], ctypes.c_double),
+    "ahead_serve_destroy": ([_vp], _int),
     "ahead_hip_star_key_index_create": ([_colp, ctypes.POINTER(_vp), _vp], _int),
     "ahead_hip_star_key_index_destroy": ([_vp], _int),
     "ahead_hip_star_key_index_status": ([_vp], _int),
@@ -600,6 +625,85 @@ def star1_pipeline(fact_fk: Column, fact_m: Column, dim_key: Column, dim_a1: Col
     return st.results()
 
 
+def star_args(n_fact: int, dims: list, preds: list, meas: list, kind: int, pre_mask: int = 0,
+              a_result: int = an.A_RESULT, a_oid_err: int = an.A_OID_ERR) -> SqArgs:
+    """The ahead_sq_args block of a star query (dims, preds, meas as StarQuery takes them)."""
+    a = SqArgs()
+    a.n_fact = n_fact
+    a.n_dims = len(dims)
+    a.n_preds = len(preds)
+    a.meas_kind = kind
+    a.meas_pre_mask = pre_mask
+    for i, d in enumerate(dims):
+        x = a.dims[i]
+        x.fk = d["fk"].desc()
+        x.key = d["key"].desc()
+        x.attr = d["attr"].desc() if d.get("attr") is not None else AheadCol()
+        x.sel = d["sel"].ptr if d.get("sel") is not None else None
+        x.sel_count = d["sel_count"].ptr if d.get("sel") is not None else None
+        x.sel_a_inv = d.get("sel_a_inv", an.A_OID_INV) & an.MASK64
+        x.flags = d.get("flags", 0)
+        ki = d.get("key_index")
+        x.key_index = ki.h if ki is not None else None
+    for i, (col, op1, th1, op2, th2, comb) in enumerate(preds):
+        a.preds[i] = SqPred(col.desc(), op1, op2, comb, 0, th1 & an.MASK64, th2 & an.MASK64)
+    for i, m in enumerate(meas):
+        a.meas[i] = m.desc()
+    a.a_result = a_result
+    a.a_oid_err = a_oid_err
+    return a
+
+
+class NativeServer:
+    """The request loop of libahead_hip.so (ahead_serve_*): slots are C++ threads with their own
+    streams, workspaces and arenas; submit() returns a ticket, wait() the query's outputs."""
+
+    def __init__(self, slots: int = 1, max_batch: int = 16, hbm_budget: int = 0, device: int = 0):
+        h = _vp()
+        check(load().ahead_serve_create(device, slots, max_batch, int(hbm_budget), ctypes.byref(h)))
+        self.h = h
+
+    def clock(self) -> float:
+        return float(load().ahead_serve_clock(self.h))
+
+    def submit(self, q: "ServeQuery") -> int:
+        t = _u64()
+        check(load().ahead_serve_submit(self.h, ctypes.byref(q), ctypes.byref(t)))
+        return int(t.value)
+
+    def wait(self, ticket: int, dims: int) -> dict:
+        """The query's outputs as numpy (copied), its times on the server's clock; the ticket is
+        released."""
+        lib = load()
+        r = ServeResult()
+        st = lib.ahead_serve_wait(self.h, ticket, ctypes.byref(r))
+        if st:
+            msg = lib.ahead_hip_last_error().decode()
+            lib.ahead_serve_release(self.h, ticket)
+            raise AheadError(st, msg)
+
+        def arr(ptr, n):
+            n = int(n)
+            if n == 0 or not ptr:
+                return np.zeros(0, dtype=np.uint64)
+            return np.ctypeslib.as_array((ctypes.c_uint64 * n).from_address(ptr)).copy()
+
+        ng = int(r.groups)
+        out = {"status": int(r.status), "groups": ng, "kept": int(r.kept), "events": int(r.events),
+               "sums": arr(r.sums, ng), "first_row": arr(r.first_row, ng),
+               "keys": {d: arr(r.keys[d], ng) for d in range(dims) if r.keys[d]},
+               "err": [arr(r.err[k], r.n_err[k]) for k in range(SQ_KINDS)],
+               "sel_err": [arr(r.sel_err[d], r.n_sel_err[d]) for d in range(dims)],
+               "t_arrival": r.t_arrival, "t_start": r.t_start, "t_finish": r.t_finish}
+        check(lib.ahead_serve_release(self.h, ticket))
+        return out
+
+    def close(self) -> None:
+        if self.h is not None:
+            check(load().ahead_serve_destroy(self.h))
+            self.h = None
+
+
 class StarQuery:
     """Argument and output blocks of one fused star query (ahead_hip_star_query), reusable.
 
@@ -612,32 +716,9 @@ class StarQuery:
                  a_result: int = an.A_RESULT, a_oid_err: int = an.A_OID_ERR, groups_cap: int = 4096,
                  err_cap: int = 1024, ev_scan: tuple | None = None, out_arena: tuple | None = None):
         self.keep = (dims, preds, meas)  # the columns and selections stay alive with the query
-        a = SqArgs()
-        a.n_fact = n_fact
-        a.n_dims = len(dims)
-        a.n_preds = len(preds)
-        a.meas_kind = kind
-        a.meas_pre_mask = pre_mask
-        for i, d in enumerate(dims):
-            x = a.dims[i]
-            x.fk = d["fk"].desc()
-            x.key = d["key"].desc()
-            x.attr = d["attr"].desc() if d.get("attr") is not None else AheadCol()
-            x.sel = d["sel"].ptr if d.get("sel") is not None else None
-            x.sel_count = d["sel_count"].ptr if d.get("sel") is not None else None
-            x.sel_a_inv = d.get("sel_a_inv", an.A_OID_INV) & an.MASK64
-            x.flags = d.get("flags", 0)
-            ki = d.get("key_index")
-            x.key_index = ki.h if ki is not None else None
-        for i, (col, op1, th1, op2, th2, comb) in enumerate(preds):
-            a.preds[i] = SqPred(col.desc(), op1, op2, comb, 0, th1 & an.MASK64, th2 & an.MASK64)
-        for i, m in enumerate(meas):
-            a.meas[i] = m.desc()
-        a.a_result = a_result
-        a.a_oid_err = a_oid_err
+        self.args = star_args(n_fact, dims, preds, meas, kind, pre_mask, a_result, a_oid_err)
         if ev_scan is not None:
-            a.ev_scan[0], a.ev_scan[1] = ev_scan
-        self.args = a
+            self.args.ev_scan[0], self.args.ev_scan[1] = ev_scan
         self.attr_dims = [i for i, d in enumerate(dims) if d.get("attr") is not None]
         self.groups_cap = groups_cap
         self.err_cap = err_cap

@@ -18,6 +18,11 @@ Three execution forms:
            (ssb.fused_batch): queries of one shape share one pass over the fact rows; each result
            is the one the query gets alone; a batch needs its queries' workspaces and one more for
            the shared scan's combined maps;
+  native   the batched form's loop in C++ (ahead_serve_*, capi.NativeServer): C++ slot threads
+           with their own streams, workspaces and arenas form the same-shape batches, admit them
+           first come, first served by the bytes the library computes for them, run the dimension
+           selections and the batch, and hand back each query's outputs after one copy per batch;
+           Python only submits arrivals and reads results;
   chain    each query is its operator chain (ssb.chain), one launch per operator, every operator
            waiting for its counts: the plain path; it needs its materialised intermediates (a bound
            per fact row measured at 600 M rows).
@@ -147,7 +152,7 @@ class Server:
     reused by that stream's later queries."""
 
     def __init__(self, tables: dict, n_fact: int, workers: int = 4, hbm_budget: float = 64e9, mode: str = "fused"):
-        assert mode in ("fused", "batched", "chain")
+        assert mode in ("fused", "batched", "native", "chain")
         self.tables = tables
         self.n_fact = n_fact
         self.workers = workers
@@ -160,9 +165,17 @@ class Server:
         self._queue: deque = deque()
         self._qcv = threading.Condition()
         self._feeding = False
+        self._native = None
+        if mode == "native":
+            from . import capi
+
+            self._native = capi.NativeServer(slots=workers, max_batch=MAX_BATCH, hbm_budget=int(hbm_budget))
 
     def close(self) -> None:
         self._ex.shutdown(wait=True)
+        if self._native is not None:
+            self._native.close()
+            self._native = None
 
     def __enter__(self) -> "Server":
         return self
@@ -230,9 +243,31 @@ class Server:
             finally:
                 self.admission.release(need)
 
+    def _run_native(self, queries: list[Query], realtime: bool) -> list[Query]:
+        specs = [q.spec() for q in queries]
+        reqs = [ssb.serve_query(self.tables, sp) for sp in specs]  # built before the clock starts
+        srv = self._native
+        c0, t0 = srv.clock(), time.perf_counter()
+        tickets = []
+        for q, rq in zip(queries, reqs):
+            if realtime:
+                delay = q.arrival - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+            tickets.append(srv.submit(rq))
+        for q, sp, tk in zip(queries, specs, tickets):
+            r = srv.wait(tk, len(sp.dims))
+            q.arrival, q.start, q.finish = r["t_arrival"] - c0, r["t_start"] - c0, r["t_finish"] - c0
+            q.result = ssb.serve_result(self.tables, sp, r)
+            if r["status"] != 0:  # the chain ran instead, here
+                q.finish = time.perf_counter() - t0
+        return queries
+
     def run(self, queries: list[Query], realtime: bool = True) -> list[Query]:
         """Replays the arrivals (in real time, or all at once) into the queue the slots serve, and
         returns the queries, completed."""
+        if self._native is not None:
+            return self._run_native(queries, realtime)
         for shape in {q.shape for q in queries}:
             self.need(shape)  # computed once per shape, before the clock starts
         t0 = time.perf_counter()

@@ -789,6 +789,50 @@ def _fused_batch_device(t: dict, qs: list, fallback: bool, ev_scan: tuple | None
     return out
 
 
+def serve_query(t: dict, q: Query):
+    """The query as the C++ request loop takes it (ahead_serve_query): its star query arguments
+    and, per dimension, the selection the server runs (one per dimension at most)."""
+    from . import capi
+
+    dims = []
+    for dq in q.dims:
+        if len(dq.sels) > 1:
+            raise ValueError(f"{q.family}: the request loop runs one selection per dimension")
+        fk, key = DIM_KEYS[dq.name]
+        dims.append({"fk": t[fk].tail, "key": t[key].tail, "attr": t[dq.attr].tail if dq.attr else None,
+                     "flags": dq.flags, "key_index": B.key_index(t[key].tail)})
+    preds = []
+    for col, p in q.preds:
+        c = t[col]
+        if len(p) == 2:
+            preds.append((c.tail, p[0], _enc(c, p[1]), B.LT, 0, B.NONE_))
+        else:
+            preds.append((c.tail, p[0], _enc(c, p[1]), p[2], _enc(c, p[3]), p[4]))
+    sq = capi.ServeQuery()
+    sq.q = capi.star_args(t[q.meas[0]].tail.n, dims, preds, [t[m].tail for m in q.meas], q.kind, q.pre_mask)
+    for d, dq in enumerate(q.dims):
+        if not dq.sels:
+            continue
+        attr, pred = dq.sels[0]
+        col = t[attr]
+        op2, th2, comb = (pred[2], _enc(col, pred[3]), pred[4]) if len(pred) > 2 else (B.LT, 0, B.NONE_)
+        sq.has_sel[d] = 1
+        sq.sel[d] = capi.ServeSel(col.tail.desc(), pred[0], op2, comb, 0, _enc(col, pred[1]), th2 & an.MASK64,
+                                  B.A_OID, B.RESOID_META.a_inv)
+    return sq
+
+
+def serve_result(t: dict, q: Query, r: dict, fallback: bool = True) -> dict:
+    """A request-loop result in fused()'s form: the selections' error vectors first, in the
+    dimensions' order, then the star query's (the chain's result with fallback when its status is
+    not 0)."""
+    E = Errors()
+    for d, dq in enumerate(q.dims):
+        if dq.sels:
+            E.add(f"select_{dq.sels[0][0]}", r["sel_err"][d])
+    return _fused_result(t, q, r, E, B, fallback)
+
+
 def fused_ws_bytes(t: dict, q: Query, ops=B) -> int:
     """Device workspace bytes of the query's fused form (its dimension maps and tables)."""
     dims = [{"fk": t[DIM_KEYS[d.name][0]].tail, "key": t[DIM_KEYS[d.name][1]].tail,

@@ -642,8 +642,9 @@ def run_serve(args):
     n_q = max(16, args.steps * 16)
     # long-lived servers: same-shape batches of fused queries and single fused queries on 1 and 4
     # slots, the plain operator chains on 1 and 4
-    forms = {"batched4": ("batched", 4), "batched1": ("batched", 1), "fused4": ("fused", 4), "fused1": ("fused", 1),
-             "chain1": ("chain", 1), "chain4": ("chain", 4)}
+    forms = {"native4": ("native", 4), "native1": ("native", 1), "batched4": ("batched", 4),
+             "batched1": ("batched", 1), "fused4": ("fused", 4), "fused1": ("fused", 1), "chain1": ("chain", 1),
+             "chain4": ("chain", 4)}
     servers = {k: serve.Server(tables, spec.n_fact, workers=w, mode=m) for k, (m, w) in forms.items()}
     for k, srv in servers.items():   # warm-up on every shape
         srv.run(serve.random_queries(4 * srv.workers, 0, 99), realtime=False)

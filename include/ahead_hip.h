@@ -433,6 +433,50 @@ int ahead_hip_star_query_batch(const ahead_sq_args *const *qs, const ahead_sq_ou
 /* Device bytes of workspace ahead_hip_star_query_batch uses for these queries; 0 for invalid. */
 size_t ahead_hip_star_query_batch_ws_bytes(const ahead_sq_args *const *qs, uint32_t nq);
 
+/* ---- served star queries (request loop) -------------------------------------------------------
+ * No counterpart in the reference (its drivers run one query at a time,
+ * tools/ssbm/ssbm-q11_continuous.cpp:69-121): the served form over a device-resident schema.
+ * `slots` host threads, each with its own HIP stream, workspace and device arena, take the query at
+ * the head of the arrival queue together with the queued queries of its shape (the same columns
+ * in the same roles; at most max_batch), admit them first come, first served once the device bytes
+ * they need (the batch's workspace, read-back area and selection lists) fit hbm_budget beside the
+ * running batches (0: no limit), run each query's dimension selections (ahead_hip_select_an on the
+ * given attribute column; the OID count stays on the device, where the star query reads it) and
+ * one ahead_hip_star_query_batch call, and copy every query's outputs to host memory in one copy.
+ * Results are exactly those of the selections and ahead_hip_star_query run one by one.          */
+typedef struct ahead_serve ahead_serve;
+typedef struct {
+    ahead_col col;              /* the dimension's attribute column the selection reads          */
+    int op1, op2, comb;
+    uint32_t reserved;
+    uint64_t th1, th2;          /* ENCODED thresholds, as selectAN's                             */
+    uint64_t a_oid, a_oid_inv;  /* A (and A^-1 mod 2^64) of the selected OIDs                     */
+} ahead_serve_sel;
+typedef struct {
+    ahead_sq_args q;            /* dims[d].sel / sel_count / sel_a_inv are filled by the server  */
+    uint32_t has_sel[AHEAD_SQ_MAX_DIMS];  /* 1: dimension d is selected by sel[d]; 0: every row   */
+    ahead_serve_sel sel[AHEAD_SQ_MAX_DIMS];
+} ahead_serve_query;
+typedef struct {
+    uint64_t status, groups, kept, events;  /* as ahead_hip_star_query's d_counts[2], [0], [1], [3] */
+    const uint64_t *sums, *first_row;       /* `groups` entries each                              */
+    const uint64_t *keys[AHEAD_SQ_MAX_DIMS];  /* attribute codes per group (NULL: no attribute)   */
+    const uint64_t *err[AHEAD_SQ_KINDS];
+    uint64_t n_err[AHEAD_SQ_KINDS];
+    const uint64_t *sel_err[AHEAD_SQ_MAX_DIMS];  /* the selections' error positions              */
+    uint64_t n_sel_err[AHEAD_SQ_MAX_DIMS];
+    double t_arrival, t_start, t_finish;    /* seconds on the server's clock (ahead_serve_clock)  */
+} ahead_serve_result;
+int ahead_serve_create(int device, uint32_t slots, uint32_t max_batch, uint64_t hbm_budget, ahead_serve **out);
+/* Copies the query; the columns it names must stay valid until its result has been waited for. */
+int ahead_serve_submit(ahead_serve *s, const ahead_serve_query *q, uint64_t *ticket);
+/* Blocks until the query has run; the result's arrays stay valid until ahead_serve_release.     */
+int ahead_serve_wait(ahead_serve *s, uint64_t ticket, ahead_serve_result *result);
+int ahead_serve_release(ahead_serve *s, uint64_t ticket);
+double ahead_serve_clock(const ahead_serve *s);
+/* Serves what is queued, then joins the slots.                                                   */
+int ahead_serve_destroy(ahead_serve *s);
+
 /* ---- data generation (seeded, counter-based; the same streams as ahead_amd/datagen.py) ------
  * Not part of the reference's operator API: the device-side generator for the synthetic
  * columns of the benchmark workloads (the reference loads SSB tables from dbgen instead).

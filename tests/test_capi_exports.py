@@ -13,7 +13,7 @@ def declared_symbols():
     for h in (ROOT / "include").glob("*.h"):
         text = h.read_text()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        names.update(re.findall(r"\b(ahead_hip_\w+)\s*\(", text))
+        names.update(re.findall(r"\b(ahead_(?:hip|serve)_\w+)\s*\(", text))
     return sorted(names)
 
 

@@ -1,4 +1,5 @@
-"""Served queries (ahead_amd.serve), in the fused form, in same-shape batches and as operator chains:
+"""Served queries (ahead_amd.serve), in the fused form, in same-shape batches (Python slots and the C++
+request loop) and as operator chains:
 every query's result, error vectors included, is the same when it runs alone (one query or batch on
 the GPU at a time) and in the mix (four slots overlapping on four streams, all arriving at once),
 and both equal the oracle's operator chain for the query; the admission budget serialises queries
@@ -13,7 +14,7 @@ pytestmark = pytest.mark.gpu
 SMALL = dict(n_fact=250_000, n_date=2556, n_cust=3_000, n_supp=200, n_part=1_400)
 
 
-@pytest.mark.parametrize("mode", ["fused", "batched", "chain"])
+@pytest.mark.parametrize("mode", ["fused", "batched", "native", "chain"])
 def test_results_alone_equal_results_in_the_mix(mode):
     spec = ssb.SsbSpec(rate=2e-3, seed=21, **SMALL)
     tables = ssb.device_tables(spec)
@@ -58,3 +59,23 @@ def test_admission_budget_serializes_when_memory_is_short(mode):
     spans = sorted({(q.start, q.finish) for q in qs})
     for (s0, f0), (s1, f1) in zip(spans, spans[1:]):
         assert s1 >= f0 - 1e-4
+
+
+def test_native_loop_admission_serializes_and_keeps_arrival_order():
+    """The C++ request loop with a budget below any batch's bytes: batches run one at a time, in the
+    arrival order of their first queries, and every result still equals the oracle's chain."""
+    spec = ssb.SsbSpec(rate=1e-3, seed=23, **SMALL)
+    tables = ssb.device_tables(spec)
+    print("serve native: admission budget of one byte", flush=True)
+    with serve.Server(tables, spec.n_fact, workers=4, mode="native", hbm_budget=1) as srv:
+        qs = srv.run(serve.random_queries(12, 0, 7), realtime=False)
+    heads = {}
+    for q in qs:
+        heads.setdefault(q.start, q.qid)
+    assert list(heads) == sorted(heads)
+    spans = sorted({(q.start, q.finish) for q in qs})
+    for (s0, f0), (s1, f1) in zip(spans, spans[1:]):
+        assert s1 >= f0 - 1e-4
+    host = OB.tables(spec)
+    for q in qs:
+        assert serve.same_result(q.result, ssb.PIPELINES[q.shape](host, ops=OB, **q.params)), (q.qid, q.shape)
