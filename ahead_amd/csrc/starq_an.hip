@@ -1233,7 +1233,11 @@ struct SqsQ {  // one query of the pass, by the pass's dimension index (device m
 struct SqsDesc {
     SqsQ q[kSqsMaxQ];
 };
-static_assert(sizeof(SqsDesc) <= 4096 - 64, "descriptor block written through kernel arguments");
+constexpr uint32_t kSqsPutQ = 8;  // descriptors written per launch (through kernel arguments)
+struct SqsDescPart {
+    SqsQ q[kSqsPutQ];
+};
+static_assert(sizeof(SqsDescPart) <= 4096 - 64 && kSqsMaxQ % kSqsPutQ == 0, "descriptor parts written through kernel arguments");
 
 struct SqsTab {
     uint32_t t[AHEAD_SQ_MAX_PREDS * 256];
@@ -3149,7 +3153,12 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
         x.gfirst = Q.a.gfirst;
         x.kept = Q.a.kept;
     }
-    hipLaunchKernelGGL((k_sq_put<SqsDesc>), dim3(1), dim3(64), 0, s, ddesc, hd);
+    for (uint32_t j = 0; j < nq; j += kSqsPutQ) {
+        SqsDescPart part{};
+        std::copy(hd.q + j, hd.q + j + kSqsPutQ, part.q);
+        hipLaunchKernelGGL((k_sq_put<SqsDescPart>), dim3(1), dim3(64), 0, s,
+                           reinterpret_cast<SqsDescPart *>(ddesc->q + j), part);
+    }
     hipLaunchKernelGGL((k_sq_put<SqsTab>), dim3(1), dim3(64), 0, s, dtab, ht);
     AHEAD_HIP_TRY(hipGetLastError());
     if (a.n > 0) {
