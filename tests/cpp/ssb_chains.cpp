@@ -6,9 +6,14 @@
 //   <dir>/tables.txt   one line per column: name kind(tiny|short|int) A rows; codes in <dir>/<name>.bin
 //   <dir>/queries.txt  the query specifications (written by tests/test_cpp_ssb.py from ahead_amd.ssb)
 // writes <dir>/out_<query>.txt: rows, the Q1.x result or the groups (sums, keys), and every
-// non-empty error vector by the name the Python chains give it (operator tag + ".e" + position).
-// Exit status 0 when every query ran; the comparison with the oracle's chains is the test's.
+// non-empty error vector by the name the Python chains give it (operator tag + ".e" + position);
+// and <dir>/out_<query>_fused.txt: the same query through the fused star query of
+// include/ahead/column_operators/StarQueryAN.hpp after the chain's dimension selections (the
+// chain's result where the fused form does not apply), its error vectors named as ahead_amd.ssb's
+// fused form names them.
+// Exit status 0 when every query ran; the comparison with the oracle is the test's.
 #include <column_operators/OperatorsAN.hpp>
+#include <column_operators/StarQueryAN.hpp>
 
 #include <cstdio>
 #include <fstream>
@@ -49,6 +54,7 @@ struct Pred {
 
 struct DimQ {
     std::string name, attr;
+    uint32_t flags = 0;
     std::vector<std::pair<std::string, Pred>> sels;
 };
 
@@ -56,6 +62,10 @@ struct Query {
     std::string name, family;
     std::vector<std::pair<std::string, Pred>> preds;
     std::vector<DimQ> dims;
+    // the fused form's measure columns, kind and early-fetch mask; the dimensions grouped by
+    uint32_t kind = 0, pre_mask = 0;
+    std::vector<std::string> meas;
+    std::vector<int> groups;
 };
 
 std::map<std::string, Col> T;
@@ -68,6 +78,10 @@ struct Errors {
         if (v != nullptr && !v->empty())
             items.push_back({name + ".e" + std::to_string(k), std::vector<uint64_t>(v->begin(), v->end())});
         delete v;
+    }
+    void add_vec(const std::string &name, const std::vector<uint64_t> &v) {
+        if (!v.empty())
+            items.push_back({name + ".e1", v});
     }
     template<typename... V>
     void add(const std::string &name, V *... vs) {
@@ -360,6 +374,82 @@ TempBAT<v2_void_t, V2> *load_column(const std::string &path, uint64_t A, size_t 
     return make_bat<V2>(v, md);
 }
 
+const std::map<std::string, std::pair<std::string, std::string>> DIM_KEYS = {
+    {"date", {"lo_orderdate", "d_datekey"}}, {"cust", {"lo_custkey", "c_custkey"}},
+    {"supp", {"lo_suppkey", "s_suppkey"}}, {"part", {"lo_partkey", "p_partkey"}}};
+const char *FUSED_KINDS[AHEAD_SQ_KINDS] = {"pred0", "pred1", "meas0", "meas1", "fk0", "fk1", "fk2", "fk3", "refetch0",
+                                           "refetch1", "refetch2", "refetch3", "attr0", "attr1", "attr2", "attr3"};
+
+ahead_col desc_of(const std::string &name) {
+    return std::visit([](auto *b) { return device::col(b->tail, device::rows(b)); }, T.at(name));
+}
+
+// The query as a fused star query (StarQueryAN.hpp) after the chain's dimension selections, with
+// the chain as the fallback; the result in the chains' form.
+Result fused_query(const Query &q, Result (*chain)(const Query &)) {
+    Result R;
+    Errors &E = R.E;
+    std::vector<StarDimAN> dims;
+    for (const DimQ &d : q.dims) {
+        const auto &[fk, key] = DIM_KEYS.at(d.name);
+        StarDimAN sd = d.attr.empty() ? starDim(col<IntB>(fk), col<IntB>(key), d.flags)
+                                      : std::visit([&](auto *a) { return starDim(col<IntB>(fk), col<IntB>(key), a, d.flags); },
+                                                   T.at(d.attr));
+        if (!d.sels.empty()) {
+            OidB *s = sel(d.sels[0].first, d.sels[0].second, E);
+            if (d.sels.size() > 1) {  // two selections on one dimension, intersected (ssbm-q13_continuous.cpp:100-110)
+                OidB *s2 = sel(d.sels[1].first, d.sels[1].second, E);
+                s = take(E, "match_" + d.name + "_selections", matchjoinAN(s, s2))->mirror_head();
+            }
+            starSelect(sd, s);
+        }
+        dims.push_back(sd);
+    }
+    std::vector<StarPredAN> preds;
+    for (const auto &[c, p] : q.preds) {
+        StarPredAN sp;
+        sp.col = desc_of(c);
+        const uint64_t A = sp.col.a;
+        sp.op1 = p.op1;
+        sp.th1 = p.v1 * A;
+        if (p.two) {
+            sp.op2 = p.op2;
+            sp.th2 = p.v2 * A;
+            sp.comb = p.comb;
+        }
+        preds.push_back(sp);
+    }
+    std::vector<ahead_col> meas;
+    for (const std::string &m : q.meas)
+        meas.push_back(desc_of(m));
+    auto out = starQueryOrChainAN(meas[0].n, dims, preds, meas, q.kind, q.pre_mask, [&] { return chain(q); });
+    if (out.chained)
+        return *out.chained;
+    const StarResultAN &F = out.fused;
+    for (int k = 0; k < AHEAD_SQ_KINDS; ++k)
+        E.add_vec(k < static_cast<int>(q.preds.size()) ? "select_" + q.preds[k].first : std::string("fused_") + FUSED_KINDS[k],
+                  F.err[k]);
+    const uint64_t inv = ext_euclidean(static_cast<uint64_t>(65117), 64);
+    R.rows = F.rows;
+    if (q.family == "q1") {
+        R.scalar = true;
+        R.result = F.sums.empty() ? 0 : F.sums[0] * inv;
+        return R;
+    }
+    R.groups = F.groups;
+    for (uint64_t v : F.sums)
+        R.sums.push_back(v * inv);
+    for (int d : q.groups) {
+        const ahead_col a = desc_of(q.dims[d].attr);
+        const uint64_t mask = a.width >= 8 ? ~0ull : ((1ull << (8 * a.width)) - 1);
+        std::vector<uint64_t> keys;
+        for (uint64_t c : F.keys[d])
+            keys.push_back((c * a.a_inv) & mask);
+        R.keys.push_back(keys);
+    }
+    return R;
+}
+
 Pred parse_pred(std::istringstream &in) {
     Pred p;
     in >> p.op1 >> p.v1;
@@ -385,10 +475,19 @@ std::vector<Query> load_queries(const std::string &path) {
             qs.back().preds.push_back({c, parse_pred(in)});
         } else if (tag == "dim") {
             DimQ d;
-            in >> d.name >> d.attr;
+            in >> d.name >> d.attr >> d.flags;
             if (d.attr == "-")
                 d.attr.clear();
             qs.back().dims.push_back(d);
+        } else if (tag == "fused") {
+            in >> qs.back().kind >> qs.back().pre_mask;
+            std::string m;
+            while (in >> m)
+                qs.back().meas.push_back(m);
+        } else if (tag == "groups") {
+            int g;
+            while (in >> g)
+                qs.back().groups.push_back(g);
         } else if (tag == "sel") {
             std::string c;
             in >> c;
@@ -460,6 +559,15 @@ int main(int argc, char **argv) {
                         static_cast<unsigned long long>(R.rows), R.E.items.size());
         } catch (const std::exception &e) {
             std::printf("%s (%s): FAILED: %s\n", q.name.c_str(), q.family.c_str(), e.what());
+            ++failed;
+        }
+        try {
+            const Result F = fused_query(q, chains.at(q.family));
+            write_result(dir + "/out_" + q.name + "_fused.txt", F);
+            std::printf("%s fused: %llu rows, %zu error vectors\n", q.name.c_str(), static_cast<unsigned long long>(F.rows),
+                        F.E.items.size());
+        } catch (const std::exception &e) {
+            std::printf("%s fused: FAILED: %s\n", q.name.c_str(), e.what());
             ++failed;
         }
         std::fflush(stdout);
