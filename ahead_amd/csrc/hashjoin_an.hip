@@ -43,7 +43,8 @@ struct HjTable {
     uint32_t *rows;   // [n_build] row lists
     uint32_t *flags;  // [0] several rows share a key, [1] table overflow, [2] unused,
                       // [3] smallest build key, [5] largest build key, [4] direct map in use,
-                      // [6] some sampled probe row finds no key
+                      // [6] some sampled probe row finds no key, [7] some build row r does not
+                      // hold key smallest + r (0 with the direct map: the map is the identity)
     uint64_t *filt;   // [kHjFiltWords] probe filter: the bit of every build key set (filt_bit)
 };
 
@@ -151,18 +152,23 @@ __global__ __launch_bounds__(kBlock) void k_hj_dmap_clear(HjTable t) {
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < range; i += gridDim.x * kBlock) t.dmap[i] = 0u;
 }
 
-// dmap[key - smallest] = first row + 1 (one row) or kDmapMulti | slot (several rows).
+// dmap[key - smallest] = first row + 1 (one row) or kDmapMulti | slot (several rows); flags[7]
+// records a row r whose key is not smallest + r (a build side of dense keys in row order makes
+// the map the identity, and the probe then computes the row instead of reading it).
 __global__ __launch_bounds__(kBlock) void k_hj_dmap_fill(BuildArgs a) {
     if (a.t.flags[4] == 0u) return;
     const uint32_t lo = a.t.flags[3];
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    bool moved = false;
     for (uint64_t j = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; j < a.n; j += stride) {
         const uint64_t k = a.key.dec_of(a.key.raw(j)) & a.key_mask;
+        moved = moved || static_cast<uint64_t>(static_cast<uint32_t>(k) - lo) != j;
         const uint32_t s = a.t.ht.find(k);
         if (s == UINT32_MAX) continue;  // not reached: every build key is in the table
         const uint32_t e = a.t.cnt[s] == 1u ? a.t.first[s] + 1u : (kDmapMulti | s);
         a.t.dmap[static_cast<uint32_t>(k) - lo] = e;
     }
+    if (__syncthreads_or(moved) && threadIdx.x == 0) atomicOr(&a.t.flags[7], 1u);
 }
 
 // offsets of the row lists: a multi-entry scan over the table slots on the row-tile engine
@@ -587,7 +593,17 @@ struct ProbeLeanOp : LeanFilter<FILT> {
                 ok[g] = ok[g] && b != UINT32_MAX && ((filt[b >> 6] >> (b & 63u)) & 1ull) != 0ull;
             }
         }
-        if (t.flags[4] != 0u) {  // direct map: one gather per row (one row per key: no flag bit)
+        if (t.flags[4] != 0u && t.flags[7] == 0u) {  // identity map: build row = key - smallest
+            const uint64_t lo = t.flags[3], span = t.flags[5] - t.flags[3];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const uint64_t rel = k[g] - lo;
+                if (ok[g] && rel <= span) {
+                    st[g].brow = static_cast<uint32_t>(rel);
+                    cnt[g] = 1;
+                }
+            }
+        } else if (t.flags[4] != 0u) {  // direct map: one gather per row (one row per key: no flag bit)
             const uint64_t lo = t.flags[3], span = t.flags[5] - t.flags[3];
             uint32_t e[G];
 #pragma unroll
