@@ -1176,8 +1176,10 @@ __global__ __launch_bounds__(kBlock) void k_sq_sel(DimPrepArgs a) {
 // in the single-query scan.
 
 constexpr uint32_t kSqsMaxQ = AHEAD_SQ_MAX_SHAPE_PASS;
-constexpr uint32_t kSqsMapWords = 18176;  // LDS words for the maps (beside groups, staging, tables)
-constexpr uint32_t kSqsGroups = 4096;     // LDS group table entries of a pass, over its queries
+// LDS of a same-shape pass: group tables for 24 queries of 280 groups (brand x year), and the maps
+// that fit beside them (a date dimension's dictionary-coded map takes 7,900 words)
+constexpr uint32_t kSqsMapWords = 10112;  // LDS words for the maps (beside groups, staging, tables)
+constexpr uint32_t kSqsGroups = 6912;     // LDS group table entries of a pass, over its queries
 constexpr uint32_t kSqsQWords = 10;       // per query in LDS: attribute offsets[4], strides[4], groups, group base
 
 struct SqsDimK {  // a dimension of the pass, in evaluation order
