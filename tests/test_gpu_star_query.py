@@ -272,3 +272,22 @@ def test_same_shape_batch_tiny_fact_tables_and_repeated_keys():
     assert [g["status"] for g in got] == [3, 3, 3]
     for q, g in zip(qs, got):
         same(g, ssb.chain(t2, q))
+
+
+def test_same_shape_full_passes_equal_single_queries():
+    """Full same-shape passes (AHEAD_SQ_MAX_SHAPE_PASS queries of P3 and of P4, and one more of
+    each, which runs outside the pass) against the same queries run alone, with flips in every
+    fact column and attribute."""
+    from ahead_amd import capi
+
+    spec = ssb.SsbSpec(rate=1e-3, seed=62, flip_cols=ALL_FLIPS, **SMALL)
+    t = ssb.device_tables(spec)
+    n = capi.SQ_MAX_SHAPE_PASS + 1
+    qs = _instances("P3", n, 63) + _instances("P4", n, 64)
+    print(f"star_query_batch {len(qs)} queries: full P3 and P4 passes and one more of each", flush=True)
+    got = ssb.fused_batch(t, qs, fallback=False)
+    for i, (q, g) in enumerate(zip(qs, got)):
+        ref = ssb.fused(t, q, fallback=False)
+        assert g["status"] == 0 and ref["status"] == 0, i
+        same(g, ref)
+        assert np.array_equal(g["first_row"], ref["first_row"]), i
