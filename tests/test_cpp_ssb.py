@@ -52,7 +52,7 @@ def read_result(path: Path) -> dict:
     r = {"keys": [], "errors": {}}
     for line in path.read_text().splitlines():
         f = line.split()
-        if f[0] in ("rows", "result", "groups"):
+        if f[0] in ("rows", "result", "groups", "status"):
             r[f[0]] = int(f[1])
         elif f[0] == "sums":
             r["sums"] = np.array([int(x) for x in f[1:]], dtype=np.uint64)
