@@ -119,3 +119,16 @@ def test_config5_alone_equals_mixed_at_600M(clean600):
         assert a.result["status"] == 0
         assert serve.same_result(a.result, m.result), (q.shape, q.params)
         assert serve.same_result(a.result, ssb.chain(t, q.spec())), (q.shape, q.params)
+
+
+def test_config5_batched_native_loop_equals_alone_at_600M(clean600):
+    """The C++ request loop over the resident 600M-row schema: 48 queries arriving at once are
+    served in same-shape batches of up to 24; every result equals the query run alone."""
+    spec, t = clean600
+    print("config 5: 48 queries through the C++ request loop vs alone", flush=True)
+    with serve.Server(t, spec.n_fact, workers=2, mode="native") as srv:
+        got = srv.run(serve.random_queries(48, 0, 72), realtime=False)
+    for q in got:
+        ref = ssb.fused(t, q.spec())
+        assert q.result["status"] == 0 and ref["status"] == 0, (q.shape, q.params)
+        assert serve.same_result(q.result, ref), (q.shape, q.params)
