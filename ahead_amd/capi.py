@@ -818,7 +818,7 @@ class StarQuery:
 
 
 SQ_MAX_BATCH = 8
-SQ_MAX_SHAPE_PASS = 24
+SQ_MAX_SHAPE_PASS = 32
 
 
 def _batch_ptrs(queries: list):

@@ -1176,11 +1176,15 @@ __global__ __launch_bounds__(kBlock) void k_sq_sel(DimPrepArgs a) {
 // in the single-query scan.
 
 constexpr uint32_t kSqsMaxQ = AHEAD_SQ_MAX_SHAPE_PASS;
-// LDS of a same-shape pass: group tables for 24 queries of 280 groups (brand x year), and the maps
-// that fit beside them (a date dimension's dictionary-coded map takes 7,900 words)
-constexpr uint32_t kSqsMapWords = 10112;  // LDS words for the maps (beside groups, staging, tables)
-constexpr uint32_t kSqsGroups = 6912;     // LDS group table entries of a pass, over its queries
 constexpr uint32_t kSqsQWords = 10;       // per query in LDS: attribute offsets[4], strides[4], groups, group base
+constexpr uint32_t kSqsFieldBits = 24;    // a row's union attribute fields, beside its 8-bit offset in the step
+// LDS of a same-shape pass: beside the staging area and the small tables, one region split per pass
+// between the queries' group tables (12 bytes per group) and the maps (the rest), sized at launch
+constexpr uint32_t kSqsLdsBytes = 160 * 1024 - 1024;
+constexpr uint32_t kSqsStaticBytes = kSqWaves * 256 * 2 * 4 + AHEAD_SQ_MAX_PREDS * 256 * 4 + kSqsMaxQ * kSqsQWords * 4 + kSqsMaxQ * 4;
+constexpr uint32_t kSqsDynBytes = kSqsLdsBytes - kSqsStaticBytes;
+constexpr uint32_t kSqsMinMapWords = 1024;
+constexpr uint32_t kSqsGroups = (kSqsDynBytes - 4 * kSqsMinMapWords) / 12;  // most groups a pass holds
 
 struct SqsDimK {  // a dimension of the pass, in evaluation order
     const uint64_t *fk;
@@ -1214,6 +1218,7 @@ struct SqsArgs {
     SqMeasK m[AHEAD_SQ_MAX_MEAS];
     const uint32_t *maps;  // presence and attribute maps; the first lds_words words go to LDS
     uint32_t lds_words;
+    uint32_t groups;       // group table entries in LDS (the maps follow them)
     const uint32_t *ptab;  // [np][256] pass bits
 };
 
@@ -1245,10 +1250,10 @@ struct SqsTab {
     uint32_t t[AHEAD_SQ_MAX_PREDS * 256];
 };
 
-struct SqsCls {  // a dictionary-coded dimension's class values by code (code 0: no query holds the key)
-    uint32_t v[256];
+struct SqsCls {  // a dictionary-coded dimension's classes by code: presence bits, attribute field
+    uint32_t v[512];  // (code 0: no query holds the key)
 };
-constexpr uint32_t kSqsDictWords = 257;  // per dimension: 256 slots of distinct values, an overflow flag
+constexpr uint32_t kSqsDictSlots = 257;  // per dimension, 64-bit: 256 slots of distinct values, an overflow flag
 
 __device__ __forceinline__ void push_event_s(const SqsQ &Q, uint64_t row, uint32_t kind, uint32_t drow) {
     const uint32_t i = __hip_atomic_fetch_add(gblw(Q.ev_count), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1353,24 +1358,23 @@ __device__ void exact_row_s(const SqsArgs &a, const SqsQ &Q, uint64_t row, unsig
 template <int ND, int NP, int PWB>
 __global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc *__restrict__ desc) {
     constexpr int kSeg = 256, R = 4;  // rows per wave step (2 chunks of 128), rows per lane
-    constexpr uint32_t OFFB = 8u;     // a staged row: offset | pass bits << OFFB, then its attribute word
-    __shared__ uint32_t s_map[kSqsMapWords];
-    __shared__ unsigned long long s_sum[kSqsGroups];
-    __shared__ uint32_t s_first[kSqsGroups];
+    constexpr uint32_t OFFB = 8u;     // a staged row: its pass bits, then offset | attribute fields << OFFB
+    extern __shared__ __align__(16) unsigned char s_dyn[];  // group sums, first rows, maps
     __shared__ uint32_t s_stage[kSqWaves * kSeg * 2];
     __shared__ uint32_t s_ptab[AHEAD_SQ_MAX_PREDS * 256];
     __shared__ uint32_t s_qd[kSqsMaxQ * kSqsQWords];
     __shared__ uint32_t s_kept[kSqsMaxQ];
-    static_assert(sizeof(s_map) + sizeof(s_sum) + sizeof(s_first) + sizeof(s_stage) + sizeof(s_ptab) + sizeof(s_qd) +
-                      sizeof(s_kept) <= 160 * 1024 - 1024,
-                  "LDS budget");
-    static_assert((1u << OFFB) == kSeg && kSqsMaxQ <= 32u - OFFB, "staged entry layout");
+    static_assert(sizeof(s_stage) + sizeof(s_ptab) + sizeof(s_qd) + sizeof(s_kept) == kSqsStaticBytes, "LDS budget");
+    unsigned long long *s_sum = reinterpret_cast<unsigned long long *>(s_dyn);
+    uint32_t *s_first = reinterpret_cast<uint32_t *>(s_dyn + 8u * a.groups);
+    uint32_t *s_map = reinterpret_cast<uint32_t *>(s_dyn + 12u * a.groups);
+    static_assert((1u << OFFB) == kSeg && kSqsMaxQ <= 32u && kSqsFieldBits <= 32u - OFFB, "staged entry layout");
     uint32_t qmask = 0;  // queries whose plan holds (no repeated key)
     for (uint32_t q = 0; q < a.nq; ++q)
         if (gbl(desc->q[q].flags)[kFlagDup] == 0u) qmask |= 1u << q;
     for (uint32_t i = threadIdx.x; i < a.lds_words; i += kSqBlock) s_map[i] = a.maps[i];
     for (uint32_t i = threadIdx.x; i < a.np * 256u; i += kSqBlock) s_ptab[i] = a.ptab[i];
-    for (uint32_t i = threadIdx.x; i < kSqsGroups; i += kSqBlock) {
+    for (uint32_t i = threadIdx.x; i < a.groups; i += kSqBlock) {
         s_sum[i] = 0ull;
         s_first[i] = 0xFFFFFFFFu;
     }
@@ -1554,16 +1558,19 @@ __global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc 
                 for (int r = 0; r < R; ++r) w[r] = a.maps[dd.map_off + (idx[r] >> sh)];
             }
             const uint32_t fmask = (1u << dd.ab) - 1u;
-            const uint32_t pmask = (1u << dd.pbits) - 1u;
+            const uint32_t pmask = dd.pbits >= 32u ? ~0u : (1u << dd.pbits) - 1u;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                uint32_t e = map_entry(w[r], idx[r], dd.lb);
-                if (dd.dict) e = s_map[dd.cls_off + e];
-                const uint32_t pres = e & pmask;
+                const uint32_t e = map_entry(w[r], idx[r], dd.lb);
+                uint32_t pres = e & pmask, fd = 0;
+                if (dd.dict) {
+                    pres = s_map[dd.cls_off + 2u * e];
+                    fd = s_map[dd.cls_off + 2u * e + 1u];
+                }
                 pass[r] &= dd.all ? (pres != 0u ? ~0u : 0u) : pres;
                 if (pass[r] == 0u) idx[r] = 0;
                 if (dd.comb && dd.ab != 0u && pass[r] != 0u) {
-                    const uint32_t f = (e >> dd.pbits) & fmask;
+                    const uint32_t f = dd.dict ? fd : (e >> dd.pbits) & fmask;
                     if (f == fmask) {
                         redo[r] |= pass[r];
                         pass[r] = 0u;
@@ -1609,8 +1616,8 @@ __global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc 
             const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
                                                             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
             if (al) {
-                stage[2 * (total + below)] = ((r >> 1) * 128u + lane * 2u + (r & 1)) | (pass[r] << OFFB);
-                stage[2 * (total + below) + 1] = attr[r];
+                stage[2 * (total + below)] = pass[r];
+                stage[2 * (total + below) + 1] = ((r >> 1) * 128u + lane * 2u + (r & 1)) | (attr[r] << OFFB);
             }
             total += static_cast<uint32_t>(__popcll(b));
         }
@@ -1622,11 +1629,11 @@ __global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc 
             const uint32_t i0 = b * 64u;
             if (i0 < total) {
                 const bool act = i0 + lane < total;
-                const uint32_t ent = act ? stage[2 * (i0 + lane)] : 0u;
+                const uint32_t ent = act ? stage[2 * (i0 + lane) + 1] : 0u;
                 pend_act = act;
                 pend_row = seg + (ent & ((1u << OFFB) - 1u));
-                pend_pass = ent >> OFFB;
-                pend_attr = act ? stage[2 * (i0 + lane) + 1] : 0u;
+                pend_pass = act ? stage[2 * (i0 + lane)] : 0u;
+                pend_attr = ent >> OFFB;
 #pragma unroll
                 for (int m = 0; m < AHEAD_SQ_MAX_MEAS; ++m)
                     if (m < static_cast<int>(a.nm)) pend_code[m] = load_code(a.m[m].col, a.m[m].width, act ? pend_row : dummy);
@@ -1662,7 +1669,7 @@ __global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc 
                 cur = m;
                 if (b < R) cur_row = seg + (b >> 1) * 128u + lane * 2u + (b & 1);
                 else if (b == R) cur_row = prev_row;
-                else cur_row = seg + (stage[2 * ((b - R - 1) * 64 + lane)] & ((1u << OFFB) - 1u));
+                else cur_row = seg + (stage[2 * ((b - R - 1) * 64 + lane) + 1] & ((1u << OFFB) - 1u));
             }
             if (cur != 0u) {
                 const uint32_t q = __ffs(cur) - 1;
@@ -1709,12 +1716,12 @@ __global__ __launch_bounds__(kSqBlock) void k_sqs_scan(SqsArgs a, const SqsDesc 
 // bit for all of them), an attribute map (mode 1: per key its attribute index over the union range
 // + 1, all ones for a corrupt attribute, 0 where no query selects it) or both in one entry (mode 2:
 // the attribute field above pbits presence bits), from the queries' own maps.  Mode 3 inserts the
-// distinct non-zero mode-2 values into a 256-slot table (an overflow sets *flag; lb must be 5);
-// mode 4 writes each key's class code (code_of[slot of its value], 0 for none).  Queries with a
-// repeated key are left out.
+// distinct (presence, field) pairs of the selected keys, as field << 32 | presence, into a 256-slot
+// table (an overflow sets table[256]; lb must be 5); mode 4 writes each key's class code
+// (code_of[slot of its pair], 0 for none).  Queries with a repeated key are left out.
 struct SqsCombineArgs {
     uint32_t span, nq, lb, mode, all, pbits, words;
-    uint32_t *table, *flag;
+    unsigned long long *table;
     uint8_t code_of[256];
     const uint32_t *qmap[kSqsMaxQ];  // each query's map of this dimension
     const uint32_t *qflags[kSqsMaxQ];
@@ -1744,20 +1751,21 @@ __global__ __launch_bounds__(kBlock) void k_sqs_combine(SqsCombineArgs a) {
                 f = v == a.qerrv[q] ? a.fall : v + a.qadd[q];
             }
             const uint32_t pv = a.all ? (pres != 0u ? 1u : 0u) : pres;
-            uint32_t e = a.mode == 1 ? f : a.mode == 0 ? pv : pv | (f << a.pbits);
-            if (a.mode >= 3 && e != 0u) {
-                uint32_t h = (e * 0x9E3779B1u) >> 24;
+            uint32_t e = a.mode == 1 ? f : a.mode == 0 || a.mode >= 3 || a.pbits >= 32u ? pv : pv | (f << a.pbits);
+            if (a.mode >= 3 && pv != 0u) {
+                const unsigned long long v = (static_cast<unsigned long long>(f) << 32) | pv;
+                uint32_t h = static_cast<uint32_t>((v * 0x9E3779B97F4A7C15ull) >> 56);
                 bool done = false;
                 for (uint32_t k = 0; k < 256u && !done; ++k, h = (h + 1u) & 255u) {
-                    const uint32_t cur = a.table[h];
-                    if (cur == e) {
+                    const unsigned long long cur = a.table[h];
+                    if (cur == v) {
                         done = true;
-                    } else if (cur == 0u && a.mode == 3) {
-                        const uint32_t old = atomicCAS(&a.table[h], 0u, e);
-                        done = old == 0u || old == e;
+                    } else if (cur == 0ull && a.mode == 3) {
+                        const unsigned long long old = atomicCAS(&a.table[h], 0ull, v);
+                        done = old == 0ull || old == v;
                     }
                 }
-                if (a.mode == 3 && !done) atomicOr(a.flag, 1u);
+                if (a.mode == 3 && !done) atomicOr(&a.table[256], 1ull);
                 e = done ? a.code_of[(h - 1u) & 255u] : 0u;
             }
             word |= e << (j << a.lb);
@@ -2822,7 +2830,7 @@ bool sqs_try_add(SqsPass &ps, uint32_t qi, const std::vector<SqQueryPlan> &plans
         if (ab > 24) return false;
         fields += ab;
     }
-    if (fields > 32) return false;  // a row's union attribute fields share one word
+    if (fields > kSqsFieldBits) return false;  // a staged row's union attribute fields share one word
     ps.qi.push_back(qi);
     ps.groups += static_cast<uint32_t>(Q.groups);
     std::copy(umin, umin + q->n_dims, ps.umin);
@@ -2843,21 +2851,31 @@ size_t sqs_region_bytes(const ahead_sq_args *const *qs, uint32_t nq) {
         maps = std::max(maps, m);
     }
     return align_up(sizeof(SqsDesc), 256) + align_up(sizeof(SqsTab), 256) +
-           align_up(AHEAD_SQ_MAX_DIMS * kSqsDictWords * 4, 256) + AHEAD_SQ_MAX_DIMS * sizeof(SqsCls) + maps;
+           align_up(AHEAD_SQ_MAX_DIMS * kSqsDictSlots * 8, 256) + AHEAD_SQ_MAX_DIMS * sizeof(SqsCls) + maps;
+}
+
+// One instantiation with its group tables and maps in dynamic LDS (the limit raised once).
+template <int ND, int NP, int PWB>
+int launch_sqs_one(const SqsArgs &a, const SqsDesc *d, uint32_t grid, hipStream_t s) {
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sqs_scan<ND, NP, PWB>),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, kSqsDynBytes);
+    AHEAD_HIP_TRY(attr);
+    const size_t dyn = 12u * static_cast<size_t>(a.groups) + 4u * static_cast<size_t>(a.lds_words);
+    hipLaunchKernelGGL((k_sqs_scan<ND, NP, PWB>), dim3(grid), dim3(kSqBlock), dyn, s, a, d);
+    return AHEAD_OK;
 }
 
 template <int ND, int NP>
-void launch_sqs_w(const SqsArgs &a, const SqsDesc *d, uint32_t pwb, uint32_t grid, hipStream_t s) {
-    if (pwb == 2) hipLaunchKernelGGL((k_sqs_scan<ND, NP, 2>), dim3(grid), dim3(kSqBlock), 0, s, a, d);
-    else hipLaunchKernelGGL((k_sqs_scan<ND, NP, 0>), dim3(grid), dim3(kSqBlock), 0, s, a, d);
+int launch_sqs_w(const SqsArgs &a, const SqsDesc *d, uint32_t pwb, uint32_t grid, hipStream_t s) {
+    return pwb == 2 ? launch_sqs_one<ND, NP, 2>(a, d, grid, s) : launch_sqs_one<ND, NP, 0>(a, d, grid, s);
 }
 
 template <int ND>
-void launch_sqs_np(const SqsArgs &a, const SqsDesc *d, uint32_t pwb, uint32_t grid, hipStream_t s) {
+int launch_sqs_np(const SqsArgs &a, const SqsDesc *d, uint32_t pwb, uint32_t grid, hipStream_t s) {
     switch (a.np) {
-        case 0: hipLaunchKernelGGL((k_sqs_scan<ND, 0, 0>), dim3(grid), dim3(kSqBlock), 0, s, a, d); break;
-        case 1: launch_sqs_w<ND, 1>(a, d, pwb, grid, s); break;
-        default: launch_sqs_w<ND, 2>(a, d, pwb, grid, s); break;
+        case 0: return launch_sqs_one<ND, 0, 0>(a, d, grid, s);
+        case 1: return launch_sqs_w<ND, 1>(a, d, pwb, grid, s);
+        default: return launch_sqs_w<ND, 2>(a, d, pwb, grid, s);
     }
 }
 
@@ -2876,8 +2894,10 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
     const uint32_t nd = q0->n_dims;
     SqsDesc *ddesc = reinterpret_cast<SqsDesc *>(region);
     SqsTab *dtab = reinterpret_cast<SqsTab *>(region + align_up(sizeof(SqsDesc), 256));
-    uint32_t *ddict = reinterpret_cast<uint32_t *>(region + align_up(sizeof(SqsDesc), 256) + align_up(sizeof(SqsTab), 256));
-    uint32_t *cmaps = ddict + align_up(AHEAD_SQ_MAX_DIMS * kSqsDictWords * 4, 256) / 4;
+    unsigned long long *ddict =
+        reinterpret_cast<unsigned long long *>(region + align_up(sizeof(SqsDesc), 256) + align_up(sizeof(SqsTab), 256));
+    uint32_t *cmaps = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(ddict) +
+                                                   align_up(AHEAD_SQ_MAX_DIMS * kSqsDictSlots * 8, 256));
 
     // map geometry per chain dimension: presence (one bit for all queries where none selects),
     // union attribute field; combined entries (presence | field) or split maps
@@ -2922,53 +2942,57 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
         c.fall = ab[d] != 0u ? (1u << ab[d]) - 1u : 0u;
         return c;
     };
-    // dictionary codes where the distinct values are few: each candidate dimension's values go
-    // into a 256-slot table on the device, read back with one wait for the pass
+    // this pass's LDS: its group tables, then the class tables and maps in kSqsMapWords words
+    const uint32_t lds_groups = (ps.groups + 63u) / 64u * 64u;
+    const uint32_t kSqsMapWords = (kSqsDynBytes - 12u * lds_groups) / 4u;
+    // dictionary codes where the distinct (presence, field) pairs are few: each candidate
+    // dimension's pairs go into a 256-slot table on the device, read back with one wait for the pass
     bool dict[AHEAD_SQ_MAX_DIMS] = {};
     uint32_t dlb[AHEAD_SQ_MAX_DIMS] = {}, ndict = 0;
     uint8_t code_of[AHEAD_SQ_MAX_DIMS][256] = {};
     SqsCls cls[AHEAD_SQ_MAX_DIMS] = {};
     bool cand[AHEAD_SQ_MAX_DIMS] = {}, any_cand = false;
     for (uint32_t d = 0; d < nd; ++d) {
-        const uint32_t vb = pbits[d] + ab[d];
-        cand[d] = vb > 4 && vb <= 32;
+        cand[d] = pbits[d] + ab[d] > 4;
         any_cand = any_cand || cand[d];
     }
     if (any_cand) {
-        if ((st = fill_u32(ddict, AHEAD_SQ_MAX_DIMS * kSqsDictWords, 0u, s)) != AHEAD_OK) return st;
+        if ((st = fill_u64(reinterpret_cast<uint64_t *>(ddict), AHEAD_SQ_MAX_DIMS * kSqsDictSlots, 0u, s)) != AHEAD_OK)
+            return st;
         for (uint32_t d = 0; d < nd; ++d) {
             if (!cand[d]) continue;
             SqsCombineArgs c = combine_args(d);
             c.mode = 3;
             c.lb = 5;
             c.words = c.span;
-            c.table = ddict + d * kSqsDictWords;
-            c.flag = c.table + 256;
+            c.table = ddict + d * kSqsDictSlots;
             hipLaunchKernelGGL(k_sqs_combine, dim3(prep_grid(c.words)), dim3(kBlock), 0, s, c);
         }
         AHEAD_HIP_TRY(hipGetLastError());
-        uint32_t h[AHEAD_SQ_MAX_DIMS * kSqsDictWords];
+        unsigned long long h[AHEAD_SQ_MAX_DIMS * kSqsDictSlots];
         AHEAD_HIP_TRY(hipMemcpyAsync(h, ddict, sizeof h, hipMemcpyDeviceToHost, s));
         AHEAD_HIP_TRY(hipStreamSynchronize(s));
         for (uint32_t d = 0; d < nd; ++d) {
-            const uint32_t *t = h + d * kSqsDictWords;
-            if (!cand[d] || t[256] != 0u) continue;
+            const unsigned long long *t = h + d * kSqsDictSlots;
+            if (!cand[d] || t[256] != 0ull) continue;
             uint32_t n = 0;
             for (uint32_t k = 0; k < 256; ++k)
-                if (t[k] != 0u) n++;
+                if (t[k] != 0ull) n++;
             if (n > 255) continue;
             uint32_t l = 0;
             while ((1u << (1u << l)) <= n) ++l;  // codes 0 .. n
             // codes narrower than the presence entry (no attribute) or than the combined entry
             // (an attribute: one lookup where the split maps take two)
             if (ab[d] == 0u ? l >= lb[d] : (cwords[d] != 0u && l >= clb[d])) continue;
+            if ((ndict + 1u) * 512u > kSqsMapWords) continue;  // class tables stay in LDS
             dict[d] = true;
             dlb[d] = l;
             uint32_t code = 0;
             for (uint32_t k = 0; k < 256; ++k) {
-                if (t[k] == 0u) continue;
+                if (t[k] == 0ull) continue;
                 code_of[d][k] = static_cast<uint8_t>(++code);
-                cls[d].v[code] = t[k];
+                cls[d].v[2 * code] = static_cast<uint32_t>(t[k]);
+                cls[d].v[2 * code + 1] = static_cast<uint32_t>(t[k] >> 32);
             }
             ++ndict;
         }
@@ -2977,7 +3001,6 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
         if (!dict[d]) continue;
         lb[d] = dlb[d];
         pwords[d] = words_of(Q0.P[d].span, dlb[d]);
-        ab[d] != 0u ? (void)0 : (void)0;
     }
     // LDS, most selective dimension (over the union) first while maps fit: a dimension with an
     // attribute takes the combined map if it fits (one LDS lookup), else the split maps if they
@@ -2988,7 +3011,7 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
     std::iota(by.begin(), by.end(), 0u);
     std::stable_sort(by.begin(), by.end(), [&](uint32_t x, uint32_t y) { return frac[x] < frac[y]; });
     bool plds[AHEAD_SQ_MAX_DIMS] = {}, alds[AHEAD_SQ_MAX_DIMS] = {};
-    uint32_t lds_words = ndict * 256u;  // the class tables of dictionary-coded dimensions come first
+    uint32_t lds_words = ndict * 512u;  // the class tables of dictionary-coded dimensions come first
     for (uint32_t d : by) {
         if (dict[d]) {
             comb[d] = true;
@@ -3026,7 +3049,7 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
     // map offsets: class tables, LDS maps, then the others
     uint32_t poff[AHEAD_SQ_MAX_DIMS] = {}, aoff[AHEAD_SQ_MAX_DIMS] = {}, coff[AHEAD_SQ_MAX_DIMS] = {}, words = 0;
     for (uint32_t d = 0; d < nd; ++d)
-        if (dict[d]) coff[d] = words, words += 256u;
+        if (dict[d]) coff[d] = words, words += 512u;
     for (uint32_t d : by) {
         if (plds[d]) poff[d] = words, words += pwords[d];
         if (alds[d]) aoff[d] = words, words += awords[d];
@@ -3074,7 +3097,7 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
         k.row_of = Q0.P[d].prep.row_of;
         SqsCombineArgs c = combine_args(d);
         if (dict[d]) {
-            c.table = ddict + d * kSqsDictWords;
+            c.table = ddict + d * kSqsDictSlots;
             std::copy(code_of[d], code_of[d] + 256, c.code_of);
             hipLaunchKernelGGL((k_sq_put<SqsCls>), dim3(1), dim3(64), 0, s, reinterpret_cast<SqsCls *>(cmaps + coff[d]),
                                cls[d]);
@@ -3119,6 +3142,7 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
     for (uint32_t m = 0; m < a.nm; ++m) a.m[m] = Q0.a.m[m];
     a.maps = cmaps;
     a.lds_words = lds_words;
+    a.groups = lds_groups;
     a.ptab = dtab->t;
 
     SqsDesc hd{};
@@ -3165,11 +3189,12 @@ int sqs_run_pass(std::vector<SqQueryPlan> &plans, const SqsPass &ps, char *regio
     AHEAD_HIP_TRY(hipGetLastError());
     if (a.n > 0) {
         switch (nd) {
-            case 1: launch_sqs_np<1>(a, ddesc, pwb, grid, s); break;
-            case 2: launch_sqs_np<2>(a, ddesc, pwb, grid, s); break;
-            case 3: launch_sqs_np<3>(a, ddesc, pwb, grid, s); break;
-            default: launch_sqs_np<4>(a, ddesc, pwb, grid, s); break;
+            case 1: st = launch_sqs_np<1>(a, ddesc, pwb, grid, s); break;
+            case 2: st = launch_sqs_np<2>(a, ddesc, pwb, grid, s); break;
+            case 3: st = launch_sqs_np<3>(a, ddesc, pwb, grid, s); break;
+            default: st = launch_sqs_np<4>(a, ddesc, pwb, grid, s); break;
         }
+        if (st != AHEAD_OK) return st;
         AHEAD_HIP_TRY(hipGetLastError());
     }
     return AHEAD_OK;

@@ -46,7 +46,7 @@ HBM_PER_ROW = {"P1": 16, "P2": 20, "P3": 20, "P4": 23}
 FUSED_OUT_BYTES = 4096 * 8 * 6 + 16 * 1024 * 8 + 4096
 # queries per batch in the batched form (ahead_hip_star_query_batch's same-shape pass,
 # AHEAD_SQ_MAX_SHAPE_PASS)
-MAX_BATCH = 24
+MAX_BATCH = 32
 
 
 @dataclass

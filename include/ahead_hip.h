@@ -427,7 +427,7 @@ size_t ahead_hip_star_query_ws_bytes(const ahead_sq_args *q);
  * reference (its drivers run one query at a time): it serves concurrent queries of a
  * device-resident schema.  ev_scan of qs[0] (if set) brackets all of the batch's scans. */
 #define AHEAD_SQ_MAX_BATCH 8
-#define AHEAD_SQ_MAX_SHAPE_PASS 24
+#define AHEAD_SQ_MAX_SHAPE_PASS 32
 int ahead_hip_star_query_batch(const ahead_sq_args *const *qs, const ahead_sq_out *const *outs, uint32_t nq,
                                ahead_hip_ws *ws, void *stream);
 /* Device bytes of workspace ahead_hip_star_query_batch uses for these queries; 0 for invalid. */
