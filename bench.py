@@ -659,7 +659,7 @@ def run_serve(args):
     # Poisson arrivals at multiples of the plain path's capacity; every form sees the same arrivals
     sweep = []
     base = sat["chain1"]["qps"]
-    for frac in (0.5, 0.9, 2.0, 4.0, 8.0, 16.0, 24.0):
+    for frac in (0.5, 0.9, 2.0, 4.0, 8.0, 16.0, 24.0, 32.0):
         rate = base * frac
         row = {"rate_qps": rate}
         for k, srv in servers.items():
