@@ -69,6 +69,7 @@ struct MergeArgs {
     uint64_t prov_cap;
     uint32_t *flags;    // [0]: a piece of the walk disagreed with its successor's entry state
     uint64_t *d_counts;
+    const uint32_t *fast;  // nonzero: the intersection path wrote every output (the sub-tile kernels return)
 };
 
 __device__ __forceinline__ uint64_t dec_at(const DCol &c, uint64_t i) { return c.dec_of(c.raw(i)); }
@@ -135,7 +136,11 @@ struct SplitArgs {
 };
 
 __global__ __launch_bounds__(kBlock) void k_mm_split(MergeArgs a, SplitArgs sp) {
+    if (*a.fast != 0u) return;
     const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    // the sub-tile kernels' flag and group sums start at zero
+    if (b == 0) a.flags[0] = 0u;
+    for (uint64_t k = b; k < a.num_grp * kMmStreams; k += static_cast<uint64_t>(gridDim.x) * blockDim.x) a.grp[k] = 0ull;
     if (b > sp.tiles) return;
     uint64_t i, st, j0;
     if (b == 0 || a.n1 == 0 || a.n2 == 0) {
@@ -619,6 +624,7 @@ __global__ __launch_bounds__(kMmThreads) void k_mm_count(MergeArgs a) {
     __shared__ uint16_t s_os[kMmDiag];  // local list-1 row of each output pair
     __shared__ uint64_t s_bound[kMmWaves];
     __shared__ uint32_t s_red[kMmWaves];
+    if (*a.fast != 0u) return;
     const uint64_t sub = blockIdx.x;
     const uint32_t t = threadIdx.x;
     SubTile T;
@@ -701,6 +707,7 @@ __global__ __launch_bounds__(kMmThreads) void k_mm_count(MergeArgs a) {
 // Exclusive prefixes of the group sums, per stream (one block), and the totals.
 __global__ __launch_bounds__(kBlock) void k_mm_gscan(MergeArgs a) {
     __shared__ uint64_t s_red[kWaves];
+    if (*a.fast != 0u) return;
     for (int s = 0; s < kMmStreams; ++s) {
         uint64_t *g = a.grp + s * a.num_grp;
         uint64_t carry = 0;
@@ -766,7 +773,7 @@ __global__ __launch_bounds__(kMmThreads) void k_mm_move(MergeArgs a) {
     __shared__ uint64_t s_red[kMmWaves];
     __shared__ uint32_t s_red32[kMmWaves];
     __shared__ uint64_t s_bound[kMmWaves];
-    if (a.flags[0] != 0u) return;  // the one-thread walk writes everything
+    if (a.flags[0] != 0u || *a.fast != 0u) return;  // the one-thread walk (or the intersection path) writes everything
     const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
     const uint64_t s0 = static_cast<uint64_t>(blockIdx.x) * kMmMoveSubs;
     const uint32_t nsub = static_cast<uint32_t>(a.num_sub - s0 < kMmMoveSubs ? a.num_sub - s0 : kMmMoveSubs);
@@ -822,7 +829,7 @@ __global__ __launch_bounds__(kMmThreads) void k_mm_move(MergeArgs a) {
 
 // The whole walk on one thread, for inputs whose chunk entry states could not be derived.
 __global__ void k_mm_fallback(MergeArgs a) {
-    if (threadIdx.x != 0 || a.flags[0] == 0u) return;
+    if (threadIdx.x != 0 || a.flags[0] == 0u || *a.fast != 0u) return;
     uint32_t cnt[kMmStreams] = {0, 0, 0, 0, 0};
     const uint64_t zero[kMmStreams] = {0, 0, 0, 0, 0};
     if (a.n1 && a.n2) (void) walk<true>(a, GlobalSrc{a}, 0, a.n1, st_pack(0, kPhTop), cnt, zero);
@@ -830,9 +837,511 @@ __global__ void k_mm_fallback(MergeArgs a) {
     for (int s = 0; s < kMmStreams; ++s) a.d_counts[s] = cnt[s];
 }
 
+// ---- the intersection path -------------------------------------------------------------------
+// When both key lists are strictly ascending in decoded value and every code of h1, t1, h2 and t2
+// is valid, no check of the walk fails and each list-1 key meets at most one list-2 key: the
+// walk's output is the sorted intersection, (h1[i], t2[j]) for every t1[i] == h2[j] in ascending
+// i, with empty error vectors.  Such inputs take streaming passes instead of the sub-tile walk:
+//  k_mi_clear  list 2's first and last keys give the key range [lo2, hi2] (the path applies when
+//              it spans at most kMiBitsPerRow bits per list-2 row), whose bitmap words it clears;
+//  k_mi_build  a bitmap of list 2's keys over the range (bit k - lo2) and, when the pairs' tails
+//              are gathered from t2, per 64-key word the list-2 row of its first key (the row of
+//              a key is that plus the set bits below it);
+//  k_mi_tails  the words the build waves share, ORed in;
+//  k_mi_count  per wave of list 1: the rows whose key is set;
+//  k_mi_scan   the waves' prefixes and the counts;
+//  k_mi_write  the pairs at the waves' prefixes.
+// The build and count passes check the conditions on every row they read (valid codes, each key
+// above the one before it); any failure clears the path's flag, the later passes return, and the
+// sub-tile kernels (which return while the flag is set) compute the outputs instead.
+constexpr uint64_t kMiBitsPerRow = 16;
+constexpr uint32_t kMiNone = 0xFFFFFFFFu;  // no bitmap word (words < 2^30)
+constexpr int kMiG = 4;  // rows per lane per step
+
+struct MiArgs {
+    uint32_t *ok;       // [0] nonzero while the path applies
+    uint64_t *range;    // [0] lo2, [1] bitmap words in use
+    uint64_t *bits;     // [words_cap]
+    uint32_t *first;    // [words_cap] list-2 row of the first key in each word that holds one
+    uint64_t words_cap;
+    uint32_t *wcnt;     // [waves1] matches per wave of list 1
+    uint32_t *wpre;     // [waves1] their exclusive prefixes
+    uint32_t *tail_w;   // [waves2] the last bitmap word of each build wave (kMiNone: none) ...
+    uint64_t *tail_m;   // [waves2] ... and the wave's bits of it, ORed in by k_mi_tails
+    uint32_t waves1, waves2;
+    uint64_t per1, per2;  // rows per wave of list 1 / list 2 (multiples of 64 * kMiG)
+};
+
+// The key range of list 2 from its first and last codes (every thread of k_mi_clear computes it;
+// thread 0 publishes it), and the bitmap words in use cleared.
+__global__ __launch_bounds__(kBlock) void k_mi_clear(MergeArgs a, MiArgs mi, uint32_t eligible) {
+    uint32_t ok = eligible && a.n1 > 0 && a.n2 > 0 ? 1u : 0u;
+    uint64_t lo = 0, words = 0;
+    if (ok) {
+        const uint64_t c0 = a.h2.raw(0), c1 = a.h2.raw(a.n2 - 1);
+        lo = a.h2.dec_of(c0);
+        const uint64_t hi = a.h2.dec_of(c1);
+        if (a.h2.bad_of(c0) || a.h2.bad_of(c1) || hi < lo) {
+            ok = 0;
+        } else {
+            words = ((hi - lo) >> 6) + 1;
+            if (words > mi.words_cap) ok = 0;
+        }
+    }
+    if (!ok) words = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        mi.range[0] = lo;
+        mi.range[1] = words;
+        mi.ok[0] = ok;
+    }
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < words; i += stride) mi.bits[i] = 0ull;
+}
+
+__device__ __forceinline__ uint64_t mi_rows_end(uint64_t begin, uint64_t per, uint64_t n) {
+    return begin + per < n ? begin + per : n;
+}
+
+// Lanes below the calling one whose ballot bit is set.
+__device__ __forceinline__ uint32_t mi_below(uint64_t ballot) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(ballot >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(ballot), 0u));
+}
+
+// Codes of N rows of a column of K codes, loaded as K (widened where they are used, so a load's
+// wait sits at its first use).  Rows that are not ok read row 0.
+template <typename K, int N>
+__device__ __forceinline__ void mi_load(const void *p, const uint64_t (&i)[N], const bool (&ok)[N], K (&x)[N]) {
+    const K *q = static_cast<const K *>(p);
+#pragma unroll
+    for (int k = 0; k < N; ++k) x[k] = q[ok[k] ? i[k] : 0ull];
+}
+
+// x needs a check of its own: it is encoded and not the column y with y's check.
+__device__ __forceinline__ bool mi_own_check(const DCol &x, const DCol &y) {
+    return x.encoded && !(x.data == y.data && x.width == y.width && y.encoded && x.a_inv == y.a_inv &&
+                          x.max_u == y.max_u && x.mask == y.mask);
+}
+
+// List 2: order and validity checks, bitmap words and first rows.  Each lane takes kMiR
+// consecutive rows of a step.  A wave's keys ascend, so it holds the last word its keys reached
+// (the open word) in registers until a key passes it, then stores it: every word a wave stores
+// holds all of its keys from that wave.  A word can hold keys of two waves only as the first's
+// last word and the second's first: the first leaves its last word to k_mi_tails (an atomic OR
+// after this kernel), so no word is stored twice.  A step's words [wf, wl] are ORed together in
+// the wave's LDS slice when they fit; a step spanning more words ORs the bits of the words between
+// the open word and wl (the wave's alone) into HBM with atomics.
+constexpr uint32_t kMiLdsWords = 512;
+constexpr int kMiR = kMiG;
+
+__device__ __forceinline__ uint64_t mi_wave_or(uint64_t x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x |= __shfl_xor(x, d, 64);
+    return x;
+}
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_mi_build(MergeArgs a, MiArgs mi) {
+    __shared__ unsigned long long s_bits[kWaves][kMiLdsWords];
+    if (mi.ok[0] == 0u) return;
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint32_t gw = blockIdx.x * kWaves + wv;
+    unsigned long long *sb = s_bits[wv];
+    for (uint32_t k = lane; k < kMiLdsWords; k += 64u) sb[k] = 0ull;
+    const uint64_t begin = static_cast<uint64_t>(gw) * mi.per2;
+    if (gw >= mi.waves2 || begin >= a.n2) return;
+    const uint64_t end = mi_rows_end(begin, mi.per2, a.n2);
+    const uint64_t lo = mi.range[0], words = mi.range[1];
+    const bool chk_t2 = mi_own_check(a.t2, a.h2);  // t2's codes are checked too
+    const bool need_first = !a.t2_from_h2;  // the write pass gathers t2 at the rows of the keys
+    bool bad = false;
+    bool has_prev = begin > 0;
+    uint64_t prev = has_prev ? a.h2.dec_of(a.h2.raw(begin - 1)) : 0ull;  // key of the row before the step
+    uint32_t ow = kMiNone;  // the open word (wave-uniform) and the wave's bits of it
+    uint64_t om = 0;
+    // a step's codes are loaded before the previous step's stores are issued (loads and stores
+    // complete in order, so a load issued after a store waits for it)
+    K cn[kMiR];
+    uint64_t cn2[kMiR] = {};
+    {
+        uint64_t rows[kMiR];
+        bool live[kMiR];
+#pragma unroll
+        for (int k = 0; k < kMiR; ++k) {
+            rows[k] = begin + static_cast<uint64_t>(lane) * kMiR + k;
+            live[k] = rows[k] < end;
+        }
+        mi_load<K, kMiR>(a.h2.data, rows, live, cn);
+        if (chk_t2) a.t2.raw_n<kMiR>(rows, live, cn2);
+    }
+    for (uint64_t seg = begin; seg < end; seg += 64u * kMiR) {
+        uint64_t rows[kMiR], c[kMiR], c2[kMiR], v[kMiR];
+        uint32_t w[kMiR];
+        bool live[kMiR], in[kMiR];
+#pragma unroll
+        for (int k = 0; k < kMiR; ++k) {
+            rows[k] = seg + static_cast<uint64_t>(lane) * kMiR + k;
+            live[k] = rows[k] < end;
+            c[k] = cn[k];
+            c2[k] = cn2[k];
+        }
+        {
+            uint64_t nrows[kMiR];
+            bool nlive[kMiR];
+#pragma unroll
+            for (int k = 0; k < kMiR; ++k) {
+                nrows[k] = seg + 64u * kMiR + static_cast<uint64_t>(lane) * kMiR + k;
+                nlive[k] = nrows[k] < end;
+            }
+            mi_load<K, kMiR>(a.h2.data, nrows, nlive, cn);
+            if (chk_t2) a.t2.raw_n<kMiR>(nrows, nlive, cn2);
+        }
+        const uint64_t last_row = (end < seg + 64u * kMiR ? end : seg + 64u * kMiR) - 1u;  // wave-uniform
+        const uint32_t last_lane = static_cast<uint32_t>((last_row - seg) / kMiR);
+        const int last_k = static_cast<int>((last_row - seg) % kMiR);
+#pragma unroll
+        for (int k = 0; k < kMiR; ++k) {
+            v[k] = a.h2.dec_of(c[k]);
+            const uint64_t off = v[k] - lo;
+            in[k] = live[k] && v[k] >= lo && (off >> 6) < words;
+            w[k] = in[k] ? static_cast<uint32_t>(off >> 6) : 0u;
+        }
+        // the key before each row: the lane's previous row, lane - 1's last row, or the step's predecessor
+        const uint64_t up = __shfl_up(v[kMiR - 1], 1, 64);
+        const uint64_t pv0 = lane == 0 ? prev : up;
+        const bool hp0 = lane > 0 || has_prev;
+#pragma unroll
+        for (int k = 0; k < kMiR; ++k) {
+            if (!live[k]) continue;
+            const uint64_t pv = k == 0 ? pv0 : v[k - 1];
+            if (a.h2.bad_of(c[k]) || (chk_t2 && a.t2.bad_of(c2[k])) || !in[k]) bad = true;
+            if ((k > 0 || hp0) && !(pv < v[k])) bad = true;
+            const bool first_of_word = k == 0 ? (!hp0 || pv < lo || ((pv - lo) >> 6) != w[0]) : w[k - 1] != w[k];
+            if (need_first && in[k] && first_of_word) mi.first[w[k]] = static_cast<uint32_t>(rows[k]);
+        }
+        uint32_t wl_own = w[0];
+#pragma unroll
+        for (int k = 1; k < kMiR; ++k) wl_own = k == last_k ? w[k] : wl_own;
+        const uint32_t wf = __builtin_amdgcn_readfirstlane(w[0]);
+        const uint32_t wl = __builtin_amdgcn_readlane(wl_own, static_cast<int>(last_lane));
+        const uint32_t span = wl >= wf ? wl - wf + 1u : ~0u;
+        if (span <= kMiLdsWords) {
+#pragma unroll
+            for (int k = 0; k < kMiR; ++k) {
+                const uint32_t o = w[k] - wf;
+                if (in[k] && o < span) atomicOr(sb + o, 1ull << ((v[k] - lo) & 63u));
+            }
+            const uint64_t x0 = sb[0], xl = sb[span - 1u];
+            for (uint32_t o = lane; o < span; o += 64u) {
+                const unsigned long long x = sb[o];
+                sb[o] = 0ull;
+                if (o > 0u && o + 1u < span && x != 0ull) mi.bits[wf + o] = x;  // all of the word's keys are in this step
+            }
+            if (ow == wf) {
+                om |= x0;
+            } else {
+                if (ow != kMiNone && lane == 0) mi.bits[ow] = om;
+                ow = wf;
+                om = x0;
+            }
+            if (span > 1u) {
+                if (lane == 0) mi.bits[ow] = om;
+                ow = wl;
+                om = xl;
+            }
+        } else if (wl > wf) {
+            uint64_t mo = 0, ml = 0;
+#pragma unroll
+            for (int k = 0; k < kMiR; ++k) {
+                if (!in[k]) continue;
+                const uint64_t bit = 1ull << ((v[k] - lo) & 63u);
+                if (w[k] == ow) mo |= bit;
+                else if (w[k] == wl) ml |= bit;
+                else atomicOr(reinterpret_cast<unsigned long long *>(mi.bits + w[k]), bit);
+            }
+            mo = mi_wave_or(mo);
+            ml = mi_wave_or(ml);
+            if (ow != kMiNone && lane == 0) mi.bits[ow] = om | mo;
+            ow = wl;
+            om = ml;
+        } else {  // out of order (the path does not apply): nothing more to store
+            bad = true;
+        }
+        uint64_t vl_own = v[0];
+#pragma unroll
+        for (int k = 1; k < kMiR; ++k) vl_own = k == last_k ? v[k] : vl_own;
+        prev = __shfl(vl_own, static_cast<int>(last_lane), 64);
+        has_prev = true;
+    }
+    if (lane == 0) {
+        mi.tail_w[gw] = ow;
+        mi.tail_m[gw] = om;
+    }
+    if (bad) mi.ok[0] = 0u;
+}
+
+// The build waves' last words, ORed into the bitmap (one thread per wave).
+__global__ __launch_bounds__(kBlock) void k_mi_tails(MergeArgs a, MiArgs mi) {
+    if (mi.ok[0] == 0u) return;
+    const uint32_t gw = blockIdx.x * kBlock + threadIdx.x;
+    if (gw >= mi.waves2 || static_cast<uint64_t>(gw) * mi.per2 >= a.n2) return;
+    const uint32_t w = mi.tail_w[gw];
+    if (w < mi.range[1]) atomicOr(reinterpret_cast<unsigned long long *>(mi.bits + w), static_cast<unsigned long long>(mi.tail_m[gw]));
+}
+
+// List 1, count pass: order and validity checks and, per wave, the rows whose key is set.  Each
+// lane takes kMiR consecutive rows of a step.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_mi_count(MergeArgs a, MiArgs mi) {
+    if (mi.ok[0] == 0u) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (gw >= mi.waves1) return;
+    const uint64_t begin = static_cast<uint64_t>(gw) * mi.per1;
+    const uint64_t end = begin < a.n1 ? mi_rows_end(begin, mi.per1, a.n1) : begin;
+    const uint64_t lo = mi.range[0], words = mi.range[1];
+    const bool chk_h1 = mi_own_check(a.h1, a.t1);  // h1's codes are checked too
+    bool bad = false;
+    bool has_prev = begin > 0 && begin < end;
+    uint64_t prev = has_prev ? a.t1.dec_of(a.t1.raw(begin - 1)) : 0ull;
+    uint32_t cnt = 0;
+    K cn[kMiR];
+    uint64_t chn[kMiR] = {};
+    {
+        uint64_t rows[kMiR];
+        bool live[kMiR];
+#pragma unroll
+        for (int k = 0; k < kMiR; ++k) {
+            rows[k] = begin + static_cast<uint64_t>(lane) * kMiR + k;
+            live[k] = rows[k] < end;
+        }
+        mi_load<K, kMiR>(a.t1.data, rows, live, cn);
+        if (chk_h1) a.h1.raw_n<kMiR>(rows, live, chn);
+    }
+    for (uint64_t seg = begin; seg < end; seg += 64u * kMiR) {
+        uint64_t rows[kMiR], c[kMiR], ch[kMiR], v[kMiR], word[kMiR];
+        bool live[kMiR], in[kMiR];
+#pragma unroll
+        for (int k = 0; k < kMiR; ++k) {
+            rows[k] = seg + static_cast<uint64_t>(lane) * kMiR + k;
+            live[k] = rows[k] < end;
+            c[k] = cn[k];
+            ch[k] = chn[k];
+        }
+        {  // the next step's codes, in flight with this step's bitmap reads
+            uint64_t nrows[kMiR];
+            bool nlive[kMiR];
+#pragma unroll
+            for (int k = 0; k < kMiR; ++k) {
+                nrows[k] = seg + 64u * kMiR + static_cast<uint64_t>(lane) * kMiR + k;
+                nlive[k] = nrows[k] < end;
+            }
+            mi_load<K, kMiR>(a.t1.data, nrows, nlive, cn);
+            if (chk_h1) a.h1.raw_n<kMiR>(nrows, nlive, chn);
+        }
+#pragma unroll
+        for (int k = 0; k < kMiR; ++k) {
+            v[k] = a.t1.dec_of(c[k]);
+            in[k] = live[k] && v[k] >= lo && ((v[k] - lo) >> 6) < words;
+        }
+#pragma unroll
+        for (int k = 0; k < kMiR; ++k) word[k] = in[k] ? mi.bits[(v[k] - lo) >> 6] : 0ull;
+        const uint64_t last_row = (end < seg + 64u * kMiR ? end : seg + 64u * kMiR) - 1u;
+        const uint32_t last_lane = static_cast<uint32_t>((last_row - seg) / kMiR);
+        const int last_k = static_cast<int>((last_row - seg) % kMiR);
+        const uint64_t up = __shfl_up(v[kMiR - 1], 1, 64);
+        const uint64_t pv0 = lane == 0 ? prev : up;
+        const bool hp0 = lane > 0 || has_prev;
+#pragma unroll
+        for (int k = 0; k < kMiR; ++k) {
+            if (!live[k]) continue;
+            const uint64_t pv = k == 0 ? pv0 : v[k - 1];
+            if (a.t1.bad_of(c[k]) || (chk_h1 && a.h1.bad_of(ch[k]))) bad = true;
+            if ((k > 0 || hp0) && !(pv < v[k])) bad = true;
+            if (in[k] && ((word[k] >> ((v[k] - lo) & 63u)) & 1ull) != 0ull) ++cnt;
+        }
+        uint64_t vl_own = v[0];
+#pragma unroll
+        for (int k = 1; k < kMiR; ++k) vl_own = k == last_k ? v[k] : vl_own;
+        prev = __shfl(vl_own, static_cast<int>(last_lane), 64);
+        has_prev = true;
+    }
+    cnt = wave_sum(cnt);
+    if (lane == 0) mi.wcnt[gw] = cnt;
+    if (bad) mi.ok[0] = 0u;
+}
+
+// Exclusive prefixes of the per-wave matches (one workgroup of 1024 threads) and the counts.
+__global__ __launch_bounds__(1024) void k_mi_scan(MergeArgs a, MiArgs mi) {
+    __shared__ uint32_t s_w[16];
+    if (mi.ok[0] == 0u) return;
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < mi.waves1; base += 8192u) {
+        uint32_t v[8], t = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t i = base + threadIdx.x * 8u + q;
+            v[q] = i < mi.waves1 ? mi.wcnt[i] : 0u;
+            t += v[q];
+        }
+        const uint32_t inc = wave_incl_scan(t);
+        if (lane == 63u) s_w[wv] = inc;
+        __syncthreads();
+        uint64_t pre = carry, tot = 0;
+#pragma unroll
+        for (int w = 0; w < 16; ++w) {
+            if (w < static_cast<int>(wv)) pre += s_w[w];
+            tot += s_w[w];
+        }
+        __syncthreads();
+        pre += inc - t;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t i = base + threadIdx.x * 8u + q;
+            if (i < mi.waves1) mi.wpre[i] = static_cast<uint32_t>(pre);
+            pre += v[q];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        a.d_counts[0] = carry;
+#pragma unroll
+        for (int s = 1; s < kMmStreams; ++s) a.d_counts[s] = 0ull;
+    }
+}
+
+// List 1, write pass: the pairs of the rows whose key is set, at the wave's prefix + their rank
+// (rows seg + g * 64 + lane, ranks from ballots).  The next step's keys are loaded while this
+// step's bitmap words are read.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_mi_write(MergeArgs a, MiArgs mi) {
+    if (mi.ok[0] == 0u) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (gw >= mi.waves1) return;
+    const uint64_t begin = static_cast<uint64_t>(gw) * mi.per1;
+    if (begin >= a.n1) return;
+    const uint64_t end = mi_rows_end(begin, mi.per1, a.n1);
+    const uint64_t lo = mi.range[0], words = mi.range[1];
+    uint64_t pos = mi.wpre[gw];
+    K c[kMiG];
+    {
+        uint64_t rows[kMiG];
+        bool live[kMiG];
+#pragma unroll
+        for (int g = 0; g < kMiG; ++g) {
+            rows[g] = begin + static_cast<uint64_t>(g) * 64u + lane;
+            live[g] = rows[g] < end;
+        }
+        mi_load<K, kMiG>(a.t1.data, rows, live, c);
+    }
+    for (uint64_t seg = begin; seg < end; seg += 64u * kMiG) {
+        uint64_t rows[kMiG], v[kMiG], word[kMiG];
+        bool live[kMiG], in[kMiG];
+#pragma unroll
+        for (int g = 0; g < kMiG; ++g) {
+            rows[g] = seg + static_cast<uint64_t>(g) * 64u + lane;
+            live[g] = rows[g] < end;
+            v[g] = a.t1.dec_of(static_cast<uint64_t>(c[g]));
+            in[g] = live[g] && v[g] >= lo && ((v[g] - lo) >> 6) < words;
+        }
+#pragma unroll
+        for (int g = 0; g < kMiG; ++g) word[g] = in[g] ? mi.bits[(v[g] - lo) >> 6] : 0ull;
+        {  // the next step's keys
+            uint64_t nrows[kMiG];
+            bool nlive[kMiG];
+#pragma unroll
+            for (int g = 0; g < kMiG; ++g) {
+                nrows[g] = seg + 64u * kMiG + static_cast<uint64_t>(g) * 64u + lane;
+                nlive[g] = nrows[g] < end;
+            }
+            mi_load<K, kMiG>(a.t1.data, nrows, nlive, c);
+        }
+#pragma unroll
+        for (int g = 0; g < kMiG; ++g) {
+            const uint32_t b = static_cast<uint32_t>((v[g] - lo) & 63u);
+            const bool hit = in[g] && ((word[g] >> b) & 1ull) != 0ull;
+            const uint64_t bal = __ballot(hit);
+            if (hit) {
+                const uint64_t p = pos + mi_below(bal);
+                if (p < a.out_cap) {
+                    const uint64_t h = a.h1_from_t1 ? enc_key(a.t1, a.a_t1, v[g]) : a.h1.raw(rows[g]);
+                    uint64_t t;
+                    if (a.t2_from_h2) {
+                        t = enc_key(a.h2, a.a_h2, v[g]);
+                    } else {
+                        const uint64_t j = static_cast<uint64_t>(mi.first[(v[g] - lo) >> 6]) +
+                                           static_cast<uint64_t>(__popcll(word[g] & ((1ull << b) - 1ull)));
+                        t = a.t2.raw(j);
+                    }
+                    store_w(a.out_h, p, a.wout_h, a.reenc_h ? h * a.f_h1 : h);
+                    store_w(a.out_t, p, a.wout_t, a.reenc_t ? t * a.f_t2 : t);
+                }
+            }
+            pos += static_cast<uint64_t>(__popcll(bal));
+        }
+    }
+}
+
 }  // namespace
 
 using namespace ahead_host;
+
+namespace {
+
+// Waves of an intersection kernel that can be resident at once (its persistent grid's cap).
+uint32_t mi_resident(const void *kernel) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0) != hipSuccess || per <= 0) per = 2;
+    return static_cast<uint32_t>(cus) * static_cast<uint32_t>(per) * kWaves;
+}
+
+// Waves of an intersection pass over n rows: at most `cap`, whole workgroups, at least 64 * kMiG
+// rows per wave.
+uint32_t mi_waves(uint64_t n, uint32_t cap) {
+    const uint64_t step = 64u * kMiG;
+    const uint64_t by_rows = (n + step - 1) / step;
+    uint64_t w = by_rows < cap ? by_rows : cap;
+    w = (w + kWaves - 1) / kWaves * kWaves;
+    return static_cast<uint32_t>(w ? w : kWaves);
+}
+
+// Calls f with a value of the unsigned type of a code width (1, 2, 4 or 8 bytes).
+template <class F>
+void by_width(uint32_t w, F f) {
+    switch (w) {
+        case 1: f(uint8_t{}); break;
+        case 2: f(uint16_t{}); break;
+        case 4: f(uint32_t{}); break;
+        default: f(uint64_t{}); break;
+    }
+}
+
+template <typename K>
+uint32_t mi_cap1() {
+    static const uint32_t c = std::min(mi_resident(reinterpret_cast<const void *>(k_mi_count<K>)),
+                                       mi_resident(reinterpret_cast<const void *>(k_mi_write<K>)));
+    return c;
+}
+
+template <typename K>
+uint32_t mi_cap2() {
+    static const uint32_t c = mi_resident(reinterpret_cast<const void *>(k_mi_build<K>));
+    return c;
+}
+
+uint64_t mi_per(uint64_t n, uint32_t waves) {
+    const uint64_t step = 64u * kMiG;
+    const uint64_t per = (n + waves - 1) / waves;
+    const uint64_t r = (per + step - 1) / step * step;
+    return r ? r : step;
+}
+
+}  // namespace
 
 extern "C" int ahead_hip_matchjoin_merge_an(const ahead_col *h1, const ahead_col *t1, const ahead_col *h2,
                                             const ahead_col *t2, int reenc_h, uint64_t f_h1, uint32_t wout_h,
@@ -903,7 +1412,32 @@ extern "C" int ahead_hip_matchjoin_merge_an(const ahead_col *h1, const ahead_col
     const size_t o_ph = o_grp + align_up(a.num_grp * 8 * kMmStreams, 256);
     const size_t o_pt = o_ph + align_up(a.prov_cap * wout_h, 256);
     const size_t o_end = o_pt + align_up(a.prov_cap * wout_t, 256);
-    if ((st = ws_ensure(ws, o_end)) != AHEAD_OK) return st;
+    // the intersection path's regions lie after the sub-tile path's, whose writes must leave the
+    // path's flag intact
+    MiArgs mi{};
+    mi.words_cap = kMiBitsPerRow * a.n2 / 64u + 2u;
+    // the path streams the key columns' codes: void key columns take the sub-tile path
+    const bool mi_eligible = t1->data != nullptr && h2->data != nullptr;
+    by_width(a.t1.width, [&](auto k) { mi.waves1 = mi_waves(a.n1, mi_cap1<decltype(k)>()); });
+    by_width(a.h2.width, [&](auto k) { mi.waves2 = mi_waves(a.n2, mi_cap2<decltype(k)>()); });
+    mi.per1 = mi_per(a.n1, mi.waves1);
+    mi.per2 = mi_per(a.n2, mi.waves2);
+    const size_t o_mi = o_end, o_bits = o_mi + 256, o_first = o_bits + align_up(mi.words_cap * 8, 256),
+                 o_wcnt = o_first + align_up(mi.words_cap * 4, 256),
+                 o_wpre = o_wcnt + align_up(static_cast<size_t>(mi.waves1) * 4, 256),
+                 o_tw = o_wpre + align_up(static_cast<size_t>(mi.waves1) * 4, 256),
+                 o_tm = o_tw + align_up(static_cast<size_t>(mi.waves2) * 4, 256),
+                 o_all = o_tm + align_up(static_cast<size_t>(mi.waves2) * 8, 256);
+    if ((st = ws_ensure(ws, o_all)) != AHEAD_OK) return st;
+    mi.ok = reinterpret_cast<uint32_t *>(ws_region(ws, o_mi));
+    mi.range = reinterpret_cast<uint64_t *>(ws_region(ws, o_mi + 64));
+    mi.bits = reinterpret_cast<uint64_t *>(ws_region(ws, o_bits));
+    mi.first = reinterpret_cast<uint32_t *>(ws_region(ws, o_first));
+    mi.wcnt = reinterpret_cast<uint32_t *>(ws_region(ws, o_wcnt));
+    mi.wpre = reinterpret_cast<uint32_t *>(ws_region(ws, o_wpre));
+    mi.tail_w = reinterpret_cast<uint32_t *>(ws_region(ws, o_tw));
+    mi.tail_m = reinterpret_cast<uint64_t *>(ws_region(ws, o_tm));
+    a.fast = mi.ok;
     a.flags = reinterpret_cast<uint32_t *>(ws_region(ws, o_flags));
     SplitArgs sp{};
     sp.diag = diag;
@@ -920,8 +1454,23 @@ extern "C" int ahead_hip_matchjoin_merge_an(const ahead_col *h1, const ahead_col
     a.prov_h = ws_region(ws, o_ph);
     a.prov_t = ws_region(ws, o_pt);
 
-    if ((st = fill_u32(a.flags, 1, 0u, s)) != AHEAD_OK) return st;
-    if ((st = fill_u64(a.grp, a.num_grp * kMmStreams, 0ull, s)) != AHEAD_OK) return st;
+    const uint64_t clear_blocks = (mi.words_cap + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_mi_clear, dim3(static_cast<uint32_t>(mi_eligible ? (clear_blocks < 1024 ? clear_blocks : 1024) : 1)),
+                       dim3(kBlock), 0, s, a, mi, mi_eligible ? 1u : 0u);
+    if (mi_eligible) {
+        by_width(a.h2.width, [&](auto k) {
+            hipLaunchKernelGGL(k_mi_build<decltype(k)>, dim3(mi.waves2 / kWaves), dim3(kBlock), 0, s, a, mi);
+        });
+        hipLaunchKernelGGL(k_mi_tails, dim3((mi.waves2 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, mi);
+        by_width(a.t1.width, [&](auto k) {
+            hipLaunchKernelGGL(k_mi_count<decltype(k)>, dim3(mi.waves1 / kWaves), dim3(kBlock), 0, s, a, mi);
+        });
+        hipLaunchKernelGGL(k_mi_scan, dim3(1), dim3(1024), 0, s, a, mi);
+        by_width(a.t1.width, [&](auto k) {
+            hipLaunchKernelGGL(k_mi_write<decltype(k)>, dim3(mi.waves1 / kWaves), dim3(kBlock), 0, s, a, mi);
+        });
+    }
+    // the sub-tile path: its kernels return when the intersection path applied
     hipLaunchKernelGGL(k_mm_split, dim3((subs + kBlock) / kBlock), dim3(kBlock), 0, s, a, sp);
     hipLaunchKernelGGL(k_mm_count, dim3(subs), dim3(kMmThreads), 0, s, a);
     hipLaunchKernelGGL(k_mm_gscan, dim3(1), dim3(kBlock), 0, s, a);

@@ -117,3 +117,84 @@ def test_merge_reencode_and_widths():
     run_both(h1, t1, h2, t2, RESOID, RESOID, RESOID, m_t2, 1, f_h1, 8, 1, f_t2, 4)
     # head-only re-encoding zeroes the tail factor, as the reference does
     run_both(h1, t1, h2, t2, RESOID, RESOID, RESOID, m_t2, 1, f_h1, 8, 1, 0, 4)
+
+
+def _span_list(n, span, seed, stream, base=1000):
+    """n strictly ascending keys in [base, base + span), both ends included."""
+    if n <= 2 or span <= 2:
+        return np.unique(np.array([base, base + span - 1], dtype=np.uint64))[:n]
+    inner = np.unique(datagen.uniform(3 * n, seed, stream, base + 1, base + span - 2))[: max(n - 2, 0)]
+    return np.union1d(np.array([base, base + span - 1], dtype=np.uint64), inner).astype(np.uint64)
+
+
+@pytest.mark.parametrize("n1,n2,bits_per_row", [(300_000, 200_000, 15.9), (300_000, 200_000, 16.2),
+                                                (70_000, 1_000_000, 1.5), (1_000_000, 3_000, 15.0),
+                                                (5, 64, 1.0), (64, 1, 1.0)])
+def test_merge_strictly_ascending_clean_lists(n1, n2, bits_per_row):
+    """Strictly ascending key lists with valid codes everywhere, whose walk is the intersection:
+    list 2's key span just below and above 16 bits per row, lists of one row and of one bitmap
+    word, mirrored lists, separate gathered tails and re-encoded heads."""
+    seed = n1 + n2
+    span = max(int(bits_per_row * n2), n2)
+    k2 = _span_list(n2, span, seed, 1)
+    n2 = k2.size
+    pick = datagen.uniform(n1 // 2 + 1, seed, 2, 0, n2 - 1).astype(np.int64)
+    k1 = np.union1d(k2[pick], datagen.uniform(n1 - n1 // 2, seed, 3, 0, 1000 + span + 500)).astype(np.uint64)
+    t1, h2 = oid_list(k1, seed, 4), oid_list(k2, seed, 5)
+    ref = run_both(t1, t1, h2, h2, RESOID, RESOID, RESOID, RESOID)
+    assert ref[0].size == np.intersect1d(k1, k2).size
+    h1 = oid_list(np.arange(k1.size, dtype=np.uint64) + 7, seed, 6)
+    t2 = datagen.encode(datagen.uniform(n2, seed, 7, 0, 0xFFFF), 63877, 4)
+    m_t2 = an.an_meta(an.RESSHORT, 63877)
+    f_h1 = (16089 * an.A_OID_INV) & an.MASK64
+    f_t2 = (233 * an.inverse(63877, 32)) & 0xFFFFFFFF
+    run_both(h1, t1, h2, t2, RESOID, RESOID, RESOID, m_t2, 1, f_h1, 8, 1, f_t2, 4)
+    run_both(h1, t1, h2, t2, RESOID, RESOID, RESOID, m_t2, 0, 0, 8, 0, 0, 4)
+
+
+@pytest.mark.parametrize("where", ["swap2", "dup2", "corrupt_t2", "swap1", "dup1", "corrupt_h1", "corrupt_t1_matched"])
+def test_merge_one_violation_in_ascending_lists(where):
+    """One row that breaks the intersection path's conditions (order, uniqueness or a valid code,
+    in either list; a corrupt head of a matched list-1 row, which the walk never checks): the
+    result is the reference walk's all the same."""
+    n1, n2 = 400_000, 300_000
+    k2 = _span_list(n2, 4 * n2, 5, 1)
+    k1 = np.union1d(k2[::3], datagen.uniform(n1 // 2, 5, 2, 0, 4 * n2 + 2000)).astype(np.uint64)
+    t1, h2 = oid_list(k1, 5, 3), oid_list(k2, 5, 4)
+    h1 = oid_list(np.arange(k1.size, dtype=np.uint64), 5, 5)
+    t2 = oid_list(np.arange(k2.size, dtype=np.uint64), 5, 6)
+    r2, r1 = k2.size * 2 // 3 + 1, k1.size // 2 + 1  # rows deep inside a list, off any word boundary
+    if where == "swap2":
+        h2[[r2, r2 + 1]] = h2[[r2 + 1, r2]]
+    elif where == "dup2":
+        h2[r2 + 1] = h2[r2]
+    elif where == "corrupt_t2":
+        t2[r2] ^= np.uint64(1 << 17)
+    elif where == "swap1":
+        t1[[r1, r1 + 1]] = t1[[r1 + 1, r1]]
+    elif where == "dup1":
+        t1[r1 + 1] = t1[r1]
+    elif where == "corrupt_h1":
+        h1[r1] ^= np.uint64(1 << 9)
+    else:
+        r = int(np.searchsorted(k1, k2[r2]))
+        assert k1[r] == k2[r2]
+        h1[r] ^= np.uint64(1 << 3)
+    run_both(h1, t1, h2, t2, RESOID, RESOID, RESOID, RESOID)
+    run_both(t1, t1, h2, h2, RESOID, RESOID, RESOID, RESOID)
+
+
+def test_merge_list2_dense_then_sparse():
+    """A list 2 whose density changes: 150,000 consecutive keys, then 2,000 keys 600 apart (a
+    stretch of it spans more bitmap words than one step stages), within 16 key values per row in
+    all; list 1 meets both stretches."""
+    dense = np.arange(10_000, 160_000, dtype=np.uint64)
+    sparse = np.uint64(160_000) + np.arange(1, 2_001, dtype=np.uint64) * np.uint64(600)
+    k2 = np.concatenate([dense, sparse])
+    assert (int(k2[-1]) - int(k2[0])) // 64 + 1 <= 16 * k2.size // 64 + 2
+    k1 = np.union1d(k2[::5], datagen.uniform(100_000, 9, 1, 0, int(k2[-1]) + 1000)).astype(np.uint64)
+    t1, h2 = oid_list(k1, 9, 2), oid_list(k2, 9, 3)
+    ref = run_both(t1, t1, h2, h2, RESOID, RESOID, RESOID, RESOID)
+    assert ref[0].size == np.intersect1d(k1, k2).size
+    t2 = oid_list(np.arange(k2.size, dtype=np.uint64) * np.uint64(3), 9, 4)
+    run_both(t1, t1, h2, t2, RESOID, RESOID, RESOID, RESOID)
