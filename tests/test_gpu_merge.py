@@ -198,3 +198,40 @@ def test_merge_list2_dense_then_sparse():
     assert ref[0].size == np.intersect1d(k1, k2).size
     t2 = oid_list(np.arange(k2.size, dtype=np.uint64) * np.uint64(3), 9, 4)
     run_both(t1, t1, h2, t2, RESOID, RESOID, RESOID, RESOID)
+
+
+@pytest.mark.parametrize("repeat", [False, True])
+def test_merge_output_capacity(repeat):
+    """Outputs smaller than the pairs: the first `cap` pairs are written, nothing past them, and the
+    count is every pair (the C ABI's contract), on the intersection path (strictly ascending keys,
+    a void head) and, with one key repeated in list 2, on the sub-tile path."""
+    import ctypes
+
+    k2 = _span_list(200_000, 600_000, 21, 1)
+    k1 = np.union1d(k2[::2], datagen.uniform(50_000, 21, 2, 0, 601_000)).astype(np.uint64)
+    t1, h2 = oid_list(k1, 21, 3), oid_list(k2, 21, 4)
+    if repeat:
+        h2[1001] = h2[1000]
+    t2 = oid_list(np.arange(k2.size, dtype=np.uint64) + 3, 21, 5)
+    ref = O.matchjoin_merge_an(O.void(k1.size, 7), O.Col(t1, RESOID), O.Col(h2, RESOID), O.Col(t2, RESOID))
+    total = ref[0].size
+    cap = total // 2
+    lib, ws = capi.load(), capi.workspace()
+    cols = [capi.Column.void(k1.size, 7), capi.Column.from_numpy(t1, RESOID), capi.Column.from_numpy(h2, RESOID),
+            capi.Column.from_numpy(t2, RESOID)]
+    d = [c.desc() for c in cols]
+    sentinel = np.full(cap + 64, 0xA5A5A5A5A5A5A5A5, dtype=np.uint64)
+    oh, ot = capi.DeviceBuffer.from_numpy(sentinel), capi.DeviceBuffer.from_numpy(sentinel)
+    errs = [capi.DeviceBuffer(8 * 16) for _ in range(4)]
+    cnt = capi.DeviceBuffer(8 * 8, zero=True)
+    capi.check(lib.ahead_hip_matchjoin_merge_an(*[ctypes.byref(x) for x in d], 0, 0, 8, 0, 0, 8, an.A_OID_ERR,
+                                                capi.Buf(oh.ptr, cap), capi.Buf(ot.ptr, cap),
+                                                *[capi.Buf(e.ptr, 16) for e in errs], cnt.ptr, ws.h,
+                                                capi.current_stream()))
+    capi.synchronize()
+    c = cnt.to_numpy(np.uint64, 5)
+    assert int(c[0]) == total and all(int(x) == r.size for x, r in zip(c[1:], ref[2:]))
+    got_h, got_t = oh.to_numpy(np.uint64, cap + 64), ot.to_numpy(np.uint64, cap + 64)
+    assert np.array_equal(got_h[:cap], ref[0][:cap].astype(np.uint64))
+    assert np.array_equal(got_t[:cap], ref[1][:cap].astype(np.uint64))
+    assert np.all(got_h[cap:] == sentinel[cap:]) and np.all(got_t[cap:] == sentinel[cap:])
