@@ -235,3 +235,22 @@ def test_merge_output_capacity(repeat):
     assert np.array_equal(got_h[:cap], ref[0][:cap].astype(np.uint64))
     assert np.array_equal(got_t[:cap], ref[1][:cap].astype(np.uint64))
     assert np.all(got_h[cap:] == sentinel[cap:]) and np.all(got_t[cap:] == sentinel[cap:])
+
+
+@pytest.mark.parametrize("kind", ["resshort", "restiny", "int", "shortint", "tinyint"])
+def test_merge_ascending_lists_of_every_key_width(kind):
+    """Strictly ascending key lists with valid codes in 4-, 2- and 1-byte key columns, encoded and
+    plain (the intersection path loads keys at the column's width); OID heads and tails."""
+    t, a, hi = {"resshort": (an.RESSHORT, 233, 65_535), "restiny": (an.RESTINY, 29, 255),
+                "int": (an.INT, 1, 3_000_000), "shortint": (an.SHORTINT, 1, 65_535),
+                "tinyint": (an.TINYINT, 1, 255)}[kind]
+    meta = an.an_meta(t, a) if t.encoded else an.plain_meta(t)
+    n = min(hi // 2, 400_000)
+    k2 = np.unique(datagen.uniform(n, 31, 1, 0, hi))
+    k1 = np.unique(datagen.uniform(n, 31, 2, 0, hi))
+    c1, c2 = datagen.encode(k1, a, t.width), datagen.encode(k2, a, t.width)
+    h1 = oid_list(np.arange(k1.size, dtype=np.uint64), 31, 3)
+    t2 = oid_list(np.arange(k2.size, dtype=np.uint64) + 11, 31, 4)
+    ref = run_both(h1, c1, c2, t2, RESOID, meta, meta, RESOID)
+    assert ref[0].size == np.intersect1d(k1, k2).size and ref[0].size > 0
+    run_both(c1, c1, c2, c2, meta, meta, meta, meta)
