@@ -94,7 +94,7 @@ int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
 using ahead_dev::kMaxSubtiles;
 struct Tiling {
     uint32_t max_subtiles = kMaxSubtiles;
-    uint32_t min_tiles = 1024;
+    uint32_t min_tiles = 512;
 };
 Tiling tiling();
 // Sub-tiles per look-back tile for a stream of `subs` sub-tiles: the largest power of two

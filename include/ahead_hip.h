@@ -83,7 +83,7 @@ size_t ahead_hip_ws_bytes(const ahead_hip_ws *ws);
 
 /* Tiling of the streaming operators: a look-back tile covers up to `max_subtiles` sub-tiles of
  * 16 KiB, as many as keep at least `min_tiles` tiles in the grid.  Process-wide; defaults 16 and
- * 1024.  Used by the autotuner and by tests that need multi-sub-tile tiles at small sizes.     */
+ * 512.  Used by the autotuner and by tests that need multi-sub-tile tiles at small sizes.      */
 int ahead_hip_set_tiling(uint32_t max_subtiles, uint32_t min_tiles);
 int ahead_hip_get_tiling(uint32_t *max_subtiles, uint32_t *min_tiles);
 
