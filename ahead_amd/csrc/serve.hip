@@ -347,7 +347,9 @@ void slot_main(ahead_serve *S) {
             adm = S->adm_next++;
         }
         if (!ok) {
-            std::lock_guard<std::mutex> lk(S->mu);
+            // the batch's admission ticket passes in its turn, so that no earlier ticket is skipped
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_adm.wait(lk, [&] { return adm == S->adm_serving; });
             for (Req *r : batch) {
                 r->rc = AHEAD_ERR_HIP;
                 r->error = "serve: slot initialisation failed: " + init_error;
