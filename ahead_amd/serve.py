@@ -231,8 +231,10 @@ class Server:
                 if not self._queue:
                     return
                 batch = form_batch(self._queue, self.max_batch)
+                # nothing that can fail lies between taking a ticket and serving it (a ticket
+                # never served would hold every later one)
+                need = self.need(batch[0].shape, len(batch))
                 ticket = self.admission.ticket()
-            need = self.need(batch[0].shape, len(batch))
             self.admission.acquire(ticket, need)
             try:
                 start = time.perf_counter() - t0
