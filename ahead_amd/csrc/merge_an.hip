@@ -27,7 +27,7 @@
 // Sub-tiles with an error position to write (rare) or whose walk left its LDS window walk again in
 // k_mm_move, writing at the final offsets directly.  Sub-tile 0 starts at (0, 0) in TOP, so if
 // every check holds the pieces are the reference's walk.  If any fails (the inputs are not sorted
-// in decoded value, e.g. a corrupted key), k_mm_fallback walks the whole input with one thread
+// in decoded value, e.g. a corrupted key), k_mm_gscan's first thread walks the whole input
 // instead: slower, same results.
 #include "rowtile.hpp"
 
@@ -721,6 +721,15 @@ __global__ __launch_bounds__(kBlock) void k_mm_gscan(MergeArgs a) {
         }
         if (threadIdx.x == 0 && a.flags[0] == 0u) a.d_counts[s] = carry;
     }
+    // inputs whose sub-tile entry states could not be derived: the whole walk on one thread (the
+    // sub-tile outputs are not moved then: k_mm_move returns)
+    if (threadIdx.x == 0 && a.flags[0] != 0u) {
+        uint32_t cnt[kMmStreams] = {0, 0, 0, 0, 0};
+        const uint64_t zero[kMmStreams] = {0, 0, 0, 0, 0};
+        if (a.n1 && a.n2) (void) walk<true>(a, GlobalSrc{a}, 0, a.n1, st_pack(0, kPhTop), cnt, zero);
+#pragma unroll
+        for (int st = 0; st < kMmStreams; ++st) a.d_counts[st] = cnt[st];
+    }
 }
 
 // Copies m provisional pairs from slot src to the final offset dst (one wave, 4 pairs per lane in
@@ -827,15 +836,6 @@ __global__ __launch_bounds__(kMmThreads) void k_mm_move(MergeArgs a) {
     }
 }
 
-// The whole walk on one thread, for inputs whose chunk entry states could not be derived.
-__global__ void k_mm_fallback(MergeArgs a) {
-    if (threadIdx.x != 0 || a.flags[0] == 0u || *a.fast != 0u) return;
-    uint32_t cnt[kMmStreams] = {0, 0, 0, 0, 0};
-    const uint64_t zero[kMmStreams] = {0, 0, 0, 0, 0};
-    if (a.n1 && a.n2) (void) walk<true>(a, GlobalSrc{a}, 0, a.n1, st_pack(0, kPhTop), cnt, zero);
-#pragma unroll
-    for (int s = 0; s < kMmStreams; ++s) a.d_counts[s] = cnt[s];
-}
 
 // ---- the intersection path -------------------------------------------------------------------
 // When both key lists are strictly ascending in decoded value and every code of h1, t1, h2 and t2
@@ -1475,7 +1475,6 @@ extern "C" int ahead_hip_matchjoin_merge_an(const ahead_col *h1, const ahead_col
     hipLaunchKernelGGL(k_mm_count, dim3(subs), dim3(kMmThreads), 0, s, a);
     hipLaunchKernelGGL(k_mm_gscan, dim3(1), dim3(kBlock), 0, s, a);
     hipLaunchKernelGGL(k_mm_move, dim3((subs + kMmMoveSubs - 1) / kMmMoveSubs), dim3(kMmThreads), 0, s, a);
-    hipLaunchKernelGGL(k_mm_fallback, dim3(1), dim3(64), 0, s, a);
     AHEAD_HIP_TRY(hipGetLastError());
     return AHEAD_OK;
 }
