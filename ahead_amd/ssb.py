@@ -684,7 +684,17 @@ def _fused_result(t: dict, q: Query, r: dict, E: "Errors", ops, fallback: bool) 
 def fused(t: dict, q: Query, ops=B, fallback: bool = True, ev_scan: tuple | None = None) -> dict:
     """The query with one fact scan (ops.star_query) after the chain's dimension selections.  The
     result has the chain's form; `status` is the fused entry point's (0: it applied).  With a
-    status other than 0 and `fallback`, the chain's result is returned (with that status)."""
+    status other than 0 and `fallback`, the chain's result is returned (with that status).  On the
+    device operators with one selection per dimension at most, the host waits once (the
+    selections' counts are read on the device; every output comes back in one copy)."""
+    if ops is B and all(len(d.sels) <= 1 for d in q.dims):
+        return _fused_batch_device(t, [q], fallback, ev_scan, single=True)[0]
+    return _fused_ops(t, q, ops, fallback, ev_scan)
+
+
+def _fused_ops(t: dict, q: Query, ops=B, fallback: bool = True, ev_scan: tuple | None = None) -> dict:
+    """fused() through the operator layer: each dimension selection returns its OID list (and
+    count) to the host before the star query runs."""
     E = Errors()
     r = ops.star_query(**_fused_args(t, q, ops, E), ev_scan=ev_scan)
     return _fused_result(t, q, r, E, ops, fallback)
@@ -707,12 +717,13 @@ _BATCH_GROUPS_CAP = 2048   # the fused form's group limit (more groups: its stat
 _BATCH_ERR_CAP = 256
 
 
-def _fused_batch_device(t: dict, qs: list, fallback: bool, ev_scan: tuple | None) -> list:
+def _fused_batch_device(t: dict, qs: list, fallback: bool, ev_scan: tuple | None, single: bool = False) -> list:
     """fused_batch on the device operators with one wait: every dimension selection (one per
     dimension at most) is launched without reading its counts back (the star queries read the OID
-    count on the device), then the batch; the selections' error vectors and all the queries'
-    outputs come back in one copy.  A query whose selection errors or outputs overflow the arena's
-    capacities runs again alone (fused() / StarQuery.results())."""
+    count on the device), then the batch (single: the one query through the single-query entry
+    point); the selections' error vectors and all the queries' outputs come back in one copy.  A
+    query whose selection errors or outputs overflow the arena's capacities runs again alone
+    (through the operator layer / StarQuery.results())."""
     import ctypes
 
     import torch
@@ -763,7 +774,10 @@ def _fused_batch_device(t: dict, qs: list, fallback: bool, ev_scan: tuple | None
                                   groups_cap=_BATCH_GROUPS_CAP, err_cap=_BATCH_ERR_CAP, out_arena=(arena, q_off[qi])))
     if ev_scan is not None:
         sqs[0].args.ev_scan[0], sqs[0].args.ev_scan[1] = ev_scan
-    capi.star_query_batch(sqs)
+    if single and len(sqs) == 1:
+        sqs[0].launch()
+    else:
+        capi.star_query_batch(sqs)
     host = torch.empty(off, dtype=torch.uint8, pin_memory=True)
     host.copy_(arena.t[:off], non_blocking=True)
     torch.cuda.current_stream().synchronize()
@@ -780,7 +794,7 @@ def _fused_batch_device(t: dict, qs: list, fallback: bool, ev_scan: tuple | None
                 rerun = True
             E.add(f"select_{attr}", h[so + 16: so + 16 + 8 * min(ne, _SEL_ERR_CAP)].view(np.uint64).copy())
         if rerun:
-            out.append(fused(t, q, fallback=fallback))
+            out.append(_fused_ops(t, q, fallback=fallback))
             continue
         r = sqs[qi].results_from(h, q_off[qi])
         if r is None:
